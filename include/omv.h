@@ -1,0 +1,261 @@
+/*
+ * omv.h -- C-ABI of the MI355X BDPT render core.
+ *
+ * The entry points replace the three C++ class interfaces the reference
+ * pathtracer's render loop sits behind (the reference has no plugin or FFI
+ * API of its own):
+ *
+ *   RenderScene   pathtracer/src/scene/RenderScene.h:109-204
+ *                 (allocate, setMesh, setInstance, setMaterial, setLight,
+ *                  setTexture, setBackground, setFalseColorSubject, commitScene)
+ *   BDPTRenderer  pathtracer/src/renderer/BDPTRenderer.h:147-155
+ *                 (setParameters, setScene, render)
+ *   PinholeCamera pathtracer/src/sensors/PinholeCamera.h + RenderSensor.h:63-96
+ *                 (ctor(w,h), setViewPoint, setYFoV, adjustToResolution,
+ *                  setRealHeight, getImpression, init/stop)
+ *
+ * Conventions (all entry points):
+ *   - plain C types and pointers only; host buffers are copied at "set" time
+ *     (the reference copies mesh arrays and textures in too,
+ *     RenderMesh.cpp:109-142, BitmapTexture.cpp:31-37);
+ *   - every function returns an int status: OMV_OK (0) or a negative
+ *     OMV_E_* code; omv_last_error() returns a thread-local message.
+ *     Out-of-range ids are reported as OMV_E_RANGE instead of being silently
+ *     ignored as the reference setters do (RenderScene.cpp:88,103,113,119,125);
+ *     the scene is left unchanged in that case, which is the reference's
+ *     observable behaviour;
+ *   - calls into one scene must not overlap; distinct scenes/renderers may be
+ *     used from distinct threads.
+ */
+#ifndef OMV_H
+#define OMV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define OMV_OK          0
+#define OMV_E_ARG      -1   /* null pointer / bad argument                 */
+#define OMV_E_RANGE    -2   /* id out of the allocated range               */
+#define OMV_E_STATE    -3   /* call not valid in the current state         */
+#define OMV_E_DEVICE   -4   /* HIP runtime error                           */
+#define OMV_E_SCENE    -5   /* commit failed: nothing renderable           */
+#define OMV_E_NOMEM    -6
+#define OMV_E_INTERNAL -7
+
+/* ------------------------------------------------------- parameter POD */
+/* Estimator extensions.  0 = the reference estimator exactly
+ * (BDPTRenderer.cpp:237-301).  Every bit below is a separately named mode
+ * that changes the estimator; none is enabled implicitly. */
+#define OMV_EXT_NONE        0u
+#define OMV_EXT_OM_ROUGH    1u   /* OpenMATERIAL complex-IOR Fresnel used as the
+                                    F term of a GGX rough conductor with
+                                    roughness params.om_roughness, instead of the
+                                    reference's ideal mirror whose connection
+                                    gate makes point-lit OM scenes black
+                                    (OpenMaterial.cpp:105-109)               */
+
+/* Mirrors RendererParameters (pathtracer/src/renderer/RendererParameters.h:24-96).
+ * omv_renderer_params_default() fills the reference defaults. */
+typedef struct omv_renderer_params {
+    int   camera_bounces;              /* cameraBouncesN   (-B)  default 10   */
+    int   light_bounces;               /* lightBouncesN    (-b)  default 10   */
+    int   max_path_length;             /* maxPathLength    (-q)  default 8    */
+    int   samples_per_pixel;           /* samplesPerPixel  (-S)  default 20   */
+    int   light_distance_attenuation;  /* (-A) 0,1,2             default 1    */
+    float light_scale_value;           /* default 100                         */
+    float light_min_distance;          /* default 0.01                        */
+    float ray_cut_pix_value;           /* default 0.002                       */
+    float gamma;                       /* post pass only; default 0.5         */
+    int   number_of_cores;             /* CPU threads (oracle only) def. 16   */
+    int   false_color_mode;            /* falseColorMode                      */
+    uint32_t seed;                     /* Philox4x32-10 key word 0            */
+    uint32_t extensions;               /* OMV_EXT_* bit set                   */
+    float om_roughness;                /* OMV_EXT_OM_ROUGH roughness (alpha=r^2)*/
+} omv_renderer_params;
+
+/* Pinhole camera state (PinholeCamera.h members + ViewPoint, RenderSensor.cpp:19-25).
+ * rotation is the row-major 3x3 whose columns are (left, up, forward). */
+typedef struct omv_camera_desc {
+    int   width, height;
+    float position[3];
+    float rotation[9];
+    float half_w, half_h, focus;
+} omv_camera_desc;
+
+/* glTF metallic-roughness material: every PbrMaterial getter the render loop
+ * reads per vertex (pathtracer/src/assets/materials/PbrMaterial.cpp:224-321). */
+#define OMV_PBR_BASE_COLOR          0
+#define OMV_PBR_METALLIC_ROUGHNESS  1
+#define OMV_PBR_NORMAL              2
+#define OMV_PBR_EMISSIVITY          3
+#define OMV_PBR_OCCLUSION           4
+#define OMV_PBR_TEXTURE_SLOTS       5
+
+#define OMV_ALPHA_BLEND   0
+#define OMV_ALPHA_MASK    1
+#define OMV_ALPHA_OPAQUE  2
+
+typedef struct omv_pbr_desc {
+    float base_color_factor[4];      /* clamped to [0,1] by the loader        */
+    float emissive_factor[3];
+    float metallic_factor;
+    float roughness_factor;
+    float normal_scale;              /* default 1                             */
+    float occlusion_strength;        /* default 1                             */
+    float ior;                       /* default 1.5                           */
+    float alpha_cutoff;              /* default 0.5                           */
+    int   alpha_mode;                /* OMV_ALPHA_*, default OPAQUE           */
+    int   double_sided;
+    int   texture_map[OMV_PBR_TEXTURE_SLOTS];     /* texture id or -1          */
+    int   texture_channel[OMV_PBR_TEXTURE_SLOTS]; /* uv channel, default 0     */
+    float texture_transform[OMV_PBR_TEXTURE_SLOTS][9]; /* KHR_texture_transform
+                                        3x3 row-major, identity by default    */
+} omv_pbr_desc;
+
+/* One OpenMATERIAL IOR table at one temperature
+ * (AssetMaterialIor::loadPropertiesFromJson, AssetMaterialIor.cpp:56-112). */
+typedef struct omv_ior_table {
+    float temperature;
+    int   n_count;
+    const float* n_wavelength;   /* metres */
+    const float* n_value;
+    int   k_count;
+    const float* k_wavelength;
+    const float* k_value;
+} omv_ior_table;
+
+/* False-colour subjects (FCSubject.h:189-482, names as in main.cpp:177). */
+#define OMV_FC_NONE   0
+#define OMV_FC_PID    1   /* "pid" primitive id          */
+#define OMV_FC_GID    2   /* "gid" geometry (mesh) id    */
+#define OMV_FC_MID    3   /* "mid" material id           */
+#define OMV_FC_RMP    4   /* "rmp" roughness map         */
+#define OMV_FC_MMP    5   /* "mmp" metallic map          */
+#define OMV_FC_MD     6   /* "md"  mesh density          */
+#define OMV_FC_SG     7   /* "sg"  surface gradient      */
+#define OMV_FC_MN     8   /* "mn"  material name         */
+#define OMV_FC_IN     9   /* "in"  inverted normals      */
+
+/* Render statistics. */
+typedef struct omv_render_stats {
+    uint64_t samples;          /* W*H*spp paths                                */
+    uint64_t rays;             /* extension + connection-visibility + mask
+                                  re-trace rays actually cast                   */
+    uint64_t camera_rays;
+    uint64_t light_rays;
+    uint64_t shadow_rays;
+    uint64_t nodes_visited;    /* BVH nodes fetched (all rays)                 */
+    uint64_t tris_tested;      /* triangle records tested (all rays)           */
+    uint64_t stack_overflows;  /* must be 0; nonzero flags a traversal error   */
+    double   render_ms;        /* device time of the render call (HIP events)  */
+    double   kernel_ms;        /* device time of the path kernel(s) only       */
+} omv_render_stats;
+
+/* ------------------------------------------------------------- errors */
+const char* omv_last_error(void);
+const char* omv_version(void);
+void omv_renderer_params_default(omv_renderer_params* p);
+
+/* ---------------------------------------------------------- scene (RenderScene) */
+typedef struct omv_scene omv_scene;
+
+int omv_scene_create(omv_scene** out);
+int omv_scene_destroy(omv_scene* s);
+/* RenderScene::allocate (RenderScene.cpp:51-80); one extra material slot
+ * (index materialsN) holds the "missing material" diffuse (1000,0,1000). */
+int omv_scene_allocate(omv_scene* s, int meshes, int instances, int materials,
+                       int textures, int lights);
+/* RenderScene::setMesh (RenderScene.cpp:82-99).  Strides are in floats.
+ * tex[i] may be NULL; texN <= 8 channels.  faces: 3*facesN uint32. */
+int omv_scene_set_mesh(omv_scene* s, int mesh_id, int vert_n, int faces_n, int tex_n,
+                       int mat_id, const uint32_t* faces,
+                       const float* vert, int vstride,
+                       const float* norm, int nstride,
+                       const float* tans, int tanstride,
+                       const float* const* tex, int texstride);
+/* RenderScene::setInstance (RenderScene.cpp:101-109); trans is a row-major 4x4,
+ * the top 3x4 is used. */
+int omv_scene_set_instance(omv_scene* s, int inst_id, const float trans[16], int mesh_id);
+/* RenderScene::setMaterial with an OpenMaterial (OpenMaterial.h); the IOR is
+ * looked up per RGB wavelength exactly as AssetMaterialIor::getIor
+ * (AssetMaterialIor.cpp:115-178) and baked at commit (temperature and the
+ * three wavelengths are constants). */
+int omv_scene_set_material_om(omv_scene* s, int mat_id, float asset_temperature,
+                              int n_tables, const omv_ior_table* tables);
+/* OpenMaterial with per-RGB complex IOR given directly (n + i k). */
+int omv_scene_set_material_om_nk(omv_scene* s, int mat_id, const float n_rgb[3],
+                                 const float k_rgb[3]);
+/* RenderScene::setMaterial with a PhysicallyBasedMaterial. */
+int omv_scene_set_material_pbr(omv_scene* s, int mat_id, const omv_pbr_desc* d);
+/* RenderScene::setTexture (RenderScene.cpp:123-127): w*h RGBA8 texels. */
+int omv_scene_set_texture(omv_scene* s, int tex_id, int w, int h, const uint8_t* rgba);
+/* fp32 RGBA texture; only read by the fp32-texture extension path. */
+int omv_scene_set_texture_f32(omv_scene* s, int tex_id, int w, int h, const float* rgba);
+/* RenderLightPoint::set (RenderLightPoint.cpp:20-26).  range<0 = infinite. */
+int omv_scene_set_light_point(omv_scene* s, int light_id, const float pos[3],
+                              const float intensity[3], float range);
+/* HDRBackground (HdrBackground.cpp:38-97): w*h RGB fp32, rotation row-major,
+ * radiance scale (main.cpp:400-405 uses the z-up->y-up matrix and pi). */
+int omv_scene_set_background_hdr(omv_scene* s, int w, int h, const float* rgb,
+                                 const float rotation[9], float scale);
+/* Background base class: constant emitted radiance (Background.h). */
+int omv_scene_set_background_constant(omv_scene* s, const float rgb[3]);
+/* RenderScene::setFalseColorSubject: "pid","gid","mid","rmp","mmp","md","sg","mn","in". */
+int omv_scene_set_false_color_subject(omv_scene* s, const char* name);
+/* RenderScene::commitScene (RenderScene.cpp:155-223): mesh normals/tangents,
+ * instance validation, light CDF, BVH build and upload.  log receives the
+ * reference's log lines, '\n'-separated (may be NULL).  Returns OMV_OK or
+ * OMV_E_SCENE when every instance is invalid. */
+int omv_scene_commit(omv_scene* s, char* log, size_t log_cap);
+/* Inspection: the committed interleaved vertex buffer of a mesh (stride
+ * 10+2*channels floats: pos3 normal3 tangent4 uv2*ch, RenderMesh.cpp:49-72). */
+int omv_scene_mesh_info(omv_scene* s, int mesh_id, int* vert_n, int* faces_n,
+                        int* stride, int* mat_id);
+int omv_scene_get_vertices(omv_scene* s, int mesh_id, float* out, size_t cap_floats);
+/* BVH statistics: node count, max depth, leaf triangle records. */
+int omv_scene_bvh_info(omv_scene* s, int* nodes, int* max_depth, int* tri_records,
+                       size_t* device_bytes);
+
+/* ----------------------------------------------------- camera (PinholeCamera) */
+typedef struct omv_camera omv_camera;
+int omv_camera_create(omv_camera** out, int width, int height);   /* PinholeCamera(w,h) */
+int omv_camera_destroy(omv_camera* c);
+/* ViewPoint(loc, xdir, ydir, zdir) + setViewPoint. */
+int omv_camera_set_viewpoint(omv_camera* c, const float pos[3], const float xdir[3],
+                             const float ydir[3], const float zdir[3]);
+int omv_camera_adjust_to_resolution(omv_camera* c);               /* :156-160 */
+int omv_camera_set_yfov(omv_camera* c, float yfov_deg);           /* :167-174 */
+int omv_camera_set_real_height(omv_camera* c, float height, float aspect);
+int omv_camera_set_focus(omv_camera* c, float focus);
+int omv_camera_get_desc(const omv_camera* c, omv_camera_desc* out);
+/* getImpression (:110-113) for one pixel, and the whole normalised film
+ * (W*H*4 floats, rgb + 1) after a render. */
+int omv_camera_get_impression(const omv_camera* c, int x, int y, float out[4]);
+int omv_camera_get_film(const omv_camera* c, float* out, size_t cap_floats);
+
+/* ------------------------------------------------- renderer (BDPTRenderer) */
+typedef struct omv_renderer omv_renderer;
+int omv_renderer_create(omv_renderer** out);
+int omv_renderer_destroy(omv_renderer* r);
+int omv_renderer_set_parameters(omv_renderer* r, const omv_renderer_params* p);
+int omv_renderer_set_scene(omv_renderer* r, omv_scene* s);
+/* BDPTRenderer::render (BDPTRenderer.cpp:372-393): init the film, run every
+ * sample on the GPU, normalise (PinholeCamera::stop).  Synchronous. */
+int omv_renderer_render(omv_renderer* r, omv_camera* c);
+/* False-colour pass (reference FC mode: main.cpp:261-270 forces spp=1,
+ * B=b=1): the camera film gets the subject's colours; ids receives four int32
+ * planes of W*H: global primitive id, mesh id, material id (-1 = missing
+ * material), instance id; -1 where the ray escapes.  tie (may be NULL) gets 1
+ * where the closest hit had an exact-t tie with a different triangle. */
+int omv_renderer_render_fc(omv_renderer* r, omv_camera* c, int32_t* ids, uint8_t* tie);
+int omv_renderer_get_stats(const omv_renderer* r, omv_render_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OMV_H */

@@ -1,0 +1,395 @@
+"""Python binding of the MI355X BDPT render core (libomv.so, C-ABI in include/omv.h).
+
+The classes mirror the reference pathtracer's host interfaces one to one:
+
+  Scene     RenderScene   (pathtracer/src/scene/RenderScene.h:109-204)
+  Camera    PinholeCamera (pathtracer/src/sensors/PinholeCamera.h)
+  Renderer  BDPTRenderer  (pathtracer/src/renderer/BDPTRenderer.h:147-155)
+
+Only the compiled HIP library is used; importing this module fails loudly when
+libomv.so is missing (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+import numpy as np
+
+_HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libomv.so"
+
+OMV_OK = 0
+EXT_NONE = 0
+EXT_OM_ROUGH = 1
+
+FC_SUBJECTS = ("pid", "gid", "mid", "rmp", "mmp", "md", "sg", "mn", "in")
+PBR_BASE_COLOR, PBR_METALLIC_ROUGHNESS, PBR_NORMAL, PBR_EMISSIVITY, PBR_OCCLUSION = range(5)
+ALPHA_BLEND, ALPHA_MASK, ALPHA_OPAQUE = range(3)
+
+
+class RendererParams(C.Structure):
+    _fields_ = [
+        ("camera_bounces", C.c_int), ("light_bounces", C.c_int), ("max_path_length", C.c_int),
+        ("samples_per_pixel", C.c_int), ("light_distance_attenuation", C.c_int),
+        ("light_scale_value", C.c_float), ("light_min_distance", C.c_float), ("ray_cut_pix_value", C.c_float),
+        ("gamma", C.c_float), ("number_of_cores", C.c_int), ("false_color_mode", C.c_int),
+        ("seed", C.c_uint32), ("extensions", C.c_uint32), ("om_roughness", C.c_float),
+    ]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int), ("position", C.c_float * 3), ("rotation", C.c_float * 9),
+        ("half_w", C.c_float), ("half_h", C.c_float), ("focus", C.c_float),
+    ]
+
+
+class PbrDesc(C.Structure):
+    _fields_ = [
+        ("base_color_factor", C.c_float * 4), ("emissive_factor", C.c_float * 3), ("metallic_factor", C.c_float),
+        ("roughness_factor", C.c_float), ("normal_scale", C.c_float), ("occlusion_strength", C.c_float),
+        ("ior", C.c_float), ("alpha_cutoff", C.c_float), ("alpha_mode", C.c_int), ("double_sided", C.c_int),
+        ("texture_map", C.c_int * 5), ("texture_channel", C.c_int * 5), ("texture_transform", (C.c_float * 9) * 5),
+    ]
+
+
+class IorTable(C.Structure):
+    _fields_ = [
+        ("temperature", C.c_float), ("n_count", C.c_int), ("n_wavelength", C.POINTER(C.c_float)),
+        ("n_value", C.POINTER(C.c_float)), ("k_count", C.c_int), ("k_wavelength", C.POINTER(C.c_float)),
+        ("k_value", C.POINTER(C.c_float)),
+    ]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [
+        ("samples", C.c_uint64), ("rays", C.c_uint64), ("camera_rays", C.c_uint64), ("light_rays", C.c_uint64),
+        ("shadow_rays", C.c_uint64), ("nodes_visited", C.c_uint64), ("tris_tested", C.c_uint64),
+        ("stack_overflows", C.c_uint64), ("render_ms", C.c_double), ("kernel_ms", C.c_double),
+    ]
+
+
+def pbr_desc(base_color=(1.0, 1.0, 1.0, 1.0), metallic=1.0, roughness=1.0, emissive=(0.0, 0.0, 0.0), normal_scale=1.0,
+             occlusion_strength=1.0, ior=1.5, alpha_cutoff=0.5, alpha_mode=ALPHA_OPAQUE, double_sided=False,
+             textures=None, channels=None, transforms=None) -> PbrDesc:
+    """PbrMaterial defaults (PbrMaterial.cpp:104-120); textures maps slot -> texture id."""
+    d = PbrDesc()
+    d.base_color_factor[:] = [float(x) for x in base_color]
+    d.emissive_factor[:] = [float(x) for x in emissive]
+    d.metallic_factor, d.roughness_factor = float(metallic), float(roughness)
+    d.normal_scale, d.occlusion_strength, d.ior, d.alpha_cutoff = float(normal_scale), float(occlusion_strength), float(ior), float(alpha_cutoff)
+    d.alpha_mode, d.double_sided = int(alpha_mode), int(bool(double_sided))
+    for s in range(5):
+        d.texture_map[s] = -1
+        d.texture_channel[s] = 0
+        d.texture_transform[s][:] = [1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0]
+    for s, t in (textures or {}).items():
+        d.texture_map[s] = int(t)
+    for s, ch in (channels or {}).items():
+        d.texture_channel[s] = int(ch)
+    for s, m in (transforms or {}).items():
+        d.texture_transform[s][:] = [float(x) for x in np.asarray(m, np.float32).reshape(9)]
+    return d
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
+
+
+def ior_tables(tables):
+    """tables: list of dicts {temperature, n: [[wl, v]...], k: [[wl, v]...]} -> (array, keepalive)."""
+    arr = (IorTable * len(tables))()
+    keep = []
+    for i, t in enumerate(tables):
+        n = np.asarray(t["n"], np.float32).reshape(-1, 2)
+        k = np.asarray(t["k"], np.float32).reshape(-1, 2)
+        nw, nv = np.ascontiguousarray(n[:, 0]), np.ascontiguousarray(n[:, 1])
+        kw, kv = np.ascontiguousarray(k[:, 0]), np.ascontiguousarray(k[:, 1])
+        keep += [nw, nv, kw, kv]
+        arr[i].temperature = float(t["temperature"])
+        arr[i].n_count, arr[i].k_count = len(nw), len(kw)
+        arr[i].n_wavelength, arr[i].n_value = _ptr(nw, C.c_float), _ptr(nv, C.c_float)
+        arr[i].k_wavelength, arr[i].k_value = _ptr(kw, C.c_float), _ptr(kv, C.c_float)
+    return arr, keep
+
+
+class SceneApi:
+    """Scene set-up shared by the product and the CPU oracle bindings (same
+    entry points, different prefix)."""
+
+    prefix = "omv"
+
+    def _fn(self, name):
+        return getattr(self.lib, f"{self.prefix}_{name}")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{self.prefix}_{what} failed ({rc}): {self._err()}")
+
+    def _err(self):
+        return ""
+
+    def allocate(self, meshes, instances, materials, textures, lights):
+        self._check(self._fn("scene_allocate")(self.h, meshes, instances, materials, textures, lights), "scene_allocate")
+
+    def set_mesh(self, mesh_id, vertices, faces, mat_id, normals=None, tangents=None, uvs=(), tex_n=None):
+        v = _f32(vertices, (-1, 3))
+        f = np.ascontiguousarray(np.asarray(faces, np.uint32).reshape(-1, 3))
+        n = _f32(normals, (-1, 3)) if normals is not None else None
+        t = _f32(tangents, (-1, 4)) if tangents is not None else None
+        uvl = [(_f32(u, (-1, 2)) if u is not None else None) for u in uvs]
+        texn = len(uvl) if tex_n is None else tex_n
+        arr = (C.POINTER(C.c_float) * max(1, texn))()
+        for i in range(texn):
+            arr[i] = _ptr(uvl[i], C.c_float) if i < len(uvl) and uvl[i] is not None else None
+        rc = self._fn("scene_set_mesh")(self.h, mesh_id, len(v), len(f), texn, mat_id, _ptr(f, C.c_uint32), _ptr(v, C.c_float), 3,
+                                         _ptr(n, C.c_float), 3, _ptr(t, C.c_float), 4, arr, 2)
+        self._check(rc, "scene_set_mesh")
+
+    def set_instance(self, inst_id, trans, mesh_id):
+        m = _f32(trans, (16,))
+        self._check(self._fn("scene_set_instance")(self.h, inst_id, _ptr(m, C.c_float), mesh_id), "scene_set_instance")
+
+    def set_material_om(self, mat_id, tables, temperature=300.0):
+        arr, keep = ior_tables(tables)
+        self._check(self._fn("scene_set_material_om")(self.h, mat_id, C.c_float(temperature), len(tables), arr), "scene_set_material_om")
+        del keep
+
+    def set_material_om_nk(self, mat_id, n, k):
+        nn, kk = _f32(n, (3,)), _f32(k, (3,))
+        self._check(self._fn("scene_set_material_om_nk")(self.h, mat_id, _ptr(nn, C.c_float), _ptr(kk, C.c_float)), "scene_set_material_om_nk")
+
+    def set_material_pbr(self, mat_id, desc: PbrDesc):
+        self._check(self._fn("scene_set_material_pbr")(self.h, mat_id, C.byref(desc)), "scene_set_material_pbr")
+
+    def set_texture(self, tex_id, rgba):
+        a = np.ascontiguousarray(np.asarray(rgba, np.uint8))
+        h, w = a.shape[0], a.shape[1]
+        self._check(self._fn("scene_set_texture")(self.h, tex_id, w, h, _ptr(a, C.c_uint8)), "scene_set_texture")
+
+    def set_texture_f32(self, tex_id, rgba):
+        a = _f32(rgba)
+        h, w = a.shape[0], a.shape[1]
+        self._check(self._fn("scene_set_texture_f32")(self.h, tex_id, w, h, _ptr(a, C.c_float)), "scene_set_texture_f32")
+
+    def set_light_point(self, light_id, pos, intensity, range_=-1.0):
+        p, i = _f32(pos, (3,)), _f32(intensity, (3,))
+        self._check(self._fn("scene_set_light_point")(self.h, light_id, _ptr(p, C.c_float), _ptr(i, C.c_float), C.c_float(range_)),
+                    "scene_set_light_point")
+
+    def set_background_hdr(self, rgb, rotation, scale):
+        a = _f32(rgb)
+        h, w = a.shape[0], a.shape[1]
+        r = _f32(rotation, (9,))
+        self._check(self._fn("scene_set_background_hdr")(self.h, w, h, _ptr(a, C.c_float), _ptr(r, C.c_float), C.c_float(scale)),
+                    "scene_set_background_hdr")
+
+    def set_background_constant(self, rgb):
+        a = _f32(rgb, (3,))
+        self._check(self._fn("scene_set_background_constant")(self.h, _ptr(a, C.c_float)), "scene_set_background_constant")
+
+    def set_false_color_subject(self, name):
+        self._check(self._fn("scene_set_false_color_subject")(self.h, name.encode()), "scene_set_false_color_subject")
+
+    def commit(self):
+        buf = C.create_string_buffer(1 << 16)
+        rc = self._fn("scene_commit")(self.h, buf, len(buf))
+        self._check(rc, "scene_commit")
+        return buf.value.decode()
+
+    def mesh_vertices(self, mesh_id):
+        vn, fn, st, mat = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self._check(self._fn("scene_mesh_info")(self.h, mesh_id, C.byref(vn), C.byref(fn), C.byref(st), C.byref(mat)), "scene_mesh_info")
+        out = np.zeros(max(1, vn.value * st.value), np.float32)
+        self._check(self._fn("scene_get_vertices")(self.h, mesh_id, _ptr(out, C.c_float), out.size), "scene_get_vertices")
+        return out[: vn.value * st.value].reshape(vn.value, st.value), mat.value
+
+
+def _load():
+    if not LIB_PATH.exists():
+        raise ImportError(f"libomv.so not built at {LIB_PATH}; run __graft_entry__.build()")
+    lib = C.CDLL(str(LIB_PATH))
+    lib.omv_last_error.restype = C.c_char_p
+    lib.omv_version.restype = C.c_char_p
+    vp = C.c_void_p
+    for name, args in {
+        "omv_scene_create": [C.POINTER(vp)], "omv_scene_destroy": [vp],
+        "omv_scene_allocate": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
+        "omv_scene_set_mesh": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_float), C.c_int,
+                               C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.POINTER(C.c_float)), C.c_int],
+        "omv_scene_set_instance": [vp, C.c_int, C.POINTER(C.c_float), C.c_int],
+        "omv_scene_set_material_om": [vp, C.c_int, C.c_float, C.c_int, C.POINTER(IorTable)],
+        "omv_scene_set_material_om_nk": [vp, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)],
+        "omv_scene_set_material_pbr": [vp, C.c_int, C.POINTER(PbrDesc)],
+        "omv_scene_set_texture": [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8)],
+        "omv_scene_set_texture_f32": [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)],
+        "omv_scene_set_light_point": [vp, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float],
+        "omv_scene_set_background_hdr": [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float],
+        "omv_scene_set_background_constant": [vp, C.POINTER(C.c_float)],
+        "omv_scene_set_false_color_subject": [vp, C.c_char_p],
+        "omv_scene_commit": [vp, C.c_char_p, C.c_size_t],
+        "omv_scene_mesh_info": [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+        "omv_scene_get_vertices": [vp, C.c_int, C.POINTER(C.c_float), C.c_size_t],
+        "omv_scene_bvh_info": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_size_t)],
+        "omv_camera_create": [C.POINTER(vp), C.c_int, C.c_int], "omv_camera_destroy": [vp],
+        "omv_camera_set_viewpoint": [vp] + [C.POINTER(C.c_float)] * 4,
+        "omv_camera_adjust_to_resolution": [vp], "omv_camera_set_yfov": [vp, C.c_float],
+        "omv_camera_set_real_height": [vp, C.c_float, C.c_float], "omv_camera_set_focus": [vp, C.c_float],
+        "omv_camera_get_desc": [vp, C.POINTER(CameraDesc)],
+        "omv_camera_get_impression": [vp, C.c_int, C.c_int, C.POINTER(C.c_float)],
+        "omv_camera_get_film": [vp, C.POINTER(C.c_float), C.c_size_t],
+        "omv_renderer_create": [C.POINTER(vp)], "omv_renderer_destroy": [vp],
+        "omv_renderer_set_parameters": [vp, C.POINTER(RendererParams)], "omv_renderer_set_scene": [vp, vp],
+        "omv_renderer_render": [vp, vp], "omv_renderer_render_fc": [vp, vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)],
+        "omv_renderer_get_stats": [vp, C.POINTER(RenderStats)],
+        "omv_renderer_params_default": [C.POINTER(RendererParams)],
+    }.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = C.c_int if name != "omv_renderer_params_default" else None
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def default_params(**kw) -> RendererParams:
+    p = RendererParams()
+    lib().omv_renderer_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Scene(SceneApi):
+    prefix = "omv"
+
+    def __init__(self):
+        self.lib = lib()
+        h = C.c_void_p()
+        self._check(self.lib.omv_scene_create(C.byref(h)), "scene_create")
+        self.h = h
+
+    def _err(self):
+        return self.lib.omv_last_error().decode()
+
+    def bvh_info(self):
+        n, d, r, b = C.c_int(), C.c_int(), C.c_int(), C.c_size_t()
+        self._check(self.lib.omv_scene_bvh_info(self.h, C.byref(n), C.byref(d), C.byref(r), C.byref(b)), "scene_bvh_info")
+        return {"nodes": n.value, "depth": d.value, "tri_records": r.value, "device_bytes": b.value}
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.omv_scene_destroy(self.h)
+            self.h = None
+
+
+class Camera:
+    """PinholeCamera: viewpoint from (position, forward, up) as main.cpp:459-476 builds it."""
+
+    def __init__(self, width, height, position=(0, 0, 0), forward=(0, 0, -1), up=(0, 1, 0), yfov=60.0):
+        self.lib = lib()
+        h = C.c_void_p()
+        self._check(self.lib.omv_camera_create(C.byref(h), width, height))
+        self.h = h
+        self.width, self.height = int(width), int(height)
+        self.look(position, forward, up)
+        self._check(self.lib.omv_camera_adjust_to_resolution(self.h))
+        self._check(self.lib.omv_camera_set_yfov(self.h, C.c_float(yfov)))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.lib.omv_last_error().decode())
+
+    def look(self, position, forward, up):
+        fwd = np.asarray(forward, np.float32)
+        upv = np.asarray(up, np.float32)
+        n = np.float32(np.sqrt(np.float32(fwd[0] * fwd[0] + fwd[1] * fwd[1]) + np.float32(fwd[2] * fwd[2])))
+        fwd = (fwd * (np.float32(1.0) / n)).astype(np.float32)
+        left = _cross32(fwd, upv)
+        upv = _cross32(left, fwd)
+        self.set_viewpoint(position, left, upv, fwd)
+
+    def set_viewpoint(self, pos, x, y, z):
+        a = [_f32(v, (3,)) for v in (pos, x, y, z)]
+        self._check(self.lib.omv_camera_set_viewpoint(self.h, *[_ptr(v, C.c_float) for v in a]))
+
+    def desc(self) -> CameraDesc:
+        d = CameraDesc()
+        self._check(self.lib.omv_camera_get_desc(self.h, C.byref(d)))
+        return d
+
+    def film(self):
+        out = np.zeros(self.width * self.height * 4, np.float32)
+        self._check(self.lib.omv_camera_get_film(self.h, _ptr(out, C.c_float), out.size))
+        return out.reshape(self.height, self.width, 4)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.omv_camera_destroy(self.h)
+            self.h = None
+
+
+def _cross32(a, b):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    return np.array([a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]], np.float32)
+
+
+class Renderer:
+    def __init__(self, params: RendererParams | None = None):
+        self.lib = lib()
+        h = C.c_void_p()
+        self._check(self.lib.omv_renderer_create(C.byref(h)))
+        self.h = h
+        self.params = params if params is not None else default_params()
+        self.set_parameters(self.params)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.lib.omv_last_error().decode())
+
+    def set_parameters(self, p: RendererParams):
+        self.params = p
+        self._check(self.lib.omv_renderer_set_parameters(self.h, C.byref(p)))
+
+    def set_scene(self, scene: Scene):
+        self._scene = scene
+        self._check(self.lib.omv_renderer_set_scene(self.h, scene.h))
+
+    def render(self, cam: Camera):
+        self._check(self.lib.omv_renderer_render(self.h, cam.h))
+        return cam.film()
+
+    def render_fc(self, cam: Camera):
+        npix = cam.width * cam.height
+        ids = np.zeros(4 * npix, np.int32)
+        tie = np.zeros(npix, np.uint8)
+        self._check(self.lib.omv_renderer_render_fc(self.h, cam.h, _ptr(ids, C.c_int32), _ptr(tie, C.c_uint8)))
+        return cam.film(), ids.reshape(4, cam.height, cam.width), tie.reshape(cam.height, cam.width)
+
+    def stats(self) -> dict:
+        s = RenderStats()
+        self._check(self.lib.omv_renderer_get_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in RenderStats._fields_}
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.omv_renderer_destroy(self.h)
+            self.h = None

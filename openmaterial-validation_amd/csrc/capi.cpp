@@ -1,0 +1,190 @@
+// capi.cpp -- the extern "C" boundary declared in include/omv.h.
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "render_core.h"
+
+struct omv_scene { omv::RenderScene s; };
+struct omv_camera { omv::PinholeCamera c; omv_camera(int w, int h) : c(w, h) {} };
+struct omv_renderer { omv::BDPTRenderer r; };
+
+#define OMV_NULLCHK(p) do { if (!(p)) { omv::setError(std::string(__func__) + ": null argument"); return OMV_E_ARG; } } while (0)
+
+extern "C" {
+
+const char* omv_last_error(void) { return omv::lastError(); }
+const char* omv_version(void) { return "omv-mi355x 0.1 (gfx950)"; }
+
+void omv_renderer_params_default(omv_renderer_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    // RendererParameters defaults (RendererParameters.h:24-96)
+    p->camera_bounces = 10;
+    p->light_bounces = 10;
+    p->max_path_length = 8;
+    p->samples_per_pixel = 20;
+    p->light_distance_attenuation = 1;
+    p->light_scale_value = 100.0f;
+    p->light_min_distance = 0.01f;
+    p->ray_cut_pix_value = 0.002f;
+    p->gamma = 0.5f;
+    p->number_of_cores = 16;
+    p->false_color_mode = 0;
+    p->seed = 0;
+    p->extensions = OMV_EXT_NONE;
+    p->om_roughness = 0.3f;
+}
+
+int omv_scene_create(omv_scene** out) {
+    OMV_NULLCHK(out);
+    *out = new (std::nothrow) omv_scene();
+    if (!*out) { omv::setError("omv_scene_create: out of memory"); return OMV_E_NOMEM; }
+    return OMV_OK;
+}
+int omv_scene_destroy(omv_scene* s) { delete s; return OMV_OK; }
+int omv_scene_allocate(omv_scene* s, int meshes, int instances, int materials, int textures, int lights) {
+    OMV_NULLCHK(s);
+    return s->s.allocate(meshes, instances, materials, textures, lights);
+}
+int omv_scene_set_mesh(omv_scene* s, int mesh_id, int vert_n, int faces_n, int tex_n, int mat_id, const uint32_t* faces,
+                       const float* vert, int vstride, const float* norm, int nstride, const float* tans, int tanstride,
+                       const float* const* tex, int texstride) {
+    OMV_NULLCHK(s);
+    return s->s.setMesh(mesh_id, vert_n, faces_n, tex_n, mat_id, faces, vert, vstride, norm, nstride, tans, tanstride, tex, texstride);
+}
+int omv_scene_set_instance(omv_scene* s, int inst_id, const float trans[16], int mesh_id) {
+    OMV_NULLCHK(s);
+    return s->s.setInstance(inst_id, trans, mesh_id);
+}
+int omv_scene_set_material_om(omv_scene* s, int mat_id, float t, int n_tables, const omv_ior_table* tables) {
+    OMV_NULLCHK(s);
+    return s->s.setMaterialOm(mat_id, t, n_tables, tables);
+}
+int omv_scene_set_material_om_nk(omv_scene* s, int mat_id, const float n[3], const float k[3]) {
+    OMV_NULLCHK(s); OMV_NULLCHK(n); OMV_NULLCHK(k);
+    return s->s.setMaterialOmNk(mat_id, n, k);
+}
+int omv_scene_set_material_pbr(omv_scene* s, int mat_id, const omv_pbr_desc* d) {
+    OMV_NULLCHK(s); OMV_NULLCHK(d);
+    return s->s.setMaterialPbr(mat_id, *d);
+}
+int omv_scene_set_texture(omv_scene* s, int tex_id, int w, int h, const uint8_t* rgba) {
+    OMV_NULLCHK(s);
+    return s->s.setTexture(tex_id, w, h, rgba);
+}
+int omv_scene_set_texture_f32(omv_scene* s, int tex_id, int w, int h, const float* rgba) {
+    OMV_NULLCHK(s);
+    return s->s.setTextureF32(tex_id, w, h, rgba);
+}
+int omv_scene_set_light_point(omv_scene* s, int light_id, const float pos[3], const float intensity[3], float range) {
+    OMV_NULLCHK(s); OMV_NULLCHK(pos); OMV_NULLCHK(intensity);
+    return s->s.setLightPoint(light_id, pos, intensity, range);
+}
+int omv_scene_set_background_hdr(omv_scene* s, int w, int h, const float* rgb, const float rot[9], float scale) {
+    OMV_NULLCHK(s);
+    return s->s.setBackgroundHdr(w, h, rgb, rot, scale);
+}
+int omv_scene_set_background_constant(omv_scene* s, const float rgb[3]) {
+    OMV_NULLCHK(s); OMV_NULLCHK(rgb);
+    return s->s.setBackgroundConstant(rgb);
+}
+int omv_scene_set_false_color_subject(omv_scene* s, const char* name) {
+    OMV_NULLCHK(s); OMV_NULLCHK(name);
+    return s->s.setFalseColorSubject(name);
+}
+int omv_scene_commit(omv_scene* s, char* log, size_t log_cap) {
+    OMV_NULLCHK(s);
+    std::string lg;
+    int rc = s->s.commit(lg);
+    if (log && log_cap) {
+        std::strncpy(log, lg.c_str(), log_cap - 1);
+        log[log_cap - 1] = 0;
+    }
+    return rc;
+}
+int omv_scene_mesh_info(omv_scene* s, int mesh_id, int* vert_n, int* faces_n, int* stride, int* mat_id) {
+    OMV_NULLCHK(s);
+    return s->s.meshInfo(mesh_id, vert_n, faces_n, stride, mat_id);
+}
+int omv_scene_get_vertices(omv_scene* s, int mesh_id, float* out, size_t cap) {
+    OMV_NULLCHK(s);
+    return s->s.vertices(mesh_id, out, cap);
+}
+int omv_scene_bvh_info(omv_scene* s, int* nodes, int* depth, int* records, size_t* bytes) {
+    OMV_NULLCHK(s);
+    return s->s.bvhInfo(nodes, depth, records, bytes);
+}
+
+int omv_camera_create(omv_camera** out, int w, int h) {
+    OMV_NULLCHK(out);
+    *out = new (std::nothrow) omv_camera(w, h);
+    if (!*out) { omv::setError("omv_camera_create: out of memory"); return OMV_E_NOMEM; }
+    return OMV_OK;
+}
+int omv_camera_destroy(omv_camera* c) { delete c; return OMV_OK; }
+int omv_camera_set_viewpoint(omv_camera* c, const float pos[3], const float x[3], const float y[3], const float z[3]) {
+    OMV_NULLCHK(c); OMV_NULLCHK(pos); OMV_NULLCHK(x); OMV_NULLCHK(y); OMV_NULLCHK(z);
+    c->c.setViewPoint(pos, x, y, z);
+    return OMV_OK;
+}
+int omv_camera_adjust_to_resolution(omv_camera* c) { OMV_NULLCHK(c); c->c.adjustToResolution(); return OMV_OK; }
+int omv_camera_set_yfov(omv_camera* c, float fov) { OMV_NULLCHK(c); return c->c.setYFoV(fov); }
+int omv_camera_set_real_height(omv_camera* c, float h, float a) { OMV_NULLCHK(c); c->c.setRealHeight(h, a); return OMV_OK; }
+int omv_camera_set_focus(omv_camera* c, float f) { OMV_NULLCHK(c); c->c.desc.focus = f; return OMV_OK; }
+int omv_camera_get_desc(const omv_camera* c, omv_camera_desc* out) {
+    OMV_NULLCHK(c); OMV_NULLCHK(out);
+    *out = c->c.desc;
+    return OMV_OK;
+}
+// PinholeCamera::getImpression (PinholeCamera.cpp:110-113)
+int omv_camera_get_impression(const omv_camera* c, int x, int y, float out[4]) {
+    OMV_NULLCHK(c); OMV_NULLCHK(out);
+    const omv_camera_desc& d = c->c.desc;
+    if (x >= 0 && x < d.width && y >= 0 && y < d.height && !c->c.film.empty()) {
+        std::memcpy(out, &c->c.film[((size_t)y * d.width + x) * 4], 4 * sizeof(float));
+    } else {
+        out[0] = out[1] = out[2] = 0.0f;
+        out[3] = 1.0f;
+    }
+    return OMV_OK;
+}
+int omv_camera_get_film(const omv_camera* c, float* out, size_t cap) {
+    OMV_NULLCHK(c); OMV_NULLCHK(out);
+    if (cap < c->c.film.size() || c->c.film.empty()) { omv::setError("omv_camera_get_film: no film or buffer too small"); return OMV_E_ARG; }
+    std::memcpy(out, c->c.film.data(), c->c.film.size() * sizeof(float));
+    return OMV_OK;
+}
+
+int omv_renderer_create(omv_renderer** out) {
+    OMV_NULLCHK(out);
+    *out = new (std::nothrow) omv_renderer();
+    if (!*out) { omv::setError("omv_renderer_create: out of memory"); return OMV_E_NOMEM; }
+    return OMV_OK;
+}
+int omv_renderer_destroy(omv_renderer* r) { delete r; return OMV_OK; }
+int omv_renderer_set_parameters(omv_renderer* r, const omv_renderer_params* p) {
+    OMV_NULLCHK(r); OMV_NULLCHK(p);
+    r->r.setParameters(*p);
+    return OMV_OK;
+}
+int omv_renderer_set_scene(omv_renderer* r, omv_scene* s) {
+    OMV_NULLCHK(r); OMV_NULLCHK(s);
+    r->r.setScene(&s->s);
+    return OMV_OK;
+}
+int omv_renderer_render(omv_renderer* r, omv_camera* c) {
+    OMV_NULLCHK(r); OMV_NULLCHK(c);
+    return r->r.render(c->c);
+}
+int omv_renderer_render_fc(omv_renderer* r, omv_camera* c, int32_t* ids, uint8_t* tie) {
+    OMV_NULLCHK(r); OMV_NULLCHK(c);
+    return r->r.renderFc(c->c, ids, tie);
+}
+int omv_renderer_get_stats(const omv_renderer* r, omv_render_stats* out) {
+    OMV_NULLCHK(r); OMV_NULLCHK(out);
+    *out = r->r.stats();
+    return OMV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,138 @@
+// render_core.h -- C++ host side of the render core, mirroring the three
+// reference classes the C-ABI replaces:
+//   omv::RenderScene    pathtracer/src/scene/RenderScene.h:109-204
+//   omv::PinholeCamera  pathtracer/src/sensors/PinholeCamera.h
+//   omv::BDPTRenderer   pathtracer/src/renderer/BDPTRenderer.h:147-155
+// The scene owns every device buffer; the renderer owns per-launch scratch.
+#ifndef OMV_RENDER_CORE_H
+#define OMV_RENDER_CORE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/omv.h"
+#include "omv_types.h"
+
+namespace omv {
+
+// Owns one device allocation.
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { release(); }
+    void release();
+    hipError_t reserve(size_t n);                        // grows, never shrinks
+    hipError_t upload(const void* src, size_t n);        // exact size
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+struct HostMesh {
+    int id = -1, vn = 0, fn = 0, texChN = 0, stride = 0, filledTexChN = 0, matId = -1;
+    std::vector<uint32_t> F;
+    std::vector<float> V;
+    int texMap[OMV_MAX_UV_CHANNELS];
+    int texOffs[OMV_MAX_UV_CHANNELS];
+    bool texDef[OMV_MAX_UV_CHANNELS];
+    bool vertexDef = false, normalDef = false, tangentDef = false, valid = false;
+    HostMesh();
+};
+
+struct HostInstance {
+    int id = -1, meshId = -1;
+    float tm[16];
+    float nrm[9];
+    float inv[12];
+};
+
+struct HostTexture {
+    int w = 0, h = 0, fmt = OMV_TEX_RGBA8;
+    std::vector<uint8_t> bytes;
+};
+
+class RenderScene {
+public:
+    int allocate(int meshes, int instances, int materials, int textures, int lights);
+    int setMesh(int meshId, int vertN, int facesN, int texN, int matId, const uint32_t* faces, const float* vert, int vstride,
+                const float* norm, int nstride, const float* tans, int tanstride, const float* const* tex, int texstride);
+    int setInstance(int instId, const float trans[16], int meshId);
+    int setMaterialOm(int matId, float temperature, int nTables, const omv_ior_table* tables);
+    int setMaterialOmNk(int matId, const float n[3], const float k[3]);
+    int setMaterialPbr(int matId, const omv_pbr_desc& d);
+    int setTexture(int texId, int w, int h, const uint8_t* rgba);
+    int setTextureF32(int texId, int w, int h, const float* rgba);
+    int setLightPoint(int lightId, const float pos[3], const float intensity[3], float range);
+    int setBackgroundHdr(int w, int h, const float* rgb, const float rot[9], float scale);
+    int setBackgroundConstant(const float rgb[3]);
+    int setFalseColorSubject(const char* name);
+    int commit(std::string& log);
+
+    bool committed() const { return committed_; }
+    const OmvSceneView& view() const { return view_; }
+    int meshInfo(int id, int* vn, int* fn, int* stride, int* mat) const;
+    int vertices(int id, float* out, size_t cap) const;
+    int bvhInfo(int* nodes, int* depth, int* records, size_t* bytes) const;
+    int stackBound() const { return stackBound_; }
+
+private:
+    std::vector<HostMesh> meshes_;
+    std::vector<HostInstance> insts_;
+    std::vector<OmvMaterial> mats_;   // materials + 1 (missing slot last)
+    std::vector<HostTexture> texs_;
+    std::vector<OmvLight> lights_;
+    std::vector<bool> lightSet_;
+    int bgType_ = OMV_BG_NONE, bgW_ = 0, bgH_ = 0;
+    std::vector<float> bgRgb_;
+    float bgRot_[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    float bgScale_ = 1.0f, bgConst_[3] = {100.0f, 100.0f, 100.0f};
+    int fcSubject_ = OMV_FC_NONE;
+    bool committed_ = false;
+    int nodeCount_ = 0, bvhDepth_ = 0, recordCount_ = 0, stackBound_ = 0;
+    size_t deviceBytes_ = 0;
+    OmvSceneView view_{};
+    DeviceBuffer dVtx_, dFaces_, dMeshes_, dInsts_, dMats_, dTexs_, dTexels_, dNodes_, dTris_, dLights_, dCdf_, dBg_;
+};
+
+class PinholeCamera {
+public:
+    PinholeCamera(int w, int h);
+    void setViewPoint(const float pos[3], const float x[3], const float y[3], const float z[3]);
+    void adjustToResolution();
+    int setYFoV(float fov);
+    void setRealHeight(float height, float aspect);
+    omv_camera_desc desc;
+    std::vector<float> film;   // W*H*4 after a render
+};
+
+class BDPTRenderer {
+public:
+    BDPTRenderer();
+    ~BDPTRenderer();
+    void setParameters(const omv_renderer_params& p) { params_ = p; }
+    void setScene(RenderScene* s) { scene_ = s; }
+    int render(PinholeCamera& cam);
+    int renderFc(PinholeCamera& cam, int32_t* ids, uint8_t* tie);
+    const omv_render_stats& stats() const { return stats_; }
+
+private:
+    int prepare(int which, uint64_t lanesWanted, int& blocks);
+    omv_renderer_params params_;
+    RenderScene* scene_ = nullptr;
+    omv_render_stats stats_{};
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+    int cus_ = 0;
+    DeviceBuffer dRad_, dLv_, dStack_, dCounters_, dFilm_, dIds_, dTie_;
+};
+
+// thread-local last error
+void setError(const std::string& msg);
+const char* lastError();
+
+}  // namespace omv
+#endif

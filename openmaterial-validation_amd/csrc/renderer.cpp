@@ -1,0 +1,185 @@
+// renderer.cpp -- omv::PinholeCamera and omv::BDPTRenderer host classes.
+// BDPTRenderer::render (BDPTRenderer.cpp:372-393) becomes: one persistent
+// sample-kernel launch per chunk of whole pixels, then the ordered film
+// reduction for that chunk; the film stays in HBM until the camera reads it.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "omv_kernels.h"
+#include "render_core.h"
+
+namespace omv {
+
+namespace {
+constexpr uint64_t kChunkSamples = 1ull << 24;   // per-sample radiance buffer: 192 MiB
+int hipFail(hipError_t e, const char* what) {
+    setError(std::string(what) + ": " + hipGetErrorString(e));
+    return OMV_E_DEVICE;
+}
+}  // namespace
+
+// PinholeCamera(w, h) (PinholeCamera.cpp:51-57).  The reference leaves the
+// focal set-up to the caller; main() always follows with adjustToResolution
+// and setYFoV(60) (main.cpp:484-487), which is the default here.
+PinholeCamera::PinholeCamera(int w, int h) {
+    std::memset(&desc, 0, sizeof(desc));
+    desc.width = std::max(1, w);
+    desc.height = std::max(1, h);
+    desc.rotation[0] = desc.rotation[4] = desc.rotation[8] = 1.0f;
+    adjustToResolution();
+    setYFoV(60.0f);
+}
+
+// ViewPoint(loc, xdir, ydir, zdir) (RenderSensor.cpp:19-25): columns x, y, z
+void PinholeCamera::setViewPoint(const float pos[3], const float x[3], const float y[3], const float z[3]) {
+    for (int i = 0; i < 3; ++i) {
+        desc.position[i] = pos[i];
+        desc.rotation[i * 3 + 0] = x[i];
+        desc.rotation[i * 3 + 1] = y[i];
+        desc.rotation[i * 3 + 2] = z[i];
+    }
+}
+
+void PinholeCamera::adjustToResolution() {   // PinholeCamera.cpp:156-160
+    desc.half_h = 0.5f * (float)desc.height;
+    desc.half_w = 0.5f * (float)desc.width;
+}
+
+int PinholeCamera::setYFoV(float fov) {      // PinholeCamera.cpp:167-174
+    if (0.0f < fov && fov < 180.0f) {
+        desc.focus = desc.half_h / std::tan(0.5f * fov * 3.14159265358979323846f / 180.0f);
+        return OMV_OK;
+    }
+    setError("Field-of-view should be greater than zero and smaller than the straight angle.");
+    return OMV_E_ARG;
+}
+
+void PinholeCamera::setRealHeight(float height, float aspect) {   // PinholeCamera.cpp:150-154
+    desc.half_h = std::max(0.001f, height);
+    desc.half_w = std::max(0.1f, aspect) * desc.half_h;
+}
+
+BDPTRenderer::BDPTRenderer() { omv_renderer_params_default(&params_); }
+
+BDPTRenderer::~BDPTRenderer() {
+    for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+int BDPTRenderer::prepare(int which, uint64_t lanesWanted, int& blocks) {
+    hipError_t e;
+    if (!stream_) {
+        if ((e = hipStreamCreate(&stream_)) != hipSuccess) return hipFail(e, "hipStreamCreate");
+        for (auto& ev : ev_) if ((e = hipEventCreate(&ev)) != hipSuccess) return hipFail(e, "hipEventCreate");
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if ((e = hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return hipFail(e, "device attribute");
+    }
+    int perCu = occupancyBlocksPerCU(which);
+    if (perCu <= 0) perCu = 1;
+    uint64_t maxBlocks = (uint64_t)cus_ * (uint64_t)perCu;
+    uint64_t want = (lanesWanted + OMV_BLOCK - 1) / OMV_BLOCK;
+    blocks = (int)std::max<uint64_t>(1, std::min(maxBlocks, want));
+    const uint64_t lanes = (uint64_t)blocks * OMV_BLOCK;
+    if ((e = dStack_.reserve(lanes * (OMV_STACK_TOTAL - OMV_STACK_LDS) * sizeof(uint32_t))) != hipSuccess) return hipFail(e, "stack spill");
+    if ((e = dCounters_.reserve(8 * sizeof(unsigned long long))) != hipSuccess) return hipFail(e, "counters");
+    return OMV_OK;
+}
+
+int BDPTRenderer::render(PinholeCamera& cam) {
+    if (!scene_ || !scene_->committed()) { setError("render: no committed scene"); return OMV_E_STATE; }
+    const omv_renderer_params& p = params_;
+    if (p.samples_per_pixel < 0 || p.camera_bounces < 0 || p.light_bounces < 0) { setError("render: negative parameter"); return OMV_E_ARG; }
+    const uint64_t W = cam.desc.width, H = cam.desc.height, npix = W * H;
+    const int spp = p.samples_per_pixel;
+    std::memset(&stats_, 0, sizeof(stats_));
+    cam.film.assign(npix * 4, 0.0f);
+    int blocks = 0;
+    const uint64_t chunkPix = std::max<uint64_t>(1, kChunkSamples / (uint64_t)std::max(spp, 1));
+    const uint64_t firstChunk = std::min(npix, chunkPix) * (uint64_t)std::max(spp, 1);
+    int rc = prepare(0, firstChunk, blocks);
+    if (rc) return rc;
+    const uint64_t lanes = (uint64_t)blocks * OMV_BLOCK;
+    const int lvCap = std::max(std::min(p.light_bounces + 1, std::max(p.max_path_length, 1)), 1);
+    hipError_t e;
+    if ((e = dLv_.reserve(lanes * (uint64_t)lvCap * 6 * sizeof(float))) != hipSuccess) return hipFail(e, "light vertex scratch");
+    if ((e = dRad_.reserve(std::min(npix, chunkPix) * (uint64_t)std::max(spp, 1) * 3 * sizeof(float))) != hipSuccess) return hipFail(e, "sample buffer");
+    if ((e = dFilm_.reserve(npix * 4 * sizeof(float))) != hipSuccess) return hipFail(e, "film");
+    if ((e = hipMemsetAsync(dCounters_.ptr, 0, 8 * sizeof(unsigned long long), stream_)) != hipSuccess) return hipFail(e, "memset");
+    // PinholeCamera::init (PinholeCamera.cpp:65-68)
+    if ((e = hipMemsetAsync(dFilm_.ptr, 0, npix * 4 * sizeof(float), stream_)) != hipSuccess) return hipFail(e, "memset film");
+    (void)hipEventRecord(ev_[0], stream_);
+    double kernelMs = 0.0;
+    if (spp > 0) {
+        for (uint64_t pix0 = 0; pix0 < npix; pix0 += chunkPix) {
+            const uint64_t np = std::min(chunkPix, npix - pix0);
+            (void)hipEventRecord(ev_[2], stream_);
+            if ((e = launchBdpt(scene_->view(), cam.desc, p, pix0 * (uint64_t)spp, np * (uint64_t)spp, dRad_.as<float>(), dLv_.as<float>(),
+                                dStack_.as<uint32_t>(), dCounters_.as<unsigned long long>(), blocks, stream_)) != hipSuccess)
+                return hipFail(e, "sample kernel launch");
+            (void)hipEventRecord(ev_[3], stream_);
+            if ((e = launchFilm(dRad_.as<float>(), spp, pix0, np, dFilm_.as<float>(), stream_)) != hipSuccess) return hipFail(e, "film kernel launch");
+            if ((e = hipEventSynchronize(ev_[3])) != hipSuccess) return hipFail(e, "sample kernel");
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ev_[2], ev_[3]);
+            kernelMs += ms;
+        }
+    } else {
+        // PinholeCamera::stop with a zero count leaves rgb = 0 and alpha = 1
+        std::vector<float> z(npix * 4, 0.0f);
+        for (uint64_t i = 0; i < npix; ++i) z[i * 4 + 3] = 1.0f;
+        if ((e = hipMemcpyAsync(dFilm_.ptr, z.data(), z.size() * 4, hipMemcpyHostToDevice, stream_)) != hipSuccess) return hipFail(e, "film");
+    }
+    (void)hipEventRecord(ev_[1], stream_);
+    if ((e = hipEventSynchronize(ev_[1])) != hipSuccess) return hipFail(e, "render");
+    float total = 0.0f;
+    (void)hipEventElapsedTime(&total, ev_[0], ev_[1]);
+    if ((e = hipMemcpy(cam.film.data(), dFilm_.ptr, npix * 4 * sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess) return hipFail(e, "film readback");
+    unsigned long long c[8] = {0};
+    if ((e = hipMemcpy(c, dCounters_.ptr, sizeof(c), hipMemcpyDeviceToHost)) != hipSuccess) return hipFail(e, "counter readback");
+    stats_.samples = npix * (uint64_t)spp;
+    stats_.rays = c[0]; stats_.camera_rays = c[1]; stats_.light_rays = c[2]; stats_.shadow_rays = c[3];
+    stats_.nodes_visited = c[4]; stats_.tris_tested = c[5]; stats_.stack_overflows = c[6];
+    stats_.render_ms = total;
+    stats_.kernel_ms = kernelMs;
+    if (stats_.stack_overflows) { setError("render: traversal stack overflow"); return OMV_E_INTERNAL; }
+    return OMV_OK;
+}
+
+int BDPTRenderer::renderFc(PinholeCamera& cam, int32_t* ids, uint8_t* tie) {
+    if (!scene_ || !scene_->committed()) { setError("render_fc: no committed scene"); return OMV_E_STATE; }
+    const uint64_t npix = (uint64_t)cam.desc.width * cam.desc.height;
+    std::memset(&stats_, 0, sizeof(stats_));
+    cam.film.assign(npix * 4, 0.0f);
+    int blocks = 0;
+    int rc = prepare(1, npix, blocks);
+    if (rc) return rc;
+    hipError_t e;
+    if ((e = dFilm_.reserve(npix * 4 * sizeof(float))) != hipSuccess) return hipFail(e, "film");
+    if ((e = dIds_.reserve(npix * 4 * sizeof(int32_t))) != hipSuccess) return hipFail(e, "ids");
+    if ((e = dTie_.reserve(npix)) != hipSuccess) return hipFail(e, "tie");
+    if ((e = hipMemsetAsync(dCounters_.ptr, 0, 8 * sizeof(unsigned long long), stream_)) != hipSuccess) return hipFail(e, "memset");
+    (void)hipEventRecord(ev_[0], stream_);
+    if ((e = launchFc(scene_->view(), cam.desc, params_.seed, dFilm_.as<float>(), dIds_.as<int32_t>(), dTie_.as<uint8_t>(),
+                      dStack_.as<uint32_t>(), dCounters_.as<unsigned long long>(), blocks, stream_)) != hipSuccess)
+        return hipFail(e, "fc kernel launch");
+    (void)hipEventRecord(ev_[1], stream_);
+    if ((e = hipEventSynchronize(ev_[1])) != hipSuccess) return hipFail(e, "fc kernel");
+    float total = 0.0f;
+    (void)hipEventElapsedTime(&total, ev_[0], ev_[1]);
+    if ((e = hipMemcpy(cam.film.data(), dFilm_.ptr, npix * 4 * sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess) return hipFail(e, "film readback");
+    if (ids && (e = hipMemcpy(ids, dIds_.ptr, npix * 4 * sizeof(int32_t), hipMemcpyDeviceToHost)) != hipSuccess) return hipFail(e, "ids readback");
+    if (tie && (e = hipMemcpy(tie, dTie_.ptr, npix, hipMemcpyDeviceToHost)) != hipSuccess) return hipFail(e, "tie readback");
+    unsigned long long c[8] = {0};
+    if ((e = hipMemcpy(c, dCounters_.ptr, sizeof(c), hipMemcpyDeviceToHost)) != hipSuccess) return hipFail(e, "counter readback");
+    stats_.samples = npix;
+    stats_.rays = c[0]; stats_.camera_rays = c[1];
+    stats_.nodes_visited = c[4]; stats_.tris_tested = c[5]; stats_.stack_overflows = c[6];
+    stats_.render_ms = total;
+    stats_.kernel_ms = total;
+    if (stats_.stack_overflows) { setError("render_fc: traversal stack overflow"); return OMV_E_INTERNAL; }
+    return OMV_OK;
+}
+
+}  // namespace omv

@@ -1,0 +1,195 @@
+"""ctypes binding of the CPU oracle (oracle/librefcpu.so).
+
+Test infrastructure only: tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use it as the checker; the product never imports it.
+The scene set-up entry points are the product's (include/omv.h) with the
+ref_ prefix, so one scene description drives both.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import importlib.util
+import pathlib
+import sys
+
+import numpy as np
+
+_HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _HERE / "librefcpu.so"
+
+
+def _pkg():
+    name = "omv_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    root = _HERE.parent / "openmaterial-validation_amd"
+    spec = importlib.util.spec_from_file_location(name, root / "__init__.py", submodule_search_locations=[str(root)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+# POD types and the shared scene set-up live in the product binding module
+# (plain ctypes structures, no library load happens by importing them).
+_p = _pkg()
+RendererParams, CameraDesc, PbrDesc, IorTable, RenderStats = _p.RendererParams, _p.CameraDesc, _p.PbrDesc, _p.IorTable, _p.RenderStats
+SceneApi = _p.SceneApi
+pbr_desc = _p.pbr_desc
+
+_lib = None
+FP = C.POINTER(C.c_float)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"oracle library not built at {LIB_PATH}")
+    L = C.CDLL(str(LIB_PATH))
+    vp = C.c_void_p
+    sig = {
+        "ref_scene_create": ([], vp), "ref_scene_destroy": ([vp], None),
+        "ref_scene_allocate": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+        "ref_scene_set_mesh": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint32), FP, C.c_int, FP, C.c_int, FP,
+                                C.c_int, C.POINTER(FP), C.c_int], C.c_int),
+        "ref_scene_set_instance": ([vp, C.c_int, FP, C.c_int], C.c_int),
+        "ref_scene_set_material_om": ([vp, C.c_int, C.c_float, C.c_int, C.POINTER(IorTable)], C.c_int),
+        "ref_scene_set_material_om_nk": ([vp, C.c_int, FP, FP], C.c_int),
+        "ref_scene_set_material_pbr": ([vp, C.c_int, C.POINTER(PbrDesc)], C.c_int),
+        "ref_scene_set_texture": ([vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8)], C.c_int),
+        "ref_scene_set_texture_f32": ([vp, C.c_int, C.c_int, C.c_int, FP], C.c_int),
+        "ref_scene_set_light_point": ([vp, C.c_int, FP, FP, C.c_float], C.c_int),
+        "ref_scene_set_background_hdr": ([vp, C.c_int, C.c_int, FP, FP, C.c_float], C.c_int),
+        "ref_scene_set_background_constant": ([vp, FP], C.c_int),
+        "ref_scene_set_false_color_subject": ([vp, C.c_char_p], C.c_int),
+        "ref_scene_commit": ([vp, C.c_char_p, C.c_size_t], C.c_int),
+        "ref_scene_mesh_info": ([vp, C.c_int] + [C.POINTER(C.c_int)] * 4, C.c_int),
+        "ref_scene_get_vertices": ([vp, C.c_int, FP, C.c_size_t], C.c_int),
+        "ref_scene_get_om_nk": ([vp, C.c_int, FP, FP], C.c_int),
+        "ref_render": ([vp, C.POINTER(RendererParams), C.POINTER(CameraDesc), C.c_int, C.c_int, C.c_int, C.c_int, FP,
+                        C.POINTER(RenderStats)], C.c_int),
+        "ref_render_samples": ([vp, C.POINTER(RendererParams), C.POINTER(CameraDesc), C.c_int, C.c_int, C.c_int, C.c_int, FP], C.c_int),
+        "ref_render_fc": ([vp, C.POINTER(RendererParams), C.POINTER(CameraDesc), FP, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)], C.c_int),
+        "ref_closest_hit": ([vp, FP, FP, C.c_float, C.c_float, C.c_int, FP, FP, FP, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                             C.POINTER(C.c_int), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
+        "ref_philox4x32_10": ([C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)], None),
+        "ref_u01": ([C.c_uint32], C.c_float),
+        "ref_fresnel": ([C.c_float, C.c_float, C.c_float, FP, FP], None),
+        "ref_evaluate_direct": ([FP, FP, FP, FP, C.c_float, C.c_float, FP], None),
+        "ref_sample_indirect": ([FP, FP, FP, FP, C.c_float, C.c_float, C.c_float, C.c_int, FP, FP, FP], C.c_int),
+        "ref_uniform_hemisphere": ([C.c_float, C.c_float, FP], None),
+        "ref_uniform_sphere": ([FP, C.c_int, FP], C.c_int),
+        "ref_ggx_hemisphere": ([C.c_float, C.c_float, C.c_float, FP], None),
+        "ref_cosine_sqrt_hemisphere": ([C.c_float, C.c_float, FP], None),
+        "ref_texture_rgba8": ([C.c_int, C.c_int, C.POINTER(C.c_uint8), C.c_float, C.c_float, FP], None),
+        "ref_hdr_radiance": ([C.c_int, C.c_int, FP, FP, C.c_float, FP, FP], None),
+        "ref_camera_ray": ([C.POINTER(CameraDesc), C.c_int, C.c_int, C.c_float, C.c_float, FP, FP], None),
+        "ref_distance_random": ([C.c_int, FP, C.c_float, FP], C.c_int),
+        "ref_get_ior": ([C.c_int, C.POINTER(IorTable), C.c_float, C.c_float, FP, FP], C.c_int),
+        "ref_gamma_correction": ([FP, C.c_int, C.c_int, C.c_float, C.c_float], None),
+        "ref_compute_camera": ([C.c_int, C.c_int, C.c_float, FP, FP, FP], None),
+        "ref_renderer_params_default": ([C.POINTER(RendererParams)], None),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def f32(a, shape=None):
+    a = np.ascontiguousarray(np.asarray(a, np.float32))
+    return a.reshape(shape) if shape is not None else a
+
+
+def ptr(a, t=C.c_float):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def default_params(**kw) -> RendererParams:
+    p = RendererParams()
+    lib().ref_renderer_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def camera_desc(width, height, position, forward, up=(0, 1, 0), yfov=60.0) -> CameraDesc:
+    """Viewpoint as main.cpp:459-476 builds it, focal set-up as adjustToResolution + setYFoV."""
+    d = CameraDesc()
+    d.width, d.height = int(width), int(height)
+    fwd = np.asarray(forward, np.float32)
+    n = np.float32(np.sqrt(np.float32(fwd[0] * fwd[0] + fwd[1] * fwd[1]) + np.float32(fwd[2] * fwd[2])))
+    fwd = (fwd * (np.float32(1.0) / n)).astype(np.float32)
+    upv = np.asarray(up, np.float32)
+    left = _p._cross32(fwd, upv)
+    upv = _p._cross32(left, fwd)
+    for i in range(3):
+        d.position[i] = float(position[i])
+        d.rotation[i * 3 + 0], d.rotation[i * 3 + 1], d.rotation[i * 3 + 2] = float(left[i]), float(upv[i]), float(fwd[i])
+    hw, hh, fo = C.c_float(), C.c_float(), C.c_float()
+    lib().ref_compute_camera(d.width, d.height, C.c_float(yfov), C.byref(hw), C.byref(hh), C.byref(fo))
+    d.half_w, d.half_h, d.focus = hw.value, hh.value, fo.value
+    return d
+
+
+class Scene(SceneApi):
+    prefix = "ref"
+
+    def __init__(self):
+        self.lib = lib()
+        self.h = C.c_void_p(self.lib.ref_scene_create())
+
+    def om_nk(self, mat_id):
+        n, k = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        rc = self.lib.ref_scene_get_om_nk(self.h, mat_id, ptr(n), ptr(k))
+        if rc != 0:
+            raise RuntimeError("not an OM material")
+        return n, k
+
+    def render(self, params, cam: CameraDesc, fp64=False, row0=0, rows=0, threads=0):
+        film = np.zeros(cam.width * cam.height * 4, np.float32)
+        st = RenderStats()
+        rc = self.lib.ref_render(self.h, C.byref(params), C.byref(cam), int(fp64), row0, rows, threads, ptr(film), C.byref(st))
+        if rc != 0:
+            raise RuntimeError(f"ref_render failed {rc}")
+        return film.reshape(cam.height, cam.width, 4), {k: getattr(st, k) for k, _ in RenderStats._fields_}
+
+    def render_samples(self, params, cam: CameraDesc, fp64=False, row0=0, rows=0, threads=0):
+        n = cam.width * cam.height * params.samples_per_pixel
+        rad = np.zeros(n * 3, np.float32)
+        rc = self.lib.ref_render_samples(self.h, C.byref(params), C.byref(cam), int(fp64), row0, rows, threads, ptr(rad))
+        if rc != 0:
+            raise RuntimeError(f"ref_render_samples failed {rc}")
+        return rad.reshape(cam.height, cam.width, params.samples_per_pixel, 3)
+
+    def render_fc(self, params, cam: CameraDesc):
+        npix = cam.width * cam.height
+        film = np.zeros(npix * 4, np.float32)
+        ids = np.zeros(npix * 4, np.int32)
+        tie = np.zeros(npix, np.uint8)
+        rc = self.lib.ref_render_fc(self.h, C.byref(params), C.byref(cam), ptr(film), ptr(ids, C.c_int32), ptr(tie, C.c_uint8))
+        if rc != 0:
+            raise RuntimeError(f"ref_render_fc failed {rc}")
+        return film.reshape(cam.height, cam.width, 4), ids.reshape(4, cam.height, cam.width), tie.reshape(cam.height, cam.width)
+
+    def closest_hit(self, org, d, tnear=0.0, tfar=3.4e38, bvh=True):
+        o, dd = f32(org, (3,)), f32(d, (3,))
+        t, u, v = C.c_float(), C.c_float(), C.c_float()
+        inst, prim, tie = C.c_int(), C.c_int(), C.c_int()
+        nodes, tris = C.c_uint64(), C.c_uint64()
+        hit = self.lib.ref_closest_hit(self.h, ptr(o), ptr(dd), C.c_float(tnear), C.c_float(tfar), int(bvh), C.byref(t), C.byref(u),
+                                       C.byref(v), C.byref(inst), C.byref(prim), C.byref(tie), C.byref(nodes), C.byref(tris))
+        if hit < 0:
+            raise RuntimeError("ref_closest_hit failed")
+        if not hit:
+            return None, nodes.value, tris.value
+        return (t.value, u.value, v.value, inst.value, prim.value, tie.value), nodes.value, tris.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.ref_scene_destroy(self.h)
+            self.h = None
