@@ -158,6 +158,10 @@ typedef struct omv_render_stats {
 
 /* ------------------------------------------------------------- errors */
 const char* omv_last_error(void);
+/* Device helpers (no reference counterpart): wait for all work on the
+ * current HIP device; report its name, compute-unit count and memory. */
+int omv_device_synchronize(void);
+int omv_device_info(char* name, size_t name_cap, int* compute_units, size_t* total_mem);
 const char* omv_version(void);
 void omv_renderer_params_default(omv_renderer_params* p);
 

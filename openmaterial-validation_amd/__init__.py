@@ -252,6 +252,7 @@ def _load():
         "omv_renderer_render": [vp, vp], "omv_renderer_render_fc": [vp, vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)],
         "omv_renderer_get_stats": [vp, C.POINTER(RenderStats)],
         "omv_renderer_params_default": [C.POINTER(RendererParams)],
+        "omv_device_synchronize": [], "omv_device_info": [C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_size_t)],
     }.items():
         f = getattr(lib, name)
         f.argtypes = args
@@ -267,6 +268,21 @@ def lib():
     if _lib is None:
         _lib = _load()
     return _lib
+
+
+def device_synchronize():
+    rc = lib().omv_device_synchronize()
+    if rc != 0:
+        raise RuntimeError(lib().omv_last_error().decode())
+
+
+def device_info() -> dict:
+    name = C.create_string_buffer(256)
+    cus, mem = C.c_int(), C.c_size_t()
+    rc = lib().omv_device_info(name, len(name), C.byref(cus), C.byref(mem))
+    if rc != 0:
+        raise RuntimeError(lib().omv_last_error().decode())
+    return {"arch": name.value.decode(), "compute_units": cus.value, "memory_bytes": mem.value}
 
 
 def default_params(**kw) -> RendererParams:

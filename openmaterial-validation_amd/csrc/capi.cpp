@@ -16,6 +16,24 @@ extern "C" {
 const char* omv_last_error(void) { return omv::lastError(); }
 const char* omv_version(void) { return "omv-mi355x 0.1 (gfx950)"; }
 
+int omv_device_synchronize(void) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) { omv::setError(std::string("hipDeviceSynchronize: ") + hipGetErrorString(e)); return OMV_E_DEVICE; }
+    return OMV_OK;
+}
+
+int omv_device_info(char* name, size_t cap, int* cus, size_t* mem) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) { omv::setError(std::string("hipGetDeviceProperties: ") + hipGetErrorString(e)); return OMV_E_DEVICE; }
+    if (name && cap) { std::strncpy(name, prop.gcnArchName, cap - 1); name[cap - 1] = 0; }
+    if (cus) *cus = prop.multiProcessorCount;
+    if (mem) *mem = prop.totalGlobalMem;
+    return OMV_OK;
+}
+
 void omv_renderer_params_default(omv_renderer_params* p) {
     if (!p) return;
     std::memset(p, 0, sizeof(*p));

@@ -138,7 +138,8 @@ def torus_knot(p=2, q=3, segments=1000, tube=50, scale=0.5, tube_radius=0.2):
     a1 = (i + 1) * (tube + 1) + j
     a2 = (i + 1) * (tube + 1) + j + 1
     a3 = i * (tube + 1) + j + 1
-    faces = np.concatenate([np.stack([a0, a1, a2], -1).reshape(-1, 3), np.stack([a0, a2, a3], -1).reshape(-1, 3)]).astype(np.uint32)
+    # counter-clockwise seen from outside: outward geometric normals
+    faces = np.concatenate([np.stack([a0, a2, a1], -1).reshape(-1, 3), np.stack([a0, a3, a2], -1).reshape(-1, 3)]).astype(np.uint32)
     return verts, faces, uvs
 
 
@@ -156,7 +157,7 @@ def grid_sphere(stacks, slices, radii=(1.0, 1.0, 1.0), bump=None):
     a1 = (i + 1) * (slices + 1) + j
     a2 = (i + 1) * (slices + 1) + j + 1
     a3 = i * (slices + 1) + j + 1
-    faces = np.concatenate([np.stack([a0, a1, a2], -1).reshape(-1, 3), np.stack([a0, a2, a3], -1).reshape(-1, 3)]).astype(np.uint32)
+    faces = np.concatenate([np.stack([a0, a2, a1], -1).reshape(-1, 3), np.stack([a0, a3, a2], -1).reshape(-1, 3)]).astype(np.uint32)
     return verts, faces, uvs
 
 
@@ -355,7 +356,7 @@ def large_instanced(width=1920, height=1080, spp=16, depth=8, count=200, tris_pe
         d.meshes.append({"vertices": v, "faces": f_, "mat_id": int(i % 8), "uvs": [uv]})
         gx, gz = i % 20, i // 20
         t = (gx * 2.6 - 24.7 + rng.uniform(-0.4, 0.4), rng.uniform(-0.5, 0.5), -gz * 2.6 - 2.0 + rng.uniform(-0.4, 0.4))
-        d.instances.append((trs(t, rng.normal(size=3), rng.uniform(0, 2 * np.pi), (1.0, 1.0, 1.0) * rng.uniform(0.7, 1.1)), i))
+        d.instances.append((trs(t, rng.normal(size=3), rng.uniform(0, 2 * np.pi), np.full(3, rng.uniform(0.7, 1.1))), i))
     for p in ((-15.0, 8.0, -5.0), (0.0, 10.0, -12.0), (15.0, 8.0, -20.0), (0.0, 4.0, 2.0)):
         d.lights.append((p, (400.0, 380.0, 350.0), -1.0))
     d.background = ("hdr", hdri(hdr_w, hdr_h), HDR_ROTATION, HDR_SCALE)
@@ -364,7 +365,7 @@ def large_instanced(width=1920, height=1080, spp=16, depth=8, count=200, tris_pe
     return d, view, params
 
 
-def mixed_test_scene(hdr=True, lights=True, tex_f32=False, masked=False):
+def mixed_test_scene(hdr=True, lights=True, tex_f32=False, masked=False, missing=True):
     """Small scene exercising every material path: a textured PBR floor, an
     OpenMATERIAL sphere, a PBR cube with a normal map, a mesh with a missing
     material, a transformed second instance, point lights and a background."""
@@ -391,7 +392,8 @@ def mixed_test_scene(hdr=True, lights=True, tex_f32=False, masked=False):
     bv, bf, buv = grid_box(3, (0.5, 0.5, 0.5))
     d.meshes.append({"vertices": bv, "faces": bf, "mat_id": 2, "uvs": [buv]})
     mv, mf = icosphere(1, 0.4)
-    d.meshes.append({"vertices": mv, "faces": mf, "mat_id": 7})   # undefined material -> missing (pink) slot
+    # undefined material -> missing (pink) slot; otherwise the OM material
+    d.meshes.append({"vertices": mv, "faces": mf, "mat_id": 7 if missing else 1})
     d.instances.append((identity(), 0))
     d.instances.append((trs((-0.9, -0.2, 0.0)), 1))
     d.instances.append((trs((1.1, -0.4, 0.3), (1, 1, 0), 0.6), 2))

@@ -681,7 +681,7 @@ static void traverseMesh(const Scene& S, const Instance& in, const V3<R>& wo, co
     int st[128]; int sp = 0; st[sp++] = 0;
     while (sp) {
         const BNode& nd = b.nodes[st[--sp]];
-        ++cnt.nodes;
+        if (nd.count == 0) ++cnt.nodes;   // interior nodes visited (the byte model's unit)
         double te;
         if (!boxHit(nd.bmin, nd.bmax, o, idr, (double)tnear, found ? (double)h.t : (double)tfar, te)) continue;
         if (nd.count > 0) { for (int i = nd.start; i < nd.start + nd.count; ++i) test(b.prims[i]); continue; }
@@ -703,7 +703,7 @@ static bool closestHit(const Scene& S, const V3<R>& o, const V3<R>& d, R tnear, 
     int st[128]; int sp = 0; st[sp++] = 0;
     while (sp) {
         const BNode& nd = S.tlas.nodes[st[--sp]];
-        ++cnt.nodes;
+        if (nd.count == 0) ++cnt.nodes;
         double te;
         if (!boxHit(nd.bmin, nd.bmax, od, idr, (double)tnear, found ? (double)h.t : (double)tfar, te)) continue;
         if (nd.count > 0) {
