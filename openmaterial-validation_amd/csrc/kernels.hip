@@ -566,11 +566,61 @@ DEV f3 omFresnelBrdf(const OmvMaterial& M, float fCos) {
     }
     return mk3(o[0], o[1], o[2]);
 }
+// Estimator extensions (omv_renderer_params.extensions)
+struct Ext { bool omRough; float omAlpha; };
+
+// OMV_EXT_OM_ROUGH: GGX rough conductor with the complex-IOR Fresnel as F.
+// Connection: f = F(|I.H|) D G2 / (4 V.N L.N), zero unless both cosines > 0.
+DEV f3 omRoughEval(const OmvMaterial& M, float alpha, const f3& I, const f3& O, const f3& N) {
+    float VdotN = -dot3(I, N), LdotN = dot3(O, N);
+    if (!(VdotN > 0.0f && LdotN > 0.0f)) return mk3(0.0f, 0.0f, 0.0f);
+    f3 H = normalize3(sub3(O, I));
+    float VdotH = fabsf(dot3(I, H)), NdotH = fabsf(dot3(N, H));
+    float D = ggxPdf(clamp01(NdotH), alpha);
+    float G = smithG2(smithG1(VdotN, alpha), smithG1(LdotN, alpha));
+    float k = D * G / (4.0f * VdotN * LdotN);
+    float o[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float rp, rs;
+        fresnel(M.n[c], M.k[c], VdotH, rp, rs);
+        o[c] = lerpr(rp, rs, 0.5f) * k;
+    }
+    return mk3(o[0], o[1], o[2]);
+}
+// Sampling: H from GGXHemisphere (2 uniforms), O = reflect(I, H); brdf times
+// the path cosine is F G |I.H| / (|I.N| |N.H|).
+DEV f3 omRoughSample(const OmvMaterial& M, float alpha, const f3& I, const SP& sp, bool inv, f3& O, Rng& r) {
+    const f3 N = sp.n;
+    O = reflect3(I, N);
+    float VdotN = -dot3(I, N);
+    if (!(VdotN > 0.0f)) return mk3(0.0f, 0.0f, 0.0f);
+    f3 B = cross3(N, sp.t);
+    f3 rd = ggxHemisphere(r, alpha * alpha);
+    f3 H = normalize3(toBasis(sp.t, B, N, rd));
+    f3 o = reflect3(I, H);
+    if (!(dot3(N, o) > 0.0f)) return mk3(0.0f, 0.0f, 0.0f);
+    O = normalize3(o);
+    float LdotN = dot3(O, N), VdotH = fabsf(dot3(I, H)), NdotH = fabsf(dot3(N, H));
+    float G = smithG2(smithG1(VdotN, alpha), smithG1(LdotN, alpha));
+    float w = G * VdotH / (VdotN * NdotH);
+    float ct = inv ? LdotN : VdotN;
+    float out[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float rp, rs;
+        fresnel(M.n[c], M.k[c], VdotH, rp, rs);
+        out[c] = lerpr(rp, rs, 0.5f) * w / ct;
+    }
+    return mk3(out[0], out[1], out[2]);
+}
+
 // RenderMaterial::getBrdf of the three materials for a connection
-DEV f3 matGetBrdf(const OmvSceneView& S, const SP& sp, const f3& inV, const f3& outV) {
+DEV f3 matGetBrdf(const OmvSceneView& S, const Ext& X, const SP& sp, const f3& inV, const f3& outV) {
     const OmvMaterial& M = S.mats[S.meshes[sp.mesh].mat_id];
     if (M.type == OMV_MAT_OM) {   // OpenMaterial::getBrdf (OpenMaterial.cpp:95-163)
         if (!M.has_base) return mk3(0.0f, 0.0f, 0.0f);
+        if (X.omRough) return omRoughEval(M, X.omAlpha, inV, outV, sp.n);
         float fCos = -dot3(inV, sp.n);
         if (fabsf(dot3(outV, sp.n) - fCos) > 1e-6f) return mk3(0.0f, 0.0f, 0.0f);
         return omFresnelBrdf(M, fCos);
@@ -590,10 +640,12 @@ DEV f3 matGetBrdf(const OmvSceneView& S, const SP& sp, const f3& inV, const f3& 
     return mk3(brdfPdf * M.diffuse[0], brdfPdf * M.diffuse[1], brdfPdf * M.diffuse[2]);
 }
 // RenderMaterial::getRayAndBrdf of the three materials (computePath, BDPTRenderer.cpp:158-159)
-DEV void matGetRayAndBrdf(const OmvSceneView& S, const SP& sp, const f3& inV, f3& outV, f3& brdf, f3& erad, bool inv, Rng& r) {
+DEV void matGetRayAndBrdf(const OmvSceneView& S, const Ext& X, const SP& sp, const f3& inV, f3& outV, f3& brdf, f3& erad, bool inv,
+                          Rng& r) {
     const OmvMaterial& M = S.mats[S.meshes[sp.mesh].mat_id];
     erad = mk3(0.0f, 0.0f, 0.0f);
     if (M.type == OMV_MAT_OM) {
+        if (X.omRough && M.has_base) { brdf = omRoughSample(M, X.omAlpha, inV, sp, inv, outV, r); return; }
         outV = omNextDirection(S, inV, sp);
         brdf = M.has_base ? omFresnelBrdf(M, dot3(outV, sp.n)) : mk3(0.0f, 0.0f, 0.0f);
         return;
@@ -612,7 +664,7 @@ DEV void matGetRayAndBrdf(const OmvSceneView& S, const SP& sp, const f3& inV, f3
     }
     f3 v = uniformHemisphere(r);
     outV = toBasis(sp.t, sp.b, sp.n, v);
-    brdf = inv ? matGetBrdf(S, sp, neg3(outV), neg3(inV)) : matGetBrdf(S, sp, inV, outV);
+    brdf = inv ? matGetBrdf(S, X, sp, neg3(outV), neg3(inV)) : matGetBrdf(S, X, sp, inV, outV);
 }
 
 // ------------------------------------------------------------ traversal
@@ -798,6 +850,7 @@ struct RP {   // renderer parameters used per sample
     int camera_bounces, light_bounces, q, atten;
     float light_scale, light_min_dist, cut;
     uint32_t seed;
+    Ext x;
 };
 
 // sceneIntersect (BDPTRenderer.cpp:75-102); tnear persists across calls of one path
@@ -924,6 +977,10 @@ DEV void flushCounters(unsigned long long* g, uint32_t rays, uint32_t cam, uint3
 extern __shared__ uint32_t omv_lds_stack[];
 
 // One sample of computePaths (BDPTRenderer.cpp:237-301) per lane, persistent grid.
+// kLights = false is the instantiation for scenes without point lights: no
+// light subpath exists (BDPTRenderer.cpp:241-242) and no connection code is
+// compiled, which frees registers for occupancy.
+template <bool kLights>
 __global__ __launch_bounds__(OMV_BLOCK) void omv_bdpt_kernel(OmvSceneView S, omv_camera_desc cam, RP P, int spp,
                                                              uint64_t sample0, uint64_t nsamples, float* __restrict__ rad,
                                                              float* __restrict__ lv, uint32_t* __restrict__ gstack,
@@ -952,7 +1009,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_bdpt_kernel(OmvSceneView S, omv
         int lpLen = 0, lightIdx = -1;
         f3 lightPos = mk3(0.0f, 0.0f, 0.0f), lightInt = mk3(0.0f, 0.0f, 0.0f);
         float lightRange = FLT_MAX;
-        if (S.num_lights > 0) {
+        if (kLights && S.num_lights > 0) {
             Rng ls; ls.init(P.seed, STREAM_LIGHT, s);
             lightIdx = sampleLight(S, ls.next());
             const OmvLight L = S.lights[lightIdx];
@@ -980,7 +1037,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_bdpt_kernel(OmvSceneView S, omv
                 computeSurfacePoint(S, h, sp, false);
                 modifyFrame(S, sp);
                 f3 outRay, brdf, erad;
-                matGetRayAndBrdf(S, sp, pDir, outRay, brdf, erad, false, ls);
+                matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, false, ls);
                 float ct = fabsf(dot3(pDir, sp.n));
                 of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
                 if (i == 1) rD = attenuation(P, length3(sub3(sp.pos, lightPos)), lightRange);
@@ -1014,7 +1071,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_bdpt_kernel(OmvSceneView S, omv
             computeSurfacePoint(S, h, sp, false);
             modifyFrame(S, sp);
             // connections of camera vertex j = i with light vertices l, l + j <= q
-            if (lightIdx >= 0) {
+            if (kLights && lightIdx >= 0) {
                 const int lEnd = min(lpLen, q - i + 1);
                 const f3 outV = neg3(pDir);
                 for (int l = 0; l < lEnd; ++l) {
@@ -1033,7 +1090,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_bdpt_kernel(OmvSceneView S, omv
                     float c = fabsf(dot3(LC, sp.n));
                     f3 lc = mk3(lRad.x * c * of.x, lRad.y * c * of.y, lRad.z * c * of.z);
                     if (intensity(lc) < P.cut) continue;
-                    f3 brdf = matGetBrdf(S, sp, LC, outV);
+                    f3 brdf = matGetBrdf(S, P.x, sp, LC, outV);
                     brdf = mk3(brdf.x * c, brdf.y * c, brdf.z * c);
                     lc = mk3(brdf.x * of.x * lRad.x, brdf.y * of.y * lRad.y, brdf.z * of.z * lRad.z);
                     if (intensity(lc) < P.cut) continue;
@@ -1042,7 +1099,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_bdpt_kernel(OmvSceneView S, omv
                 }
             }
             f3 outRay, brdf, erad;
-            matGetRayAndBrdf(S, sp, pDir, outRay, brdf, erad, true, cs);
+            matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, true, cs);
             float ct = fabsf(dot3(outRay, sp.n));
             of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
             radiance = mk3(radiance.x + of.x * erad.x, radiance.y + of.y * erad.y, radiance.z + of.z * erad.z);
@@ -1171,6 +1228,8 @@ static RP toRP(const omv_renderer_params& p) {
     r.camera_bounces = p.camera_bounces; r.light_bounces = p.light_bounces; r.q = p.max_path_length;
     r.atten = p.light_distance_attenuation; r.light_scale = p.light_scale_value; r.light_min_dist = p.light_min_distance;
     r.cut = p.ray_cut_pix_value; r.seed = p.seed;
+    r.x.omRough = (p.extensions & OMV_EXT_OM_ROUGH) != 0;
+    r.x.omAlpha = p.om_roughness * p.om_roughness;
     return r;
 }
 
@@ -1178,8 +1237,12 @@ hipError_t launchBdpt(const OmvSceneView& S, const omv_camera_desc& cam, const o
                       uint64_t nsamples, float* rad, float* lv, uint32_t* gstack, unsigned long long* counters, int blocks,
                       hipStream_t stream) {
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
-    hipLaunchKernelGGL(omv_bdpt_kernel, dim3(blocks), dim3(OMV_BLOCK), lds, stream, S, cam, toRP(p), p.samples_per_pixel,
-                       sample0, nsamples, rad, lv, gstack, counters);
+    if (S.num_lights > 0)
+        hipLaunchKernelGGL(omv_bdpt_kernel<true>, dim3(blocks), dim3(OMV_BLOCK), lds, stream, S, cam, toRP(p), p.samples_per_pixel,
+                           sample0, nsamples, rad, lv, gstack, counters);
+    else
+        hipLaunchKernelGGL(omv_bdpt_kernel<false>, dim3(blocks), dim3(OMV_BLOCK), lds, stream, S, cam, toRP(p), p.samples_per_pixel,
+                           sample0, nsamples, rad, lv, gstack, counters);
     return hipGetLastError();
 }
 
@@ -1200,8 +1263,9 @@ hipError_t launchFc(const OmvSceneView& S, const omv_camera_desc& cam, uint32_t 
 int occupancyBlocksPerCU(int which) {
     int n = 0;
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
-    hipError_t e = which == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel, OMV_BLOCK, lds)
-                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_fc_kernel, OMV_BLOCK, lds);
+    hipError_t e = which == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<true>, OMV_BLOCK, lds)
+                   : which == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<false>, OMV_BLOCK, lds)
+                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_fc_kernel, OMV_BLOCK, lds);
     if (e != hipSuccess) n = 0;
     return n;
 }

@@ -21,7 +21,7 @@ hipError_t launchBdpt(const OmvSceneView& S, const omv_camera_desc& cam, const o
 hipError_t launchFilm(const float* rad, int spp, uint64_t pix0, uint64_t npix, float* film, hipStream_t stream);
 hipError_t launchFc(const OmvSceneView& S, const omv_camera_desc& cam, uint32_t seed, float* film, int32_t* ids, uint8_t* tie,
                     uint32_t* gstack, unsigned long long* counters, int blocks, hipStream_t stream);
-// 0: sample kernel, 1: FC kernel
+// 0: sample kernel (scene with lights), 1: FC kernel, 2: sample kernel (no lights)
 int occupancyBlocksPerCU(int which);
 
 }  // namespace omv

@@ -98,7 +98,7 @@ int BDPTRenderer::render(PinholeCamera& cam) {
     int blocks = 0;
     const uint64_t chunkPix = std::max<uint64_t>(1, kChunkSamples / (uint64_t)std::max(spp, 1));
     const uint64_t firstChunk = std::min(npix, chunkPix) * (uint64_t)std::max(spp, 1);
-    int rc = prepare(0, firstChunk, blocks);
+    int rc = prepare(scene_->view().num_lights > 0 ? 0 : 2, firstChunk, blocks);
     if (rc) return rc;
     const uint64_t lanes = (uint64_t)blocks * OMV_BLOCK;
     const int lvCap = std::max(std::min(p.light_bounces + 1, std::max(p.max_path_length, 1)), 1);

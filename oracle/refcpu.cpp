@@ -1082,12 +1082,74 @@ static V3<R> omFresnelBrdf(const Material& m, R fCos) {
     return b;
 }
 
+/* Estimator extensions selected by omv_renderer_params.extensions. */
+struct Ext {
+    bool omRough = false;   // OMV_EXT_OM_ROUGH
+    float omAlpha = 0.09f;  // GGX alpha = roughness^2
+    explicit Ext(const omv_renderer_params& p) {
+        omRough = (p.extensions & OMV_EXT_OM_ROUGH) != 0;
+        omAlpha = p.om_roughness * p.om_roughness;
+    }
+};
+
+/* OMV_EXT_OM_ROUGH, connection: GGX rough conductor whose F is the complex-IOR
+ * Fresnel of fresnelReflection at |I.H|; f = F D G2 / (4 V.N L.N), zero unless
+ * both cosines are positive (the evaluateDirect convention). */
+template <class R>
+static V3<R> omRoughEval(const Material& m, R alpha, const V3<R>& I, const V3<R>& O, const V3<R>& N) {
+    R VdotN = -dot3(I, N), LdotN = dot3(O, N);
+    if (!(VdotN > R(0) && LdotN > R(0))) return mk(R(0), R(0), R(0));
+    V3<R> H = normalize3(sub(O, I));
+    R VdotH = std::abs(dot3(I, H)), NdotH = std::abs(dot3(N, H));
+    R D = ggxPdf(clamp01(NdotH), alpha);
+    R G = smithG2(smithG1(VdotN, alpha), smithG1(LdotN, alpha));
+    R k = D * G / (R(4) * VdotN * LdotN);
+    V3<R> out;
+    for (int c = 0; c < 3; ++c) {
+        R rp, rs;
+        fresnel(R(m.n[c]), R(m.k[c]), VdotH, rp, rs);
+        at(out, c) = lerpv(rp, rs, R(0.5f)) * k;
+    }
+    return out;
+}
+
+/* OMV_EXT_OM_ROUGH, sampling: half vector from GGXHemisphere (2 uniforms),
+ * O = reflect(I, H); the returned brdf times the path's cosine (|O.N| on
+ * camera paths, |I.N| on light paths) is F G |I.H| / (|I.N| |N.H|). */
+template <class R>
+static V3<R> omRoughSample(const Material& m, R alpha, const V3<R>& I, const SurfacePoint<R>& sp, bool inv, V3<R>& O,
+                           Uniforms& U) {
+    const V3<R>& N = sp.normal;
+    const V3<R> zero = mk(R(0), R(0), R(0));
+    O = reflect3(I, N);
+    R VdotN = -dot3(I, N);
+    if (!(VdotN > R(0))) return zero;
+    V3<R> B = cross3(N, sp.tangent);
+    V3<R> rd = ggxHemisphere(U, alpha * alpha);
+    V3<R> H = normalize3(toBasis(sp.tangent, B, N, rd));
+    V3<R> o = reflect3(I, H);
+    if (!(dot3(N, o) > R(0))) return zero;
+    O = normalize3(o);
+    R LdotN = dot3(O, N), VdotH = std::abs(dot3(I, H)), NdotH = std::abs(dot3(N, H));
+    R G = smithG2(smithG1(VdotN, alpha), smithG1(LdotN, alpha));
+    R w = G * VdotH / (VdotN * NdotH);
+    R ct = inv ? LdotN : VdotN;
+    V3<R> out;
+    for (int c = 0; c < 3; ++c) {
+        R rp, rs;
+        fresnel(R(m.n[c]), R(m.k[c]), VdotH, rp, rs);
+        at(out, c) = lerpv(rp, rs, R(0.5f)) * w / ct;
+    }
+    return out;
+}
+
 /* RenderMaterial::getBrdf dispatch for a connection (BDPTRenderer.cpp:287-289) */
 template <class R>
-static V3<R> matGetBrdf(const Scene& S, const SurfacePoint<R>& sp, const V3<R>& inV, const V3<R>& outV) {
+static V3<R> matGetBrdf(const Scene& S, const Ext& X, const SurfacePoint<R>& sp, const V3<R>& inV, const V3<R>& outV) {
     const Material& m = S.materials[S.meshes[sp.mesh].matId];
     if (m.type == MAT_OM) {
         if (!m.omHasBase) return mk(R(0), R(0), R(0));
+        if (X.omRough) return omRoughEval<R>(m, R(X.omAlpha), inV, outV, sp.normal);
         R fCos = -dot3(inV, sp.normal);
         if (std::abs(dot3(outV, sp.normal) - fCos) > R(1e-6f)) return mk(R(0), R(0), R(0));
         return omFresnelBrdf<R>(m, fCos);
@@ -1115,11 +1177,12 @@ static void modifyFrame(const Scene& S, SurfacePoint<R>& sp) {
 
 /* RenderMaterial::getRayAndBrdf dispatch (computePath, BDPTRenderer.cpp:158-159) */
 template <class R>
-static void matGetRayAndBrdf(const Scene& S, const SurfacePoint<R>& sp, const V3<R>& inV, V3<R>& outV, V3<R>& brdf,
-                             V3<R>& erad, bool inv, Uniforms& U) {
+static void matGetRayAndBrdf(const Scene& S, const Ext& X, const SurfacePoint<R>& sp, const V3<R>& inV, V3<R>& outV,
+                             V3<R>& brdf, V3<R>& erad, bool inv, Uniforms& U) {
     const Material& m = S.materials[S.meshes[sp.mesh].matId];
     erad = mk(R(0), R(0), R(0));
     if (m.type == MAT_OM) {
+        if (X.omRough && m.omHasBase) { brdf = omRoughSample<R>(m, R(X.omAlpha), inV, sp, inv, outV, U); return; }
         outV = omNextDirection(S, inV, sp);
         if (!m.omHasBase) { brdf = mk(R(0), R(0), R(0)); return; }
         brdf = omFresnelBrdf<R>(m, dot3(outV, sp.normal));
@@ -1139,8 +1202,8 @@ static void matGetRayAndBrdf(const Scene& S, const SurfacePoint<R>& sp, const V3
     /* DiffuseColorMaterial via RenderMaterial::getRayAndBrdf (RenderMaterial.cpp:51-72) */
     V3<R> v = uniformHemisphere<R>(U);
     outV = toBasis(sp.tangent, sp.binormal, sp.normal, v);
-    if (!inv) brdf = matGetBrdf(S, sp, inV, outV);
-    else brdf = matGetBrdf(S, sp, neg(outV), neg(inV));
+    if (!inv) brdf = matGetBrdf(S, X, sp, inV, outV);
+    else brdf = matGetBrdf(S, X, sp, neg(outV), neg(inV));
 }
 
 /* ------------------------------------------------------ renderer (A4-A12) */
@@ -1154,9 +1217,10 @@ template <class R>
 struct Tracer {
     const Scene& S;
     const omv_renderer_params& P;
+    const Ext X;
     uint64_t rays = 0, camRays = 0, lightRays = 0, shadowRays = 0;
     Counters cnt;
-    Tracer(const Scene& s, const omv_renderer_params& p) : S(s), P(p) {}
+    Tracer(const Scene& s, const omv_renderer_params& p) : S(s), P(p), X(p) {}
 
     /* sceneIntersect (BDPTRenderer.cpp:75-102); tnear persists in the caller */
     bool sceneIntersect(const V3<R>& org, const V3<R>& dir, R& tnear, R tfar, SurfacePoint<R>& sp) {
@@ -1209,7 +1273,7 @@ struct Tracer {
             Pv[i].pos = Pv[i].sp.position;
             modifyFrame(S, Pv[i].sp);
             V3<R> brdf, erad;
-            matGetRayAndBrdf(S, Pv[i].sp, d0, Pv[i].outRay, brdf, erad, inv, U);
+            matGetRayAndBrdf(S, X, Pv[i].sp, d0, Pv[i].outRay, brdf, erad, inv, U);
             R ct = std::abs(inv ? dot3(Pv[i].outRay, Pv[i].sp.normal) : dot3(d0, Pv[i].sp.normal));
             const V3<R>& of = Pv[i - 1].outFactor;
             Pv[i].outFactor = mk(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
@@ -1334,7 +1398,7 @@ struct Tracer {
                     V3<R> lc = mk(lRad.x * c * of.x, lRad.y * c * of.y, lRad.z * c * of.z);
                     if (intensity(lc) < R(P.ray_cut_pix_value)) continue;
                     V3<R> outV = neg(cp[j - 1].outRay);
-                    V3<R> brdf = matGetBrdf(S, cp[j].sp, LC, outV);
+                    V3<R> brdf = matGetBrdf(S, X, cp[j].sp, LC, outV);
                     brdf = mk(brdf.x * c, brdf.y * c, brdf.z * c);
                     lc = mk(brdf.x * of.x * lRad.x, brdf.y * of.y * lRad.y, brdf.z * of.z * lRad.z);
                     if (intensity(lc) < R(P.ray_cut_pix_value)) continue;

@@ -127,3 +127,12 @@ def test_bdpt_icosphere_reference_mode_is_black(pkg, wl, ref):
     assert np.array_equal(img[..., :3], np.zeros_like(img[..., :3]))
     assert np.array_equal(rimg, img)
     assert st["light_rays"] == 64 * 64 * 2
+
+
+def test_bdpt_icosphere_om_rough_extension(pkg, wl, ref):
+    desc, view, params = wl.plain_bdpt_icosphere()
+    view.width, view.height = 64, 64
+    params.update(samples_per_pixel=4, extensions=1, om_roughness=0.3)
+    img, rimg, st, rst = _render_pair(pkg, ref, desc, view, **params)
+    assert img[..., :3].max() > 0.0
+    _radiance_close(img, rimg)

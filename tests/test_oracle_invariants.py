@@ -360,3 +360,35 @@ def test_float_vs_double_oracle(ref, wl):
     assert np.isfinite(a).all() and np.isfinite(b).all()
     rel = abs(a[..., :3].mean() - b[..., :3].mean()) / b[..., :3].mean()
     assert rel < 0.02
+
+
+# ------------------------------------------------ OMV_EXT_OM_ROUGH extension
+def test_om_rough_tends_to_the_mirror(ref, wl):
+    """As the roughness of the rough-conductor extension goes to zero, the
+    throughput F G |I.H| / (|I.N| |N.H|) tends to the mirror's F: the near
+    normal-incidence pixel of the mirror-sphere scene returns bg * F."""
+    n, k = (0.3, 0.9, 1.4), (3.0, 5.0, 7.0)
+    d = wl.SceneDesc()
+    v, f = wl.icosphere(5, 1.0)
+    d.materials.append(("om_nk", n, k))
+    d.meshes.append({"vertices": v, "faces": f, "mat_id": 0})
+    d.instances.append((wl.identity(), 0))
+    d.background = ("const", (100.0, 100.0, 100.0))
+    s = ref.Scene()
+    d.apply(s, ref.pbr_desc)
+    cam = ref.camera_desc(9, 9, (0, 0, 6), (0, 0, -1), yfov=2.0)
+    img, _ = s.render(ref.default_params(samples_per_pixel=32, extensions=1, om_roughness=0.02), cam)
+    F = np.array([((a - 1) ** 2 + b ** 2) / ((a + 1) ** 2 + b ** 2) for a, b in zip(n, k)])
+    assert np.allclose(img[4, 4, :3], 100.0 * F, rtol=1e-2)
+
+
+def test_om_rough_lights_the_icosphere(ref, wl):
+    desc, view, params = wl.plain_bdpt_icosphere()
+    s = ref.Scene()
+    desc.apply(s, ref.pbr_desc)
+    cam = ref.camera_desc(32, 32, view.position, view.forward, view.up, view.yfov)
+    params["samples_per_pixel"] = 4
+    black, _ = s.render(ref.default_params(**params), cam)
+    lit, st = s.render(ref.default_params(extensions=1, om_roughness=0.3, **params), cam)
+    assert np.all(black[..., :3] == 0.0)
+    assert lit[..., :3].max() > 0.0 and st["shadow_rays"] > 0
