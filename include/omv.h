@@ -259,6 +259,39 @@ int omv_renderer_render(omv_renderer* r, omv_camera* c);
 int omv_renderer_render_fc(omv_renderer* r, omv_camera* c, int32_t* ids, uint8_t* tie);
 int omv_renderer_get_stats(const omv_renderer* r, omv_render_stats* out);
 
+/* ------------------------------------------------------ batched jobs */
+/* Many independent validation renders (each one BDPTRenderer::render call of
+ * main.cpp:489-521 with its own scene, camera and parameters) run as ONE
+ * persistent launch.  The reference has no batched form: its closest
+ * counterpart is the per-viewpoint loop main.cpp:489-521 run once per asset
+ * (the pathtracer/scripts shell scripts drive it per file).  Each job's film equals, bit
+ * for bit, what omv_renderer_render gives for the same scene, camera and
+ * parameters.  Scenes must stay committed and unchanged while a batch that
+ * names them exists; cameras are copied at add time. */
+typedef struct omv_batch omv_batch;
+typedef struct omv_batch_tuning {
+    int blocks_per_cu;   /* resident workgroups per CU (0 = occupancy limit) */
+    int tile_waves;      /* pixels per work item = 64 * tile_waves (>= 1)    */
+    int use_graph;       /* 1: replay a captured hipGraph, 0: plain launch   */
+} omv_batch_tuning;
+int omv_batch_create(omv_batch** out);
+int omv_batch_destroy(omv_batch* b);
+/* Adds a job; job ids count from 0 in call order. */
+int omv_batch_add_job(omv_batch* b, const omv_scene* s, const omv_camera* c,
+                      const omv_renderer_params* p, int* job_id);
+int omv_batch_job_count(const omv_batch* b, int* n);
+int omv_batch_set_tuning(omv_batch* b, const omv_batch_tuning* t);
+int omv_batch_get_tuning(const omv_batch* b, omv_batch_tuning* t);
+/* Times every (blocks_per_cu, tile_waves) candidate on the batch itself,
+ * 'reps' runs each, keeps the fastest and reports it in best (may be NULL). */
+int omv_batch_autotune(omv_batch* b, int reps, omv_batch_tuning* best);
+/* Renders every job.  Synchronous. */
+int omv_batch_run(omv_batch* b);
+/* Normalised film of one job (W*H*4 floats), as omv_camera_get_film. */
+int omv_batch_get_film(const omv_batch* b, int job_id, float* out, size_t cap_floats);
+/* Totals over all jobs of the last run; render_ms is the launch's device time. */
+int omv_batch_get_stats(const omv_batch* b, omv_render_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

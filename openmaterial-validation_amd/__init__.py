@@ -63,6 +63,10 @@ class IorTable(C.Structure):
     ]
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("blocks_per_cu", C.c_int), ("tile_waves", C.c_int), ("use_graph", C.c_int)]
+
+
 class RenderStats(C.Structure):
     _fields_ = [
         ("samples", C.c_uint64), ("rays", C.c_uint64), ("camera_rays", C.c_uint64), ("light_rays", C.c_uint64),
@@ -252,6 +256,13 @@ def _load():
         "omv_renderer_render": [vp, vp], "omv_renderer_render_fc": [vp, vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)],
         "omv_renderer_get_stats": [vp, C.POINTER(RenderStats)],
         "omv_renderer_params_default": [C.POINTER(RendererParams)],
+        "omv_batch_create": [C.POINTER(vp)], "omv_batch_destroy": [vp],
+        "omv_batch_add_job": [vp, vp, vp, C.POINTER(RendererParams), C.POINTER(C.c_int)],
+        "omv_batch_job_count": [vp, C.POINTER(C.c_int)],
+        "omv_batch_set_tuning": [vp, C.POINTER(BatchTuning)], "omv_batch_get_tuning": [vp, C.POINTER(BatchTuning)],
+        "omv_batch_autotune": [vp, C.c_int, C.POINTER(BatchTuning)], "omv_batch_run": [vp],
+        "omv_batch_get_film": [vp, C.c_int, C.POINTER(C.c_float), C.c_size_t],
+        "omv_batch_get_stats": [vp, C.POINTER(RenderStats)],
         "omv_device_synchronize": [], "omv_device_info": [C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_size_t)],
     }.items():
         f = getattr(lib, name)
@@ -408,4 +419,65 @@ class Renderer:
     def __del__(self):
         if getattr(self, "h", None):
             self.lib.omv_renderer_destroy(self.h)
+            self.h = None
+
+
+class Batch:
+    """Many render jobs (scene, camera, parameters) in one persistent launch.
+
+    Each job's film equals Renderer.render's for the same inputs.  Scenes must
+    stay alive and committed while the batch exists; cameras are copied when
+    the job is added."""
+
+    def __init__(self):
+        self.lib = lib()
+        h = C.c_void_p()
+        self._check(self.lib.omv_batch_create(C.byref(h)))
+        self.h = h
+        self._jobs = []      # (scene, width, height): keeps scenes alive
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.lib.omv_last_error().decode())
+
+    def add_job(self, scene: Scene, cam: Camera, params: RendererParams) -> int:
+        jid = C.c_int()
+        self._check(self.lib.omv_batch_add_job(self.h, scene.h, cam.h, C.byref(params), C.byref(jid)))
+        self._jobs.append((scene, cam.width, cam.height))
+        return jid.value
+
+    def __len__(self):
+        return len(self._jobs)
+
+    def set_tuning(self, blocks_per_cu=0, tile_waves=1, use_graph=True):
+        t = BatchTuning(blocks_per_cu, tile_waves, 1 if use_graph else 0)
+        self._check(self.lib.omv_batch_set_tuning(self.h, C.byref(t)))
+
+    def tuning(self) -> dict:
+        t = BatchTuning()
+        self._check(self.lib.omv_batch_get_tuning(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in BatchTuning._fields_}
+
+    def autotune(self, reps=1) -> dict:
+        t = BatchTuning()
+        self._check(self.lib.omv_batch_autotune(self.h, reps, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in BatchTuning._fields_}
+
+    def run(self):
+        self._check(self.lib.omv_batch_run(self.h))
+
+    def film(self, job: int) -> np.ndarray:
+        _, w, h = self._jobs[job]
+        out = np.empty(w * h * 4, np.float32)
+        self._check(self.lib.omv_batch_get_film(self.h, job, _ptr(out, C.c_float), out.size))
+        return out.reshape(h, w, 4)
+
+    def stats(self) -> dict:
+        s = RenderStats()
+        self._check(self.lib.omv_batch_get_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in RenderStats._fields_}
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.omv_batch_destroy(self.h)
             self.h = None

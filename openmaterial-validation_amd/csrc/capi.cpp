@@ -8,6 +8,7 @@
 struct omv_scene { omv::RenderScene s; };
 struct omv_camera { omv::PinholeCamera c; omv_camera(int w, int h) : c(w, h) {} };
 struct omv_renderer { omv::BDPTRenderer r; };
+struct omv_batch { omv::BatchRunner b; };
 
 #define OMV_NULLCHK(p) do { if (!(p)) { omv::setError(std::string(__func__) + ": null argument"); return OMV_E_ARG; } } while (0)
 
@@ -202,6 +203,51 @@ int omv_renderer_render_fc(omv_renderer* r, omv_camera* c, int32_t* ids, uint8_t
 int omv_renderer_get_stats(const omv_renderer* r, omv_render_stats* out) {
     OMV_NULLCHK(r); OMV_NULLCHK(out);
     *out = r->r.stats();
+    return OMV_OK;
+}
+
+int omv_batch_create(omv_batch** out) {
+    OMV_NULLCHK(out);
+    *out = new (std::nothrow) omv_batch();
+    if (!*out) { omv::setError("omv_batch_create: out of memory"); return OMV_E_NOMEM; }
+    return OMV_OK;
+}
+int omv_batch_destroy(omv_batch* b) { delete b; return OMV_OK; }
+int omv_batch_add_job(omv_batch* b, const omv_scene* s, const omv_camera* c, const omv_renderer_params* p, int* job_id) {
+    OMV_NULLCHK(b); OMV_NULLCHK(s); OMV_NULLCHK(c); OMV_NULLCHK(p);
+    int rc = b->b.addJob(&s->s, c->c.desc, *p);
+    if (rc == OMV_OK && job_id) *job_id = b->b.jobCount() - 1;
+    return rc;
+}
+int omv_batch_job_count(const omv_batch* b, int* n) {
+    OMV_NULLCHK(b); OMV_NULLCHK(n);
+    *n = b->b.jobCount();
+    return OMV_OK;
+}
+int omv_batch_set_tuning(omv_batch* b, const omv_batch_tuning* t) {
+    OMV_NULLCHK(b); OMV_NULLCHK(t);
+    return b->b.setTuning(*t);
+}
+int omv_batch_get_tuning(const omv_batch* b, omv_batch_tuning* t) {
+    OMV_NULLCHK(b); OMV_NULLCHK(t);
+    *t = b->b.tuning();
+    return OMV_OK;
+}
+int omv_batch_autotune(omv_batch* b, int reps, omv_batch_tuning* best) {
+    OMV_NULLCHK(b);
+    return b->b.autotune(reps, best);
+}
+int omv_batch_run(omv_batch* b) {
+    OMV_NULLCHK(b);
+    return b->b.run();
+}
+int omv_batch_get_film(const omv_batch* b, int job_id, float* out, size_t cap) {
+    OMV_NULLCHK(b); OMV_NULLCHK(out);
+    return b->b.film(job_id, out, cap);
+}
+int omv_batch_get_stats(const omv_batch* b, omv_render_stats* out) {
+    OMV_NULLCHK(b); OMV_NULLCHK(out);
+    *out = b->b.stats();
     return OMV_OK;
 }
 

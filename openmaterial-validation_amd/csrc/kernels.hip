@@ -21,6 +21,7 @@
 // off (the build passes -ffp-contract=off) and correctly rounded division
 // and square root, so the ray/hit/id path is bit-identical to the CPU oracle.
 #include <hip/hip_runtime.h>
+#include <cstring>
 #include <float.h>
 #include <stdint.h>
 
@@ -976,10 +977,144 @@ DEV void flushCounters(unsigned long long* g, uint32_t rays, uint32_t cam, uint3
 // ============================================================== kernels
 extern __shared__ uint32_t omv_lds_stack[];
 
-// One sample of computePaths (BDPTRenderer.cpp:237-301) per lane, persistent grid.
-// kLights = false is the instantiation for scenes without point lights: no
-// light subpath exists (BDPTRenderer.cpp:241-242) and no connection code is
-// compiled, which frees registers for occupancy.
+// Per-lane state of a path kernel: traversal stack, counters and this lane's
+// column of the light-vertex scratch ([vertex][component][lane]).
+struct Lane {
+    Stack st;
+    Counters cnt;
+    uint32_t rays, camRays, lightRays, shadowRays;
+    float* lv;
+    uint64_t lvStride;
+    DEV float& lvAt(int vtx, int c) { return lv[((uint64_t)vtx * 6 + c) * lvStride]; }
+};
+
+DEV void laneInit(Lane& L, uint32_t* gstack, float* lv, uint32_t gtid, uint32_t gthreads) {
+    L.st.lds = omv_lds_stack + threadIdx.x; L.st.ldsStride = blockDim.x;
+    L.st.glb = gstack + gtid; L.st.glbStride = gthreads;
+    L.st.sp = 0;
+    L.cnt.nodes = 0; L.cnt.tris = 0; L.cnt.overflow = 0;
+    L.rays = L.camRays = L.lightRays = L.shadowRays = 0;
+    L.lv = lv + gtid; L.lvStride = gthreads;
+}
+
+// One sample of BDPTRenderer::computePaths (BDPTRenderer.cpp:237-301):
+// sample s = pixel * spp + k.  kLights = false compiles no light subpath
+// and no connection code (scenes without point lights, BDPTRenderer.cpp:241-242).
+template <bool kLights>
+DEV f3 pathSample(const OmvSceneView& S, const omv_camera_desc& cam, const RP& P, int spp, uint64_t s, Lane& L) {
+    const int q = P.q;
+    const int lvCap = max(min(P.light_bounces + 1, max(q, 1)), 1);
+    const uint64_t pix = s / (uint64_t)spp;
+    const int x = (int)(pix % (uint64_t)cam.width), y = (int)(pix / (uint64_t)cam.width);
+    Rng cs; cs.init(P.seed, STREAM_CAMERA, s);
+    float u0 = cs.next(), u1 = cs.next();
+    f3 O, D;
+    cameraRay(cam, x, y, u0, u1, O, D);
+
+    // ---- light subpath (tryComputeLightPath + computePath(light), BDPTRenderer.cpp:128-206)
+    int lpLen = 0, lightIdx = -1;
+    f3 lightPos = mk3(0.0f, 0.0f, 0.0f), lightInt = mk3(0.0f, 0.0f, 0.0f);
+    float lightRange = FLT_MAX;
+    if (kLights && S.num_lights > 0) {
+        Rng ls; ls.init(P.seed, STREAM_LIGHT, s);
+        lightIdx = sampleLight(S, ls.next());
+        const OmvLight Lt = S.lights[lightIdx];
+        lightPos = ld3(Lt.pos); lightInt = ld3(Lt.intensity); lightRange = Lt.range;
+        f3 dir = uniformSphere(ls);
+        const f3 lRad = mk3(Lt.intensity[0] * P.light_scale, Lt.intensity[1] * P.light_scale, Lt.intensity[2] * P.light_scale);
+        // vertex 0: the light itself
+        L.lvAt(0, 0) = lightPos.x; L.lvAt(0, 1) = lightPos.y; L.lvAt(0, 2) = lightPos.z;
+        L.lvAt(0, 3) = lRad.x; L.lvAt(0, 4) = lRad.y; L.lvAt(0, 5) = lRad.z;
+        f3 pPos = lightPos, pDir = dir, of = mk3(1.0f, 1.0f, 1.0f);
+        float tnear = 0.0f, rD = 1.0f;
+        int n = 1;
+        // exact shortcut: light vertices with index >= max_path_length are never connected
+        for (int i = 1; i < lvCap; ++i) {
+            float delta = 3.0f * FLT_EPSILON * maxAbs(pPos);
+            f3 org = mk3(pPos.x + delta * pDir.x, pPos.y + delta * pDir.y, pPos.z + delta * pDir.z);
+            ++L.lightRays;
+            Hit h;
+            if (!sceneIntersect(S, org, pDir, tnear, FLT_MAX, h, L.st, L.cnt, L.rays)) break;
+            SP sp;
+            computeSurfacePoint(S, h, sp, false);
+            modifyFrame(S, sp);
+            f3 outRay, brdf, erad;
+            matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, false, ls);
+            float ct = fabsf(dot3(pDir, sp.n));
+            of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
+            if (i == 1) rD = attenuation(P, length3(sub3(sp.pos, lightPos)), lightRange);
+            f3 outRad = mk3(of.x * lRad.x * rD, of.y * lRad.y * rD, of.z * lRad.z * rD);
+            if (intensity(outRad) < P.cut) break;
+            L.lvAt(i, 0) = sp.pos.x; L.lvAt(i, 1) = sp.pos.y; L.lvAt(i, 2) = sp.pos.z;
+            L.lvAt(i, 3) = outRad.x; L.lvAt(i, 4) = outRad.y; L.lvAt(i, 5) = outRad.z;
+            ++n;
+            pPos = sp.pos; pDir = outRay;
+        }
+        lpLen = n;
+    }
+
+    // ---- camera subpath + connections
+    f3 radiance = mk3(0.0f, 0.0f, 0.0f), conn = mk3(0.0f, 0.0f, 0.0f);
+    f3 pPos = O, pDir = D, of = mk3(1.0f, 1.0f, 1.0f);
+    float tnear = 0.0f;
+    int n = 1;
+    bool exitScene = false;
+    for (int i = 1; i < P.camera_bounces + 1; ++i) {
+        float delta = 3.0f * FLT_EPSILON * maxAbs(pPos);
+        f3 org = mk3(pPos.x + delta * pDir.x, pPos.y + delta * pDir.y, pPos.z + delta * pDir.z);
+        ++L.camRays;
+        Hit h;
+        if (!sceneIntersect(S, org, pDir, tnear, FLT_MAX, h, L.st, L.cnt, L.rays)) { exitScene = true; break; }
+        SP sp;
+        computeSurfacePoint(S, h, sp, false);
+        modifyFrame(S, sp);
+        // connections of camera vertex j = i with light vertices l, l + j <= q
+        if (kLights && lightIdx >= 0) {
+            const int lEnd = min(lpLen, q - i + 1);
+            const f3 outV = neg3(pDir);
+            for (int l = 0; l < lEnd; ++l) {
+                f3 lPos = mk3(L.lvAt(l, 0), L.lvAt(l, 1), L.lvAt(l, 2));
+                f3 lRad;
+                f3 LC = normalize3(sub3(sp.pos, lPos));
+                if (l == 0) {
+                    float lsc = attenuation(P, length3(sub3(sp.pos, lPos)), lightRange);
+                    float k = P.light_scale * lsc;
+                    lRad = mk3(lightInt.x * k, lightInt.y * k, lightInt.z * k);
+                } else {
+                    lRad = mk3(L.lvAt(l, 3), L.lvAt(l, 4), L.lvAt(l, 5));
+                }
+                float c = fabsf(dot3(LC, sp.n));
+                f3 lc = mk3(lRad.x * c * of.x, lRad.y * c * of.y, lRad.z * c * of.z);
+                if (intensity(lc) < P.cut) continue;
+                f3 brdf = matGetBrdf(S, P.x, sp, LC, outV);
+                brdf = mk3(brdf.x * c, brdf.y * c, brdf.z * c);
+                lc = mk3(brdf.x * of.x * lRad.x, brdf.y * of.y * lRad.y, brdf.z * of.z * lRad.z);
+                if (intensity(lc) < P.cut) continue;
+                if (!isConnected(S, lPos, sp.pos, L.st, L.cnt, L.rays, L.shadowRays)) continue;
+                conn = mk3(lc.x + conn.x, lc.y + conn.y, lc.z + conn.z);
+            }
+        }
+        f3 outRay, brdf, erad;
+        matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, true, cs);
+        float ct = fabsf(dot3(outRay, sp.n));
+        of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
+        radiance = mk3(radiance.x + of.x * erad.x, radiance.y + of.y * erad.y, radiance.z + of.z * erad.z);
+        ++n;
+        pPos = sp.pos; pDir = outRay;
+        // exact shortcuts, as in the oracle: zero throughput, or no emission past q
+        if (of.x == 0.0f && of.y == 0.0f && of.z == 0.0f) break;
+        if (!S.any_emissive && n > q) break;
+    }
+    if (exitScene && S.bg_type != OMV_BG_NONE && n <= q) {
+        f3 bg = background(S, pDir);
+        bg = mk3(bg.x * of.x, bg.y * of.y, bg.z * of.z);
+        radiance = mk3(bg.x * 1.0f + radiance.x, bg.y * 1.0f + radiance.y, bg.z * 1.0f + radiance.z);
+    }
+    return mk3(conn.x * 1.0f + radiance.x, conn.y * 1.0f + radiance.y, conn.z * 1.0f + radiance.z);
+}
+
+// Plain render: one sample per lane at a time over [sample0, sample0 + nsamples),
+// persistent grid; radiance per sample for the ordered film reduction.
 template <bool kLights>
 __global__ __launch_bounds__(OMV_BLOCK) void omv_bdpt_kernel(OmvSceneView S, omv_camera_desc cam, RP P, int spp,
                                                              uint64_t sample0, uint64_t nsamples, float* __restrict__ rad,
@@ -987,139 +1122,76 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_bdpt_kernel(OmvSceneView S, omv
                                                              unsigned long long* __restrict__ counters) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack st;
-    st.lds = omv_lds_stack + threadIdx.x; st.ldsStride = blockDim.x;
-    st.glb = gstack + gtid; st.glbStride = gthreads;
-    st.sp = 0;
-    Counters cnt; cnt.nodes = 0; cnt.tris = 0; cnt.overflow = 0;
-    uint32_t rays = 0, camRays = 0, lightRays = 0, shadowRays = 0;
-    const int q = P.q;
-    const int lvCap = max(min(P.light_bounces + 1, max(q, 1)), 1);
-
+    Lane L;
+    laneInit(L, gstack, lv, gtid, gthreads);
     for (uint64_t sl = gtid; sl < nsamples; sl += gthreads) {
-        const uint64_t s = sample0 + sl;
-        const uint64_t pix = s / (uint64_t)spp;
-        const int x = (int)(pix % (uint64_t)cam.width), y = (int)(pix / (uint64_t)cam.width);
-        Rng cs; cs.init(P.seed, STREAM_CAMERA, s);
-        float u0 = cs.next(), u1 = cs.next();
-        f3 O, D;
-        cameraRay(cam, x, y, u0, u1, O, D);
-
-        // ---- light subpath (tryComputeLightPath + computePath(light), BDPTRenderer.cpp:128-206)
-        int lpLen = 0, lightIdx = -1;
-        f3 lightPos = mk3(0.0f, 0.0f, 0.0f), lightInt = mk3(0.0f, 0.0f, 0.0f);
-        float lightRange = FLT_MAX;
-        if (kLights && S.num_lights > 0) {
-            Rng ls; ls.init(P.seed, STREAM_LIGHT, s);
-            lightIdx = sampleLight(S, ls.next());
-            const OmvLight L = S.lights[lightIdx];
-            lightPos = ld3(L.pos); lightInt = ld3(L.intensity); lightRange = L.range;
-            f3 dir = uniformSphere(ls);
-            const f3 lRad = mk3(L.intensity[0] * P.light_scale, L.intensity[1] * P.light_scale, L.intensity[2] * P.light_scale);
-            // vertex 0: the light itself
-            lv[(0 * 6 + 0) * (uint64_t)gthreads + gtid] = lightPos.x;
-            lv[(0 * 6 + 1) * (uint64_t)gthreads + gtid] = lightPos.y;
-            lv[(0 * 6 + 2) * (uint64_t)gthreads + gtid] = lightPos.z;
-            lv[(0 * 6 + 3) * (uint64_t)gthreads + gtid] = lRad.x;
-            lv[(0 * 6 + 4) * (uint64_t)gthreads + gtid] = lRad.y;
-            lv[(0 * 6 + 5) * (uint64_t)gthreads + gtid] = lRad.z;
-            f3 pPos = lightPos, pDir = dir, of = mk3(1.0f, 1.0f, 1.0f);
-            float tnear = 0.0f, rD = 1.0f;
-            int n = 1;
-            // exact shortcut: light vertices with index >= max_path_length are never connected
-            for (int i = 1; i < lvCap; ++i) {
-                float delta = 3.0f * FLT_EPSILON * maxAbs(pPos);
-                f3 org = mk3(pPos.x + delta * pDir.x, pPos.y + delta * pDir.y, pPos.z + delta * pDir.z);
-                ++lightRays;
-                Hit h;
-                if (!sceneIntersect(S, org, pDir, tnear, FLT_MAX, h, st, cnt, rays)) break;
-                SP sp;
-                computeSurfacePoint(S, h, sp, false);
-                modifyFrame(S, sp);
-                f3 outRay, brdf, erad;
-                matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, false, ls);
-                float ct = fabsf(dot3(pDir, sp.n));
-                of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
-                if (i == 1) rD = attenuation(P, length3(sub3(sp.pos, lightPos)), lightRange);
-                f3 outRad = mk3(of.x * lRad.x * rD, of.y * lRad.y * rD, of.z * lRad.z * rD);
-                if (intensity(outRad) < P.cut) break;
-                lv[((uint64_t)i * 6 + 0) * gthreads + gtid] = sp.pos.x;
-                lv[((uint64_t)i * 6 + 1) * gthreads + gtid] = sp.pos.y;
-                lv[((uint64_t)i * 6 + 2) * gthreads + gtid] = sp.pos.z;
-                lv[((uint64_t)i * 6 + 3) * gthreads + gtid] = outRad.x;
-                lv[((uint64_t)i * 6 + 4) * gthreads + gtid] = outRad.y;
-                lv[((uint64_t)i * 6 + 5) * gthreads + gtid] = outRad.z;
-                ++n;
-                pPos = sp.pos; pDir = outRay;
-            }
-            lpLen = n;
-        }
-
-        // ---- camera subpath + connections
-        f3 radiance = mk3(0.0f, 0.0f, 0.0f), conn = mk3(0.0f, 0.0f, 0.0f);
-        f3 pPos = O, pDir = D, of = mk3(1.0f, 1.0f, 1.0f);
-        float tnear = 0.0f;
-        int n = 1;
-        bool exitScene = false;
-        for (int i = 1; i < P.camera_bounces + 1; ++i) {
-            float delta = 3.0f * FLT_EPSILON * maxAbs(pPos);
-            f3 org = mk3(pPos.x + delta * pDir.x, pPos.y + delta * pDir.y, pPos.z + delta * pDir.z);
-            ++camRays;
-            Hit h;
-            if (!sceneIntersect(S, org, pDir, tnear, FLT_MAX, h, st, cnt, rays)) { exitScene = true; break; }
-            SP sp;
-            computeSurfacePoint(S, h, sp, false);
-            modifyFrame(S, sp);
-            // connections of camera vertex j = i with light vertices l, l + j <= q
-            if (kLights && lightIdx >= 0) {
-                const int lEnd = min(lpLen, q - i + 1);
-                const f3 outV = neg3(pDir);
-                for (int l = 0; l < lEnd; ++l) {
-                    f3 lPos = mk3(lv[((uint64_t)l * 6 + 0) * gthreads + gtid], lv[((uint64_t)l * 6 + 1) * gthreads + gtid],
-                                  lv[((uint64_t)l * 6 + 2) * gthreads + gtid]);
-                    f3 lRad;
-                    f3 LC = normalize3(sub3(sp.pos, lPos));
-                    if (l == 0) {
-                        float lsc = attenuation(P, length3(sub3(sp.pos, lPos)), lightRange);
-                        float k = P.light_scale * lsc;
-                        lRad = mk3(lightInt.x * k, lightInt.y * k, lightInt.z * k);
-                    } else {
-                        lRad = mk3(lv[((uint64_t)l * 6 + 3) * gthreads + gtid], lv[((uint64_t)l * 6 + 4) * gthreads + gtid],
-                                   lv[((uint64_t)l * 6 + 5) * gthreads + gtid]);
-                    }
-                    float c = fabsf(dot3(LC, sp.n));
-                    f3 lc = mk3(lRad.x * c * of.x, lRad.y * c * of.y, lRad.z * c * of.z);
-                    if (intensity(lc) < P.cut) continue;
-                    f3 brdf = matGetBrdf(S, P.x, sp, LC, outV);
-                    brdf = mk3(brdf.x * c, brdf.y * c, brdf.z * c);
-                    lc = mk3(brdf.x * of.x * lRad.x, brdf.y * of.y * lRad.y, brdf.z * of.z * lRad.z);
-                    if (intensity(lc) < P.cut) continue;
-                    if (!isConnected(S, lPos, sp.pos, st, cnt, rays, shadowRays)) continue;
-                    conn = mk3(lc.x + conn.x, lc.y + conn.y, lc.z + conn.z);
-                }
-            }
-            f3 outRay, brdf, erad;
-            matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, true, cs);
-            float ct = fabsf(dot3(outRay, sp.n));
-            of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
-            radiance = mk3(radiance.x + of.x * erad.x, radiance.y + of.y * erad.y, radiance.z + of.z * erad.z);
-            ++n;
-            pPos = sp.pos; pDir = outRay;
-            // exact shortcuts, as in the oracle: zero throughput, or no emission past q
-            if (of.x == 0.0f && of.y == 0.0f && of.z == 0.0f) break;
-            if (!S.any_emissive && n > q) break;
-        }
-        if (exitScene && S.bg_type != OMV_BG_NONE && n <= q) {
-            f3 bg = background(S, pDir);
-            bg = mk3(bg.x * of.x, bg.y * of.y, bg.z * of.z);
-            radiance = mk3(bg.x * 1.0f + radiance.x, bg.y * 1.0f + radiance.y, bg.z * 1.0f + radiance.z);
-        }
-        radiance = mk3(conn.x * 1.0f + radiance.x, conn.y * 1.0f + radiance.y, conn.z * 1.0f + radiance.z);
-        rad[sl * 3 + 0] = radiance.x;
-        rad[sl * 3 + 1] = radiance.y;
-        rad[sl * 3 + 2] = radiance.z;
+        f3 r = pathSample<kLights>(S, cam, P, spp, sample0 + sl, L);
+        rad[sl * 3 + 0] = r.x;
+        rad[sl * 3 + 1] = r.y;
+        rad[sl * 3 + 2] = r.z;
     }
-    flushCounters(counters, rays, camRays, lightRays, shadowRays, cnt);
+    flushCounters(counters, L.rays, L.camRays, L.lightRays, L.shadowRays, L.cnt);
+}
+
+// Batched validation jobs in one persistent launch.  Work unit: a tile of
+// 64 x tileWaves consecutive pixels of one job, claimed by a whole wave
+// through one atomic on the work counter; each lane renders all spp samples
+// of its pixel in sample order (the reference's PinholeCamera::hit order)
+// and writes the normalised pixel, so no per-sample buffer is needed and the
+// film equals the plain render's bit for bit.
+struct OmvJobDev {
+    OmvSceneView S;
+    omv_camera_desc cam;
+    RP P;
+    int spp;
+    int pad;
+    uint64_t npix;
+    float* film;
+};
+
+__global__ __launch_bounds__(OMV_BLOCK) void omv_batch_kernel(const OmvJobDev* __restrict__ jobs, int njobs,
+                                                              const uint64_t* __restrict__ tileEnd, uint64_t totalTiles,
+                                                              int tileWaves, unsigned long long* __restrict__ work,
+                                                              float* __restrict__ lv, uint32_t* __restrict__ gstack,
+                                                              unsigned long long* __restrict__ counters) {
+    const uint32_t gthreads = gridDim.x * blockDim.x;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    Lane L;
+    laneInit(L, gstack, lv, gtid, gthreads);
+    for (;;) {
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(work, 1ull);
+        t = __shfl(t, 0, 64);
+        if (t >= totalTiles) break;
+        // job of tile t: first job whose tile range ends after t (wave-uniform)
+        int lo = 0, hi = njobs - 1;
+        while (lo < hi) { int mid = (lo + hi) >> 1; if (tileEnd[mid] > t) hi = mid; else lo = mid + 1; }
+        const int job = __builtin_amdgcn_readfirstlane(lo);
+        const OmvJobDev& J = jobs[job];
+        const uint64_t first = (job ? tileEnd[job - 1] : 0ull);
+        const uint64_t pix0 = (t - first) * 64ull * (uint64_t)tileWaves;
+        for (int w = 0; w < tileWaves; ++w) {
+            const uint64_t p = pix0 + (uint64_t)w * 64ull + (uint64_t)lane;
+            if (p >= J.npix) continue;
+            float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+            for (int k = 0; k < J.spp; ++k) {
+                f3 r = pathSample<true>(J.S, J.cam, J.P, J.spp, p * (uint64_t)J.spp + (uint64_t)k, L);
+                a0 = a0 + r.x; a1 = a1 + r.y; a2 = a2 + r.z;
+            }
+            float cnt = (float)J.spp;
+            float sc = cnt < 0.5f ? 1.0f : 1.0f / cnt;
+            float4 o; o.x = sc * a0; o.y = sc * a1; o.z = sc * a2; o.w = 1.0f;
+            reinterpret_cast<float4*>(J.film)[p] = o;
+        }
+    }
+    flushCounters(counters, L.rays, L.camRays, L.lightRays, L.shadowRays, L.cnt);
+}
+
+// Zero-fill usable inside a captured graph (no memset nodes).
+__global__ void omv_fill_u64(unsigned long long* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
 }
 
 // Film: PinholeCamera::init/hit/stop (PinholeCamera.cpp:65-108) for whole pixels.
@@ -1260,11 +1332,33 @@ hipError_t launchFc(const OmvSceneView& S, const omv_camera_desc& cam, uint32_t 
     return hipGetLastError();
 }
 
+size_t batchJobBytes() { return sizeof(OmvJobDev); }
+
+void packBatchJob(void* dst, const OmvSceneView& S, const omv_camera_desc& cam, const omv_renderer_params& p, float* film,
+                  uint64_t npix) {
+    OmvJobDev j;
+    std::memset(&j, 0, sizeof(j));
+    j.S = S; j.cam = cam; j.P = toRP(p); j.spp = p.samples_per_pixel; j.npix = npix; j.film = film;
+    std::memcpy(dst, &j, sizeof(j));
+}
+
+hipError_t launchBatch(const void* jobs, int njobs, const uint64_t* tileEnd, uint64_t totalTiles, int tileWaves,
+                       unsigned long long* counters, float* lv, uint32_t* gstack, int blocks, hipStream_t stream) {
+    // counters[0..7]: statistics, counters[8]: work counter; cleared by a
+    // kernel so that the sequence can be captured into a graph
+    hipLaunchKernelGGL(omv_fill_u64, dim3(1), dim3(64), 0, stream, counters, 16);
+    const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
+    hipLaunchKernelGGL(omv_batch_kernel, dim3(blocks), dim3(OMV_BLOCK), lds, stream, static_cast<const OmvJobDev*>(jobs), njobs,
+                       tileEnd, totalTiles, tileWaves, counters + 8, lv, gstack, counters);
+    return hipGetLastError();
+}
+
 int occupancyBlocksPerCU(int which) {
     int n = 0;
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
     hipError_t e = which == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<true>, OMV_BLOCK, lds)
                    : which == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<false>, OMV_BLOCK, lds)
+                   : which == 3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_batch_kernel, OMV_BLOCK, lds)
                                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_fc_kernel, OMV_BLOCK, lds);
     if (e != hipSuccess) n = 0;
     return n;

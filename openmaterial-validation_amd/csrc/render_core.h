@@ -130,6 +130,47 @@ private:
     DeviceBuffer dRad_, dLv_, dStack_, dCounters_, dFilm_, dIds_, dTie_;
 };
 
+// Many BDPTRenderer::render jobs in one persistent launch, replayed from a
+// captured hipGraph.  Work is handed out in tiles of 64 * tileWaves pixels;
+// jobs are laid out heaviest-per-tile first so the cheap tiles fill the tail.
+class BatchRunner {
+public:
+    BatchRunner();
+    ~BatchRunner();
+    int addJob(const RenderScene* s, const omv_camera_desc& cam, const omv_renderer_params& p);
+    int jobCount() const { return (int)jobs_.size(); }
+    int setTuning(const omv_batch_tuning& t);
+    const omv_batch_tuning& tuning() const { return tuning_; }
+    int autotune(int reps, omv_batch_tuning* best);
+    int run();
+    int film(int job, float* out, size_t cap) const;
+    const omv_render_stats& stats() const { return stats_; }
+
+private:
+    struct Job {
+        const RenderScene* scene;
+        omv_camera_desc cam;
+        omv_renderer_params p;
+        uint64_t npix, filmOff;
+        double tileCost;
+    };
+    int prepare();
+    int launch();
+    void dropGraph();
+    std::vector<Job> jobs_;
+    omv_batch_tuning tuning_{0, 1, 1};
+    bool dirty_ = true;
+    int blocks_ = 0, cus_ = 0;
+    uint64_t totalTiles_ = 0;
+    omv_render_stats stats_{};
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+    std::vector<OmvSceneView> views_;   // scene views the job table was built from
+    DeviceBuffer dJobs_, dTileEnd_, dFilm_, dLv_, dStack_, dCounters_;
+};
+
 // thread-local last error
 void setError(const std::string& msg);
 const char* lastError();

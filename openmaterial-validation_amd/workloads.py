@@ -365,6 +365,80 @@ def large_instanced(width=1920, height=1080, spp=16, depth=8, count=200, tris_pe
     return d, view, params
 
 
+def lightbox_views(vertices, tm=None):
+    """The eight lightbox viewpoints of main.cpp:434-457: for each corner p of
+    the scene bounding box with centre C, the camera sits at 2p - C and looks
+    at C (left = fwd x (0,1,0), up = left x fwd, not renormalised)."""
+    v = np.asarray(vertices, np.float64)
+    if tm is not None:
+        v = v @ np.asarray(tm, np.float64)[:3, :3].T + np.asarray(tm, np.float64)[:3, 3]
+    lo, hi = v.min(0).astype(np.float32), v.max(0).astype(np.float32)
+    c = ((lo + hi) * np.float32(0.5)).astype(np.float32)
+    views = []
+    for i in range(8):
+        p = np.array([hi[0] if i & 1 else lo[0], hi[1] if i & 2 else lo[1], hi[2] if i & 4 else lo[2]], np.float32)
+        fwd = (c - p).astype(np.float32)
+        pos = (np.float32(-2.0) * fwd + c).astype(np.float32)
+        views.append((tuple(pos.tolist()), tuple(fwd.tolist())))
+    return views
+
+
+def batched_lightbox(jobs=256, views_per_asset=8, seed=4, tri_range=(1000, 200000), res_range=(128, 512), spp_range=(8, 64),
+                     depth_range=(4, 8), hdr_w=512, hdr_h=256):
+    """Batched validation: jobs // views_per_asset assets, each one seeded
+    displaced mesh (sphere, box or torus knot) with a triangle count
+    log-uniform in tri_range and a random material (OpenMATERIAL (n,k) or PBR
+    base colour / metallic / roughness), lit by one point light and an HDRI,
+    seen from its lightbox viewpoints; every view has its own resolution
+    (square, uniform in res_range), spp and depth (uniform).  Returns
+    (assets: [SceneDesc], jobs: [(asset index, View, params)])."""
+    rng = np.random.default_rng(seed)
+    n_assets = max(1, (jobs + views_per_asset - 1) // views_per_asset)
+    sky = hdri(hdr_w, hdr_h, seed=seed + 100)
+    assets, out = [], []
+    for a in range(n_assets):
+        tris = int(round(math.exp(rng.uniform(math.log(tri_range[0]), math.log(tri_range[1])))))
+        kind = a % 3
+        freq = rng.uniform(2, 7, 3)
+        amp = rng.uniform(0.02, 0.12)
+        ph = rng.uniform(0, 2 * np.pi, 3)
+        if kind == 0:
+            stacks = max(4, int(round(math.sqrt(tris / 4))))
+            bump = lambda T, P, f=freq, am=amp, p=ph: 1.0 + am * np.sin(f[0] * T + p[0]) * np.cos(f[1] * P + p[1])
+            v, f_, uv = grid_sphere(stacks, 2 * stacks, radii=rng.uniform(0.5, 1.2, 3), bump=bump)
+        elif kind == 1:
+            n = max(2, int(round(math.sqrt(tris / 12))))
+            bump = lambda P, f=freq, am=amp, p=ph: 1.0 + am * np.sin(f[0] * P[..., 0] + p[0]) * np.sin(f[1] * P[..., 1] + p[1]) * np.sin(f[2] * P[..., 2] + p[2])
+            v, f_, uv = grid_box(n, size=rng.uniform(0.4, 1.0, 3), bump=bump)
+        else:
+            tube = max(6, int(round(math.sqrt(tris / 40))))
+            segs = max(12, int(round(tris / (2 * tube))))
+            v, f_, uv = torus_knot(segments=segs, tube=tube)
+        d = SceneDesc()
+        if rng.uniform() < 0.5:
+            n_, k_ = rng_om(int(rng.integers(1 << 30)))
+            d.materials.append(("om_nk", n_, k_))
+        else:
+            d.materials.append(("pbr", dict(base_color=tuple(rng.uniform(0.05, 0.95, 3).tolist()) + (1.0,),
+                                            metallic=float(rng.uniform(0.0, 1.0)), roughness=float(rng.uniform(0.05, 1.0)))))
+        d.meshes.append({"vertices": v, "faces": f_, "mat_id": 0, "uvs": [uv]})
+        d.instances.append((identity(), 0))
+        ext = float(np.abs(v).max())
+        d.lights.append(((2.5 * ext, 3.0 * ext, 2.0 * ext), tuple((rng.uniform(20, 60) * ext * np.ones(3)).tolist()), -1.0))
+        d.background = ("hdr", sky, HDR_ROTATION, HDR_SCALE)
+        assets.append(d)
+        for (pos, fwd) in lightbox_views(v)[:views_per_asset]:
+            if len(out) == jobs:
+                break
+            r = int(rng.integers(res_range[0], res_range[1] + 1))
+            depth = int(rng.integers(depth_range[0], depth_range[1] + 1))
+            spp = int(rng.integers(spp_range[0], spp_range[1] + 1))
+            params = dict(camera_bounces=depth, light_bounces=depth, max_path_length=depth, samples_per_pixel=spp,
+                          seed=int(len(out)))
+            out.append((a, View(r, r, pos, fwd, yfov=60.0), params))
+    return assets, out
+
+
 def mixed_test_scene(hdr=True, lights=True, tex_f32=False, masked=False, missing=True):
     """Small scene exercising every material path: a textured PBR floor, an
     OpenMATERIAL sphere, a PBR cube with a normal map, a mesh with a missing

@@ -110,3 +110,52 @@ def test_camera_desc_matches_oracle(pkg, ref):
         assert bytes(a) == bytes(b)
     cam = pkg.Camera(8, 8)
     assert pkg.lib().omv_camera_set_yfov(cam.h, C.c_float(180.0)) == -1   # PinholeCamera.cpp:169-173
+
+
+def test_batch_status_codes(pkg):
+    """Batch entry points reject bad state and arguments before any device work."""
+    L = pkg.lib()
+    b, s, c = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    assert L.omv_batch_create(C.byref(b)) == 0
+    assert L.omv_scene_create(C.byref(s)) == 0
+    assert L.omv_camera_create(C.byref(c), 8, 8) == 0
+    p = pkg.default_params()
+    jid = C.c_int(-1)
+    assert L.omv_batch_add_job(b, s, c, C.byref(p), C.byref(jid)) == -3      # scene not committed
+    assert L.omv_batch_add_job(b, None, c, C.byref(p), C.byref(jid)) == -1
+    n = C.c_int(-1)
+    assert L.omv_batch_job_count(b, C.byref(n)) == 0 and n.value == 0
+    assert L.omv_batch_run(b) == -3                                           # no jobs
+    assert L.omv_batch_autotune(b, 1, None) == -3
+    t = pkg.BatchTuning(0, 0, 1)
+    assert L.omv_batch_set_tuning(b, C.byref(t)) == -2                        # tile_waves < 1
+    t = pkg.BatchTuning(2, 4, 0)
+    assert L.omv_batch_set_tuning(b, C.byref(t)) == 0
+    t2 = pkg.BatchTuning()
+    assert L.omv_batch_get_tuning(b, C.byref(t2)) == 0
+    assert (t2.blocks_per_cu, t2.tile_waves, t2.use_graph) == (2, 4, 0)
+    out = np.zeros(4, np.float32)
+    assert L.omv_batch_get_film(b, 0, out.ctypes.data_as(C.POINTER(C.c_float)), 4) == -2
+    for f, h in ((L.omv_batch_destroy, b), (L.omv_scene_destroy, s), (L.omv_camera_destroy, c)):
+        assert f(h) == 0
+
+
+def test_batched_lightbox_workload(wl):
+    """256 jobs = 32 assets x 8 lightbox views; ranges as specified; the
+    viewpoints follow main.cpp:434-457 (camera at 2p - C looking at C)."""
+    assets, jobs = wl.batched_lightbox()
+    assert len(assets) == 32 and len(jobs) == 256
+    tris = [len(a.meshes[0]["faces"]) for a in assets]
+    assert min(tris) >= 800 and max(tris) <= 240000
+    for a, view, p in jobs:
+        assert 128 <= view.width == view.height <= 512
+        assert 8 <= p["samples_per_pixel"] <= 64 and 4 <= p["max_path_length"] <= 8
+    assert len({p["seed"] for _, _, p in jobs}) == 256
+    v = assets[0].meshes[0]["vertices"]
+    lo, hi = v.min(0), v.max(0)
+    c = (lo + hi) * np.float32(0.5)
+    pos = np.array([j[1].position for j in jobs if j[0] == 0], np.float32)
+    corners = (pos + c) * np.float32(0.5)
+    assert np.allclose(np.sort(np.abs(corners - c), axis=0), np.sort(np.abs(np.array([hi - c] * 8)), axis=0), rtol=1e-5)
+    kinds = {a.materials[0][0] for a in assets}
+    assert kinds == {"om_nk", "pbr"}

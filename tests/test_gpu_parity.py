@@ -136,3 +136,67 @@ def test_bdpt_icosphere_om_rough_extension(pkg, wl, ref):
     img, rimg, st, rst = _render_pair(pkg, ref, desc, view, **params)
     assert img[..., :3].max() > 0.0
     _radiance_close(img, rimg)
+
+
+# ---------------------------------------------------------------- full sizes
+def _oracle_rows(ref, desc, cam_desc, params, row0, rows):
+    rs = ref.Scene()
+    desc.apply(rs, ref.pbr_desc)
+    img, st = rs.render(ref.default_params(**params), cam_desc, row0=row0, rows=rows, threads=16)
+    return img[row0:row0 + rows], st
+
+
+def test_full_size_hdri_torus_knot(pkg, wl, ref):
+    """The headline workload at its full size (512x512, 64 spp, depth 8): the
+    whole frame on the GPU, a band of rows checked against the oracle, and
+    size-independent properties over the frame."""
+    desc, view, params = wl.hdri_torus_knot()
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = pkg.Camera(view.width, view.height, view.position, view.forward, view.up, view.yfov)
+    r = pkg.Renderer(pkg.default_params(**params))
+    r.set_scene(s)
+    img = r.render(cam)
+    st = r.stats()
+    assert st["samples"] == 512 * 512 * 64 and st["stack_overflows"] == 0
+    assert np.isfinite(img).all() and (img[..., :3] >= 0).all() and np.all(img[..., 3] == 1.0)
+    assert st["camera_rays"] >= st["samples"]
+    band, rst = _oracle_rows(ref, desc, cam.desc(), params, 248, 16)
+    _radiance_close(img[248:264], band, frac_min=0.98)
+
+
+def test_full_size_large_instanced_ids(pkg, wl, ref):
+    """2M-triangle, 200-instance scene at 1920x1080: primitive, mesh,
+    material and instance ids and tie flags bit-exact over the whole frame."""
+    desc, view, params = wl.large_instanced()
+    desc.fc_subject = "pid"
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = pkg.Camera(view.width, view.height, view.position, view.forward, view.up, view.yfov)
+    r = pkg.Renderer(pkg.default_params(seed=9))
+    r.set_scene(s)
+    film, ids, tie = r.render_fc(cam)
+    rs = ref.Scene()
+    desc.apply(rs, ref.pbr_desc)
+    rfilm, rids, rtie = rs.render_fc(ref.default_params(seed=9), cam.desc())
+    assert (ids[0] >= 0).mean() > 0.2
+    assert np.array_equal(ids, rids)
+    assert np.array_equal(tie, rtie)
+    assert np.array_equal(film, rfilm)
+
+
+def test_full_size_large_instanced_radiance(pkg, wl, ref):
+    """Same scene, full BDPT (4 point lights + HDRI, 16 spp, depth 8): the
+    frame on the GPU, a band of rows against the oracle."""
+    desc, view, params = wl.large_instanced()
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = pkg.Camera(view.width, view.height, view.position, view.forward, view.up, view.yfov)
+    r = pkg.Renderer(pkg.default_params(**params))
+    r.set_scene(s)
+    img = r.render(cam)
+    st = r.stats()
+    assert st["stack_overflows"] == 0 and st["shadow_rays"] > 0
+    assert np.isfinite(img).all() and (img[..., :3] >= 0).all()
+    band, _ = _oracle_rows(ref, desc, cam.desc(), params, 536, 4)
+    _radiance_close(img[536:540], band, frac_min=0.98)
