@@ -12,8 +12,13 @@ Prints one JSON line.  value = paths/s over the timed steps (inputs resident
 in HBM); roofline = the path kernel's algorithmic bytes (rays x B_ray from the
 oracle's canonical binary SAH BVH over the same rays) per launch / its mean
 HIP-event duration, against 8 TB/s; cpu_baseline = the CPU oracle (float,
-reference estimator, same Philox streams) on a band of rows of the same
-workload, all threads it was given.
+reference estimator, same Philox streams) on whole-image passes of the same
+workload, all threads it was given (at most 16).
+
+--workload batched_lightbox prints the batched form's line instead (jobs/s of
+256 lightbox jobs in one launch, beside sequential plain launches).
+--copy-probe GIB only runs a plain copy, the calibration kernel for the
+FETCH_SIZE / WRITE_SIZE counters behind roofline.traffic.
 """
 from __future__ import annotations
 
@@ -32,14 +37,15 @@ B_RAY_IN, B_RAY_OUT, B_NODE, B_TRI = 32, 16, 64, 48
 
 
 def cpu_baseline(ref, desc, view, params, cam_desc, budget_s, threads):
-    """Times the oracle on whole rows of the same workload, growing the band
-    until about budget_s of CPU work; returns paths/s and the per-ray counts."""
+    """Times the oracle on bands of whole rows of the same workload, top to
+    bottom and then again with the next seed, until about budget_s of CPU
+    work; returns paths/s and the per-ray counts of the canonical binary BVH."""
     rs = ref.Scene()
     desc.apply(rs, ref.pbr_desc)
-    p = ref.default_params(**params)
-    rows, row0 = 1, view.height // 2
-    tot_t, tot_samples, tot_rays, tot_nodes, tot_tris = 0.0, 0, 0, 0, 0
+    rows, row0, seed = 1, 0, params.get("seed", 0)
+    tot_t, tot_samples, tot_rays, tot_nodes, tot_tris, tot_rows = 0.0, 0, 0, 0, 0, 0
     while True:
+        p = ref.default_params(**dict(params, seed=seed))
         t0 = time.perf_counter()
         _, st = rs.render(p, cam_desc, row0=row0, rows=rows, threads=threads)
         dt = time.perf_counter() - t0
@@ -48,18 +54,115 @@ def cpu_baseline(ref, desc, view, params, cam_desc, budget_s, threads):
         tot_rays += st["rays"]
         tot_nodes += st["nodes_visited"]
         tot_tris += st["tris_tested"]
+        tot_rows += rows
         row0 += rows
-        if tot_t >= budget_s or row0 >= view.height:
+        if row0 >= view.height:
+            row0, seed = 0, seed + 1
+        if tot_t >= budget_s:
             break
-        rows = max(1, min(view.height - row0, int(rows * max(1.5, (budget_s - tot_t) / max(dt, 1e-3)))))
+        want = int(rows * max(1.5, (budget_s - tot_t) / max(dt, 1e-3)))
+        rows = max(1, min(view.height - row0, want))
     return {
         "paths_per_s": tot_samples / tot_t,
         "seconds": tot_t,
-        "rows": row0 - view.height // 2,
+        "rows": tot_rows,
         "samples": tot_samples,
         "rays_per_path": tot_rays / max(tot_samples, 1),
         "nodes_per_ray": tot_nodes / max(tot_rays, 1),
         "tris_per_ray": tot_tris / max(tot_rays, 1),
+    }
+
+
+def copy_probe(gib):
+    """A plain device copy of known size, the calibration kernel for the
+    FETCH_SIZE / WRITE_SIZE counters (each launch reads and writes gib GiB)."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_per_launch": n * 4, "launches": 3}), flush=True)
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of the path kernel from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy),
+    or None when there is none for this workload."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"bdpt_kernel_{workload}_pmc.txt")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def scene_of(pkg, desc):
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    return s
+
+
+def bench_batched(pkg, wl, args):
+    """Batched validation: 256 lightbox jobs (32 assets x 8 views) in one
+    persistent graph-replayed launch, autotuned; against the same jobs as
+    sequential plain renders.  value = jobs/s of the batch (films read back)."""
+    assets, jobs = wl.batched_lightbox()
+    t0 = time.perf_counter()
+    scenes = [scene_of(pkg, d) for d in assets]
+    commit_s = time.perf_counter() - t0
+    cams = [pkg.Camera(v.width, v.height, v.position, v.forward, v.up, v.yfov) for _, v, _ in jobs]
+    prm = [pkg.default_params(**p) for _, _, p in jobs]
+    b = pkg.Batch()
+    for (a, _, _), cam, p in zip(jobs, cams, prm):
+        b.add_job(scenes[a], cam, p)
+    print(f"# {len(jobs)} jobs, {len(assets)} assets, commit {commit_s:.1f}s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    best = b.autotune(reps=1)
+    print(f"# autotune {best} in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        b.run()
+    pkg.device_synchronize()
+    t0 = time.perf_counter()
+    kms, rays, samples = [], 0, 0
+    for i in range(args.steps):
+        b.run()
+        for j in range(len(jobs)):
+            b.film(j)
+        st = b.stats()
+        kms.append(st["kernel_ms"])
+        rays += st["rays"]
+        samples += st["samples"]
+        print(f"# batch step {i}: {st['kernel_ms']:.1f} ms kernel", file=sys.stderr, flush=True)
+    pkg.device_synchronize()
+    el = time.perf_counter() - t0
+    # sequential plain launches of the same jobs (one renderer per job, film read back)
+    rends = []
+    for (a, _, _), p in zip(jobs, prm):
+        r = pkg.Renderer(p)
+        r.set_scene(scenes[a])
+        rends.append(r)
+    rends[0].render(cams[0])
+    pkg.device_synchronize()
+    t0 = time.perf_counter()
+    for r, cam in zip(rends, cams):
+        r.render(cam)
+    pkg.device_synchronize()
+    seq = time.perf_counter() - t0
+    print(f"# sequential {len(jobs)} renders in {seq:.2f}s", file=sys.stderr, flush=True)
+    jobs_s = len(jobs) * args.steps / el
+    return {
+        "metric": "batched validation jobs/s on 1 GPU (vs sequential plain launches)", "value": round(jobs_s, 3),
+        "unit": "jobs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched_lightbox", "jobs": len(jobs), "assets": len(assets), "tris": "1k-200k log-uniform",
+                   "resolution": "128^2-512^2", "spp": "8-64", "depth": "4-8", "tuning": best},
+        "sequential_jobs_per_s": round(len(jobs) / seq, 3), "batch_over_sequential": round(jobs_s / (len(jobs) / seq), 3),
+        "paths_per_s": round(samples / el, 1), "mrays_per_s": round(rays / el / 1e6, 1),
+        "kernel_ms": round(float(np.mean(kms)), 3),
     }
 
 
@@ -70,30 +173,42 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="hdri_torus_knot", choices=["hdri_torus_knot", "plain_bdpt_icosphere", "large_instanced"])
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
+                    help="only run a plain device copy of GIB GiB (counter calibration)")
+    ap.add_argument("--workload", default="hdri_torus_knot",
+                    choices=["hdri_torus_knot", "plain_bdpt_icosphere", "plain_bdpt_icosphere_om_rough", "large_instanced",
+                             "batched_lightbox"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (no multi-GPU path in this build)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
 
     pkg = ge.load_package()
     wl = ge.load_workloads()
+    if args.workload == "batched_lightbox":
+        print(json.dumps(bench_batched(pkg, wl, args)), flush=True)
+        return
     if args.workload == "hdri_torus_knot":
         desc, view, params = wl.hdri_torus_knot()
         cfg = {"workload": "hdri_torus_knot", "triangles": 100000, "resolution": [512, 512], "spp": 64, "depth": 8,
                "textures": "2 x 1024^2 maps (RGBA8)", "hdri": [2048, 1024], "estimator": "reference"}
-    elif args.workload == "plain_bdpt_icosphere":
+    elif args.workload.startswith("plain_bdpt_icosphere"):
         desc, view, params = wl.plain_bdpt_icosphere()
-        cfg = {"workload": "plain_bdpt_icosphere", "triangles": 5120, "resolution": [256, 256], "spp": 16, "depth": 5,
-               "estimator": "reference"}
+        rough = args.workload.endswith("om_rough")
+        if rough:
+            params = dict(params, extensions=1, om_roughness=0.3)
+        cfg = {"workload": args.workload, "triangles": 5120, "resolution": [256, 256], "spp": 16, "depth": 5,
+               "estimator": "OM rough conductor extension (r=0.3)" if rough else "reference"}
     else:
         desc, view, params = wl.large_instanced()
         cfg = {"workload": "large_instanced", "triangles": desc.triangles(), "resolution": [1920, 1080], "spp": 16, "depth": 8,
                "estimator": "reference"}
 
     t0 = time.perf_counter()
-    scene = pkg.Scene()
-    desc.apply(scene, pkg.pbr_desc)
+    scene = scene_of(pkg, desc)
     commit_s = time.perf_counter() - t0
     cam = pkg.Camera(view.width, view.height, view.position, view.forward, view.up, view.yfov)
     r = pkg.Renderer(pkg.default_params(**params))
@@ -104,7 +219,7 @@ def main():
         r.render(cam)
         print(f"# warmup {i} done", file=sys.stderr, flush=True)
     pkg.device_synchronize()
-    kernel_ms, rays, samples = [], 0, 0
+    kernel_ms, rays, samples, nodes, tris = [], 0, 0, 0, 0
     t0 = time.perf_counter()
     for i in range(args.steps):
         r.render(cam)
@@ -112,9 +227,12 @@ def main():
         kernel_ms.append(st["kernel_ms"])
         rays += st["rays"]
         samples += st["samples"]
+        nodes += st["nodes_visited"]
+        tris += st["tris_tested"]
     pkg.device_synchronize()
     elapsed = time.perf_counter() - t0
-    print(f"# timed {args.steps} steps in {elapsed:.3f}s", file=sys.stderr, flush=True)
+    print(f"# timed {args.steps} steps in {elapsed:.3f}s; kernel ms {[round(k, 3) for k in kernel_ms]}; "
+          f"device BVH4: {nodes / max(rays, 1):.2f} nodes/ray, {tris / max(rays, 1):.2f} tris/ray", file=sys.stderr, flush=True)
     paths_per_s = samples / elapsed
     mrays = rays / elapsed / 1e6
     rays_per_launch = rays / max(args.steps, 1)
@@ -122,9 +240,9 @@ def main():
 
     cpu = None
     b_ray = None
+    threads = min(16, os.cpu_count() or 1)
     if not args.no_cpu:
         ref = ge.load_oracle()
-        threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(ref, desc, view, params, cam.desc(), args.cpu_seconds, threads)
         b_ray = B_RAY_IN + B_RAY_OUT + B_NODE * cpu["nodes_per_ray"] + B_TRI * cpu["tris_per_ray"]
         print(f"# cpu baseline {cpu}", file=sys.stderr, flush=True)
@@ -132,8 +250,9 @@ def main():
     roofline = None
     if b_ray is not None:
         achieved = rays_per_launch * b_ray / (k_ms * 1e-3) / 1e9
+        traffic = measured_traffic(args.workload)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "b_ray": round(b_ray, 1), "rays_per_launch": int(rays_per_launch), "kernel_ms": round(k_ms, 3)}
     line = {
         "metric": "BDPT paths/s (Mrays/s beside it) on 1 GPU", "value": round(paths_per_s, 1), "unit": "paths/s",
@@ -142,9 +261,9 @@ def main():
         "config": cfg, "mrays_per_s": round(mrays, 1), "rays_per_path": round(rays / max(samples, 1), 3),
         "roofline": roofline,
         "cpu_baseline": None if cpu is None else {
-            "value": round(cpu["paths_per_s"], 1), "unit": "paths/s", "cores": min(16, os.cpu_count() or 1), "kind": "port",
-            "sample": f"{cpu['rows']} rows x {view.width} px x {params['samples_per_pixel']} spp ({cpu['samples']} paths) of the same "
-                      f"workload in {cpu['seconds']:.1f}s, float oracle, reference estimator"},
+            "value": round(cpu["paths_per_s"], 1), "unit": "paths/s", "cores": threads, "kind": "port",
+            "sample": f"{cpu['rows']} rows x {view.width} px x {params['samples_per_pixel']} spp ({cpu['samples']} paths, whole-image "
+                      f"passes with successive seeds) of the same workload in {cpu['seconds']:.1f}s, float oracle, reference estimator"},
     }
     if cpu is not None:
         line["gpu_over_cpu"] = round(paths_per_s / cpu["paths_per_s"], 1)

@@ -154,6 +154,9 @@ typedef struct omv_render_stats {
     uint64_t stack_overflows;  /* must be 0; nonzero flags a traversal error   */
     double   render_ms;        /* device time of the render call (HIP events)  */
     double   kernel_ms;        /* device time of the path kernel(s) only       */
+    double   trace_ms;         /* wavefront schedule: summed device time of the
+                                  traversal-kernel launches (HIP events)       */
+    uint64_t trace_launches;   /* traversal-kernel launches timed in trace_ms  */
 } omv_render_stats;
 
 /* ------------------------------------------------------------- errors */
@@ -258,6 +261,30 @@ int omv_renderer_render(omv_renderer* r, omv_camera* c);
  * where the closest hit had an exact-t tie with a different triangle. */
 int omv_renderer_render_fc(omv_renderer* r, omv_camera* c, int32_t* ids, uint8_t* tie);
 int omv_renderer_get_stats(const omv_renderer* r, omv_render_stats* out);
+/* How omv_renderer_render schedules the work on the GPU (no reference
+ * counterpart; both give the same film bit for bit):
+ *   OMV_SCHEDULE_WAVEFRONT   (default) ray queues: gen, then per bounce a
+ *                            traversal kernel and a shading kernel
+ *                            (+ connection visibility and sum with lights);
+ *   OMV_SCHEDULE_PER_SAMPLE  one persistent kernel, one lane per sample. */
+#define OMV_SCHEDULE_WAVEFRONT   0
+#define OMV_SCHEDULE_PER_SAMPLE  1
+int omv_renderer_set_schedule(omv_renderer* r, int schedule);
+int omv_renderer_get_schedule(const omv_renderer* r, int* schedule);
+/* Launch tuning of the wavefront schedule (no reference counterpart; the
+ * film does not depend on it). */
+typedef struct omv_render_tuning {
+    int trace_keep;           /* 1..64: persistent traversal, a wave refills its
+                                 idle lanes once fewer than trace_keep are busy;
+                                 0: one pass over the queue, one ray per lane   */
+    int trace_blocks_per_cu;  /* resident traversal workgroups per CU
+                                 (0 = occupancy limit)                          */
+    int chunk_samples;        /* samples per wavefront chunk, rounded down to
+                                 whole pixels (0 = default: 2^22, 2^21 with
+                                 point lights)                                  */
+} omv_render_tuning;
+int omv_renderer_set_tuning(omv_renderer* r, const omv_render_tuning* t);
+int omv_renderer_get_tuning(const omv_renderer* r, omv_render_tuning* t);
 
 /* ------------------------------------------------------ batched jobs */
 /* Many independent validation renders (each one BDPTRenderer::render call of

@@ -27,6 +27,7 @@ EXT_OM_ROUGH = 1
 FC_SUBJECTS = ("pid", "gid", "mid", "rmp", "mmp", "md", "sg", "mn", "in")
 PBR_BASE_COLOR, PBR_METALLIC_ROUGHNESS, PBR_NORMAL, PBR_EMISSIVITY, PBR_OCCLUSION = range(5)
 ALPHA_BLEND, ALPHA_MASK, ALPHA_OPAQUE = range(3)
+SCHEDULE_WAVEFRONT, SCHEDULE_PER_SAMPLE = 0, 1
 
 
 class RendererParams(C.Structure):
@@ -63,6 +64,10 @@ class IorTable(C.Structure):
     ]
 
 
+class RenderTuning(C.Structure):
+    _fields_ = [("trace_keep", C.c_int), ("trace_blocks_per_cu", C.c_int), ("chunk_samples", C.c_int)]
+
+
 class BatchTuning(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_int), ("tile_waves", C.c_int), ("use_graph", C.c_int)]
 
@@ -72,6 +77,7 @@ class RenderStats(C.Structure):
         ("samples", C.c_uint64), ("rays", C.c_uint64), ("camera_rays", C.c_uint64), ("light_rays", C.c_uint64),
         ("shadow_rays", C.c_uint64), ("nodes_visited", C.c_uint64), ("tris_tested", C.c_uint64),
         ("stack_overflows", C.c_uint64), ("render_ms", C.c_double), ("kernel_ms", C.c_double),
+        ("trace_ms", C.c_double), ("trace_launches", C.c_uint64),
     ]
 
 
@@ -255,6 +261,8 @@ def _load():
         "omv_renderer_set_parameters": [vp, C.POINTER(RendererParams)], "omv_renderer_set_scene": [vp, vp],
         "omv_renderer_render": [vp, vp], "omv_renderer_render_fc": [vp, vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)],
         "omv_renderer_get_stats": [vp, C.POINTER(RenderStats)],
+        "omv_renderer_set_schedule": [vp, C.c_int], "omv_renderer_get_schedule": [vp, C.POINTER(C.c_int)],
+        "omv_renderer_set_tuning": [vp, C.POINTER(RenderTuning)], "omv_renderer_get_tuning": [vp, C.POINTER(RenderTuning)],
         "omv_renderer_params_default": [C.POINTER(RendererParams)],
         "omv_batch_create": [C.POINTER(vp)], "omv_batch_destroy": [vp],
         "omv_batch_add_job": [vp, vp, vp, C.POINTER(RendererParams), C.POINTER(C.c_int)],
@@ -399,6 +407,28 @@ class Renderer:
     def set_scene(self, scene: Scene):
         self._scene = scene
         self._check(self.lib.omv_renderer_set_scene(self.h, scene.h))
+
+    def set_schedule(self, schedule: int):
+        """SCHEDULE_WAVEFRONT (default) or SCHEDULE_PER_SAMPLE; same film either way."""
+        self._check(self.lib.omv_renderer_set_schedule(self.h, schedule))
+
+    def schedule(self) -> int:
+        v = C.c_int()
+        self._check(self.lib.omv_renderer_get_schedule(self.h, C.byref(v)))
+        return v.value
+
+    def set_tuning(self, **kw):
+        """trace_keep, trace_blocks_per_cu, chunk_samples (the film does not depend on them)."""
+        t = RenderTuning()
+        self._check(self.lib.omv_renderer_get_tuning(self.h, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        self._check(self.lib.omv_renderer_set_tuning(self.h, C.byref(t)))
+
+    def tuning(self) -> dict:
+        t = RenderTuning()
+        self._check(self.lib.omv_renderer_get_tuning(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in RenderTuning._fields_}
 
     def render(self, cam: Camera):
         self._check(self.lib.omv_renderer_render(self.h, cam.h))

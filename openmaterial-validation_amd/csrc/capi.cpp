@@ -205,6 +205,24 @@ int omv_renderer_get_stats(const omv_renderer* r, omv_render_stats* out) {
     *out = r->r.stats();
     return OMV_OK;
 }
+int omv_renderer_set_schedule(omv_renderer* r, int schedule) {
+    OMV_NULLCHK(r);
+    return r->r.setSchedule(schedule);
+}
+int omv_renderer_get_schedule(const omv_renderer* r, int* schedule) {
+    OMV_NULLCHK(r); OMV_NULLCHK(schedule);
+    *schedule = r->r.schedule();
+    return OMV_OK;
+}
+int omv_renderer_set_tuning(omv_renderer* r, const omv_render_tuning* t) {
+    OMV_NULLCHK(r); OMV_NULLCHK(t);
+    return r->r.setTuning(*t);
+}
+int omv_renderer_get_tuning(const omv_renderer* r, omv_render_tuning* t) {
+    OMV_NULLCHK(r); OMV_NULLCHK(t);
+    *t = r->r.tuning();
+    return OMV_OK;
+}
 
 int omv_batch_create(omv_batch** out) {
     OMV_NULLCHK(out);

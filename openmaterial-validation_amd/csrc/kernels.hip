@@ -15,964 +15,12 @@
 //
 // Samples are written per sample and reduced per pixel in sample order by a
 // second kernel, which is the reference's film summation order
-// (PinholeCamera::hit, PinholeCamera.cpp:97-108).
-//
-// Arithmetic follows the reference's operation order with FMA contraction
-// off (the build passes -ffp-contract=off) and correctly rounded division
-// and square root, so the ray/hit/id path is bit-identical to the CPU oracle.
-#include <hip/hip_runtime.h>
+// (PinholeCamera::hit, PinholeCamera.cpp:97-108).  The same per-sample
+// function drives the batched multi-job kernel, and the false-colour pass
+// lives here too.  The device building blocks are in path_device.h.
 #include <cstring>
-#include <float.h>
-#include <stdint.h>
 
-#include "../../include/omv.h"
-#include "omv_types.h"
-#include "omv_kernels.h"
-
-#pragma clang fp contract(off)
-
-#define DEV __device__ __forceinline__
-
-namespace {
-
-// ------------------------------------------------------------------ math
-struct f3 { float x, y, z; };
-struct f4 { float x, y, z, w; };
-DEV f3 mk3(float a, float b, float c) { f3 r; r.x = a; r.y = b; r.z = c; return r; }
-DEV float comp(const f3& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
-// UtilityFunctions.h:128-247, same operation order
-DEV float dot3(const f3& a, const f3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-DEV f3 cross3(const f3& a, const f3& b) { return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
-DEV f3 sub3(const f3& a, const f3& b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
-DEV f3 neg3(const f3& a) { return mk3(-a.x, -a.y, -a.z); }
-DEV float length3(const f3& a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
-DEV f3 normalize3(const f3& a) {
-    float L = length3(a);
-    if (L != 0.0f) { L = 1.0f / L; return mk3(a.x * L, a.y * L, a.z * L); }
-    return mk3(1.0f, 0.0f, 0.0f);
-}
-DEV f3 normalizeIfNeeded3(const f3& a) {
-    const float eps = 10.0f * FLT_EPSILON;
-    float sL = dot3(a, a);
-    if (sL < 1.0f + eps && sL > 1.0f - eps) return a;
-    float L = sqrtf(sL);
-    if (L != 0.0f) { L = 1.0f / L; return mk3(a.x * L, a.y * L, a.z * L); }
-    return mk3(1.0f, 0.0f, 0.0f);
-}
-DEV float mnf(float a, float b) { return a < b ? a : b; }
-DEV float mxf(float a, float b) { return a > b ? a : b; }
-DEV float clampr(float v, float lo, float hi) { return mxf(lo, mnf(v, hi)); }  // reference clamp()
-DEV float clamp01(float v) { return clampr(v, 0.0f, 1.0f); }
-DEV float lerpr(float lo, float hi, float t) { return (hi - lo) * t + lo; }    // reference lerp()
-DEV float stdmax(float a, float b) { return (a < b) ? b : a; }                 // std::max
-DEV f3 reflect3(const f3& I, const f3& N) {
-    const float w = 2.0f * dot3(I, N);
-    return mk3(I.x - w * N.x, I.y - w * N.y, I.z - w * N.z);
-}
-DEV bool refract3(const f3& I, const f3& N, float eta, f3& out) {
-    float NI = dot3(N, I);
-    float k = 1.0f - eta * eta * (1.0f - NI * NI);
-    if (k < 0.0f) { out = neg3(I); return false; }
-    k = eta * NI + sqrtf(k);
-    out = mk3(eta * I.x - k * N.x, eta * I.y - k * N.y, eta * I.z - k * N.z);
-    return true;
-}
-DEV f3 mul33(const float* A, const f3& v) {
-    return mk3(A[0] * v.x + A[1] * v.y + A[2] * v.z, A[3] * v.x + A[4] * v.y + A[5] * v.z, A[6] * v.x + A[7] * v.y + A[8] * v.z);
-}
-DEV f3 mul34(const float* A, const f3& v) {
-    return mk3(A[0] * v.x + A[1] * v.y + A[2] * v.z + A[3], A[4] * v.x + A[5] * v.y + A[6] * v.z + A[7],
-               A[8] * v.x + A[9] * v.y + A[10] * v.z + A[11]);
-}
-DEV f3 mulT33(const f3& v, const float* A) {
-    return mk3(A[0] * v.x + A[3] * v.y + A[6] * v.z, A[1] * v.x + A[4] * v.y + A[7] * v.z, A[2] * v.x + A[5] * v.y + A[8] * v.z);
-}
-DEV f3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
-DEV float intensity(const f3& r) { return 0.299f * r.x + 0.587f * r.y + 0.114f * r.z; }  // BDPTRenderer.cpp:52-55
-DEV float maxAbs(const f3& v) {                                                          // BDPTRenderer.cpp:35-40
-    float a = fabsf(v.x), b = fabsf(v.y), c = fabsf(v.z);
-    float m = (a > b && a > c) ? a : (b > c ? b : c);
-    return m < 1.0f ? 1.0f : m;
-}
-constexpr float kPi = 3.14159265358979323846f;
-
-// ------------------------------------------------------------------- RNG
-// Philox4x32-10 (Salmon et al., SC'11).  Key (seed, 0); counter
-// (call, sub-stream, sample lo, sample hi); 4 uniforms per call.
-struct Rng {
-    uint32_t k0, c0, c1, c2, c3;
-    uint32_t b0, b1, b2, b3;
-    int idx;
-    DEV void init(uint32_t seed, uint32_t stream, uint64_t sample) {
-        k0 = seed; c0 = 0; c1 = stream; c2 = (uint32_t)sample; c3 = (uint32_t)(sample >> 32); idx = 4;
-        b0 = b1 = b2 = b3 = 0;
-    }
-    DEV void refill() {
-        uint32_t x0 = c0, x1 = c1, x2 = c2, x3 = c3, ka = k0, kb = 0;
-#pragma unroll
-        for (int r = 0; r < 10; ++r) {
-            if (r) { ka += 0x9E3779B9u; kb += 0xBB67AE85u; }
-            uint32_t lo0 = 0xD2511F53u * x0, hi0 = __umulhi(0xD2511F53u, x0);
-            uint32_t lo1 = 0xCD9E8D57u * x2, hi1 = __umulhi(0xCD9E8D57u, x2);
-            uint32_t n0 = hi1 ^ x1 ^ ka, n2 = hi0 ^ x3 ^ kb;
-            x0 = n0; x1 = lo1; x2 = n2; x3 = lo0;
-        }
-        b0 = x0; b1 = x1; b2 = x2; b3 = x3;
-        ++c0; idx = 0;
-    }
-    DEV float next() {
-        if (idx == 4) refill();
-        uint32_t v = idx == 0 ? b0 : (idx == 1 ? b1 : (idx == 2 ? b2 : b3));
-        ++idx;
-        return (float)(v >> 8) * (1.0f / 16777216.0f);
-    }
-};
-enum { STREAM_CAMERA = 0, STREAM_LIGHT = 1 };
-
-// ----------------------------------------------------------------- warps
-// RandomSampler.h:66-103
-DEV f3 uniformHemisphere(Rng& r) {
-    float phi = lerpr(0.0f, 2.0f * kPi, r.next());
-    float cth = r.next();
-    float sth = sqrtf(stdmax(1.0f - cth * cth, 0.0f));
-    return mk3(sth * cosf(phi), sth * sinf(phi), cth);
-}
-DEV f3 uniformSphere(Rng& r) {   // positive octant only, bounded at 256 rounds
-    f3 v; float sl = 0.0f; int tries = 0;
-    do {
-        if (++tries > 256) { v = mk3(1.0f, 1.0f, 1.0f); sl = 3.0f; break; }
-        v.x = r.next(); v.y = r.next(); v.z = r.next();
-        sl = dot3(v, v);
-    } while (sl > 1.0f || sl < FLT_EPSILON);
-    float s = 1.0f / sqrtf(sl);
-    return mk3(s * v.x, s * v.y, s * v.z);
-}
-DEV f3 ggxHemisphere(Rng& r, float alpha2) {
-    float phi = lerpr(0.0f, 2.0f * kPi, r.next());
-    float rnd = r.next();
-    float cth = sqrtf((1.0f - rnd) / (1.0f - (1.0f - alpha2) * rnd));
-    float sth = sqrtf(stdmax(1.0f - cth * cth, 0.0f));
-    return mk3(sth * cosf(phi), sth * sinf(phi), cth);
-}
-DEV f3 cosineSqrtHemisphere(Rng& r) {
-    float phi = lerpr(0.0f, 2.0f * kPi, r.next());
-    float rnd = r.next();
-    float sth = sqrtf(rnd);
-    return mk3(sth * cosf(phi), sth * sinf(phi), sqrtf(1.0f - rnd));
-}
-// transformToNormalSpace (BRDFModel.h:25-34)
-DEV f3 toBasis(const f3& b0, const f3& b1, const f3& b2, const f3& c) {
-    f3 r = mk3(b0.x * c.x + 0.0f, b0.y * c.x + 0.0f, b0.z * c.x + 0.0f);
-    r = mk3(b1.x * c.y + r.x, b1.y * c.y + r.y, b1.z * c.y + r.z);
-    return mk3(b2.x * c.z + r.x, b2.y * c.z + r.y, b2.z * c.z + r.z);
-}
-
-// ---------------------------------------------------------------- BSDF
-// BRDFModel.h:40-80
-DEV float ggxPdf(float c, float alpha) {
-    float c2 = c * c, a2 = alpha * alpha;
-    return a2 / (kPi * powf(lerpr(1.0f, a2, c2), 2.0f));
-}
-DEV float ggxLambda(float a) { return 0.5f * (sqrtf(1.0f + powf(a, -2.0f)) - 1.0f); }
-DEV float smithG1(float c, float alpha) {
-    float a = c / (alpha * sqrtf(1.0f - c * c));
-    return (alpha > 0.0f || c < 0.99f) ? 1.0f / (1.0f + ggxLambda(a)) : 1.0f;
-}
-DEV float smithG2(float s, float m) { float p = s * m; return p / (s + m - p); }
-
-struct Bsdf {
-    f3 I, O, N, H, T;
-    float c0, c1, c2, c3;   // base colour rgba
-    bool transmissive;
-    float metal, rough, alpha, eta;
-};
-DEV void bsdfInit(Bsdf& b) {
-    b.I = b.O = b.N = b.H = b.T = mk3(0.0f, 0.0f, 0.0f);
-    b.c0 = b.c1 = b.c2 = b.c3 = 1.0f; b.transmissive = false;
-    b.metal = 0.5f; b.rough = 0.0f; b.alpha = 0.25f; b.eta = 1.0f;
-}
-// BSDFModel::evaluate (BRDFModel.h:139-207)
-DEV f3 bsdfEvaluate(const f3& vIn, const f3& vOut, const f3& N, const Bsdf& b, bool specular, bool transmitted, const f3& Hin, float P) {
-    f3 H = transmitted ? Hin : normalize3(sub3(vOut, vIn));
-    float VdotN = clampr(fabsf(dot3(vIn, N)), 0.0001f, 1.0f);
-    float LdotN = clampr(fabsf(dot3(vOut, N)), 0.0001f, 1.0f);
-    float NdotH = fabsf(dot3(N, H));
-    float VdotH = fabsf(dot3(vIn, H));
-    float LdotH = fabsf(dot3(vOut, H));
-    float p5 = powf(clamp01(1.0f - VdotH), 5.0f);
-    float col[3] = {b.c0, b.c1, b.c2};
-    float F[3], fd[3], fs[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        float cd = (1.0f - b.metal) * col[i];
-        float F0 = lerpr(0.04f, col[i], b.metal);
-        F[i] = lerpr(p5, 1.0f, F0);
-        fd[i] = (1.0f - F[i]) * cd / kPi;
-    }
-    float G = smithG2(smithG1(VdotN, b.alpha), smithG1(LdotN, b.alpha));
-#pragma unroll
-    for (int i = 0; i < 3; ++i) fs[i] = clampr(0.25f * (transmitted ? 1.0f - F[i] : F[i]) * G / (VdotN * LdotN), 0.0f, FLT_MAX);
-    if (specular) { float k = 4.0f * LdotH * VdotN / NdotH / LdotN / (1.0f - P); return mk3(k * fs[0], k * fs[1], k * fs[2]); }
-    if (transmitted) { float k = 4.0f * LdotH * VdotN / NdotH / LdotN / P; return mk3(k * fs[0], k * fs[1], k * fs[2]); }
-    float k = kPi / LdotN / P;
-    return mk3(k * fd[0], k * fd[1], k * fd[2]);
-}
-// BSDFModel::evaluateDirect (BRDFModel.h:211-261)
-DEV f3 evaluateDirect(const Bsdf& d) {
-    float VdotN = -dot3(d.I, d.N), LdotN = dot3(d.O, d.N);
-    f3 H = normalize3(sub3(d.O, d.I));
-    float VdotH = fabsf(dot3(d.I, H)), NdotH = fabsf(dot3(d.N, H));
-    if (!(VdotN > 0.0f && LdotN > 0.0f)) return mk3(0.0f, 0.0f, 0.0f);
-    float p5 = powf(clamp01(1.0f - VdotH), 5.0f);
-    float col[3] = {d.c0, d.c1, d.c2};
-    float D = ggxPdf(clamp01(NdotH), d.alpha);
-    float G = smithG2(smithG1(VdotN, d.alpha), smithG1(LdotN, d.alpha));
-    float out[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        float F0 = lerpr(0.04f, col[i], d.metal);
-        float F = lerpr(p5, 1.0f, F0);
-        float cd = (1.0f - d.metal) * col[i];
-        float fdv = (1.0f - F) * cd / kPi;
-        float fsv = clampr(0.25f * F * D * G / (VdotN * LdotN), 0.0f, FLT_MAX);
-        out[i] = fdv + fsv;
-    }
-    return mk3(out[0], out[1], out[2]);
-}
-// BSDFModel::sampleIndirect (BRDFModel.h:265-337)
-DEV f3 sampleIndirect(Bsdf& d, Rng& r) {
-    float VdotN = -dot3(d.I, d.N);
-    f3 binorm = cross3(d.N, d.T);
-    f3 b0 = d.T, b1 = binorm;
-    if (VdotN < 0.0f) {
-        if (d.transmissive) { f3 t = b0; b0 = b1; b1 = t; d.N = neg3(d.N); }
-        else return mk3(0.0f, 0.0f, 0.0f);
-    }
-    const float P = 0.8f * fabsf(VdotN) + 0.1f;
-    float rnd = r.next();
-    if (rnd < P) {
-        if (rnd < P * (1.0f - d.c3)) {
-            if (VdotN < 0.0f) d.eta = 1.0f / d.eta;
-            f3 rd = ggxHemisphere(r, powf(d.alpha, 2.0f));
-            d.H = normalize3(toBasis(b0, b1, d.N, rd));
-            if (refract3(d.I, d.H, d.eta, d.O) && dot3(d.N, d.O) < 0.0f) {
-                d.O = normalize3(d.O);
-                return bsdfEvaluate(d.I, d.O, d.N, d, false, true, d.H, P);
-            }
-            return mk3(0.0f, 0.0f, 0.0f);
-        }
-        f3 rd = cosineSqrtHemisphere(r);
-        d.O = normalize3(toBasis(b0, b1, d.N, rd));
-        return bsdfEvaluate(d.I, d.O, d.N, d, false, false, d.H, P);
-    }
-    f3 rd = ggxHemisphere(r, powf(d.alpha, 2.0f));
-    d.H = normalize3(toBasis(b0, b1, d.N, rd));
-    d.O = reflect3(d.I, d.H);
-    if (dot3(d.N, d.O) > 0.0f) {
-        d.O = normalize3(d.O);
-        return bsdfEvaluate(d.I, d.O, d.N, d, true, false, d.H, P);
-    }
-    return mk3(0.0f, 0.0f, 0.0f);
-}
-
-// ---------------------------------------------------------- complex IOR
-// fresnelReflection (OpenMaterial.cpp:20-38): unpolarised reflectance of
-// vacuum -> N = n + ik, principal complex square root.
-struct cpx { float re, im; };
-DEV cpx cmul(cpx a, cpx b) { cpx r; r.re = a.re * b.re - a.im * b.im; r.im = a.re * b.im + a.im * b.re; return r; }
-DEV cpx csqrt_(cpx z) {   // principal branch
-    float m = sqrtf(z.re * z.re + z.im * z.im);
-    float re = sqrtf(0.5f * (m + fabsf(z.re)));
-    cpx r;
-    if (re == 0.0f) { r.re = 0.0f; r.im = 0.0f; return r; }
-    if (z.re >= 0.0f) { r.re = re; r.im = z.im / (2.0f * re); }
-    else { r.re = fabsf(z.im) / (2.0f * re); r.im = z.im < 0.0f ? -re : re; }
-    return r;
-}
-DEV cpx cdiv(cpx a, cpx b) {   // Smith's algorithm
-    cpx r;
-    if (fabsf(b.re) >= fabsf(b.im)) {
-        float q = b.im / b.re, den = b.re + b.im * q;
-        r.re = (a.re + a.im * q) / den; r.im = (a.im - a.re * q) / den;
-    } else {
-        float q = b.re / b.im, den = b.im + b.re * q;
-        r.re = (a.re * q + a.im) / den; r.im = (a.im * q - a.re) / den;
-    }
-    return r;
-}
-DEV void fresnel(float n, float k, float cosT, float& Rp, float& Rs) {
-    cpx N; N.re = n; N.im = k;
-    cpx N2 = cmul(N, N);
-    cosT = fabsf(cosT);
-    const float c2 = cosT * cosT, s2 = 1.0f - c2;
-    cpx z; z.re = N2.re - s2; z.im = N2.im;
-    cpx sq = csqrt_(z);
-    cpx N2c; N2c.re = N2.re * cosT; N2c.im = N2.im * cosT;
-    cpx np, dp, ns, ds;
-    np.re = N2c.re - sq.re; np.im = N2c.im - sq.im;
-    dp.re = N2c.re + sq.re; dp.im = N2c.im + sq.im;
-    ns.re = cosT - sq.re; ns.im = -sq.im;
-    ds.re = cosT + sq.re; ds.im = sq.im;
-    cpx rp = cdiv(np, dp), rs = cdiv(ns, ds);
-    Rp = rp.re * rp.re + rp.im * rp.im;
-    Rs = rs.re * rs.re + rs.im * rs.im;
-}
-
-// ------------------------------------------------------------ textures
-// BitmapTexture::getPixCoords + texture (BitmapTexture.cpp:181-234); fp32
-// textures (an extension) use the same addressing without the 1/255 scale.
-DEV f4 texLookup(const OmvSceneView& S, int id, float u, float v) {
-    const OmvTexture t = S.texs[id];
-    f4 out;
-    if (t.w <= 0 || t.h <= 0) { out.x = out.y = out.z = 0.5f; out.w = 1.0f; return out; }
-    if (!isfinite(u)) u = 0.0f;
-    if (!isfinite(v)) v = 0.0f;
-    const float W = (float)t.w, H = (float)t.h;
-    float upage = floorf(u - 0.5f / W), vpage = floorf(v - 0.5f / H);
-    u -= upage; v -= vpage;
-    u *= W; v *= H;
-    int iu = (int)u, iv = (int)v;
-    float kui = u - (float)iu, kvi = v - (float)iv;
-    float ku = 1.0f - kui, kv = 1.0f - kvi;
-    int l = iu % t.w, r = (iu + 1) % t.w, b = iv % t.h, tt = (iv + 1) % t.h;
-    const int n = t.w * t.h - 1;
-    int i0 = min(max(b * t.w + l, 0), n), i1 = min(max(b * t.w + r, 0), n);
-    int i2 = min(max(tt * t.w + l, 0), n), i3 = min(max(tt * t.w + r, 0), n);
-    float w0 = ku * kv, w1 = kui * kv, w2 = ku * kvi, w3 = kui * kvi;
-    if (t.fmt == OMV_TEX_RGBA8) {
-        const uint32_t* px = (const uint32_t*)(S.texels + t.offset);
-        uint32_t p0 = px[i0], p1 = px[i1], p2 = px[i2], p3 = px[i3];
-        const float s = 1.0f / 255.0f;
-        float o[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            int sh = 8 * c;
-            o[c] = (float)((p0 >> sh) & 0xFF) * w0 * s + (float)((p1 >> sh) & 0xFF) * w1 * s +
-                   (float)((p2 >> sh) & 0xFF) * w2 * s + (float)((p3 >> sh) & 0xFF) * w3 * s;
-        }
-        out.x = o[0]; out.y = o[1]; out.z = o[2]; out.w = o[3];
-    } else {
-        const float4* px = (const float4*)(S.texels + t.offset);
-        float4 p0 = px[i0], p1 = px[i1], p2 = px[i2], p3 = px[i3];
-        out.x = p0.x * w0 + p1.x * w1 + p2.x * w2 + p3.x * w3;
-        out.y = p0.y * w0 + p1.y * w1 + p2.y * w2 + p3.y * w3;
-        out.z = p0.z * w0 + p1.z * w1 + p2.z * w2 + p3.z * w3;
-        out.w = p0.w * w0 + p1.w * w1 + p2.w * w2 + p3.w * w3;
-    }
-    return out;
-}
-
-// -------------------------------------------------------- surface point
-struct SP {
-    int inst, mesh, face;
-    f3 pos, n, t, b, bary;
-    float tw;
-    bool texNorm;
-    float density, gradient;
-};
-
-struct Hit { float t, u, v; int inst, prim; bool tie; };
-
-// BDPTRenderer::computeSurfacePoint + RenderMesh::computeSurfacePoint
-// (BDPTRenderer.cpp:57-73, RenderMesh.cpp:159-234)
-DEV void computeSurfacePoint(const OmvSceneView& S, const Hit& h, SP& sp, bool fcDensity) {
-    const OmvInstance& in = S.insts[h.inst];
-    const OmvMesh m = S.meshes[in.mesh_id];
-    sp.inst = h.inst; sp.mesh = in.mesh_id; sp.face = h.prim;
-    sp.bary = mk3(1.0f - h.u - h.v, h.u, h.v);
-    sp.texNorm = false;
-    const uint32_t* f = S.faces + m.face_offset + 3u * (uint32_t)h.prim;
-    const float* v0 = S.vtx + m.vtx_offset + f[0] * m.stride;
-    const float* v1 = S.vtx + m.vtx_offset + f[1] * m.stride;
-    const float* v2 = S.vtx + m.vtx_offset + f[2] * m.stride;
-    const float b0 = sp.bary.x, b1 = sp.bary.y, b2 = sp.bary.z;
-    f3 pp = mk3(b0 * v0[0] + b1 * v1[0] + b2 * v2[0], b0 * v0[1] + b1 * v1[1] + b2 * v2[1], b0 * v0[2] + b1 * v1[2] + b2 * v2[2]);
-    f3 pn = mk3(b0 * v0[3] + b1 * v1[3] + b2 * v2[3], b0 * v0[4] + b1 * v1[4] + b2 * v2[4], b0 * v0[5] + b1 * v1[5] + b2 * v2[5]);
-    f3 pt = mk3(b0 * v0[6] + b1 * v1[6] + b2 * v2[6], b0 * v0[7] + b1 * v1[7] + b2 * v2[7], b0 * v0[8] + b1 * v1[8] + b2 * v2[8]);
-    float pw = b0 * v0[9] + b1 * v1[9] + b2 * v2[9];
-    pw = pw >= 0.0f ? 1.0f : -1.0f;
-    pn = normalizeIfNeeded3(pn);
-    float tn = dot3(pn, pt);
-    pt = mk3(pt.x - tn * pn.x, pt.y - tn * pn.y, pt.z - tn * pn.z);
-    pt = normalizeIfNeeded3(pt);
-    f3 pbn = cross3(pn, pt);
-    pbn = mk3(pbn.x * pw, pbn.y * pw, pbn.z * pw);
-    sp.pos = mul34(in.xfm, pp);
-    sp.n = normalizeIfNeeded3(mul33(in.nrm, pn));
-    sp.t = normalizeIfNeeded3(mul33(in.nrm, pt));
-    sp.b = normalizeIfNeeded3(mul33(in.nrm, pbn));
-    sp.tw = pw;
-    sp.density = 0.0f; sp.gradient = 0.0f;
-    if (fcDensity) {
-        f3 V0 = mul34(in.xfm, ld3(v0)), V1 = mul34(in.xfm, ld3(v1)), V2 = mul34(in.xfm, ld3(v2));
-        f3 tfn = cross3(sub3(V1, V0), sub3(V2, V0));
-        float dens = 1.0f / (0.0f + length3(tfn) / 2.0f);
-        tfn = normalize3(tfn);
-        float ag = 0.0f;
-        const float* vv[3] = {v0, v1, v2};
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            f3 tvn = normalizeIfNeeded3(mul33(in.nrm, ld3(vv[k] + 3)));
-            ag = ag + sqrtf((1.0f - clampr(dot3(tvn, tfn), -1.0f, 1.0f)) / 2.0f);
-        }
-        sp.gradient = powf(clampr(ag / 3.0f, 1e-6f, 1.0f), 0.286f);
-        const float mnD = S.min_density, mxD = S.max_density;
-        sp.density = clamp01(1.0f - logf(mxD / clampr(dens, mnD, mxD)) / logf(mxD / mnD));
-    }
-}
-
-// hit position only (shadow rays): same arithmetic as computeSurfacePoint
-DEV f3 hitPosition(const OmvSceneView& S, const Hit& h) {
-    const OmvInstance& in = S.insts[h.inst];
-    const OmvMesh m = S.meshes[in.mesh_id];
-    const uint32_t* f = S.faces + m.face_offset + 3u * (uint32_t)h.prim;
-    const float* v0 = S.vtx + m.vtx_offset + f[0] * m.stride;
-    const float* v1 = S.vtx + m.vtx_offset + f[1] * m.stride;
-    const float* v2 = S.vtx + m.vtx_offset + f[2] * m.stride;
-    const float b0 = 1.0f - h.u - h.v, b1 = h.u, b2 = h.v;
-    f3 pp = mk3(b0 * v0[0] + b1 * v1[0] + b2 * v2[0], b0 * v0[1] + b1 * v1[1] + b2 * v2[1], b0 * v0[2] + b1 * v1[2] + b2 * v2[2]);
-    return mul34(in.xfm, pp);
-}
-
-// SurfacePoint::applyTextureNormal (SurfacePoint.cpp:19-34)
-DEV void applyTextureNormal(SP& sp, const f3& tn) {
-    if (sp.texNorm) return;
-    f3 nn = mk3(tn.x * sp.t.x + tn.y * sp.b.x + tn.z * sp.n.x, tn.x * sp.t.y + tn.y * sp.b.y + tn.z * sp.n.y,
-                tn.x * sp.t.z + tn.y * sp.b.z + tn.z * sp.n.z);
-    sp.n = normalizeIfNeeded3(nn);
-    sp.t = normalizeIfNeeded3(cross3(sp.b, sp.n));
-    f3 b = cross3(sp.n, sp.t);
-    sp.b = mk3(sp.tw * b.x, sp.tw * b.y, sp.tw * b.z);
-    sp.texNorm = true;
-}
-
-// RenderMesh::getTexCoord (RenderMesh.cpp:145-157)
-DEV bool getTexCoord(const OmvSceneView& S, int meshId, int face, const f3& bary, int ch, float& u, float& v) {
-    u = 0.0f; v = 0.0f;
-    const OmvMesh& m = S.meshes[meshId];
-    if (face < 0 || face >= (int)m.num_faces || ch < 0 || ch >= OMV_MAX_UV_CHANNELS) return false;
-    const int off = m.tex_off[ch];
-    if (off < 0) return false;
-    const uint32_t* f = S.faces + m.face_offset + 3u * (uint32_t)face;
-    const float* a = S.vtx + m.vtx_offset + f[0] * m.stride + off;
-    const float* b = S.vtx + m.vtx_offset + f[1] * m.stride + off;
-    const float* c = S.vtx + m.vtx_offset + f[2] * m.stride + off;
-    u = a[0] * bary.x + b[0] * bary.y + c[0] * bary.z;
-    v = a[1] * bary.x + b[1] * bary.y + c[1] * bary.z;
-    return true;
-}
-
-// ------------------------------------------------------------ materials
-DEV bool pbrTex(const OmvSceneView& S, const SP& sp, const OmvMaterial& M, int slot, f4& out) {
-    int id = M.tex_map[slot];
-    if (!(0 <= id && id < S.num_texs)) return false;
-    float u, v;
-    if (!getTexCoord(S, sp.mesh, sp.face, sp.bary, M.tex_ch[slot], u, v)) return false;
-    f3 uvT = mul33(M.tex_xf[slot], mk3(u, v, 1.0f));
-    out = texLookup(S, id, uvT.x, uvT.y);
-    return true;
-}
-// PhysicallyBasedMaterial::Helper (PhysicallyBasedMaterial.cpp:32-100)
-DEV void pbrInputs(const OmvSceneView& S, const SP& sp, const OmvMaterial& M, float col[4], float omr[3]) {
-    f4 t;
-    if (pbrTex(S, sp, M, OMV_PBR_BASE_COLOR, t)) { col[0] = t.x; col[1] = t.y; col[2] = t.z; col[3] = t.w; }
-    else { col[0] = col[1] = col[2] = col[3] = 1.0f; }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) col[i] = col[i] * M.bcf[i];
-    omr[0] = 1.0f; omr[1] = M.metal; omr[2] = M.rough;
-    if (pbrTex(S, sp, M, OMV_PBR_METALLIC_ROUGHNESS, t)) { omr[0] *= t.x; omr[1] *= t.z; omr[2] *= t.y; }
-    if (M.occ_separate && pbrTex(S, sp, M, OMV_PBR_OCCLUSION, t)) omr[0] = t.x;
-}
-DEV void pbrSetup(const OmvSceneView& S, const SP& sp, const OmvMaterial& M, Bsdf& b, float omr[3]) {
-    float col[4];
-    pbrInputs(S, sp, M, col, omr);
-    b.c0 = col[0]; b.c1 = col[1]; b.c2 = col[2]; b.c3 = col[3];
-    if (M.alpha_mode != OMV_ALPHA_BLEND) b.c3 = 1.0f;
-    b.transmissive = M.double_sided || b.c3 < 1.0f;
-}
-DEV void pbrFinish(const OmvMaterial& M, Bsdf& b, const float omr[3]) {
-    b.c0 = lerpr(b.c0, omr[0] * b.c0, M.occ_strength);
-    b.c1 = lerpr(b.c1, omr[0] * b.c1, M.occ_strength);
-    b.c2 = lerpr(b.c2, omr[0] * b.c2, M.occ_strength);
-    b.metal = omr[1];
-    b.rough = clampr(omr[2], 0.00001f, 0.99999f);
-    b.alpha = powf(b.rough, 2.0f);
-}
-// PhysicallyBasedMaterial::modifyFrame (:147-167); the other materials
-// apply the identity texture normal (OpenMaterial.cpp:66-70, DiffuseColorMaterial.cpp:35-39)
-DEV void modifyFrame(const OmvSceneView& S, SP& sp) {
-    const OmvMaterial& M = S.mats[S.meshes[sp.mesh].mat_id];
-    f3 tn = mk3(0.0f, 0.0f, 1.0f);
-    if (M.type == OMV_MAT_PBR) {
-        f4 t;
-        if (pbrTex(S, sp, M, OMV_PBR_NORMAL, t)) {
-            if (t.z <= 0.5f) t.z = 0.5f + FLT_EPSILON;
-            float x = 2.0f * t.x + -1.0f, y = 2.0f * t.y + -1.0f, z = 2.0f * t.z + -1.0f;
-            x = M.normal_scale * x; y = M.normal_scale * y;
-            tn = normalize3(mk3(x, y, z));
-        }
-    }
-    applyTextureNormal(sp, tn);
-}
-// emission part of PhysicallyBasedMaterial::getRayAndBrdf (:282-287, :102-123)
-DEV f3 pbrEmission(const OmvSceneView& S, const SP& sp, const OmvMaterial& M) {
-    float u, v;
-    getTexCoord(S, sp.mesh, sp.face, sp.bary, M.tex_ch[OMV_PBR_EMISSIVITY], u, v);
-    f3 uvT = mul33(M.tex_xf[OMV_PBR_EMISSIVITY], mk3(u, v, 1.0f));
-    f4 t = texLookup(S, M.tex_map[OMV_PBR_EMISSIVITY], uvT.x, uvT.y);
-    return mk3(t.x * M.ef[0], t.y * M.ef[1], t.z * M.ef[2]);
-}
-// PhysicallyBasedMaterial::isMasked (:301-316)
-DEV bool isMaskedHit(const OmvSceneView& S, int meshId, int face, const f3& bary) {
-    const OmvMaterial& M = S.mats[S.meshes[meshId].mat_id];
-    if (M.type != OMV_MAT_PBR || M.alpha_mode != OMV_ALPHA_MASK) return false;
-    float alpha = M.bcf[3];
-    if (alpha < M.alpha_cutoff) return true;
-    int id = M.tex_map[OMV_PBR_BASE_COLOR];
-    if (0 <= id && id < S.num_texs) {
-        float u, v, a;
-        if (!getTexCoord(S, meshId, face, bary, M.tex_ch[OMV_PBR_BASE_COLOR], u, v)) a = 1.0f;
-        else a = texLookup(S, id, u, v).w;
-        alpha *= a;
-    }
-    return alpha < M.alpha_cutoff;
-}
-// geometric normal through the normal matrix (OpenMaterial.cpp:76-86)
-DEV f3 faceNormal(const OmvSceneView& S, const SP& sp) {
-    const OmvMesh m = S.meshes[sp.mesh];
-    const uint32_t* f = S.faces + m.face_offset + 3u * (uint32_t)sp.face;
-    f3 P0 = ld3(S.vtx + m.vtx_offset + f[0] * m.stride), P1 = ld3(S.vtx + m.vtx_offset + f[1] * m.stride),
-       P2 = ld3(S.vtx + m.vtx_offset + f[2] * m.stride);
-    f3 fn = cross3(sub3(P1, P0), sub3(P2, P0));
-    return normalize3(mul33(S.insts[sp.inst].nrm, fn));
-}
-// OpenMaterial::defineNextDirection (OpenMaterial.cpp:72-93)
-DEV f3 omNextDirection(const OmvSceneView& S, const f3& inV, const SP& sp) {
-    f3 out = reflect3(inV, sp.n);
-    f3 tfn = faceNormal(S, sp);
-    float corr = dot3(out, tfn);
-    if (corr < 0.1f) {
-        float k = 0.1f - corr;
-        out = normalize3(mk3(tfn.x * k + out.x, tfn.y * k + out.y, tfn.z * k + out.z));
-    }
-    return out;
-}
-DEV f3 omFresnelBrdf(const OmvMaterial& M, float fCos) {
-    float o[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        float rp, rs;
-        fresnel(M.n[c], M.k[c], fCos, rp, rs);
-        o[c] = lerpr(rp, rs, 0.5f) / fabsf(fCos);
-    }
-    return mk3(o[0], o[1], o[2]);
-}
-// Estimator extensions (omv_renderer_params.extensions)
-struct Ext { bool omRough; float omAlpha; };
-
-// OMV_EXT_OM_ROUGH: GGX rough conductor with the complex-IOR Fresnel as F.
-// Connection: f = F(|I.H|) D G2 / (4 V.N L.N), zero unless both cosines > 0.
-DEV f3 omRoughEval(const OmvMaterial& M, float alpha, const f3& I, const f3& O, const f3& N) {
-    float VdotN = -dot3(I, N), LdotN = dot3(O, N);
-    if (!(VdotN > 0.0f && LdotN > 0.0f)) return mk3(0.0f, 0.0f, 0.0f);
-    f3 H = normalize3(sub3(O, I));
-    float VdotH = fabsf(dot3(I, H)), NdotH = fabsf(dot3(N, H));
-    float D = ggxPdf(clamp01(NdotH), alpha);
-    float G = smithG2(smithG1(VdotN, alpha), smithG1(LdotN, alpha));
-    float k = D * G / (4.0f * VdotN * LdotN);
-    float o[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        float rp, rs;
-        fresnel(M.n[c], M.k[c], VdotH, rp, rs);
-        o[c] = lerpr(rp, rs, 0.5f) * k;
-    }
-    return mk3(o[0], o[1], o[2]);
-}
-// Sampling: H from GGXHemisphere (2 uniforms), O = reflect(I, H); brdf times
-// the path cosine is F G |I.H| / (|I.N| |N.H|).
-DEV f3 omRoughSample(const OmvMaterial& M, float alpha, const f3& I, const SP& sp, bool inv, f3& O, Rng& r) {
-    const f3 N = sp.n;
-    O = reflect3(I, N);
-    float VdotN = -dot3(I, N);
-    if (!(VdotN > 0.0f)) return mk3(0.0f, 0.0f, 0.0f);
-    f3 B = cross3(N, sp.t);
-    f3 rd = ggxHemisphere(r, alpha * alpha);
-    f3 H = normalize3(toBasis(sp.t, B, N, rd));
-    f3 o = reflect3(I, H);
-    if (!(dot3(N, o) > 0.0f)) return mk3(0.0f, 0.0f, 0.0f);
-    O = normalize3(o);
-    float LdotN = dot3(O, N), VdotH = fabsf(dot3(I, H)), NdotH = fabsf(dot3(N, H));
-    float G = smithG2(smithG1(VdotN, alpha), smithG1(LdotN, alpha));
-    float w = G * VdotH / (VdotN * NdotH);
-    float ct = inv ? LdotN : VdotN;
-    float out[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        float rp, rs;
-        fresnel(M.n[c], M.k[c], VdotH, rp, rs);
-        out[c] = lerpr(rp, rs, 0.5f) * w / ct;
-    }
-    return mk3(out[0], out[1], out[2]);
-}
-
-// RenderMaterial::getBrdf of the three materials for a connection
-DEV f3 matGetBrdf(const OmvSceneView& S, const Ext& X, const SP& sp, const f3& inV, const f3& outV) {
-    const OmvMaterial& M = S.mats[S.meshes[sp.mesh].mat_id];
-    if (M.type == OMV_MAT_OM) {   // OpenMaterial::getBrdf (OpenMaterial.cpp:95-163)
-        if (!M.has_base) return mk3(0.0f, 0.0f, 0.0f);
-        if (X.omRough) return omRoughEval(M, X.omAlpha, inV, outV, sp.n);
-        float fCos = -dot3(inV, sp.n);
-        if (fabsf(dot3(outV, sp.n) - fCos) > 1e-6f) return mk3(0.0f, 0.0f, 0.0f);
-        return omFresnelBrdf(M, fCos);
-    }
-    if (M.type == OMV_MAT_PBR) {  // PhysicallyBasedMaterial::getBrdf (:178-245)
-        Bsdf b; bsdfInit(b);
-        float omr[3];
-        b.I = inV; b.O = outV; b.N = sp.n;
-        pbrSetup(S, sp, M, b, omr);
-        if (dot3(inV, sp.n) > 0.0f && b.transmissive) b.N = neg3(b.N);
-        pbrFinish(M, b, omr);
-        return evaluateDirect(b);
-    }
-    // DiffuseColorMaterial::getBrdf (DiffuseColorMaterial.cpp:89-104)
-    const float pdf = 1.0f / kPi;
-    const float brdfPdf = 1.0f / (pdf * kPi);
-    return mk3(brdfPdf * M.diffuse[0], brdfPdf * M.diffuse[1], brdfPdf * M.diffuse[2]);
-}
-// RenderMaterial::getRayAndBrdf of the three materials (computePath, BDPTRenderer.cpp:158-159)
-DEV void matGetRayAndBrdf(const OmvSceneView& S, const Ext& X, const SP& sp, const f3& inV, f3& outV, f3& brdf, f3& erad, bool inv,
-                          Rng& r) {
-    const OmvMaterial& M = S.mats[S.meshes[sp.mesh].mat_id];
-    erad = mk3(0.0f, 0.0f, 0.0f);
-    if (M.type == OMV_MAT_OM) {
-        if (X.omRough && M.has_base) { brdf = omRoughSample(M, X.omAlpha, inV, sp, inv, outV, r); return; }
-        outV = omNextDirection(S, inV, sp);
-        brdf = M.has_base ? omFresnelBrdf(M, dot3(outV, sp.n)) : mk3(0.0f, 0.0f, 0.0f);
-        return;
-    }
-    if (M.type == OMV_MAT_PBR) {
-        Bsdf b; bsdfInit(b);
-        float omr[3];
-        b.I = inV; b.N = sp.n; b.T = sp.t;
-        pbrSetup(S, sp, M, b, omr);
-        pbrFinish(M, b, omr);
-        b.eta = 1.0f / (M.double_sided ? 1.0f : M.ior);
-        brdf = sampleIndirect(b, r);
-        outV = b.O;
-        if (M.emissive) erad = pbrEmission(S, sp, M);
-        return;
-    }
-    f3 v = uniformHemisphere(r);
-    outV = toBasis(sp.t, sp.b, sp.n, v);
-    brdf = inv ? matGetBrdf(S, X, sp, neg3(outV), neg3(inV)) : matGetBrdf(S, X, sp, inV, outV);
-}
-
-// ------------------------------------------------------------ traversal
-struct Counters { uint32_t nodes, tris, overflow; };
-
-struct Stack {
-    uint32_t* lds;       // this lane's LDS column, stride = blockDim
-    uint32_t* glb;       // this lane's spill area, stride = spill stride
-    uint32_t ldsStride, glbStride;
-    int sp;
-    DEV void push(uint32_t e, Counters& c) {
-        if (sp < OMV_STACK_LDS) lds[sp * ldsStride] = e;
-        else if (sp < OMV_STACK_TOTAL) glb[(sp - OMV_STACK_LDS) * glbStride] = e;
-        else { ++c.overflow; return; }
-        ++sp;
-    }
-    DEV uint32_t pop() {
-        --sp;
-        return sp < OMV_STACK_LDS ? lds[sp * ldsStride] : glb[(sp - OMV_STACK_LDS) * glbStride];
-    }
-};
-
-enum : uint32_t {
-    E_KIND_SHIFT = 30,
-    E_BLAS_NODE = 0u, E_BLAS_LEAF = 1u, E_TLAS_NODE = 2u, E_INST = 3u,
-    E_END = 0xFFFFFFFFu,
-};
-
-struct Pre { int kx, ky, kz; float Sx, Sy, Sz; f3 o, d, idir; };
-
-DEV bool rayPrep(const f3& o, const f3& d, Pre& p) {
-    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
-    if (!(ax > 0.0f || ay > 0.0f || az > 0.0f)) return false;   // zero / NaN direction: no hit
-    int kz = 0; float m = ax;
-    if (ay > m) { kz = 1; m = ay; }
-    if (az > m) { kz = 2; }
-    int kx = kz + 1; if (kx == 3) kx = 0;
-    int ky = kx + 1; if (ky == 3) ky = 0;
-    float dkz = comp(d, kz);
-    if (dkz < 0.0f) { int t = kx; kx = ky; ky = t; }
-    p.kx = kx; p.ky = ky; p.kz = kz;
-    p.Sx = comp(d, kx) / dkz;
-    p.Sy = comp(d, ky) / dkz;
-    p.Sz = 1.0f / dkz;
-    p.o = o; p.d = d;
-    p.idir = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    return true;
-}
-
-// Woop, Benthin & Wald 2013; t in (tnear, tfar]
-DEV bool triIntersect(const Pre& p, const float4 r0, const float4 r1, const float4 r2, float tnear, float tfar, float& t, float& u, float& v) {
-    f3 A = mk3(r0.x - p.o.x, r0.y - p.o.y, r0.z - p.o.z);
-    f3 B = mk3(r1.x - p.o.x, r1.y - p.o.y, r1.z - p.o.z);
-    f3 C = mk3(r2.x - p.o.x, r2.y - p.o.y, r2.z - p.o.z);
-    const float Akz = comp(A, p.kz), Bkz = comp(B, p.kz), Ckz = comp(C, p.kz);
-    float Ax = comp(A, p.kx) - p.Sx * Akz, Ay = comp(A, p.ky) - p.Sy * Akz;
-    float Bx = comp(B, p.kx) - p.Sx * Bkz, By = comp(B, p.ky) - p.Sy * Bkz;
-    float Cx = comp(C, p.kx) - p.Sx * Ckz, Cy = comp(C, p.ky) - p.Sy * Ckz;
-    float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
-    if (U == 0.0f || V == 0.0f || W == 0.0f) {
-        double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
-        U = (float)(CxBy - CyBx);
-        double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
-        V = (float)(AxCy - AyCx);
-        double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
-        W = (float)(BxAy - ByAx);
-    }
-    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
-    float det = U + V + W;
-    if (det == 0.0f) return false;
-    float Az = p.Sz * Akz, Bz = p.Sz * Bkz, Cz = p.Sz * Ckz;
-    float T = U * Az + V * Bz + W * Cz;
-    float rcp = 1.0f / det;
-    float tt = T * rcp;
-    if (!(tt > tnear && tt <= tfar)) return false;
-    t = tt; u = V * rcp; v = W * rcp;
-    return true;
-}
-
-DEV void offer(Hit& h, bool& found, float t, float u, float v, int inst, int prim) {
-    if (!found || t < h.t) { h.t = t; h.u = u; h.v = v; h.inst = inst; h.prim = prim; h.tie = false; found = true; return; }
-    if (t == h.t) {
-        if (inst != h.inst || prim != h.prim) h.tie = true;
-        if (inst < h.inst || (inst == h.inst && prim < h.prim)) { h.u = u; h.v = v; h.inst = inst; h.prim = prim; }
-    }
-}
-
-// 4-wide slab test, conservative (Ize 2013: tmax scaled by 1 + 2*gamma(3))
-DEV void boxTest4(const OmvNode4& nd, const Pre& p, float tnear, float tfar, float tent[4], bool hit[4]) {
-    const float kUp = 1.0000004f;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        float t0x = (nd.lox[c] - p.o.x) * p.idir.x, t1x = (nd.hix[c] - p.o.x) * p.idir.x;
-        float t0y = (nd.loy[c] - p.o.y) * p.idir.y, t1y = (nd.hiy[c] - p.o.y) * p.idir.y;
-        float t0z = (nd.loz[c] - p.o.z) * p.idir.z, t1z = (nd.hiz[c] - p.o.z) * p.idir.z;
-        float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tnear));
-        float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tfar));
-        tent[c] = tmin;
-        hit[c] = nd.child[c] != OMV_EMPTY_CHILD && tmin <= tmax * kUp;
-    }
-}
-
-DEV void loadNode(const OmvNode4* nodes, uint32_t idx, OmvNode4& nd) {
-    const float4* src = reinterpret_cast<const float4*>(nodes + idx);
-    float4* dst = reinterpret_cast<float4*>(&nd);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) dst[i] = src[i];
-}
-
-// push the hit children nearest-last so the nearest is popped first
-DEV void pushSorted(Stack& st, Counters& c, uint32_t e[4], float t[4], bool h[4]) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) if (!h[i]) t[i] = FLT_MAX;
-#define OMV_CSWAP(a, b) if (t[b] > t[a]) { float tt = t[a]; t[a] = t[b]; t[b] = tt; uint32_t ee = e[a]; e[a] = e[b]; e[b] = ee; bool hh = h[a]; h[a] = h[b]; h[b] = hh; }
-    // descending by t (farthest first)
-    OMV_CSWAP(0, 1) OMV_CSWAP(2, 3) OMV_CSWAP(0, 2) OMV_CSWAP(1, 3) OMV_CSWAP(1, 2)
-#undef OMV_CSWAP
-#pragma unroll
-    for (int i = 0; i < 4; ++i) if (h[i]) st.push(e[i], c);
-}
-
-// Closest hit of a world-space ray over the two-level BVH.
-DEV bool traceClosest(const OmvSceneView& S, const f3& o, const f3& d, float tnear, float tfar, Hit& h, Stack& st, Counters& c) {
-    bool found = false;
-    h.tie = false; h.t = tfar;
-    Pre wp;
-    if (!rayPrep(o, d, wp)) return false;
-    Pre op = wp;
-    int curInst = -1;
-    st.sp = 0;
-    st.push((E_TLAS_NODE << E_KIND_SHIFT) | (uint32_t)S.tlas_root, c);
-    while (st.sp > 0) {
-        uint32_t e = st.pop();
-        if (e == E_END) { curInst = -1; continue; }
-        const uint32_t kind = e >> E_KIND_SHIFT;
-        const uint32_t idx = e & 0x3FFFFFFFu;
-        const float tb = found ? h.t : tfar;
-        if (kind == E_BLAS_LEAF) {
-            const uint32_t first = idx >> 3, cnt = (idx & 7u) + 1u;
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const float4* rec = reinterpret_cast<const float4*>(S.tris) + 3u * (first + k);
-                float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-                ++c.tris;
-                float t, u, v;
-                if (triIntersect(op, r0, r1, r2, tnear, found ? h.t : tfar, t, u, v)) offer(h, found, t, u, v, curInst, __float_as_int(r0.w));
-            }
-            continue;
-        }
-        if (kind == E_INST) {
-            const int inst = (int)(idx & 0x1FFFFFFFu);
-            const OmvInstance& in = S.insts[inst];
-            const float* I = in.inv;
-            f3 oo = mk3(I[0] * o.x + I[1] * o.y + I[2] * o.z + I[3], I[4] * o.x + I[5] * o.y + I[6] * o.z + I[7],
-                        I[8] * o.x + I[9] * o.y + I[10] * o.z + I[11]);
-            f3 od = mk3(I[0] * d.x + I[1] * d.y + I[2] * d.z, I[4] * d.x + I[5] * d.y + I[6] * d.z, I[8] * d.x + I[9] * d.y + I[10] * d.z);
-            if (!rayPrep(oo, od, op)) continue;
-            curInst = inst;
-            st.push(E_END, c);
-            st.push((E_BLAS_NODE << E_KIND_SHIFT) | (uint32_t)S.meshes[in.mesh_id].blas_root, c);
-            continue;
-        }
-        // internal node (TLAS uses the world ray, BLAS the object ray)
-        const bool tl = kind == E_TLAS_NODE;
-        OmvNode4 nd;
-        loadNode(S.nodes, idx, nd);
-        ++c.nodes;
-        float tent[4]; bool hit[4]; uint32_t ent[4];
-        boxTest4(nd, tl ? wp : op, tnear, tb, tent, hit);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            int32_t ch = nd.child[k];
-            if (ch >= 0) ent[k] = ((tl ? E_TLAS_NODE : E_BLAS_NODE) << E_KIND_SHIFT) | (uint32_t)ch;
-            else if (tl) ent[k] = (E_INST << E_KIND_SHIFT) | (uint32_t)(-(ch + 1));
-            else ent[k] = (E_BLAS_LEAF << E_KIND_SHIFT) | ((uint32_t)(-(ch + 1)) << 3) | ((nd.count[k] - 1u) & 7u);
-        }
-        pushSorted(st, c, ent, tent, hit);
-    }
-    return found;
-}
-
-// --------------------------------------------------------- path helpers
-struct RP {   // renderer parameters used per sample
-    int camera_bounces, light_bounces, q, atten;
-    float light_scale, light_min_dist, cut;
-    uint32_t seed;
-    Ext x;
-};
-
-// sceneIntersect (BDPTRenderer.cpp:75-102); tnear persists across calls of one path
-DEV bool sceneIntersect(const OmvSceneView& S, const f3& org, const f3& dir, float& tnear, float tfar, Hit& h, Stack& st, Counters& c, uint32_t& rays) {
-    const float t1 = tfar;
-    for (int itr = 10; itr > 0; --itr) {
-        ++rays;
-        if (!traceClosest(S, org, dir, tnear, tfar, h, st, c)) return false;
-        const int mesh = S.insts[h.inst].mesh_id;
-        const OmvMaterial& M = S.mats[S.meshes[mesh].mat_id];
-        if (M.type != OMV_MAT_PBR || M.alpha_mode != OMV_ALPHA_MASK) return true;
-        if (!isMaskedHit(S, mesh, h.prim, mk3(1.0f - h.u - h.v, h.u, h.v))) return true;
-        f3 o2 = mk3(org.x + h.t * dir.x, org.y + h.t * dir.y, org.z + h.t * dir.z);
-        float v = maxAbs(o2);
-        v = v > h.t ? v : h.t;
-        float delta = 3.0f * FLT_EPSILON * v;
-        tnear = h.t + delta;
-        tfar = t1;
-        if (tfar <= tnear) return false;
-    }
-    return false;
-}
-
-// computeLightAttenuation (BDPTRenderer.cpp:104-126)
-DEV float attenuation(const RP& P, float d, float attD) {
-    if (attD >= FLT_MAX) {
-        d = d < P.light_min_dist ? P.light_min_dist : d;
-        return P.atten == 1 ? 1.0f / d : (P.atten == 2 ? 1.0f / (d * d) : 1.0f);
-    }
-    float rD = mxf(1.0f - d / attD, 0.0f);
-    return P.atten == 1 ? rD : (P.atten == 2 ? sqrtf(rD) : (d < attD ? 1.0f : 0.0f));
-}
-
-// isConnected (BDPTRenderer.cpp:208-235)
-DEV bool isConnected(const OmvSceneView& S, const f3& A, const f3& B, Stack& st, Counters& c, uint32_t& rays, uint32_t& shadow) {
-    f3 dir = sub3(B, A);
-    float len = length3(dir);
-    float maxsz = maxAbs(A);
-    float safeD = 2.0f * 3.0f * FLT_EPSILON * maxsz;
-    if (len <= safeD) return true;
-    dir = mk3(dir.x / len, dir.y / len, dir.z / len);
-    float tnear = safeD;
-    f3 org = mk3(A.x + 0.0f * dir.x, A.y + 0.0f * dir.y, A.z + 0.0f * dir.z);
-    float tfar = len + (0.1f * len > safeD ? 0.1f * len : safeD);
-    Hit h;
-    ++shadow;
-    if (!sceneIntersect(S, org, dir, tnear, tfar, h, st, c, rays)) return true;
-    f3 dd = sub3(hitPosition(S, h), B);
-    float sd = dot3(dd, dd);
-    maxsz = maxAbs(B);
-    safeD = 10.0f * FLT_EPSILON * maxsz;
-    return sd <= safeD * safeD;
-}
-
-// HDRBackground::getRadiance (HdrBackground.cpp:55-97) / Background::getRadiance
-DEV f3 background(const OmvSceneView& S, const f3& dir) {
-    if (S.bg_type == OMV_BG_CONSTANT) return mk3(S.bg_const[0], S.bg_const[1], S.bg_const[2]);
-    f3 rdir = mulT33(dir, S.bg_rot);
-    f3 rxy = normalize3(mk3(rdir.x, rdir.y, 0.0f));
-    float xang = acosf(clampr(rxy.x, -1.0f, 1.0f));
-    xang = rxy.y < 0.0f ? 2.0f * kPi - xang : xang;
-    float yang = asinf(clampr(rdir.z, -1.0f, 1.0f));
-    yang += 0.5f * kPi;
-    float u = xang / (2.0f * kPi);
-    float v = 1.0f - yang / kPi;
-    const int W = S.bg_w, H = S.bg_h;
-    float upage = floorf(u - 0.5f / (float)W), vpage = floorf(v - 0.5f / (float)H);
-    u -= upage; v -= vpage; u *= (float)W; v *= (float)H;
-    int iu = (int)u, iv = (int)v;
-    float kui = u - (float)iu, kvi = v - (float)iv, ku = 1.0f - kui, kv = 1.0f - kvi;
-    int l = iu % W, r = (iu + 1) % W, b = iv % H, t = (iv + 1) % H;
-    const int n = W * H - 1;
-    int i0 = min(max(b * W + l, 0), n), i1 = min(max(b * W + r, 0), n), i2 = min(max(t * W + l, 0), n), i3 = min(max(t * W + r, 0), n);
-    float w0 = ku * kv, w1 = kui * kv, w2 = ku * kvi, w3 = kui * kvi;
-    const float4* px = reinterpret_cast<const float4*>(S.bg);
-    float4 p0 = px[i0], p1 = px[i1], p2 = px[i2], p3 = px[i3];
-    f3 val = mk3(p0.x * w0 + p1.x * w1 + p2.x * w2 + p3.x * w3, p0.y * w0 + p1.y * w1 + p2.y * w2 + p3.y * w3,
-                 p0.z * w0 + p1.z * w1 + p2.z * w2 + p3.z * w3);
-    return mk3(val.x * S.bg_scale, val.y * S.bg_scale, val.z * S.bg_scale);
-}
-
-// DistanceRandom::getValue (DistanceRandom.h:203-220)
-DEV int sampleLight(const OmvSceneView& S, float u) {
-    const int n = S.num_lights;
-    if (n <= 0) return -1;
-    float d = u * S.light_max;
-    if (n == 1) return 0;
-    if (d <= S.light_cdf[0]) return 0;
-    if (d >= S.light_cdf[n - 2]) return n - 1;
-    int L = 0, R = n - 1;
-    while (L < R) { int C = (L + R) / 2; if (d <= S.light_cdf[C]) R = C; else L = C + 1; }
-    return (L + R) / 2;
-}
-
-// PinholeCamera::getRay (PinholeCamera.cpp:80-95)
-DEV void cameraRay(const omv_camera_desc& c, int x, int y, float u0, float u1, f3& O, f3& D) {
-    float px = (float)x + 0.5f - c.half_w;
-    float py = (float)y + 0.5f - c.half_h;
-    float dx = lerpr(-0.5f, 0.5f, u0);
-    float dy = lerpr(-0.5f, 0.5f, u1);
-    f3 cr = normalize3(mk3(px + dx, py + dy, c.focus));
-    D = mul33(c.rotation, cr);
-    O = ld3(c.position);
-}
-
-DEV uint64_t waveSum(uint64_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-DEV void flushCounters(unsigned long long* g, uint32_t rays, uint32_t cam, uint32_t light, uint32_t shadow, const Counters& c) {
-    uint64_t v[7] = {rays, cam, light, shadow, c.nodes, c.tris, c.overflow};
-#pragma unroll
-    for (int i = 0; i < 7; ++i) {
-        uint64_t s = waveSum(v[i]);
-        if ((threadIdx.x & 63) == 0 && s) atomicAdd(g + i, (unsigned long long)s);
-    }
-}
-
-}  // namespace
+#include "path_device.h"
 
 // ============================================================== kernels
 extern __shared__ uint32_t omv_lds_stack[];
@@ -1295,15 +343,6 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_fc_kernel(OmvSceneView S, omv_c
 // ----------------------------------------------------------- launchers
 namespace omv {
 
-static RP toRP(const omv_renderer_params& p) {
-    RP r;
-    r.camera_bounces = p.camera_bounces; r.light_bounces = p.light_bounces; r.q = p.max_path_length;
-    r.atten = p.light_distance_attenuation; r.light_scale = p.light_scale_value; r.light_min_dist = p.light_min_distance;
-    r.cut = p.ray_cut_pix_value; r.seed = p.seed;
-    r.x.omRough = (p.extensions & OMV_EXT_OM_ROUGH) != 0;
-    r.x.omAlpha = p.om_roughness * p.om_roughness;
-    return r;
-}
 
 hipError_t launchBdpt(const OmvSceneView& S, const omv_camera_desc& cam, const omv_renderer_params& p, uint64_t sample0,
                       uint64_t nsamples, float* rad, float* lv, uint32_t* gstack, unsigned long long* counters, int blocks,

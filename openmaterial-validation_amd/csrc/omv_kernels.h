@@ -30,6 +30,29 @@ void packBatchJob(void* dst, const OmvSceneView& S, const omv_camera_desc& cam, 
                   uint64_t npix);
 hipError_t launchBatch(const void* jobs, int njobs, const uint64_t* tileEnd, uint64_t totalTiles, int tileWaves,
                        unsigned long long* counters, float* lv, uint32_t* gstack, int blocks, hipStream_t stream);
+// Wavefront render of one chunk of whole pixels (wavefront.hip).
+struct WfLaunch {
+    OmvSceneView S;
+    omv_camera_desc cam;
+    omv_renderer_params p;
+    uint64_t sample0;          // first sample of the chunk
+    uint32_t nslots;           // samples in the chunk
+    int lvCap, lc;             // light vertices kept, connections per camera vertex
+    void* arena;               // wavefrontArenaBytes(nslots, lvCap, lc, lights)
+    uint32_t* qcount;          // wavefrontCounterCount(B, lvCap) counters
+    float* rad;                // 3 floats per sample of the chunk
+    uint32_t* gstack;          // spill stacks of blocksTrace * OMV_BLOCK lanes
+    unsigned long long* counters;
+    int blocksGen, blocksTrace, blocksShade;
+    int traceKeep;             // > 0: persistent traversal, refill below this many busy lanes
+    hipEvent_t* traceEv;       // optional: an event pair around every traversal launch
+    int traceEvCap;            // events available from traceEv
+    int* traceEvUsed;          // events recorded so far (in/out)
+};
+size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights);
+int wavefrontCounterCount(int B, int lvCap);
+void wavefrontBlocksPerCU(int out[4]);   // trace, shade, shade_light, gen
+hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream);
 // 0: sample kernel (scene with lights), 1: FC kernel, 2: sample kernel (no lights), 3: batch kernel
 int occupancyBlocksPerCU(int which);
 

@@ -115,19 +115,30 @@ public:
     ~BDPTRenderer();
     void setParameters(const omv_renderer_params& p) { params_ = p; }
     void setScene(RenderScene* s) { scene_ = s; }
+    int setSchedule(int s);
+    int schedule() const { return schedule_; }
+    int setTuning(const omv_render_tuning& t);
+    const omv_render_tuning& tuning() const { return tuning_; }
     int render(PinholeCamera& cam);
     int renderFc(PinholeCamera& cam, int32_t* ids, uint8_t* tie);
     const omv_render_stats& stats() const { return stats_; }
 
 private:
     int prepare(int which, uint64_t lanesWanted, int& blocks);
+    int initStream();
+    int renderSamples(PinholeCamera& cam, double& kernelMs);
+    int renderWavefront(PinholeCamera& cam, double& kernelMs);
     omv_renderer_params params_;
+    int schedule_ = OMV_SCHEDULE_WAVEFRONT;
+    omv_render_tuning tuning_{0, 0, 0};
+    int wfPerCu_[4] = {0, 0, 0, 0};
     RenderScene* scene_ = nullptr;
     omv_render_stats stats_{};
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
     int cus_ = 0;
-    DeviceBuffer dRad_, dLv_, dStack_, dCounters_, dFilm_, dIds_, dTie_;
+    DeviceBuffer dRad_, dLv_, dStack_, dCounters_, dFilm_, dIds_, dTie_, dArena_, dQcount_;
+    std::vector<hipEvent_t> traceEv_;
 };
 
 // Many BDPTRenderer::render jobs in one persistent launch, replayed from a

@@ -13,6 +13,8 @@ namespace omv {
 
 namespace {
 constexpr uint64_t kChunkSamples = 1ull << 24;   // per-sample radiance buffer: 192 MiB
+constexpr uint64_t kWfSlots = 1ull << 22;         // wavefront slots per chunk (~0.7 GiB of state)
+constexpr uint64_t kWfSlotsLights = 1ull << 21;   // with lights (light vertices + connections)
 int hipFail(hipError_t e, const char* what) {
     setError(std::string(what) + ": " + hipGetErrorString(e));
     return OMV_E_DEVICE;
@@ -64,18 +66,40 @@ BDPTRenderer::BDPTRenderer() { omv_renderer_params_default(&params_); }
 
 BDPTRenderer::~BDPTRenderer() {
     for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
+    for (auto& e : traceEv_) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-int BDPTRenderer::prepare(int which, uint64_t lanesWanted, int& blocks) {
-    hipError_t e;
-    if (!stream_) {
-        if ((e = hipStreamCreate(&stream_)) != hipSuccess) return hipFail(e, "hipStreamCreate");
-        for (auto& ev : ev_) if ((e = hipEventCreate(&ev)) != hipSuccess) return hipFail(e, "hipEventCreate");
-        int dev = 0;
-        (void)hipGetDevice(&dev);
-        if ((e = hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return hipFail(e, "device attribute");
+int BDPTRenderer::setSchedule(int s) {
+    if (s != OMV_SCHEDULE_WAVEFRONT && s != OMV_SCHEDULE_PER_SAMPLE) { setError("set_schedule: unknown schedule"); return OMV_E_ARG; }
+    schedule_ = s;
+    return OMV_OK;
+}
+
+int BDPTRenderer::setTuning(const omv_render_tuning& t) {
+    if (t.trace_keep < 0 || t.trace_keep > 64 || t.trace_blocks_per_cu < 0 || t.chunk_samples < 0) {
+        setError("set_tuning: value out of range");
+        return OMV_E_RANGE;
     }
+    tuning_ = t;
+    return OMV_OK;
+}
+
+int BDPTRenderer::initStream() {
+    hipError_t e;
+    if (stream_) return OMV_OK;
+    if ((e = hipStreamCreate(&stream_)) != hipSuccess) return hipFail(e, "hipStreamCreate");
+    for (auto& ev : ev_) if ((e = hipEventCreate(&ev)) != hipSuccess) return hipFail(e, "hipEventCreate");
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if ((e = hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return hipFail(e, "device attribute");
+    return OMV_OK;
+}
+
+int BDPTRenderer::prepare(int which, uint64_t lanesWanted, int& blocks) {
+    int rc = initStream();
+    if (rc) return rc;
+    hipError_t e;
     int perCu = occupancyBlocksPerCU(which);
     if (perCu <= 0) perCu = 1;
     uint64_t maxBlocks = (uint64_t)cus_ * (uint64_t)perCu;
@@ -95,36 +119,19 @@ int BDPTRenderer::render(PinholeCamera& cam) {
     const int spp = p.samples_per_pixel;
     std::memset(&stats_, 0, sizeof(stats_));
     cam.film.assign(npix * 4, 0.0f);
-    int blocks = 0;
-    const uint64_t chunkPix = std::max<uint64_t>(1, kChunkSamples / (uint64_t)std::max(spp, 1));
-    const uint64_t firstChunk = std::min(npix, chunkPix) * (uint64_t)std::max(spp, 1);
-    int rc = prepare(scene_->view().num_lights > 0 ? 0 : 2, firstChunk, blocks);
+    int rc = initStream();
     if (rc) return rc;
-    const uint64_t lanes = (uint64_t)blocks * OMV_BLOCK;
-    const int lvCap = std::max(std::min(p.light_bounces + 1, std::max(p.max_path_length, 1)), 1);
     hipError_t e;
-    if ((e = dLv_.reserve(lanes * (uint64_t)lvCap * 6 * sizeof(float))) != hipSuccess) return hipFail(e, "light vertex scratch");
-    if ((e = dRad_.reserve(std::min(npix, chunkPix) * (uint64_t)std::max(spp, 1) * 3 * sizeof(float))) != hipSuccess) return hipFail(e, "sample buffer");
     if ((e = dFilm_.reserve(npix * 4 * sizeof(float))) != hipSuccess) return hipFail(e, "film");
+    if ((e = dCounters_.reserve(8 * sizeof(unsigned long long))) != hipSuccess) return hipFail(e, "counters");
     if ((e = hipMemsetAsync(dCounters_.ptr, 0, 8 * sizeof(unsigned long long), stream_)) != hipSuccess) return hipFail(e, "memset");
     // PinholeCamera::init (PinholeCamera.cpp:65-68)
     if ((e = hipMemsetAsync(dFilm_.ptr, 0, npix * 4 * sizeof(float), stream_)) != hipSuccess) return hipFail(e, "memset film");
     (void)hipEventRecord(ev_[0], stream_);
     double kernelMs = 0.0;
     if (spp > 0) {
-        for (uint64_t pix0 = 0; pix0 < npix; pix0 += chunkPix) {
-            const uint64_t np = std::min(chunkPix, npix - pix0);
-            (void)hipEventRecord(ev_[2], stream_);
-            if ((e = launchBdpt(scene_->view(), cam.desc, p, pix0 * (uint64_t)spp, np * (uint64_t)spp, dRad_.as<float>(), dLv_.as<float>(),
-                                dStack_.as<uint32_t>(), dCounters_.as<unsigned long long>(), blocks, stream_)) != hipSuccess)
-                return hipFail(e, "sample kernel launch");
-            (void)hipEventRecord(ev_[3], stream_);
-            if ((e = launchFilm(dRad_.as<float>(), spp, pix0, np, dFilm_.as<float>(), stream_)) != hipSuccess) return hipFail(e, "film kernel launch");
-            if ((e = hipEventSynchronize(ev_[3])) != hipSuccess) return hipFail(e, "sample kernel");
-            float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, ev_[2], ev_[3]);
-            kernelMs += ms;
-        }
+        rc = schedule_ == OMV_SCHEDULE_WAVEFRONT ? renderWavefront(cam, kernelMs) : renderSamples(cam, kernelMs);
+        if (rc) return rc;
     } else {
         // PinholeCamera::stop with a zero count leaves rgb = 0 and alpha = 1
         std::vector<float> z(npix * 4, 0.0f);
@@ -144,6 +151,103 @@ int BDPTRenderer::render(PinholeCamera& cam) {
     stats_.render_ms = total;
     stats_.kernel_ms = kernelMs;
     if (stats_.stack_overflows) { setError("render: traversal stack overflow"); return OMV_E_INTERNAL; }
+    return OMV_OK;
+}
+
+// Per-sample schedule: one persistent sample-kernel launch per chunk of
+// whole pixels, then the ordered film reduction for that chunk.
+int BDPTRenderer::renderSamples(PinholeCamera& cam, double& kernelMs) {
+    const omv_renderer_params& p = params_;
+    const uint64_t npix = (uint64_t)cam.desc.width * cam.desc.height;
+    const int spp = p.samples_per_pixel;
+    int blocks = 0;
+    const uint64_t chunkPix = std::max<uint64_t>(1, kChunkSamples / (uint64_t)spp);
+    const uint64_t firstChunk = std::min(npix, chunkPix) * (uint64_t)spp;
+    int rc = prepare(scene_->view().num_lights > 0 ? 0 : 2, firstChunk, blocks);
+    if (rc) return rc;
+    const uint64_t lanes = (uint64_t)blocks * OMV_BLOCK;
+    const int lvCap = std::max(std::min(p.light_bounces + 1, std::max(p.max_path_length, 1)), 1);
+    hipError_t e;
+    if ((e = dLv_.reserve(lanes * (uint64_t)lvCap * 6 * sizeof(float))) != hipSuccess) return hipFail(e, "light vertex scratch");
+    if ((e = dRad_.reserve(std::min(npix, chunkPix) * (uint64_t)spp * 3 * sizeof(float))) != hipSuccess) return hipFail(e, "sample buffer");
+    for (uint64_t pix0 = 0; pix0 < npix; pix0 += chunkPix) {
+        const uint64_t np = std::min(chunkPix, npix - pix0);
+        (void)hipEventRecord(ev_[2], stream_);
+        if ((e = launchBdpt(scene_->view(), cam.desc, p, pix0 * (uint64_t)spp, np * (uint64_t)spp, dRad_.as<float>(), dLv_.as<float>(),
+                            dStack_.as<uint32_t>(), dCounters_.as<unsigned long long>(), blocks, stream_)) != hipSuccess)
+            return hipFail(e, "sample kernel launch");
+        (void)hipEventRecord(ev_[3], stream_);
+        if ((e = launchFilm(dRad_.as<float>(), spp, pix0, np, dFilm_.as<float>(), stream_)) != hipSuccess) return hipFail(e, "film kernel launch");
+        if ((e = hipEventSynchronize(ev_[3])) != hipSuccess) return hipFail(e, "sample kernel");
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, ev_[2], ev_[3]);
+        kernelMs += ms;
+    }
+    return OMV_OK;
+}
+
+// Wavefront schedule: chunks of whole pixels, each run as ray-queue rounds
+// (wavefront.hip), then the ordered film reduction for that chunk.
+int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
+    const omv_renderer_params& p = params_;
+    const OmvSceneView& S = scene_->view();
+    const bool lights = S.num_lights > 0;
+    const uint64_t npix = (uint64_t)cam.desc.width * cam.desc.height;
+    const int spp = p.samples_per_pixel;
+    const int lvCap = std::max(std::min(p.light_bounces + 1, std::max(p.max_path_length, 1)), 1);
+    const int lc = std::max(1, std::min(lvCap, std::max(p.max_path_length, 1)));
+    const uint64_t maxSlots = tuning_.chunk_samples > 0 ? (uint64_t)tuning_.chunk_samples : (lights ? kWfSlotsLights : kWfSlots);
+    const uint64_t chunkPix = std::max<uint64_t>(1, maxSlots / (uint64_t)spp);
+    const uint64_t slots = std::min(npix, chunkPix) * (uint64_t)spp;
+    if (slots * (uint64_t)lc >= (1ull << 32)) { setError("render: too many samples per pixel for one chunk"); return OMV_E_RANGE; }
+    if (!wfPerCu_[0]) wavefrontBlocksPerCU(wfPerCu_);
+    const int blocksTrace = cus_ * (tuning_.trace_blocks_per_cu > 0 ? std::min(tuning_.trace_blocks_per_cu, wfPerCu_[0]) : wfPerCu_[0]);
+    hipError_t e;
+    if ((e = dStack_.reserve((uint64_t)blocksTrace * OMV_BLOCK * (OMV_STACK_TOTAL - OMV_STACK_LDS) * sizeof(uint32_t))) != hipSuccess)
+        return hipFail(e, "stack spill");
+    if ((e = dArena_.reserve(wavefrontArenaBytes((uint32_t)slots, lvCap, lc, lights))) != hipSuccess) return hipFail(e, "wavefront state");
+    if ((e = dQcount_.reserve(wavefrontCounterCount(p.camera_bounces, lvCap) * sizeof(uint32_t))) != hipSuccess) return hipFail(e, "queue counters");
+    if ((e = dRad_.reserve(slots * 3 * sizeof(float))) != hipSuccess) return hipFail(e, "sample buffer");
+    WfLaunch L;
+    std::memset(&L, 0, sizeof(L));
+    L.S = S; L.cam = cam.desc; L.p = p; L.lvCap = lvCap; L.lc = lc;
+    L.arena = dArena_.ptr; L.qcount = dQcount_.as<uint32_t>(); L.rad = dRad_.as<float>(); L.gstack = dStack_.as<uint32_t>();
+    L.counters = dCounters_.as<unsigned long long>();
+    L.blocksTrace = blocksTrace;
+    L.blocksShade = cus_ * (lights ? wfPerCu_[1] : wfPerCu_[1]);
+    L.blocksGen = cus_ * wfPerCu_[3];
+    L.traceKeep = tuning_.trace_keep;
+    // an event pair around every traversal launch: (light + 2 camera rounds) per chunk
+    const uint64_t chunks = (npix + chunkPix - 1) / chunkPix;
+    const size_t evNeed = (size_t)std::min<uint64_t>(1u << 14, chunks * 2 * (uint64_t)(lvCap + 2 * p.camera_bounces));
+    while (traceEv_.size() < evNeed) {
+        hipEvent_t ev;
+        if ((e = hipEventCreate(&ev)) != hipSuccess) return hipFail(e, "hipEventCreate");
+        traceEv_.push_back(ev);
+    }
+    int evUsed = 0;
+    L.traceEv = traceEv_.data(); L.traceEvCap = (int)(evNeed & ~(size_t)1); L.traceEvUsed = &evUsed;
+    (void)hipEventRecord(ev_[2], stream_);
+    for (uint64_t pix0 = 0; pix0 < npix; pix0 += chunkPix) {
+        const uint64_t np = std::min(chunkPix, npix - pix0);
+        L.sample0 = pix0 * (uint64_t)spp;
+        L.nslots = (uint32_t)(np * (uint64_t)spp);
+        if ((e = launchWavefront(L, stream_)) != hipSuccess) return hipFail(e, "wavefront launch");
+        if ((e = launchFilm(dRad_.as<float>(), spp, pix0, np, dFilm_.as<float>(), stream_)) != hipSuccess) return hipFail(e, "film kernel launch");
+    }
+    (void)hipEventRecord(ev_[3], stream_);
+    if ((e = hipEventSynchronize(ev_[3])) != hipSuccess) return hipFail(e, "wavefront kernels");
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, ev_[2], ev_[3]);
+    kernelMs += ms;
+    double tr = 0.0;
+    for (int k = 0; k + 1 < evUsed; k += 2) {
+        float t = 0.0f;
+        (void)hipEventElapsedTime(&t, traceEv_[k], traceEv_[k + 1]);
+        tr += t;
+    }
+    stats_.trace_ms = tr;
+    stats_.trace_launches = (uint64_t)(evUsed / 2);
     return OMV_OK;
 }
 

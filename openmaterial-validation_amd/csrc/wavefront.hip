@@ -1,0 +1,708 @@
+// wavefront.hip -- the render loop of BDPTRenderer (BDPTRenderer.cpp:128-301)
+// as a sequence of small gfx950 kernels over compacted ray queues.
+//
+// Every sample keeps a slot of path state in HBM (structure of arrays, slot
+// = sample - first sample of the chunk).  A chunk runs as
+//
+//   gen             camera ray + light-subpath start of every slot
+//   light rounds    trace(light rays) -> shade_light   (vertex i = 1..)
+//   camera rounds   trace(camera rays) -> shade_cam    (vertex i = 1..B)
+//                   [ -> trace(shadow rays) -> connect  when there are lights ]
+//
+// so each trace launch sees only rays that exist, whatever the lengths of
+// the paths around them, and the traversal kernel stays small (registers,
+// code) next to the material code.  Shading appends the next ray with one
+// atomic per wave.  The per-sample arithmetic and the order in which every
+// sum is formed are those of the per-sample kernel (kernels.hip), so the
+// film is identical to it bit for bit; only the schedule differs.
+//
+// Philox stream positions persist between kernels as "draws consumed".
+#include <algorithm>
+#include <cstring>
+
+#include "path_device.h"
+
+extern __shared__ uint32_t omv_lds_stack[];
+
+namespace {
+
+// ----------------------------------------------------------- state layout
+struct WfQueue {       // one ray queue
+    float4* o;         // origin xyz, tnear
+    float4* d;         // direction xyz, as_float(slot)
+};
+struct WfShadowQ {     // connection visibility rays (isConnected, BDPTRenderer.cpp:208-235)
+    float4* o;         // A xyz (light vertex), tnear = safeD
+    float4* d;         // unit direction xyz, tfar
+    float4* b;         // B xyz (camera vertex), as_float(slot * lc + l)
+};
+struct WfHits {
+    float4* h;         // t, u, v, tnear after the masked re-trace loop
+    int2* id;          // instance (-1: miss), primitive | tie << 31
+};
+struct WfState {
+    uint32_t n;        // slots
+    float4* thr;       // camera throughput xyz
+    float4* radc;      // camera radiance xyz, as_float(vertex count)
+    float4* conn;      // connection sum xyz
+    uint32_t* csn;     // camera stream draws consumed
+    uint32_t* flags;   // 0 running, 1 ends after this round's connections, 2 written
+    uint32_t* nconn;   // connections staged this round
+    float4* lthr;      // light throughput xyz, first-hit attenuation rD
+    uint32_t* lsn;     // light stream draws consumed
+    int* lidx;         // sampled light
+    int* lplen;        // light subpath vertex count
+    float* lv;         // light vertices [vertex][6][slot]
+    float* cc;         // staged connection contributions [l][3][slot]
+    uint8_t* vis;      // [l][slot] 0 none/blocked, 1 pending, 2 visible
+    int lvCap, lc;     // light vertex capacity, connections per camera vertex
+};
+
+DEV uint32_t rngConsumed(const Rng& r) { return 4u * r.c0 + (uint32_t)r.idx - 4u; }
+DEV void rngRestore(Rng& r, uint32_t seed, uint32_t stream, uint64_t sample, uint32_t n) {
+    r.init(seed, stream, sample);
+    if (n == 0u) return;
+    const uint32_t block = (n - 1u) >> 2;
+    r.c0 = block;
+    r.refill();
+    r.idx = (int)(n - 4u * block);
+}
+
+// Slot of the calling lane in a queue: one atomic per wave.  Lanes that do
+// not append pass want = false; all active lanes must make the call.
+DEV uint32_t waveAppend(uint32_t* counter, bool want) {
+    const uint64_t m = __ballot(want);
+    if (m == 0ull) return 0u;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0u;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+DEV void stackInit(Stack& st, uint32_t* gstack, uint32_t gtid, uint32_t gthreads) {
+    st.lds = omv_lds_stack + threadIdx.x; st.ldsStride = blockDim.x;
+    st.glb = gstack + gtid; st.glbStride = gthreads;
+    st.sp = 0;
+}
+
+DEV f3 xyz(const float4& v) { return mk3(v.x, v.y, v.z); }
+DEV float4 f4of(const f3& v, float w) { float4 r; r.x = v.x; r.y = v.y; r.z = v.z; r.w = w; return r; }
+
+}  // namespace
+
+// ------------------------------------------------------------------ gen
+// Camera ray of every slot (PinholeCamera::getRay, 2 draws) and, with
+// lights, the light-subpath start (tryComputeLightPath, BDPTRenderer.cpp:194-206).
+template <bool kLights>
+__global__ __launch_bounds__(OMV_BLOCK) void omv_wf_gen(OmvSceneView S, omv_camera_desc cam, RP P, int spp, uint64_t sample0,
+                                                        WfState W, WfQueue camQ, WfQueue lightQ, uint32_t* __restrict__ camCount,
+                                                        uint32_t* __restrict__ lightCount, float* __restrict__ rad) {
+    const uint32_t gthreads = gridDim.x * blockDim.x;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = W.n;
+    const bool camRays = P.camera_bounces >= 1;
+    if (gtid == 0) {
+        *camCount = camRays ? n : 0u;
+        if (kLights) *lightCount = W.lvCap > 1 ? n : 0u;
+    }
+    for (uint32_t slot = gtid; slot < n; slot += gthreads) {
+        const uint64_t s = sample0 + slot;
+        const uint64_t pix = s / (uint64_t)spp;
+        const int x = (int)(pix % (uint64_t)cam.width), y = (int)(pix / (uint64_t)cam.width);
+        Rng cs; cs.init(P.seed, STREAM_CAMERA, s);
+        float u0 = cs.next(), u1 = cs.next();
+        f3 O, D;
+        cameraRay(cam, x, y, u0, u1, O, D);
+        W.thr[slot] = f4of(mk3(1.0f, 1.0f, 1.0f), 0.0f);
+        W.radc[slot] = f4of(mk3(0.0f, 0.0f, 0.0f), __int_as_float(1));
+        W.conn[slot] = f4of(mk3(0.0f, 0.0f, 0.0f), 0.0f);
+        W.csn[slot] = rngConsumed(cs);
+        W.flags[slot] = 0u;
+        if (camRays) {
+            float delta = 3.0f * FLT_EPSILON * maxAbs(O);
+            f3 org = mk3(O.x + delta * D.x, O.y + delta * D.y, O.z + delta * D.z);
+            camQ.o[slot] = f4of(org, 0.0f);
+            camQ.d[slot] = f4of(D, __uint_as_float(slot));
+        } else {
+            // no camera vertex: conn * 1 + radiance with both zero
+            rad[(uint64_t)slot * 3 + 0] = 0.0f;
+            rad[(uint64_t)slot * 3 + 1] = 0.0f;
+            rad[(uint64_t)slot * 3 + 2] = 0.0f;
+            W.flags[slot] = 2u;
+        }
+        if (kLights) {
+            Rng ls; ls.init(P.seed, STREAM_LIGHT, s);
+            const int li = sampleLight(S, ls.next());
+            const OmvLight Lt = S.lights[li];
+            f3 dir = uniformSphere(ls);
+            const uint64_t N = n;
+            W.lv[(0 * 6 + 0) * N + slot] = Lt.pos[0];
+            W.lv[(0 * 6 + 1) * N + slot] = Lt.pos[1];
+            W.lv[(0 * 6 + 2) * N + slot] = Lt.pos[2];
+            W.lv[(0 * 6 + 3) * N + slot] = Lt.intensity[0] * P.light_scale;
+            W.lv[(0 * 6 + 4) * N + slot] = Lt.intensity[1] * P.light_scale;
+            W.lv[(0 * 6 + 5) * N + slot] = Lt.intensity[2] * P.light_scale;
+            W.lidx[slot] = li;
+            W.lplen[slot] = 1;
+            W.lthr[slot] = f4of(mk3(1.0f, 1.0f, 1.0f), 1.0f);
+            W.lsn[slot] = rngConsumed(ls);
+            if (W.lvCap > 1) {
+                const f3 lp = ld3(Lt.pos);
+                float delta = 3.0f * FLT_EPSILON * maxAbs(lp);
+                f3 org = mk3(lp.x + delta * dir.x, lp.y + delta * dir.y, lp.z + delta * dir.z);
+                lightQ.o[slot] = f4of(org, 0.0f);
+                lightQ.d[slot] = f4of(dir, __uint_as_float(slot));
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- trace
+// sceneIntersect (BDPTRenderer.cpp:75-102) over a queue.  kKind 0: camera
+// rays, 1: light rays (closest hit + the re-trace tnear), 2: connection
+// rays (the visibility decision of isConnected, BDPTRenderer.cpp:221-234).
+template <int kKind>
+__global__ __launch_bounds__(OMV_BLOCK) void omv_wf_trace(OmvSceneView S, WfQueue q, WfShadowQ sq, const uint32_t* __restrict__ count,
+                                                          WfHits H, uint8_t* __restrict__ vis, uint32_t nslots, int lc,
+                                                          uint32_t* __restrict__ gstack, unsigned long long* __restrict__ counters) {
+    const uint32_t gthreads = gridDim.x * blockDim.x;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st;
+    stackInit(st, gstack, gtid, gthreads);
+    Counters cnt; cnt.nodes = 0; cnt.tris = 0; cnt.overflow = 0;
+    uint32_t rays = 0, nCam = 0, nLight = 0, nShadow = 0;
+    const uint32_t total = *count;
+    for (uint32_t e = gtid; e < total; e += gthreads) {
+        Hit h;
+        if (kKind == 2) {
+            const float4 a = sq.o[e], dd = sq.d[e], b = sq.b[e];
+            float tnear = a.w;
+            ++nShadow;
+            bool visible;
+            if (!sceneIntersect(S, xyz(a), xyz(dd), tnear, dd.w, h, st, cnt, rays)) {
+                visible = true;
+            } else {
+                f3 df = sub3(hitPosition(S, h), xyz(b));
+                float sd = dot3(df, df);
+                float safeD = 10.0f * FLT_EPSILON * maxAbs(xyz(b));
+                visible = sd <= safeD * safeD;
+            }
+            const uint32_t id = __float_as_uint(b.w);
+            const uint32_t slot = id / (uint32_t)lc, l = id - slot * (uint32_t)lc;
+            vis[(uint64_t)l * nslots + slot] = visible ? 2 : 0;
+        } else {
+            const float4 o = q.o[e], d = q.d[e];
+            float tnear = o.w;
+            if (kKind == 0) ++nCam; else ++nLight;
+            const bool hit = sceneIntersect(S, xyz(o), xyz(d), tnear, FLT_MAX, h, st, cnt, rays);
+            float4 r; r.x = h.t; r.y = h.u; r.z = h.v; r.w = tnear;
+            int2 id; id.x = hit ? h.inst : -1; id.y = h.prim | (h.tie ? (int)0x80000000 : 0);
+            H.h[e] = r;
+            H.id[e] = id;
+        }
+    }
+    flushCounters(counters, rays, nCam, nLight, nShadow, cnt);
+}
+
+// Persistent form of the trace kernel: each lane keeps one ray and refills
+// from the queue (one atomic per wave) as soon as its ray is finished, once
+// fewer than `keep` lanes of the wave are still busy, so lanes do not idle
+// behind the longest traversal of their wave.  Same per-ray arithmetic
+// (Trav + the masked re-trace loop of sceneIntersect) and the same outputs.
+template <int kKind>
+__global__ __launch_bounds__(OMV_BLOCK) void omv_wf_trace_pt(OmvSceneView S, WfQueue q, WfShadowQ sq, const uint32_t* __restrict__ count,
+                                                             uint32_t* __restrict__ fetch, int keep, WfHits H, uint8_t* __restrict__ vis,
+                                                             uint32_t nslots, int lc, uint32_t* __restrict__ gstack,
+                                                             unsigned long long* __restrict__ counters) {
+    const uint32_t gthreads = gridDim.x * blockDim.x;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    Stack st;
+    stackInit(st, gstack, gtid, gthreads);
+    Counters cnt; cnt.nodes = 0; cnt.tris = 0; cnt.overflow = 0;
+    uint32_t rays = 0, nCam = 0, nLight = 0, nShadow = 0;
+    const uint32_t total = *count;
+    Trav T;
+    bool busy = false, drained = false;
+    uint32_t e = 0;
+    int itr = 0;
+    f3 org = mk3(0.0f, 0.0f, 0.0f), dir = org, B = org;
+    float t1 = 0.0f;
+    for (;;) {
+        // refill idle lanes
+        const bool need = !busy && !drained;
+        const uint64_t m = __ballot(need);
+        if (m) {
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t base = 0u;
+            if (lane == leader) base = atomicAdd(fetch, (uint32_t)__popcll(m));
+            base = __shfl(base, leader, 64);
+            if (need) {
+                const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                if (k < total) {
+                    e = k;
+                    float tn;
+                    if (kKind == 2) {
+                        const float4 a = sq.o[e], dd = sq.d[e], b = sq.b[e];
+                        org = xyz(a); dir = xyz(dd); B = xyz(b); tn = a.w; t1 = dd.w;
+                        ++nShadow;
+                    } else {
+                        const float4 o = q.o[e], d = q.d[e];
+                        org = xyz(o); dir = xyz(d); tn = o.w; t1 = FLT_MAX;
+                        if (kKind == 0) ++nCam; else ++nLight;
+                    }
+                    itr = 10;
+                    ++rays;
+                    T.start(S, org, dir, tn, t1, st, cnt);
+                    busy = true;
+                } else {
+                    drained = true;
+                }
+            }
+        }
+        if (__ballot(busy) == 0ull) break;
+        // traverse until too few lanes of the wave are busy
+        for (;;) {
+            if (busy && st.sp > 0) T.step(S, st, cnt);
+            if (busy && st.sp == 0) {
+                // one traceClosest call finished: sceneIntersect's masked re-trace loop
+                bool done = true, hit = T.found;
+                if (hit) {
+                    const int mesh = S.insts[T.h.inst].mesh_id;
+                    const OmvMaterial& M = S.mats[S.meshes[mesh].mat_id];
+                    if (M.type == OMV_MAT_PBR && M.alpha_mode == OMV_ALPHA_MASK &&
+                        isMaskedHit(S, mesh, T.h.prim, mk3(1.0f - T.h.u - T.h.v, T.h.u, T.h.v))) {
+                        f3 o2 = mk3(org.x + T.h.t * dir.x, org.y + T.h.t * dir.y, org.z + T.h.t * dir.z);
+                        float v = maxAbs(o2);
+                        v = v > T.h.t ? v : T.h.t;
+                        float delta = 3.0f * FLT_EPSILON * v;
+                        const float tn = T.h.t + delta;
+                        hit = false;
+                        T.tnear = tn;
+                        if (!(t1 <= tn) && --itr > 0) {
+                            ++rays;
+                            T.start(S, org, dir, tn, t1, st, cnt);
+                            done = false;
+                        }
+                    }
+                }
+                if (done) {
+                    if (kKind == 2) {
+                        bool visible = true;
+                        if (hit) {
+                            f3 df = sub3(hitPosition(S, T.h), B);
+                            float sd = dot3(df, df);
+                            float safeD = 10.0f * FLT_EPSILON * maxAbs(B);
+                            visible = sd <= safeD * safeD;
+                        }
+                        const uint32_t id = __float_as_uint(sq.b[e].w);
+                        const uint32_t slot = id / (uint32_t)lc, l = id - slot * (uint32_t)lc;
+                        vis[(uint64_t)l * nslots + slot] = visible ? 2 : 0;
+                    } else {
+                        float4 r; r.x = T.h.t; r.y = T.h.u; r.z = T.h.v; r.w = T.tnear;
+                        int2 id; id.x = hit ? T.h.inst : -1; id.y = T.h.prim | (T.h.tie ? (int)0x80000000 : 0);
+                        H.h[e] = r;
+                        H.id[e] = id;
+                    }
+                    busy = false;
+                }
+            }
+            if (__popcll(__ballot(busy)) < keep) break;
+        }
+    }
+    flushCounters(counters, rays, nCam, nLight, nShadow, cnt);
+}
+
+// ---------------------------------------------------------- shade light
+// Light-subpath vertex i (computePath with inv = false, BDPTRenderer.cpp:149-188).
+__global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_light(OmvSceneView S, RP P, uint64_t sample0, int i, WfState W, WfQueue in,
+                                                                const uint32_t* __restrict__ inCount, WfHits H, WfQueue out,
+                                                                uint32_t* __restrict__ outCount) {
+    const uint32_t gthreads = gridDim.x * blockDim.x;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t total = *inCount;
+    const uint64_t N = W.n;
+    for (uint32_t e = gtid; e < total; e += gthreads) {
+        const float4 dd = in.d[e];
+        const uint32_t slot = __float_as_uint(dd.w);
+        const int2 id = H.id[e];
+        bool emit = false;
+        f3 nPos = mk3(0.0f, 0.0f, 0.0f), nDir = nPos;
+        float nTnear = 0.0f;
+        if (id.x >= 0) {
+            const float4 hr = H.h[e];
+            Hit h; h.t = hr.x; h.u = hr.y; h.v = hr.z; h.inst = id.x; h.prim = id.y & 0x7FFFFFFF; h.tie = id.y < 0;
+            SP sp;
+            computeSurfacePoint(S, h, sp, false);
+            modifyFrame(S, sp);
+            Rng ls;
+            rngRestore(ls, P.seed, STREAM_LIGHT, sample0 + slot, W.lsn[slot]);
+            const OmvLight Lt = S.lights[W.lidx[slot]];
+            const f3 lightPos = ld3(Lt.pos);
+            const f3 lRad = mk3(Lt.intensity[0] * P.light_scale, Lt.intensity[1] * P.light_scale, Lt.intensity[2] * P.light_scale);
+            const f3 pDir = xyz(dd);
+            f3 outRay, brdf, erad;
+            matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, false, ls);
+            float4 lt = W.lthr[slot];
+            f3 of = xyz(lt);
+            float rD = lt.w;
+            float ct = fabsf(dot3(pDir, sp.n));
+            of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
+            if (i == 1) rD = attenuation(P, length3(sub3(sp.pos, lightPos)), Lt.range);
+            f3 outRad = mk3(of.x * lRad.x * rD, of.y * lRad.y * rD, of.z * lRad.z * rD);
+            if (!(intensity(outRad) < P.cut)) {
+                W.lv[((uint64_t)i * 6 + 0) * N + slot] = sp.pos.x;
+                W.lv[((uint64_t)i * 6 + 1) * N + slot] = sp.pos.y;
+                W.lv[((uint64_t)i * 6 + 2) * N + slot] = sp.pos.z;
+                W.lv[((uint64_t)i * 6 + 3) * N + slot] = outRad.x;
+                W.lv[((uint64_t)i * 6 + 4) * N + slot] = outRad.y;
+                W.lv[((uint64_t)i * 6 + 5) * N + slot] = outRad.z;
+                W.lplen[slot] = i + 1;
+                W.lthr[slot] = f4of(of, rD);
+                W.lsn[slot] = rngConsumed(ls);
+                if (i + 1 < W.lvCap) {
+                    emit = true;
+                    float delta = 3.0f * FLT_EPSILON * maxAbs(sp.pos);
+                    nPos = mk3(sp.pos.x + delta * outRay.x, sp.pos.y + delta * outRay.y, sp.pos.z + delta * outRay.z);
+                    nDir = outRay;
+                    nTnear = hr.w;
+                }
+            }
+        }
+        const uint32_t k = waveAppend(outCount, emit);
+        if (emit) {
+            out.o[k] = f4of(nPos, nTnear);
+            out.d[k] = f4of(nDir, __uint_as_float(slot));
+        }
+    }
+}
+
+// ---------------------------------------------------------- shade camera
+// Camera vertex i (computePath with inv = true, BDPTRenderer.cpp:149-176),
+// the connections of that vertex staged for the visibility trace
+// (BDPTRenderer.cpp:266-297), and the escape term (:251-258).
+template <bool kLights>
+__global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_cam(OmvSceneView S, RP P, uint64_t sample0, int i, WfState W, WfQueue in,
+                                                              const uint32_t* __restrict__ inCount, WfHits H, WfQueue out,
+                                                              uint32_t* __restrict__ outCount, WfShadowQ sq, uint32_t* __restrict__ sqCount,
+                                                              float* __restrict__ rad) {
+    const uint32_t gthreads = gridDim.x * blockDim.x;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t total = *inCount;
+    const uint64_t N = W.n;
+    const int q = P.q;
+    for (uint32_t e = gtid; e < total; e += gthreads) {
+        const float4 dd = in.d[e];
+        const uint32_t slot = __float_as_uint(dd.w);
+        const int2 id = H.id[e];
+        const f3 pDir = xyz(dd);
+        const float4 rc = W.radc[slot];
+        f3 radiance = xyz(rc);
+        int n = __float_as_int(rc.w);
+        f3 of = xyz(W.thr[slot]);
+        bool emit = false;
+        f3 nPos = mk3(0.0f, 0.0f, 0.0f), nDir = nPos;
+        float nTnear = 0.0f;
+        if (id.x < 0) {
+            // escape (BDPTRenderer.cpp:251-258); no connections this round
+            if (kLights) W.nconn[slot] = 0u;
+            if (S.bg_type != OMV_BG_NONE && n <= q) {
+                f3 bg = background(S, pDir);
+                bg = mk3(bg.x * of.x, bg.y * of.y, bg.z * of.z);
+                radiance = mk3(bg.x * 1.0f + radiance.x, bg.y * 1.0f + radiance.y, bg.z * 1.0f + radiance.z);
+            }
+            const f3 conn = xyz(W.conn[slot]);
+            rad[(uint64_t)slot * 3 + 0] = conn.x * 1.0f + radiance.x;
+            rad[(uint64_t)slot * 3 + 1] = conn.y * 1.0f + radiance.y;
+            rad[(uint64_t)slot * 3 + 2] = conn.z * 1.0f + radiance.z;
+            W.flags[slot] = 2u;
+        } else {
+            const float4 hr = H.h[e];
+            Hit h; h.t = hr.x; h.u = hr.y; h.v = hr.z; h.inst = id.x; h.prim = id.y & 0x7FFFFFFF; h.tie = id.y < 0;
+            SP sp;
+            computeSurfacePoint(S, h, sp, false);
+            modifyFrame(S, sp);
+            if (kLights) {
+                const int li = W.lidx[slot];
+                const OmvLight Lt = S.lights[li];
+                const f3 lightInt = ld3(Lt.intensity);
+                const int lEnd = min(W.lplen[slot], q - i + 1);
+                const f3 outV = neg3(pDir);
+                W.nconn[slot] = (uint32_t)max(lEnd, 0);
+                for (int l = 0; l < lEnd; ++l) {
+                    f3 lPos = mk3(W.lv[((uint64_t)l * 6 + 0) * N + slot], W.lv[((uint64_t)l * 6 + 1) * N + slot],
+                                  W.lv[((uint64_t)l * 6 + 2) * N + slot]);
+                    f3 lRad;
+                    f3 LC = normalize3(sub3(sp.pos, lPos));
+                    if (l == 0) {
+                        float lsc = attenuation(P, length3(sub3(sp.pos, lPos)), Lt.range);
+                        float k = P.light_scale * lsc;
+                        lRad = mk3(lightInt.x * k, lightInt.y * k, lightInt.z * k);
+                    } else {
+                        lRad = mk3(W.lv[((uint64_t)l * 6 + 3) * N + slot], W.lv[((uint64_t)l * 6 + 4) * N + slot],
+                                   W.lv[((uint64_t)l * 6 + 5) * N + slot]);
+                    }
+                    float c = fabsf(dot3(LC, sp.n));
+                    f3 lc = mk3(lRad.x * c * of.x, lRad.y * c * of.y, lRad.z * c * of.z);
+                    uint8_t v = 0;
+                    bool ray = false;
+                    f3 A = lPos, B = sp.pos, dir = mk3(0.0f, 0.0f, 0.0f);
+                    float tnear = 0.0f, tfar = 0.0f;
+                    if (!(intensity(lc) < P.cut)) {
+                        f3 brdf = matGetBrdf(S, P.x, sp, LC, outV);
+                        brdf = mk3(brdf.x * c, brdf.y * c, brdf.z * c);
+                        lc = mk3(brdf.x * of.x * lRad.x, brdf.y * of.y * lRad.y, brdf.z * of.z * lRad.z);
+                        if (!(intensity(lc) < P.cut)) {
+                            // isConnected(A = light vertex, B = camera vertex) up to the ray cast
+                            dir = sub3(B, A);
+                            float len = length3(dir);
+                            float safeD = 2.0f * 3.0f * FLT_EPSILON * maxAbs(A);
+                            if (len <= safeD) {
+                                v = 2;
+                            } else {
+                                dir = mk3(dir.x / len, dir.y / len, dir.z / len);
+                                A = mk3(A.x + 0.0f * dir.x, A.y + 0.0f * dir.y, A.z + 0.0f * dir.z);
+                                tnear = safeD;
+                                tfar = len + (0.1f * len > safeD ? 0.1f * len : safeD);
+                                v = 1;
+                                ray = true;
+                            }
+                            W.cc[((uint64_t)l * 3 + 0) * N + slot] = lc.x;
+                            W.cc[((uint64_t)l * 3 + 1) * N + slot] = lc.y;
+                            W.cc[((uint64_t)l * 3 + 2) * N + slot] = lc.z;
+                        }
+                    }
+                    W.vis[(uint64_t)l * N + slot] = v;
+                    const uint32_t k = waveAppend(sqCount, ray);
+                    if (ray) {
+                        sq.o[k] = f4of(A, tnear);
+                        sq.d[k] = f4of(dir, tfar);
+                        sq.b[k] = f4of(B, __uint_as_float(slot * (uint32_t)W.lc + (uint32_t)l));
+                    }
+                }
+            }
+            Rng cs;
+            rngRestore(cs, P.seed, STREAM_CAMERA, sample0 + slot, W.csn[slot]);
+            f3 outRay, brdf, erad;
+            matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, true, cs);
+            float ct = fabsf(dot3(outRay, sp.n));
+            of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
+            radiance = mk3(radiance.x + of.x * erad.x, radiance.y + of.y * erad.y, radiance.z + of.z * erad.z);
+            ++n;
+            W.csn[slot] = rngConsumed(cs);
+            W.thr[slot] = f4of(of, 0.0f);
+            W.radc[slot] = f4of(radiance, __int_as_float(n));
+            // continue unless the loop ends (i == B) or an exact shortcut stops it
+            const bool zero = of.x == 0.0f && of.y == 0.0f && of.z == 0.0f;
+            const bool cont = i < P.camera_bounces && !zero && (S.any_emissive || n <= q);
+            if (cont) {
+                emit = true;
+                float delta = 3.0f * FLT_EPSILON * maxAbs(sp.pos);
+                nPos = mk3(sp.pos.x + delta * outRay.x, sp.pos.y + delta * outRay.y, sp.pos.z + delta * outRay.z);
+                nDir = outRay;
+                nTnear = hr.w;
+            } else if (kLights) {
+                W.flags[slot] = 1u;   // written by the connect kernel of this round
+            } else {
+                const f3 conn = xyz(W.conn[slot]);
+                rad[(uint64_t)slot * 3 + 0] = conn.x * 1.0f + radiance.x;
+                rad[(uint64_t)slot * 3 + 1] = conn.y * 1.0f + radiance.y;
+                rad[(uint64_t)slot * 3 + 2] = conn.z * 1.0f + radiance.z;
+                W.flags[slot] = 2u;
+            }
+        }
+        const uint32_t k = waveAppend(outCount, emit);
+        if (emit) {
+            out.o[k] = f4of(nPos, nTnear);
+            out.d[k] = f4of(nDir, __uint_as_float(slot));
+        }
+    }
+}
+
+// -------------------------------------------------------------- connect
+// Adds the visible connections of this round in light-vertex order and
+// writes the sample of paths that ended this round.
+__global__ __launch_bounds__(OMV_BLOCK) void omv_wf_connect(WfState W, WfQueue in, const uint32_t* __restrict__ inCount,
+                                                            float* __restrict__ rad) {
+    const uint32_t gthreads = gridDim.x * blockDim.x;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t total = *inCount;
+    const uint64_t N = W.n;
+    for (uint32_t e = gtid; e < total; e += gthreads) {
+        const uint32_t slot = __float_as_uint(in.d[e].w);
+        const uint32_t fl = W.flags[slot];
+        if (fl == 2u) continue;
+        const uint32_t nc = W.nconn[slot];
+        f3 conn = xyz(W.conn[slot]);
+        for (uint32_t l = 0; l < nc; ++l) {
+            if (W.vis[(uint64_t)l * N + slot] != 2) continue;
+            f3 lc = mk3(W.cc[((uint64_t)l * 3 + 0) * N + slot], W.cc[((uint64_t)l * 3 + 1) * N + slot],
+                        W.cc[((uint64_t)l * 3 + 2) * N + slot]);
+            conn = mk3(lc.x + conn.x, lc.y + conn.y, lc.z + conn.z);
+        }
+        W.conn[slot] = f4of(conn, 0.0f);
+        if (fl == 1u) {
+            const f3 radiance = xyz(W.radc[slot]);
+            rad[(uint64_t)slot * 3 + 0] = conn.x * 1.0f + radiance.x;
+            rad[(uint64_t)slot * 3 + 1] = conn.y * 1.0f + radiance.y;
+            rad[(uint64_t)slot * 3 + 2] = conn.z * 1.0f + radiance.z;
+            W.flags[slot] = 2u;
+        }
+    }
+}
+
+// Paths that end when the camera loop runs out (i == B) are written by
+// shade/connect above; nothing is left over after the last round.
+
+__global__ void omv_wf_fill_u32(uint32_t* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// ------------------------------------------------------------ launchers
+namespace omv {
+
+namespace {
+template <class K>
+int occ(K k, size_t lds) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, OMV_BLOCK, lds) != hipSuccess || n < 1) n = 1;
+    return n;
+}
+constexpr size_t kAlign = 256;
+size_t up(size_t b) { return (b + kAlign - 1) & ~(kAlign - 1); }
+}  // namespace
+
+size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights) {
+    const size_t N = nslots;
+    size_t b = 6 * up(N * 16)                        // three ray queues
+               + up(N * 16) + up(N * 8)              // hits
+               + 3 * up(N * 16) + 3 * up(N * 4);     // thr, radc, conn, csn, flags, nconn
+    if (lights)
+        b += up(N * 16) + 3 * up(N * 4) + up(N * (size_t)lvCap * 24) + up(N * (size_t)lc * 12) + up(N * (size_t)lc) +
+             3 * up(N * (size_t)lc * 16);
+    return b;
+}
+
+int wavefrontCounterCount(int B, int lvCap) { return 2 * lvCap + 4 * B + 16; }
+
+void wavefrontBlocksPerCU(int out[4]) {
+    const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
+    out[0] = std::min(occ(omv_wf_trace<0>, lds), occ(omv_wf_trace_pt<2>, lds));
+    out[1] = occ(omv_wf_shade_cam<true>, 0);
+    out[2] = occ(omv_wf_shade_light, 0);
+    out[3] = occ(omv_wf_gen<true>, 0);
+}
+
+// One chunk of whole pixels: samples [sample0, sample0 + nslots).
+hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
+    const OmvSceneView& S = L.S;
+    const RP P = toRP(L.p);
+    const bool lights = S.num_lights > 0;
+    const uint32_t n = L.nslots;
+    const size_t N = n;
+    uint8_t* cur = static_cast<uint8_t*>(L.arena);
+    auto take = [&](size_t bytes) { uint8_t* p = cur; cur += up(bytes); return p; };
+    auto queue = [&]() { WfQueue q; q.o = reinterpret_cast<float4*>(take(N * 16)); q.d = reinterpret_cast<float4*>(take(N * 16)); return q; };
+    WfQueue qa = queue(), qb = queue(), qc = queue();
+    WfHits H;
+    H.h = reinterpret_cast<float4*>(take(N * 16));
+    H.id = reinterpret_cast<int2*>(take(N * 8));
+    WfState W;
+    std::memset(&W, 0, sizeof(W));
+    W.n = n; W.lvCap = L.lvCap; W.lc = L.lc;
+    W.thr = reinterpret_cast<float4*>(take(N * 16));
+    W.radc = reinterpret_cast<float4*>(take(N * 16));
+    W.conn = reinterpret_cast<float4*>(take(N * 16));
+    W.csn = reinterpret_cast<uint32_t*>(take(N * 4));
+    W.flags = reinterpret_cast<uint32_t*>(take(N * 4));
+    W.nconn = reinterpret_cast<uint32_t*>(take(N * 4));
+    WfShadowQ sq;
+    std::memset(&sq, 0, sizeof(sq));
+    if (lights) {
+        W.lthr = reinterpret_cast<float4*>(take(N * 16));
+        W.lsn = reinterpret_cast<uint32_t*>(take(N * 4));
+        W.lidx = reinterpret_cast<int*>(take(N * 4));
+        W.lplen = reinterpret_cast<int*>(take(N * 4));
+        W.lv = reinterpret_cast<float*>(take(N * (size_t)L.lvCap * 24));
+        W.cc = reinterpret_cast<float*>(take(N * (size_t)L.lc * 12));
+        W.vis = reinterpret_cast<uint8_t*>(take(N * (size_t)L.lc));
+        sq.o = reinterpret_cast<float4*>(take(N * (size_t)L.lc * 16));
+        sq.d = reinterpret_cast<float4*>(take(N * (size_t)L.lc * 16));
+        sq.b = reinterpret_cast<float4*>(take(N * (size_t)L.lc * 16));
+    }
+    // queue counters: light round i reads lightCnt[i] (i = 1 .. lvCap-1) and
+    // appends to lightCnt[i+1]; camera round i reads camCnt[i] (i = 1 .. B),
+    // appends to camCnt[i+1] and stages connection rays in shCnt[i]
+    const int B = L.p.camera_bounces;
+    const int ncnt = wavefrontCounterCount(B, L.lvCap);
+    uint32_t* lightCnt = L.qcount;
+    uint32_t* camCnt = lightCnt + (L.lvCap + 2);
+    uint32_t* shCnt = camCnt + (B + 2);
+    uint32_t* fetchCnt = shCnt + (B + 2);            // one per persistent traversal launch
+    int nf = 0;
+    const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
+    const dim3 blk(OMV_BLOCK);
+    auto mark = [&]() {
+        if (L.traceEv && L.traceEvUsed && *L.traceEvUsed < L.traceEvCap) (void)hipEventRecord(L.traceEv[(*L.traceEvUsed)++], stream);
+    };
+    hipLaunchKernelGGL(omv_wf_fill_u32, dim3((ncnt + 63) / 64), dim3(64), 0, stream, L.qcount, ncnt);
+    // camera rays of round 1 go to qa, light rays of round 1 to qb
+    if (lights)
+        hipLaunchKernelGGL(omv_wf_gen<true>, dim3(L.blocksGen), blk, 0, stream, S, L.cam, P, L.p.samples_per_pixel, L.sample0, W, qa, qb,
+                           camCnt + 1, lightCnt + 1, L.rad);
+    else
+        hipLaunchKernelGGL(omv_wf_gen<false>, dim3(L.blocksGen), blk, 0, stream, S, L.cam, P, L.p.samples_per_pixel, L.sample0, W, qa, qb,
+                           camCnt + 1, lightCnt + 1, L.rad);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (lights) {
+        WfQueue in = qb, out = qc;
+        for (int i = 1; i < L.lvCap; ++i) {
+            mark();
+            if (L.traceKeep > 0)
+                hipLaunchKernelGGL(omv_wf_trace_pt<1>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, lightCnt + i, fetchCnt + (nf++),
+                                   L.traceKeep, H, W.vis, n, L.lc, L.gstack, L.counters);
+            else
+                hipLaunchKernelGGL(omv_wf_trace<1>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, lightCnt + i, H, W.vis, n, L.lc, L.gstack,
+                                   L.counters);
+            mark();
+            hipLaunchKernelGGL(omv_wf_shade_light, dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, lightCnt + i, H, out,
+                               lightCnt + i + 1);
+            std::swap(in, out);
+        }
+    }
+    WfQueue in = qa, out = qb;
+    for (int i = 1; i <= B; ++i) {
+        mark();
+        if (L.traceKeep > 0)
+            hipLaunchKernelGGL(omv_wf_trace_pt<0>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, camCnt + i, fetchCnt + (nf++), L.traceKeep,
+                               H, W.vis, n, L.lc, L.gstack, L.counters);
+        else
+            hipLaunchKernelGGL(omv_wf_trace<0>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, camCnt + i, H, W.vis, n, L.lc, L.gstack,
+                               L.counters);
+        mark();
+        if (lights) {
+            hipLaunchKernelGGL(omv_wf_shade_cam<true>, dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i, H, out,
+                               camCnt + i + 1, sq, shCnt + i, L.rad);
+            mark();
+            if (L.traceKeep > 0)
+                hipLaunchKernelGGL(omv_wf_trace_pt<2>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, shCnt + i, fetchCnt + (nf++),
+                                   L.traceKeep, H, W.vis, n, L.lc, L.gstack, L.counters);
+            else
+                hipLaunchKernelGGL(omv_wf_trace<2>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, shCnt + i, H, W.vis, n, L.lc, L.gstack,
+                                   L.counters);
+            mark();
+            hipLaunchKernelGGL(omv_wf_connect, dim3(L.blocksShade), blk, 0, stream, W, in, camCnt + i, L.rad);
+        } else {
+            hipLaunchKernelGGL(omv_wf_shade_cam<false>, dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i, H, out,
+                               camCnt + i + 1, sq, shCnt + i, L.rad);
+        }
+        std::swap(in, out);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace omv

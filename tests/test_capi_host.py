@@ -159,3 +159,15 @@ def test_batched_lightbox_workload(wl):
     assert np.allclose(np.sort(np.abs(corners - c), axis=0), np.sort(np.abs(np.array([hi - c] * 8)), axis=0), rtol=1e-5)
     kinds = {a.materials[0][0] for a in assets}
     assert kinds == {"om_nk", "pbr"}
+
+
+def test_schedule_switch(pkg):
+    L = pkg.lib()
+    r = C.c_void_p()
+    assert L.omv_renderer_create(C.byref(r)) == 0
+    v = C.c_int(-1)
+    assert L.omv_renderer_get_schedule(r, C.byref(v)) == 0 and v.value == pkg.SCHEDULE_WAVEFRONT
+    assert L.omv_renderer_set_schedule(r, 7) == -1
+    assert L.omv_renderer_set_schedule(r, pkg.SCHEDULE_PER_SAMPLE) == 0
+    assert L.omv_renderer_get_schedule(r, C.byref(v)) == 0 and v.value == pkg.SCHEDULE_PER_SAMPLE
+    assert L.omv_renderer_destroy(r) == 0

@@ -1,0 +1,86 @@
+"""The two GPU schedules of omv_renderer_render give the same film.
+
+The wavefront schedule (ray queues, one traversal kernel per bounce) and the
+per-sample schedule (one lane per sample) evaluate the same per-sample
+arithmetic in the same order, so films must be equal bit for bit, and the
+ray counts (camera, light, connection, total) must be equal too.  Cases cover
+lights on/off, HDRI, masked materials (re-trace loop), the missing material,
+the OM rough-conductor extension, instancing, and parameter edges
+(B = 0, b = 0, q = 1, spp = 1).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _both(pkg, desc, view, **params):
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = pkg.Camera(view.width, view.height, view.position, view.forward, view.up, view.yfov)
+    out = []
+    for sched in (pkg.SCHEDULE_WAVEFRONT, pkg.SCHEDULE_PER_SAMPLE):
+        r = pkg.Renderer(pkg.default_params(**params))
+        r.set_scene(s)
+        r.set_schedule(sched)
+        assert r.schedule() == sched
+        img = r.render(cam)
+        out.append((img, r.stats()))
+    return out
+
+
+def _same(out):
+    (a, sa), (b, sb) = out
+    assert np.all(np.isfinite(a))
+    assert np.array_equal(a, b), f"films differ at {np.argwhere(a != b)[:5].tolist()}"
+    for k in ("samples", "rays", "camera_rays", "light_rays", "shadow_rays", "nodes_visited", "tris_tested"):
+        assert sa[k] == sb[k], (k, sa[k], sb[k])
+    assert sa["stack_overflows"] == 0
+
+
+CASES = {
+    "mixed_lights_hdr_masked": (dict(hdr=True, lights=True, masked=True), dict(samples_per_pixel=4, camera_bounces=5, light_bounces=4,
+                                                                              max_path_length=6, seed=3)),
+    "mixed_lights_only": (dict(hdr=False, lights=True), dict(samples_per_pixel=3, camera_bounces=4, light_bounces=4, max_path_length=5, seed=9)),
+    "mixed_hdr_only_missing": (dict(hdr=True, lights=False, missing=True), dict(samples_per_pixel=3, camera_bounces=6, light_bounces=2,
+                                                                               max_path_length=6, seed=1)),
+    "mixed_om_rough": (dict(hdr=True, lights=True), dict(samples_per_pixel=2, camera_bounces=4, light_bounces=3, max_path_length=5, seed=2,
+                                                         extensions=1, om_roughness=0.25)),
+    "edge_B0": (dict(hdr=True, lights=True), dict(samples_per_pixel=2, camera_bounces=0, light_bounces=3, max_path_length=4, seed=4)),
+    "edge_b0": (dict(hdr=True, lights=True), dict(samples_per_pixel=2, camera_bounces=3, light_bounces=0, max_path_length=4, seed=5)),
+    "edge_q1": (dict(hdr=True, lights=True), dict(samples_per_pixel=2, camera_bounces=3, light_bounces=3, max_path_length=1, seed=6)),
+    "edge_spp1_deep": (dict(hdr=True, lights=True, masked=True), dict(samples_per_pixel=1, camera_bounces=10, light_bounces=10,
+                                                                      max_path_length=8, seed=7)),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_schedules_agree_mixed(pkg, wl, name):
+    kw, params = CASES[name]
+    desc, view = wl.mixed_test_scene(**kw)
+    view.width, view.height = 64, 48
+    _same(_both(pkg, desc, view, **params))
+
+
+def test_schedules_agree_torus_knot(pkg, wl):
+    desc, view, params = wl.hdri_torus_knot(width=96, height=96, spp=4, tex=128, hdr_w=256, hdr_h=128, segments=200, tube=16)
+    _same(_both(pkg, desc, view, **params))
+
+
+def test_schedules_agree_icosphere_rough(pkg, wl):
+    desc, view, params = wl.plain_bdpt_icosphere()
+    view.width, view.height = 64, 64
+    _same(_both(pkg, desc, view, **dict(params, samples_per_pixel=4, extensions=1, om_roughness=0.3)))
+
+
+def test_schedules_agree_instanced(pkg, wl):
+    desc, view, params = wl.large_instanced(width=120, height=68, spp=2, count=24, tris_per=2000, hdr_w=256, hdr_h=128)
+    _same(_both(pkg, desc, view, **params))
+
+
+def test_wavefront_multi_chunk(pkg, wl):
+    """More samples than one wavefront chunk holds (2^22 without lights):
+    chunk boundaries fall on whole pixels and the film is unchanged."""
+    desc, view = wl.mixed_test_scene(hdr=True, lights=False)
+    view.width, view.height = 512, 520
+    _same(_both(pkg, desc, view, samples_per_pixel=16, camera_bounces=3, light_bounces=2, max_path_length=4, seed=8))
