@@ -280,7 +280,7 @@ typedef struct omv_render_tuning {
     int trace_blocks_per_cu;  /* resident traversal workgroups per CU
                                  (0 = occupancy limit)                          */
     int chunk_samples;        /* samples per wavefront chunk, rounded down to
-                                 whole pixels (0 = default: 2^22, 2^21 with
+                                 whole pixels (0 = default: 2^24, 2^23 with
                                  point lights)                                  */
 } omv_render_tuning;
 int omv_renderer_set_tuning(omv_renderer* r, const omv_render_tuning* t);

@@ -13,8 +13,12 @@ namespace omv {
 
 namespace {
 constexpr uint64_t kChunkSamples = 1ull << 24;   // per-sample radiance buffer: 192 MiB
-constexpr uint64_t kWfSlots = 1ull << 22;         // wavefront slots per chunk (~0.7 GiB of state)
-constexpr uint64_t kWfSlotsLights = 1ull << 21;   // with lights (light vertices + connections)
+// Wavefront samples per chunk.  Every chunk pays the latency-bound tail of
+// its last bounce rounds (a few long paths), so chunks are as large as HBM
+// comfortably allows: ~2.9 GB of state at 2^24 samples, ~7 GB at 2^23 with
+// point lights (light vertices, staged connections, connection rays).
+constexpr uint64_t kWfSlots = 1ull << 24;
+constexpr uint64_t kWfSlotsLights = 1ull << 23;
 int hipFail(hipError_t e, const char* what) {
     setError(std::string(what) + ": " + hipGetErrorString(e));
     return OMV_E_DEVICE;

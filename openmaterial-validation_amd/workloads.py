@@ -365,6 +365,44 @@ def large_instanced(width=1920, height=1080, spp=16, depth=8, count=200, tris_pe
     return d, view, params
 
 
+def material_fit(width=256, height=256, spp=32, depth=8, tex=512, hdr_w=2048, hdr_h=1024, seed_target=5000, seed_init=5001,
+                 segments=1000, tube=50):
+    """Differentiable material fit: the HDRI torus-knot scene at width x
+    height with fp32 base-colour (texture 0, RGBA) and metallic-roughness
+    (texture 1: G roughness, B metallic) maps of tex^2 texels, plus an
+    OpenMATERIAL mirror sphere (material 1) beside the knot, which the HDRI
+    makes observable through escape paths.  Parameters come from value noise
+    seeded by seed_target (hidden target) and seed_init (initial guess).
+    Returns (target SceneDesc, initial SceneDesc, View, params, ids) with
+    ids = {"base_color": 0, "metal_rough": 1, "om": 1}."""
+    v, f, uv = torus_knot(segments=segments, tube=tube)
+    sv, sf = icosphere(3, 0.45)
+    sky = hdri(hdr_w, hdr_h)
+
+    def desc(seed):
+        rng = np.random.default_rng(seed)
+        base = np.stack([0.05 + 0.9 * value_noise(tex, tex, seed * 10 + c) for c in range(3)] + [np.ones((tex, tex))], -1)
+        rough = 0.05 + 0.95 * value_noise(tex, tex, seed * 10 + 5)
+        metal = 0.05 + 0.95 * value_noise(tex, tex, seed * 10 + 6)
+        mr = np.stack([np.ones_like(rough), rough, metal, np.ones_like(rough)], -1)
+        d = SceneDesc()
+        d.textures.append(("f32", base.astype(np.float32)))
+        d.textures.append(("f32", mr.astype(np.float32)))
+        d.materials.append(("pbr", dict(base_color=(1.0, 1.0, 1.0, 1.0), metallic=1.0, roughness=1.0, textures={0: 0, 1: 1})))
+        n, k = rng_om(int(rng.integers(1 << 30)))
+        d.materials.append(("om_nk", n, k))
+        d.meshes.append({"vertices": v, "faces": f, "mat_id": 0, "uvs": [uv]})
+        d.meshes.append({"vertices": sv, "faces": sf, "mat_id": 1})
+        d.instances.append((identity(), 0))
+        d.instances.append((trs((1.75, 0.55, 0.6)), 1))
+        d.background = ("hdr", sky, HDR_ROTATION, HDR_SCALE)
+        return d
+
+    view = View(width, height, (0.3, 0.5, 5.2), (-0.05, -0.1, -1.0), yfov=42.0)
+    params = dict(camera_bounces=depth, light_bounces=depth, max_path_length=depth, samples_per_pixel=spp, seed=0)
+    return desc(seed_target), desc(seed_init), view, params, {"base_color": 0, "metal_rough": 1, "om": 1}
+
+
 def lightbox_views(vertices, tm=None):
     """The eight lightbox viewpoints of main.cpp:434-457: for each corner p of
     the scene bounding box with centre C, the camera sits at 2p - C and looks

@@ -81,6 +81,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -418,13 +419,11 @@ static int getIor(const std::vector<IorTable>& tabs, float T, float wl, float& n
 static const float kRgbWl[3] = {6.50000004e-07f, 5.10000007e-07f, 4.40000008e-07f};
 
 /* ------------------------------------------------------------ textures */
-/* BitmapTexture::getPixCoords + texture (BitmapTexture.cpp:181-234) */
-template <class R>
-static void texLookup(const Texture& t, float u, float v, R out[4]) {
-    if (t.w <= 0 || t.h <= 0 || (!t.f32 && t.rgba.empty()) || (t.f32 && t.rgbaf.empty())) {
-        out[0] = R(0.5f); out[1] = R(0.5f); out[2] = R(0.5f); out[3] = R(1.0f);
-        return;
-    }
+/* BitmapTexture::getPixCoords (BitmapTexture.cpp:181-208): the four texels
+ * and bilinear weights of (u, v), texel-centre wrap addressing.  Returns
+ * false for a missing image (texture() then returns 0.5 grey). */
+static bool texAddr(const Texture& t, float u, float v, int id[4], float w[4]) {
+    if (t.w <= 0 || t.h <= 0 || (!t.f32 && t.rgba.empty()) || (t.f32 && t.rgbaf.empty())) return false;
     if (!std::isfinite(u)) u = 0.0f;   // guard: the reference indexes out of bounds here
     if (!std::isfinite(v)) v = 0.0f;
     const float W = (float)t.w, H = (float)t.h;
@@ -436,22 +435,31 @@ static void texLookup(const Texture& t, float u, float v, R out[4]) {
     float kui = u - (float)iu, kvi = v - (float)iv;
     float ku = 1.0f - kui, kv = 1.0f - kvi;
     int l = iu % t.w, r = (iu + 1) % t.w, b = iv % t.h, tt = (iv + 1) % t.h;
-    int id[4] = {b * t.w + l, b * t.w + r, tt * t.w + l, tt * t.w + r};
-    float w[4] = {ku * kv, kui * kv, ku * kvi, kui * kvi};
+    id[0] = b * t.w + l; id[1] = b * t.w + r; id[2] = tt * t.w + l; id[3] = tt * t.w + r;
+    w[0] = ku * kv; w[1] = kui * kv; w[2] = ku * kvi; w[3] = kui * kvi;
     for (int i = 0; i < 4; ++i) id[i] = std::min(std::max(id[i], 0), t.w * t.h - 1);
+    return true;
+}
+
+/* BitmapTexture::texture (BitmapTexture.cpp:210-234).  The weighted sum is
+ * formed in R (float for the float oracle, double for the FP64 oracle). */
+template <class R>
+static void texLookup(const Texture& t, float u, float v, R out[4]) {
+    int id[4];
+    float w[4];
+    if (!texAddr(t, u, v, id, w)) {
+        out[0] = R(0.5f); out[1] = R(0.5f); out[2] = R(0.5f); out[3] = R(1.0f);
+        return;
+    }
     if (!t.f32) {
-        const float s = 1.0f / 255.0f;
-        for (int i = 0; i < 4; ++i) {
-            float acc = (float)t.rgba[(size_t)id[0] * 4 + i] * w[0] * s + (float)t.rgba[(size_t)id[1] * 4 + i] * w[1] * s +
-                        (float)t.rgba[(size_t)id[2] * 4 + i] * w[2] * s + (float)t.rgba[(size_t)id[3] * 4 + i] * w[3] * s;
-            out[i] = R(acc);
-        }
+        const R s = R(1.0f / 255.0f);
+        for (int i = 0; i < 4; ++i)
+            out[i] = R(t.rgba[(size_t)id[0] * 4 + i]) * R(w[0]) * s + R(t.rgba[(size_t)id[1] * 4 + i]) * R(w[1]) * s +
+                     R(t.rgba[(size_t)id[2] * 4 + i]) * R(w[2]) * s + R(t.rgba[(size_t)id[3] * 4 + i]) * R(w[3]) * s;
     } else {
-        for (int i = 0; i < 4; ++i) {
-            float acc = t.rgbaf[(size_t)id[0] * 4 + i] * w[0] + t.rgbaf[(size_t)id[1] * 4 + i] * w[1] +
-                        t.rgbaf[(size_t)id[2] * 4 + i] * w[2] + t.rgbaf[(size_t)id[3] * 4 + i] * w[3];
-            out[i] = R(acc);
-        }
+        for (int i = 0; i < 4; ++i)
+            out[i] = R(t.rgbaf[(size_t)id[0] * 4 + i]) * R(w[0]) + R(t.rgbaf[(size_t)id[1] * 4 + i]) * R(w[1]) +
+                     R(t.rgbaf[(size_t)id[2] * 4 + i]) * R(w[2]) + R(t.rgbaf[(size_t)id[3] * 4 + i]) * R(w[3]);
     }
 }
 
@@ -852,8 +860,9 @@ template <class R> struct Bsdf {
     R color[4];
     bool transmissive;
     R metal, rough, alpha, eta;
+    int lobe;   /* lobe sampleIndirect took: 0 none, 1 diffuse, 2 reflection, 3 transmission */
     Bsdf() { I = O = N = H = T = mk(R(0), R(0), R(0)); color[0] = color[1] = color[2] = color[3] = R(1);
-             transmissive = false; metal = R(0.5f); rough = R(0); alpha = R(0.25f); eta = R(1); }
+             transmissive = false; metal = R(0.5f); rough = R(0); alpha = R(0.25f); eta = R(1); lobe = 0; }
 };
 
 /* BSDFModel::evaluate (BRDFModel.h:139-207) */
@@ -926,12 +935,14 @@ static V3<R> sampleIndirect(Bsdf<R>& d, Uniforms& U) {
             d.H = normalize3(toBasis(b0, b1, d.N, rd));
             if (refract3(d.I, d.H, d.eta, d.O) && dot3(d.N, d.O) < R(0)) {
                 d.O = normalize3(d.O);
+                d.lobe = 3;
                 return bsdfEvaluate(d.I, d.O, d.N, d.color, d.metal, d.alpha, false, true, d.H, P);
             }
             return z;
         }
         V3<R> rd = cosineSqrtHemisphere<R>(U);
         d.O = normalize3(toBasis(b0, b1, d.N, rd));
+        d.lobe = 1;
         return bsdfEvaluate(d.I, d.O, d.N, d.color, d.metal, d.alpha, false, false, d.H, P);
     }
     V3<R> rd = ggxHemisphere(U, std::pow(d.alpha, R(2)));
@@ -939,9 +950,95 @@ static V3<R> sampleIndirect(Bsdf<R>& d, Uniforms& U) {
     d.O = reflect3(d.I, d.H);
     if (dot3(d.N, d.O) > R(0)) {
         d.O = normalize3(d.O);
+        d.lobe = 2;
         return bsdfEvaluate(d.I, d.O, d.N, d.color, d.metal, d.alpha, true, false, d.H, P);
     }
     return z;
+}
+
+/* ------------------------------------------- detached lobe values (adjoint)
+ * With the sampled lobe and direction held fixed, the estimator's weight of
+ * a PBR bounce is w_ref(theta0) * f(theta) / f(theta0), where f is the
+ * sampled lobe's integrand up to factors that do not depend on the material
+ * (the reference's weight times its sampling density): diffuse
+ * (1-F)(1-metal)color, reflection F*D*G, transmission (1-F)*D*G, with the
+ * geometry bsdfEvaluate uses.  The gradient of the bounce is the weight
+ * times d ln f / d(color_c, metal, alpha). */
+template <class R> struct LobeGeo { int lobe; R p5, NH, VN, LN; };
+
+template <class R>
+static LobeGeo<R> lobeGeo(const Bsdf<R>& d) {
+    LobeGeo<R> g;
+    g.lobe = d.lobe;
+    V3<R> H = d.lobe == 3 ? d.H : normalize3(sub(d.O, d.I));
+    g.VN = clampv(std::abs(dot3(d.I, d.N)), R(0.0001f), R(1));
+    g.LN = clampv(std::abs(dot3(d.O, d.N)), R(0.0001f), R(1));
+    g.NH = std::abs(dot3(d.N, H));
+    R VH = std::abs(dot3(d.I, H));
+    g.p5 = std::pow(clamp01(R(1) - VH), R(5));
+    return g;
+}
+
+template <class R>
+static void lobeValue(const LobeGeo<R>& g, const R color[3], R metal, R alpha, R f[3]) {
+    R D = R(1), G = R(1);
+    if (g.lobe >= 2) { D = ggxPdf(g.NH, alpha); G = smithG2(smithG1(g.VN, alpha), smithG1(g.LN, alpha)); }
+    for (int c = 0; c < 3; ++c) {
+        R F0 = lerpv(R(0.04f), color[c], metal);
+        R F = lerpv(g.p5, R(1), F0);
+        if (g.lobe == 1) f[c] = (R(1) - F) * (R(1) - metal) * color[c];
+        else if (g.lobe == 2) f[c] = F * D * G;
+        else if (g.lobe == 3) f[c] = (R(1) - F) * D * G;
+        else f[c] = R(0);
+    }
+}
+
+/* d ln G1 / d alpha of smithG1 */
+template <class R>
+static R dlnSmithG1(R c, R alpha) {
+    if (!(alpha > R(0) || c < R(0.99f))) return R(0);
+    R a = c / (alpha * std::sqrt(R(1) - c * c));
+    R ai2 = std::pow(a, R(-2));
+    R lam = R(0.5f) * (std::sqrt(R(1) + ai2) - R(1));
+    R dlam = ai2 / (R(2) * alpha * std::sqrt(R(1) + ai2));
+    return -dlam / (R(1) + lam);
+}
+
+/* Planted mistakes for the tests that show the adjoint checks have teeth:
+ * bit 0 drops d ln D / d alpha, bit 1 drops the base-colour term of F. */
+static int gAdjPlant = 0;
+
+/* d ln f_c / d(color_c, metal, alpha) for the three lobes; 0 where f_c == 0 */
+template <class R>
+static void lobeDlog(const LobeGeo<R>& g, const R color[3], R metal, R alpha, R dColor[3], R dMetal[3], R dAlpha[3]) {
+    R dlnDG = R(0);
+    if (g.lobe >= 2) {
+        R c2 = g.NH * g.NH;
+        R L = lerpv(R(1), alpha * alpha, c2);
+        R dlnD = (gAdjPlant & 1) ? R(0) : R(2) / alpha - R(4) * alpha * c2 / L;
+        R x = smithG1(g.VN, alpha), y = smithG1(g.LN, alpha);
+        R dx = x * dlnSmithG1(g.VN, alpha), dy = y * dlnSmithG1(g.LN, alpha);
+        R den = x + y - x * y;
+        R dlnG = dx / x + dy / y - (dx + dy - dx * y - x * dy) / den;
+        dlnDG = dlnD + dlnG;
+    }
+    R f[3];
+    lobeValue(g, color, metal, alpha, f);
+    for (int c = 0; c < 3; ++c) {
+        dColor[c] = dMetal[c] = dAlpha[c] = R(0);
+        if (!(f[c] != R(0))) continue;
+        R F0 = lerpv(R(0.04f), color[c], metal);
+        R F = lerpv(g.p5, R(1), F0);
+        R dFdC = (gAdjPlant & 2) ? R(0) : (R(1) - g.p5) * metal, dFdM = (R(1) - g.p5) * (color[c] - R(0.04f));
+        if (g.lobe == 1) {
+            dColor[c] = -dFdC / (R(1) - F) + R(1) / color[c];
+            dMetal[c] = -dFdM / (R(1) - F) - R(1) / (R(1) - metal);
+        } else if (g.lobe == 2) {
+            dColor[c] = dFdC / F; dMetal[c] = dFdM / F; dAlpha[c] = dlnDG;
+        } else {
+            dColor[c] = -dFdC / (R(1) - F); dMetal[c] = -dFdM / (R(1) - F); dAlpha[c] = dlnDG;
+        }
+    }
 }
 
 /* fresnelReflection (OpenMaterial.cpp:20-38), std::complex as the reference */
@@ -960,6 +1057,29 @@ static void fresnel(R n, R k, R cosT, R& Rp, R& Rs) {
     C ds = cosT + std::sqrt(N2 - s2);
     C rs = ns / ds;
     Rs = std::real(rs * std::conj(rs));
+}
+
+/* d(Rp, Rs)/d(n, k) of fresnel() at fixed cos: with N2 = (n + ik)^2 and
+ * u = sqrt(N2 - s^2), d rp/d N2 = c (2u^2 - N2) / (u (N2 c + u)^2),
+ * d rs/d N2 = -c / (u (c + u)^2), d N2/dn = 2N, d N2/dk = 2iN, and
+ * d|r|^2 = 2 Re(conj(r) dr). */
+template <class R>
+static void fresnelGrad(R n, R k, R cosT, R& dRpdn, R& dRpdk, R& dRsdn, R& dRsdk) {
+    typedef std::complex<R> C;
+    C N(n, k);
+    C N2 = N * N;
+    cosT = std::abs(cosT);
+    const R c2 = cosT * cosT, s2 = R(1) - c2;
+    C u = std::sqrt(N2 - s2);
+    C a = N2 * cosT;
+    C rp = (a - u) / (a + u), rs = (C(cosT) - u) / (C(cosT) + u);
+    C drp = cosT * (R(2) * u * u - N2) / (u * (a + u) * (a + u));
+    C drs = -cosT / (u * (C(cosT) + u) * (C(cosT) + u));
+    C dn = R(2) * N, dk = C(0, 2) * N;
+    dRpdn = R(2) * std::real(std::conj(rp) * drp * dn);
+    dRpdk = R(2) * std::real(std::conj(rp) * drp * dk);
+    dRsdn = R(2) * std::real(std::conj(rs) * drs * dn);
+    dRsdk = R(2) * std::real(std::conj(rs) * drs * dk);
 }
 
 /* ---------------------------------------------------------- materials */
@@ -1175,28 +1295,60 @@ static void modifyFrame(const Scene& S, SurfacePoint<R>& sp) {
     else applyTextureNormal(sp, mk(R(0), R(0), R(1)));
 }
 
-/* RenderMaterial::getRayAndBrdf dispatch (computePath, BDPTRenderer.cpp:158-159) */
+/* What the adjoint needs from one PBR bounce: the sampled Bsdf state and the
+ * material inputs before occlusion. */
+template <class R> struct BounceInfo {
+    int kind = 0;            /* 0 other, 1 PBR, 2 OM mirror */
+    Bsdf<R> b;
+    R colorPre[3] = {R(0), R(0), R(0)}, omr[3] = {R(0), R(0), R(0)};
+    R fCos = R(0);
+};
+
+/* RenderMaterial::getRayAndBrdf dispatch (computePath, BDPTRenderer.cpp:158-159).
+ * E (optional): the scene whose material parameters the weight is evaluated
+ * with, sampling staying with S ("detached" sampling, used for finite
+ * differences of the adjoint): PBR weights are scaled by f(E)/f(S) of the
+ * sampled lobe, OM mirror weights use E's (n, k). */
 template <class R>
 static void matGetRayAndBrdf(const Scene& S, const Ext& X, const SurfacePoint<R>& sp, const V3<R>& inV, V3<R>& outV,
-                             V3<R>& brdf, V3<R>& erad, bool inv, Uniforms& U) {
+                             V3<R>& brdf, V3<R>& erad, bool inv, Uniforms& U, const Scene* E = nullptr,
+                             BounceInfo<R>* info = nullptr) {
     const Material& m = S.materials[S.meshes[sp.mesh].matId];
     erad = mk(R(0), R(0), R(0));
     if (m.type == MAT_OM) {
         if (X.omRough && m.omHasBase) { brdf = omRoughSample<R>(m, R(X.omAlpha), inV, sp, inv, outV, U); return; }
         outV = omNextDirection(S, inV, sp);
         if (!m.omHasBase) { brdf = mk(R(0), R(0), R(0)); return; }
-        brdf = omFresnelBrdf<R>(m, dot3(outV, sp.normal));
+        const Material& me = E ? E->materials[E->meshes[sp.mesh].matId] : m;
+        brdf = omFresnelBrdf<R>(me, dot3(outV, sp.normal));
+        if (info) { info->kind = 2; info->fCos = dot3(outV, sp.normal); }
         return;
     }
     if (m.type == MAT_PBR) {
         Bsdf<R> b; R omr[3];
         b.I = inV; b.N = sp.normal; b.T = sp.tangent;
         pbrPrepare(S, sp, m.pbr, b, omr);
+        R pre[3] = {b.color[0], b.color[1], b.color[2]};
         pbrFinish(m.pbr, b, omr);
         b.eta = R(1) / (m.pbr.double_sided ? R(1) : R(m.pbr.ior));
         brdf = sampleIndirect(b, U);
         outV = b.O;
         if (m.pbrEmissive) erad = pbrEmission(S, sp, m.pbr);
+        if (E && b.lobe != 0) {
+            const Material& me = E->materials[E->meshes[sp.mesh].matId];
+            Bsdf<R> be; R omre[3];
+            pbrPrepare(*E, sp, me.pbr, be, omre);
+            pbrFinish(me.pbr, be, omre);
+            LobeGeo<R> g = lobeGeo(b);
+            R f0[3], f1[3];
+            lobeValue(g, b.color, b.metal, b.alpha, f0);
+            lobeValue(g, be.color, be.metal, be.alpha, f1);
+            for (int c = 0; c < 3; ++c) at(brdf, c) = f0[c] != R(0) ? at(brdf, c) * (f1[c] / f0[c]) : R(0);
+        }
+        if (info) {
+            info->kind = 1; info->b = b;
+            for (int c = 0; c < 3; ++c) { info->colorPre[c] = pre[c]; info->omr[c] = omr[c]; }
+        }
         return;
     }
     /* DiffuseColorMaterial via RenderMaterial::getRayAndBrdf (RenderMaterial.cpp:51-72) */
@@ -1218,6 +1370,7 @@ struct Tracer {
     const Scene& S;
     const omv_renderer_params& P;
     const Ext X;
+    const Scene* E = nullptr;   /* detached evaluation scene (see matGetRayAndBrdf) */
     uint64_t rays = 0, camRays = 0, lightRays = 0, shadowRays = 0;
     Counters cnt;
     Tracer(const Scene& s, const omv_renderer_params& p) : S(s), P(p), X(p) {}
@@ -1273,7 +1426,7 @@ struct Tracer {
             Pv[i].pos = Pv[i].sp.position;
             modifyFrame(S, Pv[i].sp);
             V3<R> brdf, erad;
-            matGetRayAndBrdf(S, X, Pv[i].sp, d0, Pv[i].outRay, brdf, erad, inv, U);
+            matGetRayAndBrdf(S, X, Pv[i].sp, d0, Pv[i].outRay, brdf, erad, inv, U, E);
             R ct = std::abs(inv ? dot3(Pv[i].outRay, Pv[i].sp.normal) : dot3(d0, Pv[i].sp.normal));
             const V3<R>& of = Pv[i - 1].outFactor;
             Pv[i].outFactor = mk(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
@@ -1398,7 +1551,7 @@ struct Tracer {
                     V3<R> lc = mk(lRad.x * c * of.x, lRad.y * c * of.y, lRad.z * c * of.z);
                     if (intensity(lc) < R(P.ray_cut_pix_value)) continue;
                     V3<R> outV = neg(cp[j - 1].outRay);
-                    V3<R> brdf = matGetBrdf(S, X, cp[j].sp, LC, outV);
+                    V3<R> brdf = matGetBrdf(E ? *E : S, X, cp[j].sp, LC, outV);
                     brdf = mk(brdf.x * c, brdf.y * c, brdf.z * c);
                     lc = mk(brdf.x * of.x * lRad.x, brdf.y * of.y * lRad.y, brdf.z * of.z * lRad.z);
                     if (intensity(lc) < R(P.ray_cut_pix_value)) continue;
@@ -1409,6 +1562,90 @@ struct Tracer {
         }
         R scale = totalPN ? R(1) / R(totalPN) : R(1);
         return mk(radiance.x * scale, radiance.y * scale, radiance.z * scale);
+    }
+
+    /* Gradient accumulators of the adjoint: per texture (w*h*4) and per
+     * material slot (dn[3], dk[3]); empty = not a target. */
+    struct AdjGrad {
+        std::vector<std::vector<R>> tex, om;
+    };
+
+    void scatterTex(const omv_pbr_desc& d, const SurfacePoint<R>& sp, int slot, int ch, R g, AdjGrad& G) {
+        const int id = d.texture_map[slot];
+        if (!(0 <= id && id < (int)G.tex.size()) || G.tex[id].empty() || !(g != R(0))) return;
+        R u, v;
+        if (!getTexCoord(S, sp, d.texture_channel[slot], u, v)) return;
+        V3<R> uvT = mul33(d.texture_transform[slot], mk(u, v, R(1)));
+        int ids[4];
+        float w[4];
+        if (!texAddr(S.textures[id], (float)uvT.x, (float)uvT.y, ids, w)) return;
+        for (int i = 0; i < 4; ++i) G.tex[id][(size_t)ids[i] * 4 + ch] += g * R(w[i]);
+    }
+
+    /* Path-replay adjoint of one camera sample (computePath with inv = true,
+     * BDPTRenderer.cpp:149-176, in a scene without point lights).  C is the
+     * sample's radiance and A = d loss / d C.  Every term of C at or after
+     * bounce i carries that bounce's weight g_i, so the bounce adds
+     * A . R_i . d ln g_i / d theta, R_i being C minus the terms before i. */
+    void adjointSample(const V3<R>& O, const V3<R>& Rd, Stream& cs, const V3<R>& C, const R A[3], AdjGrad& G) {
+        Uniforms U; U.s = &cs;
+        const int q = P.max_path_length;
+        R rem[3] = {C.x, C.y, C.z};
+        V3<R> pos = O, dir = Rd, of = mk(R(1), R(1), R(1));
+        R tnear = R(0);
+        int n = 1;
+        for (int i = 1; i < P.camera_bounces + 1; ++i) {
+            R delta = R(3) * feps<R>() * maxAbs(pos);
+            V3<R> org = mk(pos.x + delta * dir.x, pos.y + delta * dir.y, pos.z + delta * dir.z);
+            SurfacePoint<R> sp;
+            if (!sceneIntersect(org, dir, tnear, std::numeric_limits<R>::max(), sp)) break;
+            modifyFrame(S, sp);
+            V3<R> outV, brdf, erad;
+            BounceInfo<R> bi;
+            matGetRayAndBrdf(S, X, sp, dir, outV, brdf, erad, true, U, nullptr, &bi);
+            R wc[3];
+            for (int c = 0; c < 3; ++c) wc[c] = A[c] * rem[c];
+            const Material& m = S.materials[S.meshes[sp.mesh].matId];
+            if (bi.kind == 1 && bi.b.lobe != 0) {
+                const Bsdf<R>& b = bi.b;
+                const omv_pbr_desc& d = m.pbr;
+                LobeGeo<R> g = lobeGeo(b);
+                R dC[3], dM[3], dA[3];
+                lobeDlog(g, b.color, b.metal, b.alpha, dC, dM, dA);
+                R gMetal = R(0), gAlpha = R(0);
+                const R occ = (bi.omr[0] - R(1)) * R(d.occlusion_strength) + R(1);
+                for (int c = 0; c < 3; ++c) {
+                    scatterTex(d, sp, OMV_PBR_BASE_COLOR, c, wc[c] * dC[c] * occ * R(d.base_color_factor[c]), G);
+                    gMetal += wc[c] * dM[c];
+                    gAlpha += wc[c] * dA[c];
+                }
+                scatterTex(d, sp, OMV_PBR_METALLIC_ROUGHNESS, 2, gMetal * R(d.metallic_factor), G);
+                const R r2 = bi.omr[2];
+                if (r2 > R(0.00001f) && r2 < R(0.99999f))
+                    scatterTex(d, sp, OMV_PBR_METALLIC_ROUGHNESS, 1, gAlpha * R(2) * b.rough * R(d.roughness_factor), G);
+            } else if (bi.kind == 2) {
+                const int slot = S.meshes[sp.mesh].matId;
+                if (slot < (int)G.om.size() && !G.om[slot].empty()) {
+                    for (int c = 0; c < 3; ++c) {
+                        R rp, rs, dpn, dpk, dsn, dsk;
+                        fresnel(R(m.n[c]), R(m.k[c]), bi.fCos, rp, rs);
+                        fresnelGrad(R(m.n[c]), R(m.k[c]), bi.fCos, dpn, dpk, dsn, dsk);
+                        const R F = rp + rs;
+                        if (!(F != R(0))) continue;
+                        G.om[slot][c] += wc[c] * (dpn + dsn) / F;
+                        G.om[slot][3 + c] += wc[c] * (dpk + dsk) / F;
+                    }
+                }
+            }
+            R ct = std::abs(dot3(outV, sp.normal));
+            of = mk(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
+            rem[0] -= of.x * erad.x; rem[1] -= of.y * erad.y; rem[2] -= of.z * erad.z;
+            n++;
+            if (of.x == R(0) && of.y == R(0) && of.z == R(0)) break;
+            if (!S.anyEmissive && n > q) break;
+            pos = sp.position;
+            dir = outV;
+        }
     }
 
     /* FC paint (FCSubject.h:153-482 + the FC branches of the three materials) */
@@ -1500,13 +1737,15 @@ static void cameraRay(const omv_camera_desc& c, int x, int y, float u0, float u1
 
 template <class R>
 static void renderRows(const Scene& S, const omv_renderer_params& P, const omv_camera_desc& cam, int row0, int rows,
-                       int threads, float* film, float* samplesOut, omv_render_stats* st) {
+                       int threads, float* film, float* samplesOut, omv_render_stats* st, const Scene* E = nullptr,
+                       double* film64 = nullptr) {
     const int W = cam.width, H = cam.height, spp = std::max(P.samples_per_pixel, 0);
     const int rEnd = std::min(H, row0 + rows);
     std::atomic<int> next(row0);
     std::atomic<uint64_t> rays(0), cr(0), lr(0), sr(0), nodes(0), tris(0);
     auto worker = [&]() {
         Tracer<R> T(S, P);
+        T.E = E;
         std::vector<Vtx<R>> cp(P.camera_bounces + 2), lp(P.light_bounces + 2);
         int row;
         while ((row = next.fetch_add(1)) < rEnd) {
@@ -1524,12 +1763,16 @@ static void renderRows(const Scene& S, const omv_renderer_params& P, const omv_c
                     /* PinholeCamera::hit (:97-108): film += radiance in sample order */
                     acc[0] = acc[0] + rad.x; acc[1] = acc[1] + rad.y; acc[2] = acc[2] + rad.z;
                 }
+                /* PinholeCamera::stop (:70-78) */
+                R cnt = R(spp);
+                R sc = cnt < R(0.5f) ? R(1) : R(1) / cnt;
                 if (film) {
                     float* f = film + (size_t)id * 4;
-                    /* PinholeCamera::stop (:70-78) */
-                    R cnt = R(spp);
-                    R sc = cnt < R(0.5f) ? R(1) : R(1) / cnt;
                     f[0] = (float)(sc * acc[0]); f[1] = (float)(sc * acc[1]); f[2] = (float)(sc * acc[2]); f[3] = 1.0f;
+                }
+                if (film64) {
+                    double* f = film64 + (size_t)id * 4;
+                    f[0] = (double)(sc * acc[0]); f[1] = (double)(sc * acc[1]); f[2] = (double)(sc * acc[2]); f[3] = 1.0;
                 }
             }
         }
@@ -1546,6 +1789,54 @@ static void renderRows(const Scene& S, const omv_renderer_params& P, const omv_c
         st->rays = rays; st->camera_rays = cr; st->light_rays = lr; st->shadow_rays = sr;
         st->nodes_visited = nodes; st->tris_tested = tris;
     }
+}
+
+/* The adjoint over the whole image: d loss / d film (W*H*4, rgb used) in,
+ * gradients of the target textures and OM materials out (added to). */
+template <class R>
+static void adjointImage(const Scene& S, const omv_renderer_params& P, const omv_camera_desc& cam, const float* dfilm,
+                         const std::vector<int>& texT, const std::vector<int>& omT, int threads,
+                         std::vector<std::vector<double>>& texOut, std::vector<std::vector<double>>& omOut) {
+    const int W = cam.width, H = cam.height, spp = std::max(P.samples_per_pixel, 0);
+    std::atomic<int> next(0);
+    std::mutex mu;
+    auto worker = [&]() {
+        Tracer<R> T(S, P);
+        typename Tracer<R>::AdjGrad G;
+        G.tex.resize(S.textures.size());
+        G.om.resize(S.materials.size());
+        for (int id : texT) G.tex[id].assign((size_t)S.textures[id].w * S.textures[id].h * 4, R(0));
+        for (int id : omT) G.om[id].assign(6, R(0));
+        std::vector<Vtx<R>> cp(P.camera_bounces + 2), lp(P.light_bounces + 2);
+        int row;
+        while ((row = next.fetch_add(1)) < H) {
+            for (int x = 0; x < W; ++x) {
+                const int id = row * W + x;
+                const R sc = R(spp) < R(0.5f) ? R(1) : R(1) / R(spp);
+                R A[3] = {R(dfilm[(size_t)id * 4 + 0]) * sc, R(dfilm[(size_t)id * 4 + 1]) * sc, R(dfilm[(size_t)id * 4 + 2]) * sc};
+                if (A[0] == R(0) && A[1] == R(0) && A[2] == R(0)) continue;
+                for (int k = 0; k < spp; ++k) {
+                    const uint64_t s = (uint64_t)id * spp + k;
+                    Stream cs(P.seed, STREAM_CAMERA, s);
+                    float u0 = cs.next(), u1 = cs.next();
+                    V3<R> O, D;
+                    cameraRay<R>(cam, x, row, u0, u1, O, D);
+                    V3<R> C = T.computePaths(O, D, s, cs, cp.data(), lp.data());
+                    if (C.x == R(0) && C.y == R(0) && C.z == R(0)) continue;
+                    Stream cs2(P.seed, STREAM_CAMERA, s);
+                    (void)cs2.next(); (void)cs2.next();
+                    T.adjointSample(O, D, cs2, C, A, G);
+                }
+            }
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        for (int id : texT) for (size_t i = 0; i < G.tex[id].size(); ++i) texOut[id][i] += (double)G.tex[id][i];
+        for (int id : omT) for (int i = 0; i < 6; ++i) omOut[id][i] += (double)G.om[id][i];
+    };
+    int nt = std::max(1, std::min(threads > 0 ? threads : 1, H));
+    std::vector<std::thread> th;
+    for (int i = 0; i < nt; ++i) th.emplace_back(worker);
+    for (auto& t : th) t.join();
 }
 
 }  // namespace refcpu
@@ -1857,6 +2148,58 @@ int ref_render(ref_scene* r, const omv_renderer_params* p, const omv_camera_desc
     else renderRows<float>(r->s, *p, *cam, row0, rows, threads, film, nullptr, st);
     return OMV_OK;
 }
+
+/* Detached render for finite differences of the adjoint: directions and
+ * lobes sampled with `sample`'s materials, weights evaluated with `eval`'s
+ * (same geometry, same texture sizes). */
+int ref_render_detached(ref_scene* sample, ref_scene* eval, const omv_renderer_params* p, const omv_camera_desc* cam, int fp64,
+                        int threads, double* film) {
+    if (!sample || !eval || !p || !cam || !sample->s.committed || !eval->s.committed) return OMV_E_ARG;
+    if (sample->s.materials.size() != eval->s.materials.size() || sample->s.meshes.size() != eval->s.meshes.size()) return OMV_E_ARG;
+    if (fp64) renderRows<double>(sample->s, *p, *cam, 0, cam->height, threads, nullptr, nullptr, nullptr, &eval->s, film);
+    else renderRows<float>(sample->s, *p, *cam, 0, cam->height, threads, nullptr, nullptr, nullptr, &eval->s, film);
+    return OMV_OK;
+}
+
+/* Path-replay adjoint of the camera-path estimator (scenes without point
+ * lights): dfilm = d loss / d film (W*H*4 floats, rgb used); the gradients
+ * of the listed textures (w*h*4 doubles each) and OM materials (6 doubles
+ * each: dn rgb, dk rgb) are added to the caller's arrays. */
+int ref_adjoint(ref_scene* r, const omv_renderer_params* p, const omv_camera_desc* cam, int fp64, int threads, const float* dfilm,
+                int ntex, const int* tex_ids, double* const* tex_grads, int nom, const int* om_ids, double* om_grads) {
+    if (!r || !p || !cam || !dfilm || !r->s.committed) return OMV_E_ARG;
+    const Scene& S = r->s;
+    if (!S.lights.empty()) return OMV_E_STATE;
+    std::vector<int> texT, omT;
+    for (int i = 0; i < ntex; ++i) {
+        if (!(0 <= tex_ids[i] && tex_ids[i] < (int)S.textures.size()) || !tex_grads || !tex_grads[i]) return OMV_E_RANGE;
+        texT.push_back(tex_ids[i]);
+    }
+    for (int i = 0; i < nom; ++i) {
+        if (!(0 <= om_ids[i] && om_ids[i] < (int)S.materials.size()) || S.materials[om_ids[i]].type != MAT_OM || !om_grads) return OMV_E_RANGE;
+        omT.push_back(om_ids[i]);
+    }
+    std::vector<std::vector<double>> texOut(S.textures.size()), omOut(S.materials.size());
+    for (int id : texT) texOut[id].assign((size_t)S.textures[id].w * S.textures[id].h * 4, 0.0);
+    for (int id : omT) omOut[id].assign(6, 0.0);
+    if (fp64) adjointImage<double>(S, *p, *cam, dfilm, texT, omT, threads, texOut, omOut);
+    else adjointImage<float>(S, *p, *cam, dfilm, texT, omT, threads, texOut, omOut);
+    for (int i = 0; i < ntex; ++i) {
+        const std::vector<double>& g = texOut[tex_ids[i]];
+        for (size_t j = 0; j < g.size(); ++j) tex_grads[i][j] += g[j];
+    }
+    for (int i = 0; i < nom; ++i)
+        for (int j = 0; j < 6; ++j) om_grads[(size_t)i * 6 + j] += omOut[om_ids[i]][j];
+    return OMV_OK;
+}
+
+void ref_adjoint_plant(int mask) { gAdjPlant = mask; }
+
+/* d(Rp, Rs)/d(n, k) in double: out = {dRp/dn, dRp/dk, dRs/dn, dRs/dk} */
+void ref_fresnel_grad(double n, double k, double c, double out[4]) {
+    fresnelGrad<double>(n, k, c, out[0], out[1], out[2], out[3]);
+}
+void ref_fresnel64(double n, double k, double c, double out[2]) { fresnel<double>(n, k, c, out[0], out[1]); }
 
 int ref_render_samples(ref_scene* r, const omv_renderer_params* p, const omv_camera_desc* cam, int fp64, int row0, int rows,
                        int threads, float* rad) {

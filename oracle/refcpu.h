@@ -67,6 +67,14 @@ int  ref_closest_hit(ref_scene* s, const float org[3], const float dir[3], float
 void ref_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 float ref_u01(uint32_t x);
 void ref_fresnel(float n, float k, float cos_theta, float* rp, float* rs);
+void ref_fresnel64(double n, double k, double cos_theta, double out[2]);
+void ref_fresnel_grad(double n, double k, double cos_theta, double out[4]);
+void ref_adjoint_plant(int mask);   /* test aid: plants a known mistake in the adjoint */
+int  ref_render_detached(ref_scene* sample, ref_scene* eval, const omv_renderer_params* p, const omv_camera_desc* cam,
+                         int fp64, int threads, double* film);
+int  ref_adjoint(ref_scene* s, const omv_renderer_params* p, const omv_camera_desc* cam, int fp64, int threads,
+                 const float* dfilm, int ntex, const int* tex_ids, double* const* tex_grads, int nom, const int* om_ids,
+                 double* om_grads);
 void ref_evaluate_direct(const float I[3], const float O[3], const float N[3], const float color[4],
                          float metallic, float alpha, float out[3]);
 /* BSDFModel::sampleIndirect with the uniforms given explicitly; returns the

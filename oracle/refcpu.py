@@ -91,6 +91,13 @@ def lib():
         "ref_gamma_correction": ([FP, C.c_int, C.c_int, C.c_float, C.c_float], None),
         "ref_compute_camera": ([C.c_int, C.c_int, C.c_float, FP, FP, FP], None),
         "ref_renderer_params_default": ([C.POINTER(RendererParams)], None),
+        "ref_fresnel64": ([C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)], None),
+        "ref_fresnel_grad": ([C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)], None),
+        "ref_adjoint_plant": ([C.c_int], None),
+        "ref_render_detached": ([vp, vp, C.POINTER(RendererParams), C.POINTER(CameraDesc), C.c_int, C.c_int, C.POINTER(C.c_double)],
+                                C.c_int),
+        "ref_adjoint": ([vp, C.POINTER(RendererParams), C.POINTER(CameraDesc), C.c_int, C.c_int, FP, C.c_int, C.POINTER(C.c_int),
+                         C.POINTER(C.POINTER(C.c_double)), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -176,6 +183,32 @@ class Scene(SceneApi):
             raise RuntimeError(f"ref_render_fc failed {rc}")
         return film.reshape(cam.height, cam.width, 4), ids.reshape(4, cam.height, cam.width), tie.reshape(cam.height, cam.width)
 
+    def render_detached(self, eval_scene, params, cam: CameraDesc, fp64=True, threads=0):
+        """Directions and lobes sampled with this scene's materials, weights
+        evaluated with eval_scene's (the functional the adjoint differentiates)."""
+        film = np.zeros(cam.width * cam.height * 4, np.float64)
+        rc = self.lib.ref_render_detached(self.h, eval_scene.h, C.byref(params), C.byref(cam), int(fp64), threads,
+                                          film.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc != 0:
+            raise RuntimeError(f"ref_render_detached failed {rc}")
+        return film.reshape(cam.height, cam.width, 4)
+
+    def adjoint(self, params, cam: CameraDesc, dfilm, textures=(), om_materials=(), tex_shapes=None, fp64=True, threads=0):
+        """Path-replay gradients: dfilm = d loss / d film (H, W, 4).  Returns
+        ({tex_id: (h, w, 4) float64}, {mat_id: (dn[3], dk[3])})."""
+        d = np.ascontiguousarray(np.asarray(dfilm, np.float32).reshape(-1))
+        tex_ids = (C.c_int * max(1, len(textures)))(*textures)
+        grads = [np.zeros(int(np.prod(tex_shapes[t])), np.float64) for t in textures]
+        gp = (C.POINTER(C.c_double) * max(1, len(textures)))(*[g.ctypes.data_as(C.POINTER(C.c_double)) for g in grads])
+        om_ids = (C.c_int * max(1, len(om_materials)))(*om_materials)
+        omg = np.zeros(6 * max(1, len(om_materials)), np.float64)
+        rc = self.lib.ref_adjoint(self.h, C.byref(params), C.byref(cam), int(fp64), threads, ptr(d), len(textures), tex_ids, gp,
+                                  len(om_materials), om_ids, omg.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc != 0:
+            raise RuntimeError(f"ref_adjoint failed {rc}")
+        return ({t: g.reshape(tex_shapes[t]) for t, g in zip(textures, grads)},
+                {m: (omg[6 * i:6 * i + 3].copy(), omg[6 * i + 3:6 * i + 6].copy()) for i, m in enumerate(om_materials)})
+
     def closest_hit(self, org, d, tnear=0.0, tfar=3.4e38, bvh=True):
         o, dd = f32(org, (3,)), f32(d, (3,))
         t, u, v = C.c_float(), C.c_float(), C.c_float()
@@ -193,3 +226,16 @@ class Scene(SceneApi):
         if getattr(self, "h", None):
             self.lib.ref_scene_destroy(self.h)
             self.h = None
+
+
+def fresnel64(n, k, c):
+    out = (C.c_double * 2)()
+    lib().ref_fresnel64(n, k, c, out)
+    return out[0], out[1]
+
+
+def fresnel_grad(n, k, c):
+    """(dRp/dn, dRp/dk, dRs/dn, dRs/dk) in double."""
+    out = (C.c_double * 4)()
+    lib().ref_fresnel_grad(n, k, c, out)
+    return tuple(out)

@@ -78,9 +78,60 @@ def test_schedules_agree_instanced(pkg, wl):
     _same(_both(pkg, desc, view, **params))
 
 
-def test_wavefront_multi_chunk(pkg, wl):
-    """More samples than one wavefront chunk holds (2^22 without lights):
-    chunk boundaries fall on whole pixels and the film is unchanged."""
-    desc, view = wl.mixed_test_scene(hdr=True, lights=False)
-    view.width, view.height = 512, 520
-    _same(_both(pkg, desc, view, samples_per_pixel=16, camera_bounces=3, light_bounces=2, max_path_length=4, seed=8))
+@pytest.mark.parametrize("lights", [False, True])
+def test_wavefront_multi_chunk(pkg, wl, lights):
+    """Several wavefront chunks (chunk_samples not a multiple of spp): chunk
+    boundaries fall on whole pixels and the film is unchanged."""
+    desc, view = wl.mixed_test_scene(hdr=True, lights=lights)
+    view.width, view.height = 200, 130
+    params = dict(samples_per_pixel=5, camera_bounces=3, light_bounces=2, max_path_length=4, seed=8)
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = pkg.Camera(view.width, view.height, view.position, view.forward, view.up, view.yfov)
+    out = []
+    for sched in (pkg.SCHEDULE_WAVEFRONT, pkg.SCHEDULE_PER_SAMPLE):
+        r = pkg.Renderer(pkg.default_params(**params))
+        r.set_scene(s)
+        r.set_schedule(sched)
+        if sched == pkg.SCHEDULE_WAVEFRONT:
+            r.set_tuning(chunk_samples=20001)
+        out.append((r.render(cam), r.stats()))
+    _same(out)
+
+
+@pytest.mark.parametrize("tuning", [dict(trace_keep=1), dict(trace_keep=32), dict(trace_keep=64),
+                                    dict(trace_keep=48, trace_blocks_per_cu=1, chunk_samples=1000)])
+def test_wavefront_tuning_agrees(pkg, wl, tuning):
+    """Launch tuning (persistent traversal with refill, fewer resident blocks,
+    small chunks) changes the schedule only: the film and the counts equal the
+    per-sample kernel's."""
+    desc, view = wl.mixed_test_scene(hdr=True, lights=True, masked=True)
+    view.width, view.height = 64, 48
+    params = dict(samples_per_pixel=3, camera_bounces=5, light_bounces=4, max_path_length=6, seed=12)
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = pkg.Camera(view.width, view.height, view.position, view.forward, view.up, view.yfov)
+    out = []
+    for sched in (pkg.SCHEDULE_WAVEFRONT, pkg.SCHEDULE_PER_SAMPLE):
+        r = pkg.Renderer(pkg.default_params(**params))
+        r.set_scene(s)
+        r.set_schedule(sched)
+        if sched == pkg.SCHEDULE_WAVEFRONT:
+            r.set_tuning(**tuning)
+        out.append((r.render(cam), r.stats()))
+    _same(out)
+    assert out[0][1]["trace_launches"] > 0 and out[0][1]["trace_ms"] > 0
+
+
+def test_wavefront_tuning_knot(pkg, wl):
+    desc, view, params = wl.hdri_torus_knot(width=96, height=96, spp=4, tex=128, hdr_w=256, hdr_h=128, segments=200, tube=16)
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = pkg.Camera(view.width, view.height, view.position, view.forward, view.up, view.yfov)
+    imgs = []
+    for keep in (0, 40):
+        r = pkg.Renderer(pkg.default_params(**params))
+        r.set_scene(s)
+        r.set_tuning(trace_keep=keep)
+        imgs.append(r.render(cam))
+    assert np.array_equal(imgs[0], imgs[1])
