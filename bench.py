@@ -73,6 +73,32 @@ def cpu_baseline(ref, desc, view, params, cam_desc, budget_s, threads):
     }
 
 
+def rmse_check(ref, desc, view, params, cam_desc, gpu_img, rows, threads):
+    """Equal-RMSE comparison on a band of `rows` rows through the image centre:
+    the GPU image and the float oracle at the same spp and seed, each scored
+    against the FP64 oracle at 16x spp with an independent seed."""
+    rs = ref.Scene()
+    desc.apply(rs, ref.pbr_desc)
+    row0 = max(0, view.height // 2 - rows // 2)
+    p = ref.default_params(**params)
+    t0 = time.perf_counter()
+    cpu, _ = rs.render(p, cam_desc, row0=row0, rows=rows, threads=threads)
+    cpu_s = time.perf_counter() - t0
+    p64 = ref.default_params(**dict(params, samples_per_pixel=16 * params["samples_per_pixel"], seed=params.get("seed", 0) + 1000003))
+    t0 = time.perf_counter()
+    hi, _ = rs.render(p64, cam_desc, fp64=True, row0=row0, rows=rows, threads=threads)
+    ref_s = time.perf_counter() - t0
+    band = slice(row0, row0 + rows)
+    g = gpu_img[band, :, :3].astype(np.float64)
+    c = cpu[band, :, :3].astype(np.float64)
+    h = hi[band, :, :3].astype(np.float64)
+    rg = float(np.sqrt(np.mean((g - h) ** 2)))
+    rc = float(np.sqrt(np.mean((c - h) ** 2)))
+    return {"rows": [row0, row0 + rows], "spp": params["samples_per_pixel"], "reference_spp": 16 * params["samples_per_pixel"],
+            "reference": "FP64 oracle, independent seed", "rmse_gpu": rg, "rmse_cpu": rc, "ratio": rg / rc if rc > 0 else None,
+            "cpu_band_s": round(cpu_s, 2), "reference_s": round(ref_s, 2)}
+
+
 def copy_probe(gib):
     """A plain device copy of known size, the calibration kernel for the
     FETCH_SIZE / WRITE_SIZE counters (each launch reads and writes gib GiB)."""
@@ -166,6 +192,160 @@ def bench_batched(pkg, wl, args):
     }
 
 
+def bench_material_fit(pkg, wl, args):
+    """Differentiable material fit: forward render + path-replay backward of an
+    L2 image loss against a target rendered with hidden parameters (seed
+    5000) from the initial ones (seed 5001).  value = (forward + backward)
+    paths/s; a short Adam fit shows the loss falling; --fd N checks N texels
+    (seed 5002) and the OM (n, k) against central finite differences of the
+    FP64 oracle's detached functional (h = 1e-3)."""
+    import importlib.util
+    import torch
+    spec = importlib.util.spec_from_file_location(
+        "omv_amd.torch_adjoint", os.path.join(os.path.dirname(os.path.abspath(__file__)), "openmaterial-validation_amd", "torch_adjoint.py"))
+    ta = importlib.util.module_from_spec(spec)
+    sys.modules["omv_amd.torch_adjoint"] = ta
+    spec.loader.exec_module(ta)
+    dt, di, view, params, ids = wl.material_fit()
+    dev = torch.device("cuda")
+    st, si = pkg.Scene(), pkg.Scene()
+    dt.apply(st, pkg.pbr_desc)
+    di.apply(si, pkg.pbr_desc)
+    cam = pkg.Camera(view.width, view.height, view.position, view.forward, view.up, view.yfov)
+    rt = pkg.Renderer(pkg.default_params(**params))
+    rt.set_scene(st)
+    T = torch.from_numpy(rt.render(cam)).to(dev)
+    r = pkg.Renderer(pkg.default_params(**params))
+    r.set_scene(si)
+    fit = ta.MaterialFit(si, r, cam, ids["base_color"], ids["metal_rough"], ids["om"])
+    base = torch.tensor(di.textures[0][1], device=dev, requires_grad=True)
+    mr = torch.tensor(di.textures[1][1], device=dev, requires_grad=True)
+    n = torch.tensor(np.asarray(di.materials[1][1], np.float32), device=dev, requires_grad=True)
+    k = torch.tensor(np.asarray(di.materials[1][2], np.float32), device=dev, requires_grad=True)
+
+    def step():
+        for t in (base, mr, n, k):
+            t.grad = None
+        img = fit(base, mr, n, k)
+        loss = ((img[..., :3] - T[..., :3]) ** 2).sum()
+        loss.backward()
+        return float(loss)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    # forward and backward device times on their own
+    fwd_ms, bwd_ms = [], []
+    d = torch.ones_like(T)
+    for _ in range(3):
+        r.render(cam)
+        fwd_ms.append(r.stats()["render_ms"])
+        r.backward(cam, d.data_ptr())
+        bwd_ms.append(r.stats()["kernel_ms"])
+    samples = view.width * view.height * params["samples_per_pixel"]
+    # a short fit: Adam on all parameters, clamped to the valid ranges
+    opt = torch.optim.Adam([base, mr, n, k], lr=0.02)
+    losses = []
+    for it in range(args.fit_steps):
+        opt.zero_grad()
+        img = fit(base, mr, n, k)
+        loss = ((img[..., :3] - T[..., :3]) ** 2).sum()
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            base.clamp_(0.0, 1.0); mr.clamp_(0.0, 1.0); n.clamp_(0.05, 3.0); k.clamp_(0.0, 10.0)
+        losses.append(float(loss))
+    line = {
+        "metric": "differentiable material fit: (forward + backward) paths/s on 1 GPU", "value": round(samples * args.steps / el, 1),
+        "unit": "paths/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "material_fit", "resolution": [view.width, view.height], "spp": params["samples_per_pixel"],
+                   "depth": params["max_path_length"], "textures": "512^2 fp32 base colour + metallic-roughness", "om_sphere": True,
+                   "sampling": "detached"},
+        "forward_ms": round(float(np.median(fwd_ms)), 3), "backward_ms": round(float(np.median(bwd_ms)), 3),
+        "fit_loss": [round(losses[0], 3), round(losses[-1], 3)] if losses else None, "fit_steps": args.fit_steps,
+    }
+    if args.fd > 0:
+        line["fd_check"] = fd_check(pkg, wl, args.fd, dt, di, view, params, ids, T.cpu().numpy())
+    return line
+
+
+def fd_check(pkg, wl, count, dt, di, view, params, ids, T):
+    """GPU adjoint vs central differences (h = 1e-3) of the FP64 oracle's
+    detached functional, on `count` texels drawn with seed 5002 among those the
+    loss sees, and on the OM (n, k)."""
+    import copy
+    import torch
+    ref = ge.load_oracle()
+    dev = torch.device("cuda")
+    s = pkg.Scene()
+    di.apply(s, pkg.pbr_desc)
+    cam = pkg.Camera(view.width, view.height, view.position, view.forward, view.up, view.yfov)
+    r = pkg.Renderer(pkg.default_params(**params))
+    r.set_scene(s)
+    img = r.render(cam)
+    dfilm = torch.from_numpy(np.ascontiguousarray(2.0 * (img - T) * np.array([1, 1, 1, 0], np.float32))).to(dev)
+    tex = di.textures[0][1].shape[0]
+    gb = torch.zeros((tex, tex, 4), device=dev)
+    gmr = torch.zeros((tex, tex, 4), device=dev)
+    gom = torch.zeros(6, device=dev)
+    r.backward(cam, dfilm.data_ptr(), textures=[(0, gb.data_ptr()), (1, gmr.data_ptr())], om_materials=[1], om_grads_ptr=gom.data_ptr())
+    g = {0: gb.cpu().numpy(), 1: gmr.cpu().numpy()}
+    gom = gom.cpu().numpy()
+    rs = ref.Scene()
+    di.apply(rs, ref.pbr_desc)
+    p = ref.default_params(**params)
+    cd = cam.desc()
+    threads = min(16, os.cpu_count() or 1)
+
+    def loss_of(d):
+        e = ref.Scene()
+        d.apply(e, ref.pbr_desc)
+        out = rs.render_detached(e, p, cd, fp64=True, threads=threads)
+        return float(((out[..., :3] - T[..., :3]) ** 2).sum())
+
+    rng = np.random.default_rng(5002)
+    cands = [(t, c) for t, cs in ((0, (0, 1, 2)), (1, (1, 2))) for c in cs]
+    picks = []
+    while len(picks) < count:
+        t, c = cands[int(rng.integers(len(cands)))]
+        nz = np.argwhere(np.abs(g[t][..., c]) > 1e-3 * np.abs(g[t][..., c]).max())
+        y, x = nz[int(rng.integers(len(nz)))]
+        picks.append((t, int(y), int(x), c))
+    rel = []
+    for t, y, x, c in picks:
+        vals = []
+        for sgn in (1.0, -1.0):
+            d = copy.deepcopy(di)
+            a = d.textures[t][1]
+            a[y, x, c] = np.float32(a[y, x, c] + np.float32(sgn * 1e-3))
+            vals += [float(a[y, x, c]), loss_of(d)]
+        fd = (vals[1] - vals[3]) / (vals[0] - vals[2])
+        rel.append(abs(g[t][y, x, c] - fd) / max(abs(fd), 1e-12))
+        print(f"# fd texel {t},{y},{x},{c}: adjoint {g[t][y, x, c]:.6g} fd {fd:.6g}", file=sys.stderr, flush=True)
+    om_rel = []
+    for j in range(6):
+        vals = []
+        for sgn in (1.0, -1.0):
+            d = copy.deepcopy(di)
+            nn, kk = np.array(d.materials[1][1], np.float32), np.array(d.materials[1][2], np.float32)
+            arr = nn if j < 3 else kk
+            arr[j % 3] = np.float32(arr[j % 3] + np.float32(sgn * 1e-3))
+            d.materials[1] = ("om_nk", nn, kk)
+            vals += [float(arr[j % 3]), loss_of(d)]
+        fd = (vals[1] - vals[3]) / (vals[0] - vals[2])
+        om_rel.append(abs(gom[j] - fd) / max(abs(fd), 1e-12))
+        print(f"# fd om {j}: adjoint {gom[j]:.6g} fd {fd:.6g}", file=sys.stderr, flush=True)
+    rel = np.array(rel)
+    return {"texels": count, "seed": 5002, "h": 1e-3, "median_rel": float(np.median(rel)), "p90_rel": float(np.quantile(rel, 0.9)),
+            "max_rel": float(rel.max()), "within_1e-3": float(np.mean(rel < 1e-3)), "om_max_rel": float(max(om_rel))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,7 +357,11 @@ def main():
                     help="only run a plain device copy of GIB GiB (counter calibration)")
     ap.add_argument("--workload", default="hdri_torus_knot",
                     choices=["hdri_torus_knot", "plain_bdpt_icosphere", "plain_bdpt_icosphere_om_rough", "large_instanced",
-                             "batched_lightbox"])
+                             "batched_lightbox", "material_fit"])
+    ap.add_argument("--fit-steps", type=int, default=20, help="material_fit: Adam steps after the timed region")
+    ap.add_argument("--rmse", type=int, default=0, metavar="ROWS",
+                    help="equal-RMSE check of GPU and CPU images against the FP64 oracle at 16x spp on ROWS rows")
+    ap.add_argument("--fd", type=int, default=0, help="material_fit: finite-difference check on this many texels")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (no multi-GPU path in this build)", file=sys.stderr)
@@ -190,6 +374,9 @@ def main():
     wl = ge.load_workloads()
     if args.workload == "batched_lightbox":
         print(json.dumps(bench_batched(pkg, wl, args)), flush=True)
+        return
+    if args.workload == "material_fit":
+        print(json.dumps(bench_material_fit(pkg, wl, args)), flush=True)
         return
     if args.workload == "hdri_torus_knot":
         desc, view, params = wl.hdri_torus_knot()
@@ -275,6 +462,10 @@ def main():
     }
     if cpu is not None:
         line["gpu_over_cpu"] = round(paths_per_s / cpu["paths_per_s"], 1)
+    if args.rmse > 0:
+        ref = ge.load_oracle()
+        line["equal_rmse"] = rmse_check(ref, desc, view, params, cam.desc(), r.render(cam), args.rmse, threads)
+        print(f"# equal rmse {line['equal_rmse']}", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
 
 

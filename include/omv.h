@@ -219,6 +219,13 @@ int omv_scene_set_false_color_subject(omv_scene* s, const char* name);
  * reference's log lines, '\n'-separated (may be NULL).  Returns OMV_OK or
  * OMV_E_SCENE when every instance is invalid. */
 int omv_scene_commit(omv_scene* s, char* log, size_t log_cap);
+/* In-place parameter updates of a committed scene (no reference counterpart:
+ * the optimiser loop of the adjoint form changes material parameters between
+ * renders without a new commit).  texels: same size and format as set, host
+ * or device pointer (w*h RGBA8 or w*h*4 floats); the OM (n, k) replace the
+ * baked per-RGB IOR of an OpenMATERIAL material. */
+int omv_scene_update_texture(omv_scene* s, int tex_id, const void* texels);
+int omv_scene_update_material_om_nk(omv_scene* s, int mat_id, const float n_rgb[3], const float k_rgb[3]);
 /* Inspection: the committed interleaved vertex buffer of a mesh (stride
  * 10+2*channels floats: pos3 normal3 tangent4 uv2*ch, RenderMesh.cpp:49-72). */
 int omv_scene_mesh_info(omv_scene* s, int mesh_id, int* vert_n, int* faces_n,
@@ -280,11 +287,43 @@ typedef struct omv_render_tuning {
     int trace_blocks_per_cu;  /* resident traversal workgroups per CU
                                  (0 = occupancy limit)                          */
     int chunk_samples;        /* samples per wavefront chunk, rounded down to
-                                 whole pixels (0 = default: 2^24, 2^23 with
-                                 point lights)                                  */
+                                 whole pixels (0 = default: 2^25)               */
 } omv_render_tuning;
 int omv_renderer_set_tuning(omv_renderer* r, const omv_render_tuning* t);
 int omv_renderer_get_tuning(const omv_renderer* r, omv_render_tuning* t);
+
+/* Film post pass of main.cpp:513-519 on the device, on the camera's last
+ * film from this renderer: gammaCorrection(A = 1, gamma)
+ * (ToneMapping.cpp:24-54), then when denoise != 0 the 3x3 median filter
+ * SimpleDenoiserMedian<1> (Denoiser.h:24-68), and the 8-bit truncation of
+ * saveImage (main.cpp:88-93).  out_f32 (W*H*4 floats) and out_u8 (W*H*4
+ * bytes) are host or device pointers, either may be NULL. */
+int omv_renderer_post(omv_renderer* r, omv_camera* c, float gamma, int denoise, float* out_f32, uint8_t* out_u8);
+
+/* ---------------------------------------------------------- adjoint */
+/* Gradient targets of omv_renderer_backward.  All pointers are device
+ * pointers; gradients are ADDED to them.  texture_grads[i] holds w*h*4
+ * floats for texture texture_ids[i] (the gradient with respect to the texel
+ * values as floats, byte/255 for RGBA8); om_grads holds 6 floats per OM
+ * material (d/dn rgb, d/dk rgb). */
+typedef struct omv_adjoint_targets {
+    int n_textures;
+    const int* texture_ids;       /* host array */
+    float* const* texture_grads;  /* host array of device pointers */
+    int n_om_materials;
+    const int* om_material_ids;   /* host array */
+    float* om_grads;              /* device pointer, 6 * n_om_materials floats */
+} omv_adjoint_targets;
+/* Path-replay backward pass of omv_renderer_render (no reference
+ * counterpart; the reference has no derivatives): given d_film = d loss /
+ * d film (device pointer, W*H*4 floats, rgb used) for the camera's last
+ * film, adds d loss / d parameter for the base-colour (rgb) and
+ * metallic-roughness (G roughness, B metallic) texels of PBR materials and
+ * the (n, k) of OpenMATERIAL mirrors.  Detached sampling: directions and
+ * lobes are those the current parameters sample; the derivative is that of
+ * the weights.  Scenes with point lights and the OMV_EXT_OM_ROUGH
+ * extension return OMV_E_STATE. */
+int omv_renderer_backward(omv_renderer* r, omv_camera* c, const float* d_film, const omv_adjoint_targets* t);
 
 /* ------------------------------------------------------ batched jobs */
 /* Many independent validation renders (each one BDPTRenderer::render call of
@@ -297,9 +336,15 @@ int omv_renderer_get_tuning(const omv_renderer* r, omv_render_tuning* t);
  * names them exists; cameras are copied at add time. */
 typedef struct omv_batch omv_batch;
 typedef struct omv_batch_tuning {
-    int blocks_per_cu;   /* resident workgroups per CU (0 = occupancy limit) */
-    int tile_waves;      /* pixels per work item = 64 * tile_waves (>= 1)    */
-    int use_graph;       /* 1: replay a captured hipGraph, 0: plain launch   */
+    int blocks_per_cu;   /* per-pixel kernel: resident workgroups per CU
+                            (0 = occupancy limit)                          */
+    int tile_waves;      /* per-pixel kernel: pixels per work item =
+                            64 * tile_waves (>= 1)                         */
+    int use_graph;       /* 1: replay a captured hipGraph, 0: plain launch */
+    int streams;         /* 0: one persistent per-pixel kernel over all
+                            jobs; N >= 1: each job's wavefront launch
+                            sequence, jobs dealt over N streams so that one
+                            job's last bounce rounds overlap other jobs    */
 } omv_batch_tuning;
 int omv_batch_create(omv_batch** out);
 int omv_batch_destroy(omv_batch* b);

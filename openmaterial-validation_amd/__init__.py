@@ -68,8 +68,13 @@ class RenderTuning(C.Structure):
     _fields_ = [("trace_keep", C.c_int), ("trace_blocks_per_cu", C.c_int), ("chunk_samples", C.c_int)]
 
 
+class AdjointTargets(C.Structure):
+    _fields_ = [("n_textures", C.c_int), ("texture_ids", C.POINTER(C.c_int)), ("texture_grads", C.POINTER(C.c_void_p)),
+                ("n_om_materials", C.c_int), ("om_material_ids", C.POINTER(C.c_int)), ("om_grads", C.c_void_p)]
+
+
 class BatchTuning(C.Structure):
-    _fields_ = [("blocks_per_cu", C.c_int), ("tile_waves", C.c_int), ("use_graph", C.c_int)]
+    _fields_ = [("blocks_per_cu", C.c_int), ("tile_waves", C.c_int), ("use_graph", C.c_int), ("streams", C.c_int)]
 
 
 class RenderStats(C.Structure):
@@ -261,6 +266,10 @@ def _load():
         "omv_renderer_set_parameters": [vp, C.POINTER(RendererParams)], "omv_renderer_set_scene": [vp, vp],
         "omv_renderer_render": [vp, vp], "omv_renderer_render_fc": [vp, vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)],
         "omv_renderer_get_stats": [vp, C.POINTER(RenderStats)],
+        "omv_renderer_backward": [vp, vp, C.c_void_p, C.POINTER(AdjointTargets)],
+        "omv_renderer_post": [vp, vp, C.c_float, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_uint8)],
+        "omv_scene_update_texture": [vp, C.c_int, C.c_void_p],
+        "omv_scene_update_material_om_nk": [vp, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)],
         "omv_renderer_set_schedule": [vp, C.c_int], "omv_renderer_get_schedule": [vp, C.POINTER(C.c_int)],
         "omv_renderer_set_tuning": [vp, C.POINTER(RenderTuning)], "omv_renderer_get_tuning": [vp, C.POINTER(RenderTuning)],
         "omv_renderer_params_default": [C.POINTER(RendererParams)],
@@ -314,6 +323,25 @@ def default_params(**kw) -> RendererParams:
 
 class Scene(SceneApi):
     prefix = "omv"
+
+    def update_texture(self, tex_id: int, texels):
+        """Replace a committed texture's texels in place: a numpy array (host)
+        or an int device address (e.g. a torch tensor's data_ptr())."""
+        if isinstance(texels, int):
+            src = C.c_void_p(texels)
+        else:
+            arr = np.ascontiguousarray(texels)
+            self._keep = arr
+            src = C.c_void_p(arr.ctypes.data)
+        rc = self.lib.omv_scene_update_texture(self.h, tex_id, src)
+        if rc != 0:
+            raise RuntimeError(self.lib.omv_last_error().decode())
+
+    def update_om(self, mat_id: int, n, k):
+        nn, kk = _f32(n, (3,)), _f32(k, (3,))
+        rc = self.lib.omv_scene_update_material_om_nk(self.h, mat_id, _ptr(nn, C.c_float), _ptr(kk, C.c_float))
+        if rc != 0:
+            raise RuntimeError(self.lib.omv_last_error().decode())
 
     def __init__(self):
         self.lib = lib()
@@ -434,6 +462,26 @@ class Renderer:
         self._check(self.lib.omv_renderer_render(self.h, cam.h))
         return cam.film()
 
+    def backward(self, cam: Camera, d_film_ptr: int, textures=(), om_materials=(), om_grads_ptr: int = 0):
+        """Path-replay adjoint of the last render: d_film_ptr = device address of
+        d loss / d film (H*W*4 floats); textures = [(tex_id, grad device address)];
+        om_materials = [mat_id] with om_grads_ptr -> 6 floats each.  Adds."""
+        t = AdjointTargets()
+        ids = (C.c_int * max(1, len(textures)))(*[i for i, _ in textures])
+        ptrs = (C.c_void_p * max(1, len(textures)))(*[g for _, g in textures])
+        oms = (C.c_int * max(1, len(om_materials)))(*om_materials)
+        t.n_textures, t.texture_ids, t.texture_grads = len(textures), ids, ptrs
+        t.n_om_materials, t.om_material_ids, t.om_grads = len(om_materials), oms, C.c_void_p(om_grads_ptr or None)
+        self._check(self.lib.omv_renderer_backward(self.h, cam.h, C.c_void_p(d_film_ptr), C.byref(t)))
+
+    def post(self, cam: Camera, gamma=0.5, denoise=False):
+        """main.cpp's post pass on the device: (gamma-corrected [+ median] float film, 8-bit image)."""
+        n = cam.width * cam.height * 4
+        f = np.zeros(n, np.float32)
+        u = np.zeros(n, np.uint8)
+        self._check(self.lib.omv_renderer_post(self.h, cam.h, C.c_float(gamma), int(bool(denoise)), _ptr(f, C.c_float), _ptr(u, C.c_uint8)))
+        return f.reshape(cam.height, cam.width, 4), u.reshape(cam.height, cam.width, 4)
+
     def render_fc(self, cam: Camera):
         npix = cam.width * cam.height
         ids = np.zeros(4 * npix, np.int32)
@@ -479,8 +527,8 @@ class Batch:
     def __len__(self):
         return len(self._jobs)
 
-    def set_tuning(self, blocks_per_cu=0, tile_waves=1, use_graph=True):
-        t = BatchTuning(blocks_per_cu, tile_waves, 1 if use_graph else 0)
+    def set_tuning(self, blocks_per_cu=0, tile_waves=1, use_graph=True, streams=0):
+        t = BatchTuning(blocks_per_cu, tile_waves, 1 if use_graph else 0, streams)
         self._check(self.lib.omv_batch_set_tuning(self.h, C.byref(t)))
 
     def tuning(self) -> dict:

@@ -29,6 +29,11 @@ BatchRunner::BatchRunner() = default;
 BatchRunner::~BatchRunner() {
     dropGraph();
     for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
+    for (auto& r : sres_) {
+        if (r.done) (void)hipEventDestroy(r.done);
+        if (r.s) (void)hipStreamDestroy(r.s);
+    }
+    if (fork_) (void)hipEventDestroy(fork_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -60,7 +65,10 @@ int BatchRunner::addJob(const RenderScene* s, const omv_camera_desc& cam, const 
 }
 
 int BatchRunner::setTuning(const omv_batch_tuning& t) {
-    if (t.blocks_per_cu < 0 || t.tile_waves < 1 || t.tile_waves > 64) { setError("batch: tuning out of range"); return OMV_E_RANGE; }
+    if (t.blocks_per_cu < 0 || t.tile_waves < 1 || t.tile_waves > 64 || t.streams < 0 || t.streams > 16) {
+        setError("batch: tuning out of range");
+        return OMV_E_RANGE;
+    }
     tuning_ = t;
     tuning_.use_graph = t.use_graph ? 1 : 0;
     dirty_ = true;
@@ -88,6 +96,17 @@ int BatchRunner::prepare() {
     std::vector<int> order(jobs_.size());
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return jobs_[a].tileCost > jobs_[b].tileCost; });
+    order_ = order;
+    if (tuning_.streams > 0) {
+        if ((e = dFilm_.reserve(filmFloats * sizeof(float))) != hipSuccess) return hipFail(e, "film");
+        if ((e = dCounters_.reserve(16 * sizeof(unsigned long long))) != hipSuccess) return hipFail(e, "counters");
+        views_.resize(jobs_.size());
+        for (size_t i = 0; i < jobs_.size(); ++i) views_[i] = jobs_[i].scene->view();
+        int rc = prepareWavefront(order);
+        if (rc) return rc;
+        dirty_ = false;
+        return OMV_OK;
+    }
     const uint64_t tilePix = 64ull * (uint64_t)tuning_.tile_waves;
     std::vector<uint64_t> tileEnd(jobs_.size());
     uint64_t t = 0;
@@ -137,7 +156,104 @@ int BatchRunner::prepare() {
     return OMV_OK;
 }
 
+// Wavefront jobs: per-stream state sized for the largest job chunk, the
+// whole sequence optionally captured into one graph (fork/join over streams).
+int BatchRunner::prepareWavefront(const std::vector<int>& order) {
+    hipError_t e;
+    const int ns = tuning_.streams;
+    while ((int)sres_.size() < ns) {
+        sres_.emplace_back();
+        StreamRes& r = sres_.back();
+        if ((e = hipStreamCreate(&r.s)) != hipSuccess) return hipFail(e, "hipStreamCreate");
+        if ((e = hipEventCreateWithFlags(&r.done, hipEventDisableTiming)) != hipSuccess) return hipFail(e, "hipEventCreate");
+    }
+    if (!fork_ && (e = hipEventCreateWithFlags(&fork_, hipEventDisableTiming)) != hipSuccess) return hipFail(e, "hipEventCreate");
+    if (!wfPerCu_[0]) wavefrontBlocksPerCU(wfPerCu_);
+    wfMaxSlots_ = 1ull << 22;
+    const int blocksTrace = cus_ * wfPerCu_[0];
+    size_t arena = 0, rad = 0;
+    int ncnt = 0;
+    for (int j : order) {
+        const Job& J = jobs_[j];
+        const int spp = std::max(J.p.samples_per_pixel, 0);
+        if (spp == 0) continue;
+        const bool lights = J.scene->view().num_lights > 0;
+        const int lvCap = lvCapOf(J.p);
+        const int lc = std::max(1, std::min(lvCap, std::max(J.p.max_path_length, 1)));
+        const uint64_t chunkPix = std::max<uint64_t>(1, wfMaxSlots_ / (uint64_t)spp);
+        const uint64_t slots = std::min(J.npix, chunkPix) * (uint64_t)spp;
+        if (slots * (uint64_t)lc >= (1ull << 32)) { setError("batch: too many samples per pixel for one chunk"); return OMV_E_RANGE; }
+        arena = std::max(arena, wavefrontArenaBytes((uint32_t)slots, lvCap, lc, lights));
+        rad = std::max(rad, (size_t)slots * 3 * sizeof(float));
+        ncnt = std::max(ncnt, wavefrontCounterCount(J.p.camera_bounces, lvCap));
+    }
+    for (int k = 0; k < ns; ++k) {
+        StreamRes& r = sres_[k];
+        if ((e = r.arena.reserve(std::max<size_t>(arena, 256))) != hipSuccess) return hipFail(e, "wavefront state");
+        if ((e = r.rad.reserve(std::max<size_t>(rad, 16))) != hipSuccess) return hipFail(e, "sample buffer");
+        if ((e = r.qcount.reserve(std::max(ncnt, 1) * sizeof(uint32_t))) != hipSuccess) return hipFail(e, "queue counters");
+        if ((e = r.gstack.reserve((size_t)blocksTrace * OMV_BLOCK * (OMV_STACK_TOTAL - OMV_STACK_LDS) * sizeof(uint32_t))) != hipSuccess)
+            return hipFail(e, "stack spill");
+    }
+    if (tuning_.use_graph) {
+        if ((e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal)) != hipSuccess) return hipFail(e, "begin capture");
+        int rc = launchWavefrontJobs();
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(stream_, &g);
+        if (rc != OMV_OK || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc != OMV_OK ? rc : hipFail(e, "graph capture");
+        }
+        graph_ = g;
+        if ((e = hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0)) != hipSuccess) { dropGraph(); return hipFail(e, "graph instantiate"); }
+    }
+    return OMV_OK;
+}
+
+int BatchRunner::launchWavefrontJobs() {
+    hipError_t e;
+    const int ns = tuning_.streams;
+    if ((e = launchFillU64(dCounters_.as<unsigned long long>(), 16, stream_)) != hipSuccess) return hipFail(e, "counter fill");
+    if ((e = hipEventRecord(fork_, stream_)) != hipSuccess) return hipFail(e, "fork");
+    for (int k = 0; k < ns; ++k)
+        if ((e = hipStreamWaitEvent(sres_[k].s, fork_, 0)) != hipSuccess) return hipFail(e, "fork wait");
+    for (size_t i = 0; i < order_.size(); ++i) {
+        const Job& J = jobs_[order_[i]];
+        StreamRes& r = sres_[i % ns];
+        float* film = dFilm_.as<float>() + J.filmOff;
+        const int spp = std::max(J.p.samples_per_pixel, 0);
+        if (spp == 0) {   // PinholeCamera::stop with a zero count: rgb 0, alpha 1
+            if ((e = launchFilm(r.rad.as<float>(), 0, 0, J.npix, film, r.s)) != hipSuccess) return hipFail(e, "film kernel launch");
+            continue;
+        }
+        const int lvCap = lvCapOf(J.p);
+        WfLaunch L;
+        std::memset(&L, 0, sizeof(L));
+        L.S = J.scene->view(); L.cam = J.cam; L.p = J.p; L.lvCap = lvCap;
+        L.lc = std::max(1, std::min(lvCap, std::max(J.p.max_path_length, 1)));
+        L.arena = r.arena.ptr; L.qcount = r.qcount.as<uint32_t>(); L.rad = r.rad.as<float>(); L.gstack = r.gstack.as<uint32_t>();
+        L.counters = dCounters_.as<unsigned long long>();
+        L.blocksTrace = cus_ * wfPerCu_[0];
+        L.blocksShade = cus_ * wfPerCu_[1];
+        L.blocksGen = cus_ * wfPerCu_[3];
+        const uint64_t chunkPix = std::max<uint64_t>(1, wfMaxSlots_ / (uint64_t)spp);
+        for (uint64_t pix0 = 0; pix0 < J.npix; pix0 += chunkPix) {
+            const uint64_t np = std::min(chunkPix, J.npix - pix0);
+            L.sample0 = pix0 * (uint64_t)spp;
+            L.nslots = (uint32_t)(np * (uint64_t)spp);
+            if ((e = launchWavefront(L, r.s)) != hipSuccess) return hipFail(e, "wavefront launch");
+            if ((e = launchFilm(r.rad.as<float>(), spp, pix0, np, film, r.s)) != hipSuccess) return hipFail(e, "film kernel launch");
+        }
+    }
+    for (int k = 0; k < ns; ++k) {
+        if ((e = hipEventRecord(sres_[k].done, sres_[k].s)) != hipSuccess) return hipFail(e, "join");
+        if ((e = hipStreamWaitEvent(stream_, sres_[k].done, 0)) != hipSuccess) return hipFail(e, "join wait");
+    }
+    return OMV_OK;
+}
+
 int BatchRunner::launch() {
+    if (tuning_.streams > 0) return launchWavefrontJobs();
     hipError_t e = launchBatch(dJobs_.ptr, (int)jobs_.size(), dTileEnd_.as<uint64_t>(), totalTiles_, tuning_.tile_waves,
                                dCounters_.as<unsigned long long>(), dLv_.as<float>(), dStack_.as<uint32_t>(), blocks_, stream_);
     if (e != hipSuccess) return hipFail(e, "batch kernel launch");
@@ -187,19 +303,21 @@ int BatchRunner::autotune(int reps, omv_batch_tuning* best) {
     omv_batch_tuning bestT = start;
     double bestMs = 1e300;
     const int tiles[] = {1, 2, 4};
+    std::vector<omv_batch_tuning> cands;
+    for (int ns : {1, 2, 4, 8}) cands.push_back(omv_batch_tuning{0, 1, start.use_graph, ns});
     for (int bpc = 1; bpc <= occ; ++bpc)
-        for (int tw : tiles) {
-            omv_batch_tuning t{bpc, tw, start.use_graph};
-            int rc = setTuning(t);
+        for (int tw : tiles) cands.push_back(omv_batch_tuning{bpc, tw, start.use_graph, 0});
+    for (const omv_batch_tuning& t : cands) {
+        int rc = setTuning(t);
+        if (rc) return rc;
+        double m = 1e300;
+        for (int r = 0; r < reps; ++r) {
+            rc = run();
             if (rc) return rc;
-            double m = 1e300;
-            for (int r = 0; r < reps; ++r) {
-                rc = run();
-                if (rc) return rc;
-                m = std::min(m, stats_.render_ms);
-            }
-            if (m < bestMs) { bestMs = m; bestT = t; }
+            m = std::min(m, stats_.render_ms);
         }
+        if (m < bestMs) { bestMs = m; bestT = t; }
+    }
     int rc = setTuning(bestT);
     if (rc) return rc;
     if (best) *best = bestT;

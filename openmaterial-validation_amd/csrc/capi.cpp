@@ -205,6 +205,22 @@ int omv_renderer_get_stats(const omv_renderer* r, omv_render_stats* out) {
     *out = r->r.stats();
     return OMV_OK;
 }
+int omv_renderer_post(omv_renderer* r, omv_camera* c, float gamma, int denoise, float* out_f32, uint8_t* out_u8) {
+    OMV_NULLCHK(r); OMV_NULLCHK(c);
+    return r->r.post(c->c, gamma, denoise != 0, out_f32, out_u8);
+}
+int omv_renderer_backward(omv_renderer* r, omv_camera* c, const float* d_film, const omv_adjoint_targets* t) {
+    OMV_NULLCHK(r); OMV_NULLCHK(c); OMV_NULLCHK(t);
+    return r->r.backward(c->c, d_film, *t);
+}
+int omv_scene_update_texture(omv_scene* s, int tex_id, const void* texels) {
+    OMV_NULLCHK(s);
+    return s->s.updateTexture(tex_id, texels);
+}
+int omv_scene_update_material_om_nk(omv_scene* s, int mat_id, const float n[3], const float k[3]) {
+    OMV_NULLCHK(s); OMV_NULLCHK(n); OMV_NULLCHK(k);
+    return s->s.updateMaterialOmNk(mat_id, n, k);
+}
 int omv_renderer_set_schedule(omv_renderer* r, int schedule) {
     OMV_NULLCHK(r);
     return r->r.setSchedule(schedule);

@@ -23,144 +23,6 @@
 #include "path_device.h"
 
 // ============================================================== kernels
-extern __shared__ uint32_t omv_lds_stack[];
-
-// Per-lane state of a path kernel: traversal stack, counters and this lane's
-// column of the light-vertex scratch ([vertex][component][lane]).
-struct Lane {
-    Stack st;
-    Counters cnt;
-    uint32_t rays, camRays, lightRays, shadowRays;
-    float* lv;
-    uint64_t lvStride;
-    DEV float& lvAt(int vtx, int c) { return lv[((uint64_t)vtx * 6 + c) * lvStride]; }
-};
-
-DEV void laneInit(Lane& L, uint32_t* gstack, float* lv, uint32_t gtid, uint32_t gthreads) {
-    L.st.lds = omv_lds_stack + threadIdx.x; L.st.ldsStride = blockDim.x;
-    L.st.glb = gstack + gtid; L.st.glbStride = gthreads;
-    L.st.sp = 0;
-    L.cnt.nodes = 0; L.cnt.tris = 0; L.cnt.overflow = 0;
-    L.rays = L.camRays = L.lightRays = L.shadowRays = 0;
-    L.lv = lv + gtid; L.lvStride = gthreads;
-}
-
-// One sample of BDPTRenderer::computePaths (BDPTRenderer.cpp:237-301):
-// sample s = pixel * spp + k.  kLights = false compiles no light subpath
-// and no connection code (scenes without point lights, BDPTRenderer.cpp:241-242).
-template <bool kLights>
-DEV f3 pathSample(const OmvSceneView& S, const omv_camera_desc& cam, const RP& P, int spp, uint64_t s, Lane& L) {
-    const int q = P.q;
-    const int lvCap = max(min(P.light_bounces + 1, max(q, 1)), 1);
-    const uint64_t pix = s / (uint64_t)spp;
-    const int x = (int)(pix % (uint64_t)cam.width), y = (int)(pix / (uint64_t)cam.width);
-    Rng cs; cs.init(P.seed, STREAM_CAMERA, s);
-    float u0 = cs.next(), u1 = cs.next();
-    f3 O, D;
-    cameraRay(cam, x, y, u0, u1, O, D);
-
-    // ---- light subpath (tryComputeLightPath + computePath(light), BDPTRenderer.cpp:128-206)
-    int lpLen = 0, lightIdx = -1;
-    f3 lightPos = mk3(0.0f, 0.0f, 0.0f), lightInt = mk3(0.0f, 0.0f, 0.0f);
-    float lightRange = FLT_MAX;
-    if (kLights && S.num_lights > 0) {
-        Rng ls; ls.init(P.seed, STREAM_LIGHT, s);
-        lightIdx = sampleLight(S, ls.next());
-        const OmvLight Lt = S.lights[lightIdx];
-        lightPos = ld3(Lt.pos); lightInt = ld3(Lt.intensity); lightRange = Lt.range;
-        f3 dir = uniformSphere(ls);
-        const f3 lRad = mk3(Lt.intensity[0] * P.light_scale, Lt.intensity[1] * P.light_scale, Lt.intensity[2] * P.light_scale);
-        // vertex 0: the light itself
-        L.lvAt(0, 0) = lightPos.x; L.lvAt(0, 1) = lightPos.y; L.lvAt(0, 2) = lightPos.z;
-        L.lvAt(0, 3) = lRad.x; L.lvAt(0, 4) = lRad.y; L.lvAt(0, 5) = lRad.z;
-        f3 pPos = lightPos, pDir = dir, of = mk3(1.0f, 1.0f, 1.0f);
-        float tnear = 0.0f, rD = 1.0f;
-        int n = 1;
-        // exact shortcut: light vertices with index >= max_path_length are never connected
-        for (int i = 1; i < lvCap; ++i) {
-            float delta = 3.0f * FLT_EPSILON * maxAbs(pPos);
-            f3 org = mk3(pPos.x + delta * pDir.x, pPos.y + delta * pDir.y, pPos.z + delta * pDir.z);
-            ++L.lightRays;
-            Hit h;
-            if (!sceneIntersect(S, org, pDir, tnear, FLT_MAX, h, L.st, L.cnt, L.rays)) break;
-            SP sp;
-            computeSurfacePoint(S, h, sp, false);
-            modifyFrame(S, sp);
-            f3 outRay, brdf, erad;
-            matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, false, ls);
-            float ct = fabsf(dot3(pDir, sp.n));
-            of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
-            if (i == 1) rD = attenuation(P, length3(sub3(sp.pos, lightPos)), lightRange);
-            f3 outRad = mk3(of.x * lRad.x * rD, of.y * lRad.y * rD, of.z * lRad.z * rD);
-            if (intensity(outRad) < P.cut) break;
-            L.lvAt(i, 0) = sp.pos.x; L.lvAt(i, 1) = sp.pos.y; L.lvAt(i, 2) = sp.pos.z;
-            L.lvAt(i, 3) = outRad.x; L.lvAt(i, 4) = outRad.y; L.lvAt(i, 5) = outRad.z;
-            ++n;
-            pPos = sp.pos; pDir = outRay;
-        }
-        lpLen = n;
-    }
-
-    // ---- camera subpath + connections
-    f3 radiance = mk3(0.0f, 0.0f, 0.0f), conn = mk3(0.0f, 0.0f, 0.0f);
-    f3 pPos = O, pDir = D, of = mk3(1.0f, 1.0f, 1.0f);
-    float tnear = 0.0f;
-    int n = 1;
-    bool exitScene = false;
-    for (int i = 1; i < P.camera_bounces + 1; ++i) {
-        float delta = 3.0f * FLT_EPSILON * maxAbs(pPos);
-        f3 org = mk3(pPos.x + delta * pDir.x, pPos.y + delta * pDir.y, pPos.z + delta * pDir.z);
-        ++L.camRays;
-        Hit h;
-        if (!sceneIntersect(S, org, pDir, tnear, FLT_MAX, h, L.st, L.cnt, L.rays)) { exitScene = true; break; }
-        SP sp;
-        computeSurfacePoint(S, h, sp, false);
-        modifyFrame(S, sp);
-        // connections of camera vertex j = i with light vertices l, l + j <= q
-        if (kLights && lightIdx >= 0) {
-            const int lEnd = min(lpLen, q - i + 1);
-            const f3 outV = neg3(pDir);
-            for (int l = 0; l < lEnd; ++l) {
-                f3 lPos = mk3(L.lvAt(l, 0), L.lvAt(l, 1), L.lvAt(l, 2));
-                f3 lRad;
-                f3 LC = normalize3(sub3(sp.pos, lPos));
-                if (l == 0) {
-                    float lsc = attenuation(P, length3(sub3(sp.pos, lPos)), lightRange);
-                    float k = P.light_scale * lsc;
-                    lRad = mk3(lightInt.x * k, lightInt.y * k, lightInt.z * k);
-                } else {
-                    lRad = mk3(L.lvAt(l, 3), L.lvAt(l, 4), L.lvAt(l, 5));
-                }
-                float c = fabsf(dot3(LC, sp.n));
-                f3 lc = mk3(lRad.x * c * of.x, lRad.y * c * of.y, lRad.z * c * of.z);
-                if (intensity(lc) < P.cut) continue;
-                f3 brdf = matGetBrdf(S, P.x, sp, LC, outV);
-                brdf = mk3(brdf.x * c, brdf.y * c, brdf.z * c);
-                lc = mk3(brdf.x * of.x * lRad.x, brdf.y * of.y * lRad.y, brdf.z * of.z * lRad.z);
-                if (intensity(lc) < P.cut) continue;
-                if (!isConnected(S, lPos, sp.pos, L.st, L.cnt, L.rays, L.shadowRays)) continue;
-                conn = mk3(lc.x + conn.x, lc.y + conn.y, lc.z + conn.z);
-            }
-        }
-        f3 outRay, brdf, erad;
-        matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, true, cs);
-        float ct = fabsf(dot3(outRay, sp.n));
-        of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
-        radiance = mk3(radiance.x + of.x * erad.x, radiance.y + of.y * erad.y, radiance.z + of.z * erad.z);
-        ++n;
-        pPos = sp.pos; pDir = outRay;
-        // exact shortcuts, as in the oracle: zero throughput, or no emission past q
-        if (of.x == 0.0f && of.y == 0.0f && of.z == 0.0f) break;
-        if (!S.any_emissive && n > q) break;
-    }
-    if (exitScene && S.bg_type != OMV_BG_NONE && n <= q) {
-        f3 bg = background(S, pDir);
-        bg = mk3(bg.x * of.x, bg.y * of.y, bg.z * of.z);
-        radiance = mk3(bg.x * 1.0f + radiance.x, bg.y * 1.0f + radiance.y, bg.z * 1.0f + radiance.z);
-    }
-    return mk3(conn.x * 1.0f + radiance.x, conn.y * 1.0f + radiance.y, conn.z * 1.0f + radiance.z);
-}
-
 // Plain render: one sample per lane at a time over [sample0, sample0 + nsamples),
 // persistent grid; radiance per sample for the ordered film reduction.
 template <bool kLights>
@@ -379,6 +241,11 @@ void packBatchJob(void* dst, const OmvSceneView& S, const omv_camera_desc& cam, 
     std::memset(&j, 0, sizeof(j));
     j.S = S; j.cam = cam; j.P = toRP(p); j.spp = p.samples_per_pixel; j.npix = npix; j.film = film;
     std::memcpy(dst, &j, sizeof(j));
+}
+
+hipError_t launchFillU64(unsigned long long* p, int n, hipStream_t stream) {
+    hipLaunchKernelGGL(omv_fill_u64, dim3((n + 63) / 64), dim3(64), 0, stream, p, n);
+    return hipGetLastError();
 }
 
 hipError_t launchBatch(const void* jobs, int njobs, const uint64_t* tileEnd, uint64_t totalTiles, int tileWaves,

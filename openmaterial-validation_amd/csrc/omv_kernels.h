@@ -25,6 +25,7 @@ hipError_t launchFc(const OmvSceneView& S, const omv_camera_desc& cam, uint32_t 
 // writes one record of batchJobBytes() bytes.  tileEnd[j] is the exclusive
 // end of job j's range of work tiles (64 * tileWaves pixels each).
 // counters: 16 u64 (8 statistics + the work counter), cleared by the launch.
+hipError_t launchFillU64(unsigned long long* p, int n, hipStream_t stream);   // zero-fill, graph-capturable
 size_t batchJobBytes();
 void packBatchJob(void* dst, const OmvSceneView& S, const omv_camera_desc& cam, const omv_renderer_params& p, float* film,
                   uint64_t npix);
@@ -53,6 +54,15 @@ size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights);
 int wavefrontCounterCount(int B, int lvCap);
 void wavefrontBlocksPerCU(int out[4]);   // trace, shade, shade_light, gen
 hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream);
+// Path-replay adjoint over samples [sample0, sample0 + nsamples) (adjoint.hip).
+int adjointBlocksPerCU();
+hipError_t launchAdjoint(const OmvSceneView& S, const omv_camera_desc& cam, const omv_renderer_params& p, uint64_t sample0,
+                         uint64_t nsamples, const float* dfilm, const int* texSlot, float* const* texGrad, const int* omSlot,
+                         float* omGrad, float* lv, uint32_t* gstack, unsigned long long* counters, int blocks, hipStream_t stream);
+// Film post pass (post.hip): gamma (+ 3x3 median when denoise) into outF, and
+// the 8-bit truncation of outF into outU8 when non-null; tmp holds w*h float4.
+hipError_t launchPost(const float* film, float* tmp, float* outF, uint8_t* outU8, int w, int h, float A, float gamma, bool denoise,
+                      hipStream_t stream);
 // 0: sample kernel (scene with lights), 1: FC kernel, 2: sample kernel (no lights), 3: batch kernel
 int occupancyBlocksPerCU(int which);
 

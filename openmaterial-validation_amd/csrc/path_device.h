@@ -175,11 +175,12 @@ struct Bsdf {
     float c0, c1, c2, c3;   // base colour rgba
     bool transmissive;
     float metal, rough, alpha, eta;
+    int lobe;               // sampled lobe: 0 none, 1 diffuse, 2 reflection, 3 transmission
 };
 DEV void bsdfInit(Bsdf& b) {
     b.I = b.O = b.N = b.H = b.T = mk3(0.0f, 0.0f, 0.0f);
     b.c0 = b.c1 = b.c2 = b.c3 = 1.0f; b.transmissive = false;
-    b.metal = 0.5f; b.rough = 0.0f; b.alpha = 0.25f; b.eta = 1.0f;
+    b.metal = 0.5f; b.rough = 0.0f; b.alpha = 0.25f; b.eta = 1.0f; b.lobe = 0;
 }
 // BSDFModel::evaluate (BRDFModel.h:139-207)
 DEV f3 bsdfEvaluate(const f3& vIn, const f3& vOut, const f3& N, const Bsdf& b, bool specular, bool transmitted, const f3& Hin, float P) {
@@ -247,12 +248,14 @@ DEV f3 sampleIndirect(Bsdf& d, Rng& r) {
             d.H = normalize3(toBasis(b0, b1, d.N, rd));
             if (refract3(d.I, d.H, d.eta, d.O) && dot3(d.N, d.O) < 0.0f) {
                 d.O = normalize3(d.O);
+                d.lobe = 3;
                 return bsdfEvaluate(d.I, d.O, d.N, d, false, true, d.H, P);
             }
             return mk3(0.0f, 0.0f, 0.0f);
         }
         f3 rd = cosineSqrtHemisphere(r);
         d.O = normalize3(toBasis(b0, b1, d.N, rd));
+        d.lobe = 1;
         return bsdfEvaluate(d.I, d.O, d.N, d, false, false, d.H, P);
     }
     f3 rd = ggxHemisphere(r, powf(d.alpha, 2.0f));
@@ -260,9 +263,67 @@ DEV f3 sampleIndirect(Bsdf& d, Rng& r) {
     d.O = reflect3(d.I, d.H);
     if (dot3(d.N, d.O) > 0.0f) {
         d.O = normalize3(d.O);
+        d.lobe = 2;
         return bsdfEvaluate(d.I, d.O, d.N, d, true, false, d.H, P);
     }
     return mk3(0.0f, 0.0f, 0.0f);
+}
+
+// ----------------------------------------------- detached lobe derivatives
+// With the sampled lobe and direction held fixed, a PBR bounce weighs
+// w_ref(theta0) f(theta) / f(theta0), f being the sampled lobe's integrand up
+// to material-independent factors: diffuse (1-F)(1-metal)color, reflection
+// F*D*G, transmission (1-F)*D*G, on bsdfEvaluate's geometry.  These give
+// d ln f / d(color_c, metal, alpha) for the path-replay adjoint.
+struct LobeGeo { int lobe; float p5, NH, VN, LN; };
+DEV LobeGeo lobeGeo(const Bsdf& d) {
+    LobeGeo g;
+    g.lobe = d.lobe;
+    f3 H = d.lobe == 3 ? d.H : normalize3(sub3(d.O, d.I));
+    g.VN = clampr(fabsf(dot3(d.I, d.N)), 0.0001f, 1.0f);
+    g.LN = clampr(fabsf(dot3(d.O, d.N)), 0.0001f, 1.0f);
+    g.NH = fabsf(dot3(d.N, H));
+    float VH = fabsf(dot3(d.I, H));
+    g.p5 = powf(clamp01(1.0f - VH), 5.0f);
+    return g;
+}
+DEV float dlnSmithG1(float c, float alpha) {
+    if (!(alpha > 0.0f || c < 0.99f)) return 0.0f;
+    float a = c / (alpha * sqrtf(1.0f - c * c));
+    float ai2 = powf(a, -2.0f);
+    float lam = 0.5f * (sqrtf(1.0f + ai2) - 1.0f);
+    float dlam = ai2 / (2.0f * alpha * sqrtf(1.0f + ai2));
+    return -dlam / (1.0f + lam);
+}
+DEV void lobeDlog(const LobeGeo& g, const float color[3], float metal, float alpha, float dC[3], float dM[3], float dA[3]) {
+    float dlnDG = 0.0f, DG = 1.0f;
+    if (g.lobe >= 2) {
+        float c2 = g.NH * g.NH;
+        float L = lerpr(1.0f, alpha * alpha, c2);
+        float dlnD = 2.0f / alpha - 4.0f * alpha * c2 / L;
+        float x = smithG1(g.VN, alpha), y = smithG1(g.LN, alpha);
+        float dx = x * dlnSmithG1(g.VN, alpha), dy = y * dlnSmithG1(g.LN, alpha);
+        float den = x + y - x * y;
+        dlnDG = dlnD + dx / x + dy / y - (dx + dy - dx * y - x * dy) / den;
+        DG = ggxPdf(g.NH, alpha) * smithG2(x, y);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        dC[c] = dM[c] = dA[c] = 0.0f;
+        float F0 = lerpr(0.04f, color[c], metal);
+        float F = lerpr(g.p5, 1.0f, F0);
+        float f = g.lobe == 1 ? (1.0f - F) * (1.0f - metal) * color[c] : (g.lobe == 2 ? F * DG : (1.0f - F) * DG);
+        if (g.lobe == 0 || !(f != 0.0f)) continue;
+        float dFdC = (1.0f - g.p5) * metal, dFdM = (1.0f - g.p5) * (color[c] - 0.04f);
+        if (g.lobe == 1) {
+            dC[c] = -dFdC / (1.0f - F) + 1.0f / color[c];
+            dM[c] = -dFdM / (1.0f - F) - 1.0f / (1.0f - metal);
+        } else if (g.lobe == 2) {
+            dC[c] = dFdC / F; dM[c] = dFdM / F; dA[c] = dlnDG;
+        } else {
+            dC[c] = -dFdC / (1.0f - F); dM[c] = -dFdM / (1.0f - F); dA[c] = dlnDG;
+        }
+    }
 }
 
 // ---------------------------------------------------------- complex IOR
@@ -307,14 +368,48 @@ DEV void fresnel(float n, float k, float cosT, float& Rp, float& Rs) {
     Rp = rp.re * rp.re + rp.im * rp.im;
     Rs = rs.re * rs.re + rs.im * rs.im;
 }
+// d ln(Rp + Rs) / d(n, k) at fixed cos: with N2 = (n+ik)^2, u = sqrt(N2 - s^2),
+// d rp/dN2 = c (2u^2 - N2) / (u (N2 c + u)^2), d rs/dN2 = -c / (u (c + u)^2),
+// dN2/dn = 2N, dN2/dk = 2iN, d|r|^2 = 2 Re(conj(r) dr).
+DEV void fresnelDlog(float n, float k, float cosT, float& dn, float& dk) {
+    cpx N; N.re = n; N.im = k;
+    cpx N2 = cmul(N, N);
+    cosT = fabsf(cosT);
+    const float c2 = cosT * cosT, s2 = 1.0f - c2;
+    cpx z; z.re = N2.re - s2; z.im = N2.im;
+    cpx u = csqrt_(z);
+    cpx a; a.re = N2.re * cosT; a.im = N2.im * cosT;
+    cpx num, den;
+    num.re = a.re - u.re; num.im = a.im - u.im; den.re = a.re + u.re; den.im = a.im + u.im;
+    cpx rp = cdiv(num, den);
+    num.re = cosT - u.re; num.im = -u.im; den.re = cosT + u.re; den.im = u.im;
+    cpx rs = cdiv(num, den);
+    cpx apu; apu.re = a.re + u.re; apu.im = a.im + u.im;
+    cpx cpu; cpu.re = cosT + u.re; cpu.im = u.im;
+    cpx uu = cmul(u, u);
+    cpx t1; t1.re = cosT * (2.0f * uu.re - N2.re); t1.im = cosT * (2.0f * uu.im - N2.im);
+    cpx drp = cdiv(t1, cmul(u, cmul(apu, apu)));
+    cpx mc; mc.re = -cosT; mc.im = 0.0f;
+    cpx drs = cdiv(mc, cmul(u, cmul(cpu, cpu)));
+    cpx dN2n; dN2n.re = 2.0f * N.re; dN2n.im = 2.0f * N.im;
+    cpx dN2k; dN2k.re = -2.0f * N.im; dN2k.im = 2.0f * N.re;
+    cpx crp; crp.re = rp.re; crp.im = -rp.im;
+    cpx crs; crs.re = rs.re; crs.im = -rs.im;
+    const float F = (rp.re * rp.re + rp.im * rp.im) + (rs.re * rs.re + rs.im * rs.im);
+    if (!(F != 0.0f)) { dn = dk = 0.0f; return; }
+    float dRn = 2.0f * (cmul(crp, cmul(drp, dN2n)).re + cmul(crs, cmul(drs, dN2n)).re);
+    float dRk = 2.0f * (cmul(crp, cmul(drp, dN2k)).re + cmul(crs, cmul(drs, dN2k)).re);
+    dn = dRn / F;
+    dk = dRk / F;
+}
 
 // ------------------------------------------------------------ textures
 // BitmapTexture::getPixCoords + texture (BitmapTexture.cpp:181-234); fp32
 // textures (an extension) use the same addressing without the 1/255 scale.
-DEV f4 texLookup(const OmvSceneView& S, int id, float u, float v) {
-    const OmvTexture t = S.texs[id];
-    f4 out;
-    if (t.w <= 0 || t.h <= 0) { out.x = out.y = out.z = 0.5f; out.w = 1.0f; return out; }
+// The four texels and bilinear weights of (u, v) (BitmapTexture::getPixCoords,
+// texel-centre wrap addressing); false for a missing image.
+DEV bool texAddress(const OmvTexture& t, float u, float v, int id[4], float w[4]) {
+    if (t.w <= 0 || t.h <= 0) return false;
     if (!isfinite(u)) u = 0.0f;
     if (!isfinite(v)) v = 0.0f;
     const float W = (float)t.w, H = (float)t.h;
@@ -326,28 +421,36 @@ DEV f4 texLookup(const OmvSceneView& S, int id, float u, float v) {
     float ku = 1.0f - kui, kv = 1.0f - kvi;
     int l = iu % t.w, r = (iu + 1) % t.w, b = iv % t.h, tt = (iv + 1) % t.h;
     const int n = t.w * t.h - 1;
-    int i0 = min(max(b * t.w + l, 0), n), i1 = min(max(b * t.w + r, 0), n);
-    int i2 = min(max(tt * t.w + l, 0), n), i3 = min(max(tt * t.w + r, 0), n);
-    float w0 = ku * kv, w1 = kui * kv, w2 = ku * kvi, w3 = kui * kvi;
+    id[0] = min(max(b * t.w + l, 0), n); id[1] = min(max(b * t.w + r, 0), n);
+    id[2] = min(max(tt * t.w + l, 0), n); id[3] = min(max(tt * t.w + r, 0), n);
+    w[0] = ku * kv; w[1] = kui * kv; w[2] = ku * kvi; w[3] = kui * kvi;
+    return true;
+}
+DEV f4 texLookup(const OmvSceneView& S, int id, float u, float v) {
+    const OmvTexture t = S.texs[id];
+    f4 out;
+    int i[4];
+    float w[4];
+    if (!texAddress(t, u, v, i, w)) { out.x = out.y = out.z = 0.5f; out.w = 1.0f; return out; }
     if (t.fmt == OMV_TEX_RGBA8) {
         const uint32_t* px = (const uint32_t*)(S.texels + t.offset);
-        uint32_t p0 = px[i0], p1 = px[i1], p2 = px[i2], p3 = px[i3];
+        uint32_t p0 = px[i[0]], p1 = px[i[1]], p2 = px[i[2]], p3 = px[i[3]];
         const float s = 1.0f / 255.0f;
         float o[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             int sh = 8 * c;
-            o[c] = (float)((p0 >> sh) & 0xFF) * w0 * s + (float)((p1 >> sh) & 0xFF) * w1 * s +
-                   (float)((p2 >> sh) & 0xFF) * w2 * s + (float)((p3 >> sh) & 0xFF) * w3 * s;
+            o[c] = (float)((p0 >> sh) & 0xFF) * w[0] * s + (float)((p1 >> sh) & 0xFF) * w[1] * s +
+                   (float)((p2 >> sh) & 0xFF) * w[2] * s + (float)((p3 >> sh) & 0xFF) * w[3] * s;
         }
         out.x = o[0]; out.y = o[1]; out.z = o[2]; out.w = o[3];
     } else {
         const float4* px = (const float4*)(S.texels + t.offset);
-        float4 p0 = px[i0], p1 = px[i1], p2 = px[i2], p3 = px[i3];
-        out.x = p0.x * w0 + p1.x * w1 + p2.x * w2 + p3.x * w3;
-        out.y = p0.y * w0 + p1.y * w1 + p2.y * w2 + p3.y * w3;
-        out.z = p0.z * w0 + p1.z * w1 + p2.z * w2 + p3.z * w3;
-        out.w = p0.w * w0 + p1.w * w1 + p2.w * w2 + p3.w * w3;
+        float4 p0 = px[i[0]], p1 = px[i[1]], p2 = px[i[2]], p3 = px[i[3]];
+        out.x = p0.x * w[0] + p1.x * w[1] + p2.x * w[2] + p3.x * w[3];
+        out.y = p0.y * w[0] + p1.y * w[1] + p2.y * w[2] + p3.y * w[3];
+        out.z = p0.z * w[0] + p1.z * w[1] + p2.z * w[2] + p3.z * w[3];
+        out.w = p0.w * w[0] + p1.w * w[1] + p2.w * w[2] + p3.w * w[3];
     }
     return out;
 }
@@ -631,14 +734,23 @@ DEV f3 matGetBrdf(const OmvSceneView& S, const Ext& X, const SP& sp, const f3& i
     return mk3(brdfPdf * M.diffuse[0], brdfPdf * M.diffuse[1], brdfPdf * M.diffuse[2]);
 }
 // RenderMaterial::getRayAndBrdf of the three materials (computePath, BDPTRenderer.cpp:158-159)
+// What the adjoint needs from a bounce (kind 1 PBR: the sampled Bsdf state and
+// the occlusion input; kind 2 OM mirror: the Fresnel cosine).
+struct BounceInfo {
+    int kind;
+    Bsdf b;
+    float occ, roughRaw, fCos;
+};
 DEV void matGetRayAndBrdf(const OmvSceneView& S, const Ext& X, const SP& sp, const f3& inV, f3& outV, f3& brdf, f3& erad, bool inv,
-                          Rng& r) {
+                          Rng& r, BounceInfo* info = nullptr) {
     const OmvMaterial& M = S.mats[S.meshes[sp.mesh].mat_id];
     erad = mk3(0.0f, 0.0f, 0.0f);
+    if (info) info->kind = 0;
     if (M.type == OMV_MAT_OM) {
         if (X.omRough && M.has_base) { brdf = omRoughSample(M, X.omAlpha, inV, sp, inv, outV, r); return; }
         outV = omNextDirection(S, inV, sp);
         brdf = M.has_base ? omFresnelBrdf(M, dot3(outV, sp.n)) : mk3(0.0f, 0.0f, 0.0f);
+        if (info && M.has_base) { info->kind = 2; info->fCos = dot3(outV, sp.n); }
         return;
     }
     if (M.type == OMV_MAT_PBR) {
@@ -651,6 +763,7 @@ DEV void matGetRayAndBrdf(const OmvSceneView& S, const Ext& X, const SP& sp, con
         brdf = sampleIndirect(b, r);
         outV = b.O;
         if (M.emissive) erad = pbrEmission(S, sp, M);
+        if (info) { info->kind = 1; info->b = b; info->occ = omr[0]; info->roughRaw = omr[2]; }
         return;
     }
     f3 v = uniformHemisphere(r);
@@ -744,13 +857,13 @@ DEV void offer(Hit& h, bool& found, float t, float u, float v, int inst, int pri
 }
 
 // 4-wide slab test, conservative (Ize 2013: tmax scaled by 1 + 2*gamma(3))
-DEV void boxTest4(const OmvNode4& nd, const Pre& p, float tnear, float tfar, float tent[4], bool hit[4]) {
+DEV void boxTest4(const OmvNode4& nd, const f3& o, const f3& idir, float tnear, float tfar, float tent[4], bool hit[4]) {
     const float kUp = 1.0000004f;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        float t0x = (nd.lox[c] - p.o.x) * p.idir.x, t1x = (nd.hix[c] - p.o.x) * p.idir.x;
-        float t0y = (nd.loy[c] - p.o.y) * p.idir.y, t1y = (nd.hiy[c] - p.o.y) * p.idir.y;
-        float t0z = (nd.loz[c] - p.o.z) * p.idir.z, t1z = (nd.hiz[c] - p.o.z) * p.idir.z;
+        float t0x = (nd.lox[c] - o.x) * idir.x, t1x = (nd.hix[c] - o.x) * idir.x;
+        float t0y = (nd.loy[c] - o.y) * idir.y, t1y = (nd.hiy[c] - o.y) * idir.y;
+        float t0z = (nd.loz[c] - o.z) * idir.z, t1z = (nd.hiz[c] - o.z) * idir.z;
         float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tnear));
         float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tfar));
         tent[c] = tmin;
@@ -784,7 +897,8 @@ DEV void pushSorted(Stack& st, Counters& c, uint32_t e[4], float t[4], bool h[4]
 // against the current closest t, triangles accept t in (tnear, t_closest],
 // and offer() keeps the (t, instance, primitive) minimum.
 struct Trav {
-    Pre wp, op;
+    f3 wo, wd, widir;   // world ray (TLAS boxes, instance transforms)
+    Pre op;             // object ray of the current instance (BLAS boxes, triangles)
     Hit h;
     float tnear, tfar;
     int curInst;
@@ -794,8 +908,10 @@ struct Trav {
         h.tie = false; h.t = tf; h.u = 0.0f; h.v = 0.0f; h.inst = -1; h.prim = 0;
         tnear = tn; tfar = tf;
         st.sp = 0;
-        if (!rayPrep(o, d, wp)) return false;
-        op = wp;
+        const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+        if (!(ax > 0.0f || ay > 0.0f || az > 0.0f)) return false;   // zero / NaN direction: no hit (as rayPrep)
+        wo = o; wd = d;
+        widir = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
         curInst = -1;
         st.push((E_TLAS_NODE << E_KIND_SHIFT) | (uint32_t)S.tlas_root, c);
         return true;
@@ -821,7 +937,7 @@ struct Trav {
             const int inst = (int)(idx & 0x1FFFFFFFu);
             const OmvInstance& in = S.insts[inst];
             const float* I = in.inv;
-            const f3 o = wp.o, d = wp.d;
+            const f3 o = wo, d = wd;
             f3 oo = mk3(I[0] * o.x + I[1] * o.y + I[2] * o.z + I[3], I[4] * o.x + I[5] * o.y + I[6] * o.z + I[7],
                         I[8] * o.x + I[9] * o.y + I[10] * o.z + I[11]);
             f3 od = mk3(I[0] * d.x + I[1] * d.y + I[2] * d.z, I[4] * d.x + I[5] * d.y + I[6] * d.z, I[8] * d.x + I[9] * d.y + I[10] * d.z);
@@ -837,7 +953,8 @@ struct Trav {
         loadNode(S.nodes, idx, nd);
         ++c.nodes;
         float tent[4]; bool hit[4]; uint32_t ent[4];
-        boxTest4(nd, tl ? wp : op, tnear, tb, tent, hit);
+        if (tl) boxTest4(nd, wo, widir, tnear, tb, tent, hit);
+        else boxTest4(nd, op.o, op.idir, tnear, tb, tent, hit);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             int32_t ch = nd.child[k];
@@ -992,6 +1109,150 @@ inline RP toRP(const omv_renderer_params& p) {
     r.x.omRough = (p.extensions & OMV_EXT_OM_ROUGH) != 0;
     r.x.omAlpha = p.om_roughness * p.om_roughness;
     return r;
+}
+
+}  // namespace
+
+// ================================================= per-sample path function
+// Dynamic LDS of every path kernel: the traversal stacks, [entry][lane].
+extern __shared__ uint32_t omv_lds_stack[];
+
+namespace {
+
+// Per-lane state of a path kernel: traversal stack, counters and this lane's
+// column of the light-vertex scratch ([vertex][component][lane]).
+struct Lane {
+    Stack st;
+    Counters cnt;
+    uint32_t rays, camRays, lightRays, shadowRays;
+    float* lv;
+    uint64_t lvStride;
+    DEV float& lvAt(int vtx, int c) { return lv[((uint64_t)vtx * 6 + c) * lvStride]; }
+};
+
+DEV void laneInit(Lane& L, uint32_t* gstack, float* lv, uint32_t gtid, uint32_t gthreads) {
+    L.st.lds = omv_lds_stack + threadIdx.x; L.st.ldsStride = blockDim.x;
+    L.st.glb = gstack + gtid; L.st.glbStride = gthreads;
+    L.st.sp = 0;
+    L.cnt.nodes = 0; L.cnt.tris = 0; L.cnt.overflow = 0;
+    L.rays = L.camRays = L.lightRays = L.shadowRays = 0;
+    L.lv = lv + gtid; L.lvStride = gthreads;
+}
+
+// One sample of BDPTRenderer::computePaths (BDPTRenderer.cpp:237-301):
+// sample s = pixel * spp + k.  kLights = false compiles no light subpath
+// and no connection code (scenes without point lights, BDPTRenderer.cpp:241-242).
+template <bool kLights>
+DEV f3 pathSample(const OmvSceneView& S, const omv_camera_desc& cam, const RP& P, int spp, uint64_t s, Lane& L) {
+    const int q = P.q;
+    const int lvCap = max(min(P.light_bounces + 1, max(q, 1)), 1);
+    const uint64_t pix = s / (uint64_t)spp;
+    const int x = (int)(pix % (uint64_t)cam.width), y = (int)(pix / (uint64_t)cam.width);
+    Rng cs; cs.init(P.seed, STREAM_CAMERA, s);
+    float u0 = cs.next(), u1 = cs.next();
+    f3 O, D;
+    cameraRay(cam, x, y, u0, u1, O, D);
+
+    // ---- light subpath (tryComputeLightPath + computePath(light), BDPTRenderer.cpp:128-206)
+    int lpLen = 0, lightIdx = -1;
+    f3 lightPos = mk3(0.0f, 0.0f, 0.0f), lightInt = mk3(0.0f, 0.0f, 0.0f);
+    float lightRange = FLT_MAX;
+    if (kLights && S.num_lights > 0) {
+        Rng ls; ls.init(P.seed, STREAM_LIGHT, s);
+        lightIdx = sampleLight(S, ls.next());
+        const OmvLight Lt = S.lights[lightIdx];
+        lightPos = ld3(Lt.pos); lightInt = ld3(Lt.intensity); lightRange = Lt.range;
+        f3 dir = uniformSphere(ls);
+        const f3 lRad = mk3(Lt.intensity[0] * P.light_scale, Lt.intensity[1] * P.light_scale, Lt.intensity[2] * P.light_scale);
+        // vertex 0: the light itself
+        L.lvAt(0, 0) = lightPos.x; L.lvAt(0, 1) = lightPos.y; L.lvAt(0, 2) = lightPos.z;
+        L.lvAt(0, 3) = lRad.x; L.lvAt(0, 4) = lRad.y; L.lvAt(0, 5) = lRad.z;
+        f3 pPos = lightPos, pDir = dir, of = mk3(1.0f, 1.0f, 1.0f);
+        float tnear = 0.0f, rD = 1.0f;
+        int n = 1;
+        // exact shortcut: light vertices with index >= max_path_length are never connected
+        for (int i = 1; i < lvCap; ++i) {
+            float delta = 3.0f * FLT_EPSILON * maxAbs(pPos);
+            f3 org = mk3(pPos.x + delta * pDir.x, pPos.y + delta * pDir.y, pPos.z + delta * pDir.z);
+            ++L.lightRays;
+            Hit h;
+            if (!sceneIntersect(S, org, pDir, tnear, FLT_MAX, h, L.st, L.cnt, L.rays)) break;
+            SP sp;
+            computeSurfacePoint(S, h, sp, false);
+            modifyFrame(S, sp);
+            f3 outRay, brdf, erad;
+            matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, false, ls);
+            float ct = fabsf(dot3(pDir, sp.n));
+            of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
+            if (i == 1) rD = attenuation(P, length3(sub3(sp.pos, lightPos)), lightRange);
+            f3 outRad = mk3(of.x * lRad.x * rD, of.y * lRad.y * rD, of.z * lRad.z * rD);
+            if (intensity(outRad) < P.cut) break;
+            L.lvAt(i, 0) = sp.pos.x; L.lvAt(i, 1) = sp.pos.y; L.lvAt(i, 2) = sp.pos.z;
+            L.lvAt(i, 3) = outRad.x; L.lvAt(i, 4) = outRad.y; L.lvAt(i, 5) = outRad.z;
+            ++n;
+            pPos = sp.pos; pDir = outRay;
+        }
+        lpLen = n;
+    }
+
+    // ---- camera subpath + connections
+    f3 radiance = mk3(0.0f, 0.0f, 0.0f), conn = mk3(0.0f, 0.0f, 0.0f);
+    f3 pPos = O, pDir = D, of = mk3(1.0f, 1.0f, 1.0f);
+    float tnear = 0.0f;
+    int n = 1;
+    bool exitScene = false;
+    for (int i = 1; i < P.camera_bounces + 1; ++i) {
+        float delta = 3.0f * FLT_EPSILON * maxAbs(pPos);
+        f3 org = mk3(pPos.x + delta * pDir.x, pPos.y + delta * pDir.y, pPos.z + delta * pDir.z);
+        ++L.camRays;
+        Hit h;
+        if (!sceneIntersect(S, org, pDir, tnear, FLT_MAX, h, L.st, L.cnt, L.rays)) { exitScene = true; break; }
+        SP sp;
+        computeSurfacePoint(S, h, sp, false);
+        modifyFrame(S, sp);
+        // connections of camera vertex j = i with light vertices l, l + j <= q
+        if (kLights && lightIdx >= 0) {
+            const int lEnd = min(lpLen, q - i + 1);
+            const f3 outV = neg3(pDir);
+            for (int l = 0; l < lEnd; ++l) {
+                f3 lPos = mk3(L.lvAt(l, 0), L.lvAt(l, 1), L.lvAt(l, 2));
+                f3 lRad;
+                f3 LC = normalize3(sub3(sp.pos, lPos));
+                if (l == 0) {
+                    float lsc = attenuation(P, length3(sub3(sp.pos, lPos)), lightRange);
+                    float k = P.light_scale * lsc;
+                    lRad = mk3(lightInt.x * k, lightInt.y * k, lightInt.z * k);
+                } else {
+                    lRad = mk3(L.lvAt(l, 3), L.lvAt(l, 4), L.lvAt(l, 5));
+                }
+                float c = fabsf(dot3(LC, sp.n));
+                f3 lc = mk3(lRad.x * c * of.x, lRad.y * c * of.y, lRad.z * c * of.z);
+                if (intensity(lc) < P.cut) continue;
+                f3 brdf = matGetBrdf(S, P.x, sp, LC, outV);
+                brdf = mk3(brdf.x * c, brdf.y * c, brdf.z * c);
+                lc = mk3(brdf.x * of.x * lRad.x, brdf.y * of.y * lRad.y, brdf.z * of.z * lRad.z);
+                if (intensity(lc) < P.cut) continue;
+                if (!isConnected(S, lPos, sp.pos, L.st, L.cnt, L.rays, L.shadowRays)) continue;
+                conn = mk3(lc.x + conn.x, lc.y + conn.y, lc.z + conn.z);
+            }
+        }
+        f3 outRay, brdf, erad;
+        matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, true, cs);
+        float ct = fabsf(dot3(outRay, sp.n));
+        of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
+        radiance = mk3(radiance.x + of.x * erad.x, radiance.y + of.y * erad.y, radiance.z + of.z * erad.z);
+        ++n;
+        pPos = sp.pos; pDir = outRay;
+        // exact shortcuts, as in the oracle: zero throughput, or no emission past q
+        if (of.x == 0.0f && of.y == 0.0f && of.z == 0.0f) break;
+        if (!S.any_emissive && n > q) break;
+    }
+    if (exitScene && S.bg_type != OMV_BG_NONE && n <= q) {
+        f3 bg = background(S, pDir);
+        bg = mk3(bg.x * of.x, bg.y * of.y, bg.z * of.z);
+        radiance = mk3(bg.x * 1.0f + radiance.x, bg.y * 1.0f + radiance.y, bg.z * 1.0f + radiance.z);
+    }
+    return mk3(conn.x * 1.0f + radiance.x, conn.y * 1.0f + radiance.y, conn.z * 1.0f + radiance.z);
 }
 
 }  // namespace

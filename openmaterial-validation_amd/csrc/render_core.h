@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <deque>
 #include <string>
 #include <vector>
 
@@ -78,6 +79,10 @@ public:
     int vertices(int id, float* out, size_t cap) const;
     int bvhInfo(int* nodes, int* depth, int* records, size_t* bytes) const;
     int stackBound() const { return stackBound_; }
+    int updateTexture(int texId, const void* src);
+    int updateMaterialOmNk(int matId, const float n[3], const float k[3]);
+    int textureSize(int texId, int* w, int* h) const;
+    bool isOm(int matId) const { return matId >= 0 && matId < (int)mats_.size() && mats_[matId].type == OMV_MAT_OM; }
 
 private:
     std::vector<HostMesh> meshes_;
@@ -94,6 +99,7 @@ private:
     bool committed_ = false;
     int nodeCount_ = 0, bvhDepth_ = 0, recordCount_ = 0, stackBound_ = 0;
     size_t deviceBytes_ = 0;
+    std::vector<uint64_t> texOffsets_;   // byte offsets in the device texel pool
     OmvSceneView view_{};
     DeviceBuffer dVtx_, dFaces_, dMeshes_, dInsts_, dMats_, dTexs_, dTexels_, dNodes_, dTris_, dLights_, dCdf_, dBg_;
 };
@@ -121,6 +127,8 @@ public:
     const omv_render_tuning& tuning() const { return tuning_; }
     int render(PinholeCamera& cam);
     int renderFc(PinholeCamera& cam, int32_t* ids, uint8_t* tie);
+    int backward(PinholeCamera& cam, const float* dFilm, const omv_adjoint_targets& t);
+    int post(PinholeCamera& cam, float gamma, bool denoise, float* outF, uint8_t* outU8);
     const omv_render_stats& stats() const { return stats_; }
 
 private:
@@ -137,7 +145,8 @@ private:
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
     int cus_ = 0;
-    DeviceBuffer dRad_, dLv_, dStack_, dCounters_, dFilm_, dIds_, dTie_, dArena_, dQcount_;
+    DeviceBuffer dRad_, dLv_, dStack_, dCounters_, dFilm_, dIds_, dTie_, dArena_, dQcount_, dDfilm_, dAdjMaps_, dPost_;
+    const PinholeCamera* filmOf_ = nullptr;   // camera whose film dFilm_ holds
     std::vector<hipEvent_t> traceEv_;
 };
 
@@ -166,10 +175,22 @@ private:
         double tileCost;
     };
     int prepare();
+    int prepareWavefront(const std::vector<int>& order);
     int launch();
+    int launchWavefrontJobs();
     void dropGraph();
     std::vector<Job> jobs_;
-    omv_batch_tuning tuning_{0, 1, 1};
+    std::vector<int> order_;            // heaviest job first
+    omv_batch_tuning tuning_{0, 1, 1, 0};
+    struct StreamRes {
+        hipStream_t s = nullptr;
+        hipEvent_t done = nullptr;
+        DeviceBuffer arena, qcount, rad, gstack;
+    };
+    std::deque<StreamRes> sres_;
+    hipEvent_t fork_ = nullptr;
+    int wfPerCu_[4] = {0, 0, 0, 0};
+    uint64_t wfMaxSlots_ = 0;
     bool dirty_ = true;
     int blocks_ = 0, cus_ = 0;
     uint64_t totalTiles_ = 0;

@@ -15,10 +15,11 @@ namespace {
 constexpr uint64_t kChunkSamples = 1ull << 24;   // per-sample radiance buffer: 192 MiB
 // Wavefront samples per chunk.  Every chunk pays the latency-bound tail of
 // its last bounce rounds (a few long paths), so chunks are as large as HBM
-// comfortably allows: ~2.9 GB of state at 2^24 samples, ~7 GB at 2^23 with
-// point lights (light vertices, staged connections, connection rays).
-constexpr uint64_t kWfSlots = 1ull << 24;
-constexpr uint64_t kWfSlotsLights = 1ull << 23;
+// comfortably allows: ~6 GB of state at 2^25 samples, ~28 GB with point
+// lights (light vertices, staged connections, connection rays) -- a tenth
+// of one MI355X's 288 GB; chunk_samples lowers it.
+constexpr uint64_t kWfSlots = 1ull << 25;
+constexpr uint64_t kWfSlotsLights = 1ull << 25;
 int hipFail(hipError_t e, const char* what) {
     setError(std::string(what) + ": " + hipGetErrorString(e));
     return OMV_E_DEVICE;
@@ -154,7 +155,25 @@ int BDPTRenderer::render(PinholeCamera& cam) {
     stats_.nodes_visited = c[4]; stats_.tris_tested = c[5]; stats_.stack_overflows = c[6];
     stats_.render_ms = total;
     stats_.kernel_ms = kernelMs;
+    filmOf_ = &cam;
     if (stats_.stack_overflows) { setError("render: traversal stack overflow"); return OMV_E_INTERNAL; }
+    return OMV_OK;
+}
+
+int BDPTRenderer::post(PinholeCamera& cam, float gamma, bool denoise, float* outF, uint8_t* outU8) {
+    if (filmOf_ != &cam || !dFilm_.ptr) { setError("post: this camera has no film from this renderer"); return OMV_E_STATE; }
+    const uint64_t npix = (uint64_t)cam.desc.width * cam.desc.height;
+    hipError_t e;
+    if ((e = dPost_.reserve(npix * (16 + 16 + 4))) != hipSuccess) return hipFail(e, "post buffers");
+    float* tmp = dPost_.as<float>();
+    float* f = tmp + npix * 4;
+    uint8_t* u8 = reinterpret_cast<uint8_t*>(f + npix * 4);
+    if ((e = launchPost(dFilm_.as<float>(), tmp, f, outU8 ? u8 : nullptr, cam.desc.width, cam.desc.height, 1.0f, gamma, denoise, stream_)) !=
+        hipSuccess)
+        return hipFail(e, "post kernels");
+    if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return hipFail(e, "post");
+    if (outF && (e = hipMemcpy(outF, f, npix * 16, hipMemcpyDefault)) != hipSuccess) return hipFail(e, "post readback");
+    if (outU8 && (e = hipMemcpy(outU8, u8, npix * 4, hipMemcpyDefault)) != hipSuccess) return hipFail(e, "post readback");
     return OMV_OK;
 }
 
@@ -252,6 +271,77 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     }
     stats_.trace_ms = tr;
     stats_.trace_launches = (uint64_t)(evUsed / 2);
+    return OMV_OK;
+}
+
+// Path-replay adjoint (adjoint.hip) of the last render's estimator.
+int BDPTRenderer::backward(PinholeCamera& cam, const float* dFilm, const omv_adjoint_targets& t) {
+    if (!scene_ || !scene_->committed()) { setError("backward: no committed scene"); return OMV_E_STATE; }
+    const OmvSceneView& S = scene_->view();
+    if (S.num_lights > 0) { setError("backward: scenes with point lights are not differentiated"); return OMV_E_STATE; }
+    if (params_.extensions & OMV_EXT_OM_ROUGH) { setError("backward: the OM rough-conductor extension is not differentiated"); return OMV_E_STATE; }
+    if (!dFilm) { setError("backward: null d_film"); return OMV_E_ARG; }
+    if (t.n_textures < 0 || t.n_om_materials < 0 || (t.n_textures && (!t.texture_ids || !t.texture_grads)) ||
+        (t.n_om_materials && (!t.om_material_ids || !t.om_grads))) {
+        setError("backward: bad targets"); return OMV_E_ARG;
+    }
+    std::vector<int> texSlot(S.num_texs, -1), omSlot(S.num_mats, -1);
+    std::vector<float*> texGrad;
+    for (int i = 0; i < t.n_textures; ++i) {
+        const int id = t.texture_ids[i];
+        int w = 0, h = 0;
+        if (scene_->textureSize(id, &w, &h) != OMV_OK || w <= 0 || !t.texture_grads[i]) { setError("backward: bad texture target"); return OMV_E_RANGE; }
+        if (texSlot[id] >= 0) { setError("backward: texture listed twice"); return OMV_E_ARG; }
+        texSlot[id] = (int)texGrad.size();
+        texGrad.push_back(t.texture_grads[i]);
+    }
+    for (int i = 0; i < t.n_om_materials; ++i) {
+        const int id = t.om_material_ids[i];
+        if (!scene_->isOm(id) || omSlot[id] >= 0) { setError("backward: bad OM material target"); return OMV_E_RANGE; }
+        omSlot[id] = i;
+    }
+    const omv_renderer_params& p = params_;
+    const uint64_t npix = (uint64_t)cam.desc.width * cam.desc.height;
+    const uint64_t nsamples = npix * (uint64_t)std::max(p.samples_per_pixel, 0);
+    std::memset(&stats_, 0, sizeof(stats_));
+    int rc = initStream();
+    if (rc) return rc;
+    if (nsamples == 0) return OMV_OK;
+    const int blocks = cus_ * adjointBlocksPerCU();
+    const uint64_t lanes = (uint64_t)blocks * OMV_BLOCK;
+    hipError_t e;
+    if ((e = dStack_.reserve(lanes * (OMV_STACK_TOTAL - OMV_STACK_LDS) * sizeof(uint32_t))) != hipSuccess) return hipFail(e, "stack spill");
+    if ((e = dLv_.reserve(lanes * 6 * sizeof(float))) != hipSuccess) return hipFail(e, "lane scratch");
+    if ((e = dCounters_.reserve(8 * sizeof(unsigned long long))) != hipSuccess) return hipFail(e, "counters");
+    if ((e = dDfilm_.reserve(npix * 4 * sizeof(float))) != hipSuccess) return hipFail(e, "d_film");
+    // slot maps and the pointer table in one upload
+    std::vector<uint8_t> maps(texSlot.size() * 4 + omSlot.size() * 4 + texGrad.size() * sizeof(float*) + 16);
+    size_t oTex = 0, oOm = texSlot.size() * 4, oPtr = ((oOm + omSlot.size() * 4 + 15) / 16) * 16;
+    maps.resize(oPtr + std::max<size_t>(1, texGrad.size()) * sizeof(float*));
+    std::memcpy(maps.data() + oTex, texSlot.data(), texSlot.size() * 4);
+    std::memcpy(maps.data() + oOm, omSlot.data(), omSlot.size() * 4);
+    if (!texGrad.empty()) std::memcpy(maps.data() + oPtr, texGrad.data(), texGrad.size() * sizeof(float*));
+    if ((e = dAdjMaps_.upload(maps.data(), maps.size())) != hipSuccess) return hipFail(e, "adjoint maps");
+    if ((e = hipMemcpyAsync(dDfilm_.ptr, dFilm, npix * 4 * sizeof(float), hipMemcpyDefault, stream_)) != hipSuccess) return hipFail(e, "d_film copy");
+    if ((e = hipMemsetAsync(dCounters_.ptr, 0, 8 * sizeof(unsigned long long), stream_)) != hipSuccess) return hipFail(e, "memset");
+    uint8_t* mb = dAdjMaps_.as<uint8_t>();
+    (void)hipEventRecord(ev_[0], stream_);
+    if ((e = launchAdjoint(S, cam.desc, p, 0, nsamples, dDfilm_.as<float>(), reinterpret_cast<const int*>(mb + oTex),
+                           reinterpret_cast<float* const*>(mb + oPtr), reinterpret_cast<const int*>(mb + oOm), t.om_grads, dLv_.as<float>(),
+                           dStack_.as<uint32_t>(), dCounters_.as<unsigned long long>(), blocks, stream_)) != hipSuccess)
+        return hipFail(e, "adjoint kernel launch");
+    (void)hipEventRecord(ev_[1], stream_);
+    if ((e = hipEventSynchronize(ev_[1])) != hipSuccess) return hipFail(e, "adjoint kernel");
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, ev_[0], ev_[1]);
+    unsigned long long c[8] = {0};
+    if ((e = hipMemcpy(c, dCounters_.ptr, sizeof(c), hipMemcpyDeviceToHost)) != hipSuccess) return hipFail(e, "counter readback");
+    stats_.samples = nsamples;
+    stats_.rays = c[0]; stats_.camera_rays = c[1];
+    stats_.nodes_visited = c[4]; stats_.tris_tested = c[5]; stats_.stack_overflows = c[6];
+    stats_.render_ms = ms;
+    stats_.kernel_ms = ms;
+    if (stats_.stack_overflows) { setError("backward: traversal stack overflow"); return OMV_E_INTERNAL; }
     return OMV_OK;
 }
 

@@ -616,6 +616,8 @@ int RenderScene::commit(std::string& log) {
         dtex[i].offset = texels.size();
         texels.insert(texels.end(), t.bytes.begin(), t.bytes.end());
     }
+    texOffsets_.resize(dtex.size());
+    for (size_t i = 0; i < dtex.size(); ++i) texOffsets_[i] = dtex[i].offset;
     hipError_t e;
     if ((e = dVtx_.upload(vtx.data(), vtx.size() * 4)) != hipSuccess) return hipFail(e, "upload vertices");
     if ((e = dFaces_.upload(faces.data(), faces.size() * 4)) != hipSuccess) return hipFail(e, "upload faces");
@@ -655,6 +657,36 @@ int RenderScene::commit(std::string& log) {
     deviceBytes_ = dVtx_.bytes + dFaces_.bytes + dMeshes_.bytes + dInsts_.bytes + dMats_.bytes + dTexs_.bytes + dTexels_.bytes +
                    dNodes_.bytes + dTris_.bytes + dLights_.bytes + dCdf_.bytes + dBg_.bytes;
     committed_ = true;
+    return OMV_OK;
+}
+
+int RenderScene::updateTexture(int texId, const void* src) {
+    if (!committed_) { setError("update_texture: scene not committed"); return OMV_E_STATE; }
+    if (texId < 0 || texId >= (int)texs_.size()) { setError("update_texture: texture id out of range"); return OMV_E_RANGE; }
+    HostTexture& t = texs_[texId];
+    if (t.bytes.empty()) { setError("update_texture: texture has no image"); return OMV_E_STATE; }
+    if (!src) { setError("update_texture: null texels"); return OMV_E_ARG; }
+    hipError_t e = hipMemcpy(dTexels_.as<uint8_t>() + texOffsets_[texId], src, t.bytes.size(), hipMemcpyDefault);
+    if (e == hipSuccess) e = hipMemcpy(t.bytes.data(), src, t.bytes.size(), hipMemcpyDefault);
+    if (e != hipSuccess) return hipFail(e, "update_texture");
+    return OMV_OK;
+}
+
+int RenderScene::updateMaterialOmNk(int matId, const float n[3], const float k[3]) {
+    if (!committed_) { setError("update_material: scene not committed"); return OMV_E_STATE; }
+    if (!isOm(matId)) { setError("update_material: not an OpenMATERIAL material"); return OMV_E_RANGE; }
+    OmvMaterial& m = mats_[matId];
+    for (int c = 0; c < 3; ++c) { m.n[c] = n[c]; m.k[c] = k[c]; }
+    m.has_base = 1;
+    hipError_t e = hipMemcpy(dMats_.as<uint8_t>() + (size_t)matId * sizeof(OmvMaterial), &m, sizeof(OmvMaterial), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hipFail(e, "update_material");
+    return OMV_OK;
+}
+
+int RenderScene::textureSize(int texId, int* w, int* h) const {
+    if (texId < 0 || texId >= (int)texs_.size()) return OMV_E_RANGE;
+    *w = texs_[texId].bytes.empty() ? 0 : texs_[texId].w;
+    *h = texs_[texId].bytes.empty() ? 0 : texs_[texId].h;
     return OMV_OK;
 }
 

@@ -22,7 +22,9 @@
 
 #include "path_device.h"
 
-extern __shared__ uint32_t omv_lds_stack[];
+// Traversal kernels are held to 5 waves per SIMD (<= 102 VGPRs, no spill):
+// more resident rays hide more of the dependent node-fetch latency.
+#define OMV_TRACE_WAVES 5
 
 namespace {
 
@@ -164,7 +166,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_gen(OmvSceneView S, omv_came
 // rays, 1: light rays (closest hit + the re-trace tnear), 2: connection
 // rays (the visibility decision of isConnected, BDPTRenderer.cpp:221-234).
 template <int kKind>
-__global__ __launch_bounds__(OMV_BLOCK) void omv_wf_trace(OmvSceneView S, WfQueue q, WfShadowQ sq, const uint32_t* __restrict__ count,
+__global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSceneView S, WfQueue q, WfShadowQ sq, const uint32_t* __restrict__ count,
                                                           WfHits H, uint8_t* __restrict__ vis, uint32_t nslots, int lc,
                                                           uint32_t* __restrict__ gstack, unsigned long long* __restrict__ counters) {
     const uint32_t gthreads = gridDim.x * blockDim.x;

@@ -2362,6 +2362,30 @@ void ref_gamma_correction(float* img, int w, int h, float A, float gamma) {
         p[3] = 1.0f;
     }
 }
+/* SimpleDenoiserMedian<1> (Denoiser.h:24-68): interior pixels take the
+ * per-channel median of their 3x3 window (std::sort, element 4), the border
+ * keeps its value, alpha 1 inside */
+void ref_median3(float* img, int w, int h) {
+    std::vector<float> out(img, img + (size_t)w * h * 4);
+    for (int i = 1; i < h - 1; ++i)
+        for (int j = 1; j < w - 1; ++j) {
+            float win[3][9];
+            int id = 0;
+            for (int y = i - 1; y <= i + 1; ++y)
+                for (int x = j - 1; x <= j + 1; ++x, ++id)
+                    for (int c = 0; c < 3; ++c) win[c][id] = img[((size_t)y * w + x) * 4 + c];
+            for (int c = 0; c < 3; ++c) {
+                std::sort(win[c], win[c] + 9);
+                out[((size_t)i * w + j) * 4 + c] = win[c][4];
+            }
+            out[((size_t)i * w + j) * 4 + 3] = 1.0f;
+        }
+    std::memcpy(img, out.data(), out.size() * sizeof(float));
+}
+/* saveImage's conversion (main.cpp:88-93): static_cast<unsigned char>(255.0f * f) */
+void ref_to_u8(const float* img, size_t n, uint8_t* out) {
+    for (size_t i = 0; i < n; ++i) out[i] = static_cast<unsigned char>(255.0f * img[i]);
+}
 void ref_compute_camera(int w, int h, float yfov, float* hw, float* hh, float* focus) {
     /* adjustToResolution + setYFoV (PinholeCamera.cpp:156-174) */
     float H = 0.5f * (float)std::max(1, h), W = 0.5f * (float)std::max(1, w);

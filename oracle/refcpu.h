@@ -69,6 +69,8 @@ float ref_u01(uint32_t x);
 void ref_fresnel(float n, float k, float cos_theta, float* rp, float* rs);
 void ref_fresnel64(double n, double k, double cos_theta, double out[2]);
 void ref_fresnel_grad(double n, double k, double cos_theta, double out[4]);
+void ref_median3(float* img, int w, int h);
+void ref_to_u8(const float* img, size_t n, uint8_t* out);
 void ref_adjoint_plant(int mask);   /* test aid: plants a known mistake in the adjoint */
 int  ref_render_detached(ref_scene* sample, ref_scene* eval, const omv_renderer_params* p, const omv_camera_desc* cam,
                          int fp64, int threads, double* film);

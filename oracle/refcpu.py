@@ -94,6 +94,8 @@ def lib():
         "ref_fresnel64": ([C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)], None),
         "ref_fresnel_grad": ([C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)], None),
         "ref_adjoint_plant": ([C.c_int], None),
+        "ref_median3": ([FP, C.c_int, C.c_int], None),
+        "ref_to_u8": ([FP, C.c_size_t, C.POINTER(C.c_uint8)], None),
         "ref_render_detached": ([vp, vp, C.POINTER(RendererParams), C.POINTER(CameraDesc), C.c_int, C.c_int, C.POINTER(C.c_double)],
                                 C.c_int),
         "ref_adjoint": ([vp, C.POINTER(RendererParams), C.POINTER(CameraDesc), C.c_int, C.c_int, FP, C.c_int, C.POINTER(C.c_int),
@@ -239,3 +241,15 @@ def fresnel_grad(n, k, c):
     out = (C.c_double * 4)()
     lib().ref_fresnel_grad(n, k, c, out)
     return tuple(out)
+
+
+def post(film, gamma=0.5, denoise=False):
+    """main.cpp's post pass on the host: gammaCorrection, optional median, 8-bit."""
+    img = np.ascontiguousarray(np.array(film, np.float32))
+    h, w = img.shape[:2]
+    lib().ref_gamma_correction(ptr(img), w, h, C.c_float(1.0), C.c_float(gamma))
+    if denoise:
+        lib().ref_median3(ptr(img), w, h)
+    u8 = np.zeros(img.size, np.uint8)
+    lib().ref_to_u8(ptr(img), img.size, u8.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return img, u8.reshape(img.shape)

@@ -2,8 +2,9 @@
 
 Bar: every job's film is bit-identical to Renderer.render's for the same
 scene, camera and parameters (same Philox streams, same per-pixel summation
-order), whatever the tuning and whether the launch is replayed from a
-captured graph; one job is also checked against the CPU oracle at the
+order), whatever the tuning (one persistent per-pixel kernel, or the jobs'
+wavefront sequences over several streams) and whether the launch is replayed
+from a captured graph; one job is also checked against the CPU oracle at the
 radiance tolerance of the plain path.
 """
 import numpy as np
@@ -45,7 +46,10 @@ def _batch(pkg, jobs, scenes):
 
 @pytest.mark.parametrize("tuning", [dict(blocks_per_cu=0, tile_waves=1, use_graph=True),
                                     dict(blocks_per_cu=1, tile_waves=4, use_graph=False),
-                                    dict(blocks_per_cu=2, tile_waves=2, use_graph=True)])
+                                    dict(blocks_per_cu=2, tile_waves=2, use_graph=True),
+                                    dict(streams=1, use_graph=False),
+                                    dict(streams=3, use_graph=True),
+                                    dict(streams=4, use_graph=True)])
 def test_batch_equals_plain(pkg, wl, tuning):
     _, jobs, scenes = _small_batch(pkg, wl)
     plain = _plain_films(pkg, jobs, scenes)
@@ -80,11 +84,13 @@ def test_batch_mixed_lights_and_zero_spp(pkg, wl):
         r.set_scene(scenes[si])
         plain.append(r.render(cam))
         b.add_job(scenes[si], cam, pkg.default_params(**p))
-    b.run()
-    for j in range(len(jobs)):
-        assert np.array_equal(b.film(j), plain[j])
-    z = b.film(2)
-    assert np.all(z[..., :3] == 0) and np.all(z[..., 3] == 1)
+    for streams in (0, 2):
+        b.set_tuning(streams=streams)
+        b.run()
+        for j in range(len(jobs)):
+            assert np.array_equal(b.film(j), plain[j]), (streams, j)
+        z = b.film(2)
+        assert np.all(z[..., :3] == 0) and np.all(z[..., 3] == 1)
 
 
 def test_batch_job_against_oracle(pkg, wl, ref):

@@ -200,3 +200,27 @@ def test_full_size_large_instanced_radiance(pkg, wl, ref):
     assert np.isfinite(img).all() and (img[..., :3] >= 0).all()
     band, _ = _oracle_rows(ref, desc, cam.desc(), params, 536, 4)
     _radiance_close(img[536:540], band, frac_min=0.98)
+
+
+@pytest.mark.parametrize("denoise", [False, True])
+def test_post_pass_matches_oracle(pkg, wl, ref, denoise):
+    """Device gammaCorrection (+ median) + 8-bit against the host post pass on
+    the same film: floats within 2 ulp (device powf vs glibc powf), the median
+    selects the same window element, 8-bit values equal except where the
+    float sits within 2 ulp of a k/255 step."""
+    desc, view = wl.mixed_test_scene(hdr=True, lights=True)
+    view.width, view.height = 40, 30
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = pkg.Camera(view.width, view.height, view.position, view.forward, view.up, view.yfov)
+    r = pkg.Renderer(pkg.default_params(samples_per_pixel=2, camera_bounces=3, light_bounces=3, max_path_length=4))
+    r.set_scene(s)
+    film = r.render(cam)
+    f, u = r.post(cam, gamma=0.5, denoise=denoise)
+    rf, ru = ref.post(film, gamma=0.5, denoise=denoise)
+    ulp = np.abs(f.view(np.int32).astype(np.int64) - rf.view(np.int32).astype(np.int64))
+    assert ulp.max() <= 2, ulp.max()
+    diff = u.astype(int) != ru.astype(int)
+    near = np.abs(np.float32(255.0) * rf - np.round(np.float32(255.0) * rf)) <= 2 * np.spacing(np.float32(255.0) * rf)
+    assert not np.any(diff & ~near)
+    assert (diff.mean() < 0.01)

@@ -392,3 +392,26 @@ def test_om_rough_lights_the_icosphere(ref, wl):
     lit, st = s.render(ref.default_params(extensions=1, om_roughness=0.3, **params), cam)
     assert np.all(black[..., :3] == 0.0)
     assert lit[..., :3].max() > 0.0 and st["shadow_rays"] > 0
+
+
+def test_median3_matches_numpy(ref):
+    """SimpleDenoiserMedian<1>: interior = per-channel median of the 3x3
+    window (numpy's median of 9 values is the 5th order statistic), border
+    unchanged."""
+    rng = np.random.default_rng(4)
+    img = rng.random((13, 17, 4)).astype(np.float32)
+    out = img.copy()
+    ref.lib().ref_median3(ref.ptr(out), 17, 13)
+    want = img.copy()
+    for y in range(1, 12):
+        for x in range(1, 16):
+            want[y, x, :3] = np.median(img[y - 1:y + 2, x - 1:x + 2, :3].reshape(9, 3), axis=0)
+            want[y, x, 3] = 1.0
+    assert np.array_equal(out, want)
+
+
+def test_post_u8_truncates(ref):
+    f = np.array([0.0, 1.0, 0.5, 0.999, 1.0 / 255.0, 0.00392], np.float32)
+    u = np.zeros(f.size, np.uint8)
+    ref.lib().ref_to_u8(ref.ptr(f), f.size, u.ctypes.data_as(C.POINTER(C.c_uint8)))
+    assert list(u) == [int(np.float32(255.0) * v) for v in f]
