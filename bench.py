@@ -73,10 +73,12 @@ def cpu_baseline(ref, desc, view, params, cam_desc, budget_s, threads):
     }
 
 
-def rmse_check(ref, desc, view, params, cam_desc, gpu_img, rows, threads):
+def rmse_check(ref, desc, view, params, cam_desc, gpu_img, rows, threads, plain_img=None):
     """Equal-RMSE comparison on a band of `rows` rows through the image centre:
     the GPU image and the float oracle at the same spp and seed, each scored
-    against the FP64 oracle at 16x spp with an independent seed."""
+    against the FP64 oracle at 16x spp with an independent seed.  plain_img:
+    the GPU image of the reference estimator at the same spp, scored too when
+    the measured estimator is an extension."""
     rs = ref.Scene()
     desc.apply(rs, ref.pbr_desc)
     row0 = max(0, view.height // 2 - rows // 2)
@@ -94,9 +96,14 @@ def rmse_check(ref, desc, view, params, cam_desc, gpu_img, rows, threads):
     h = hi[band, :, :3].astype(np.float64)
     rg = float(np.sqrt(np.mean((g - h) ** 2)))
     rc = float(np.sqrt(np.mean((c - h) ** 2)))
-    return {"rows": [row0, row0 + rows], "spp": params["samples_per_pixel"], "reference_spp": 16 * params["samples_per_pixel"],
-            "reference": "FP64 oracle, independent seed", "rmse_gpu": rg, "rmse_cpu": rc, "ratio": rg / rc if rc > 0 else None,
-            "cpu_band_s": round(cpu_s, 2), "reference_s": round(ref_s, 2)}
+    out = {"rows": [row0, row0 + rows], "spp": params["samples_per_pixel"], "reference_spp": 16 * params["samples_per_pixel"],
+           "reference": "FP64 oracle, independent seed", "rmse_gpu": rg, "rmse_cpu": rc, "ratio": rg / rc if rc > 0 else None,
+           "cpu_band_s": round(cpu_s, 2), "reference_s": round(ref_s, 2)}
+    if plain_img is not None:
+        rp = float(np.sqrt(np.mean((plain_img[band, :, :3].astype(np.float64) - h) ** 2)))
+        out["rmse_gpu_reference_estimator"] = rp
+        out["rmse_reduction"] = rp / rg if rg > 0 else None
+    return out
 
 
 def copy_probe(gib):
@@ -165,16 +172,16 @@ def bench_batched(pkg, wl, args):
         print(f"# batch step {i}: {st['kernel_ms']:.1f} ms kernel", file=sys.stderr, flush=True)
     pkg.device_synchronize()
     el = time.perf_counter() - t0
-    # sequential plain launches of the same jobs (one renderer per job, film read back)
-    rends = []
-    for (a, _, _), p in zip(jobs, prm):
-        r = pkg.Renderer(p)
-        r.set_scene(scenes[a])
-        rends.append(r)
-    rends[0].render(cams[0])
+    # sequential plain launches of the same jobs (one renderer, re-pointed at
+    # each job's scene and parameters, film read back)
+    r = pkg.Renderer(prm[0])
+    r.set_scene(scenes[jobs[0][0]])
+    r.render(cams[0])
     pkg.device_synchronize()
     t0 = time.perf_counter()
-    for r, cam in zip(rends, cams):
+    for (a, _, _), cam, p in zip(jobs, cams, prm):
+        r.set_parameters(p)
+        r.set_scene(scenes[a])
         r.render(cam)
     pkg.device_synchronize()
     seq = time.perf_counter() - t0
@@ -362,6 +369,8 @@ def main():
     ap.add_argument("--rmse", type=int, default=0, metavar="ROWS",
                     help="equal-RMSE check of GPU and CPU images against the FP64 oracle at 16x spp on ROWS rows")
     ap.add_argument("--fd", type=int, default=0, help="material_fit: finite-difference check on this many texels")
+    ap.add_argument("--estimator", default="reference", choices=["reference", "nee"],
+                    help="hdri_torus_knot: the reference estimator, or the HDRI next-event-estimation extension")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (no multi-GPU path in this build)", file=sys.stderr)
@@ -382,6 +391,9 @@ def main():
         desc, view, params = wl.hdri_torus_knot()
         cfg = {"workload": "hdri_torus_knot", "triangles": 100000, "resolution": [512, 512], "spp": 64, "depth": 8,
                "textures": "2 x 1024^2 maps (RGBA8)", "hdri": [2048, 1024], "estimator": "reference"}
+        if args.estimator == "nee":
+            params = dict(params, extensions=params.get("extensions", 0) | 2)
+            cfg["estimator"] = "HDRI next-event estimation + MIS extension"
     elif args.workload.startswith("plain_bdpt_icosphere"):
         desc, view, params = wl.plain_bdpt_icosphere()
         rough = args.workload.endswith("om_rough")
@@ -458,13 +470,19 @@ def main():
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["paths_per_s"], 1), "unit": "paths/s", "cores": threads, "kind": "port",
             "sample": f"{cpu['rows']} rows x {view.width} px x {params['samples_per_pixel']} spp ({cpu['samples']} paths, whole-image "
-                      f"passes with successive seeds) of the same workload in {cpu['seconds']:.1f}s, float oracle, reference estimator"},
+                      f"passes with successive seeds) of the same workload in {cpu['seconds']:.1f}s, float oracle, {cfg['estimator']} "
+                      f"estimator"},
     }
     if cpu is not None:
         line["gpu_over_cpu"] = round(paths_per_s / cpu["paths_per_s"], 1)
     if args.rmse > 0:
         ref = ge.load_oracle()
-        line["equal_rmse"] = rmse_check(ref, desc, view, params, cam.desc(), r.render(cam), args.rmse, threads)
+        plain = None
+        if params.get("extensions", 0) & 2:   # NEE estimates the same image: score the reference estimator beside it
+            rp = pkg.Renderer(pkg.default_params(**dict(params, extensions=params["extensions"] & ~2)))
+            rp.set_scene(scene)
+            plain = rp.render(cam)
+        line["equal_rmse"] = rmse_check(ref, desc, view, params, cam.desc(), r.render(cam), args.rmse, threads, plain)
         print(f"# equal rmse {line['equal_rmse']}", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
 

@@ -58,6 +58,13 @@ extern "C" {
                                     reference's ideal mirror whose connection
                                     gate makes point-lit OM scenes black
                                     (OpenMaterial.cpp:105-109)               */
+#define OMV_EXT_HDRI_NEE    2u   /* HDR background importance sampling: at every
+                                    opaque PBR camera vertex one direction drawn
+                                    from the environment's alias tables, with a
+                                    visibility ray, combined with the escape
+                                    term by the balance heuristic against the
+                                    reference's own BSDF sampling density; same
+                                    expected image as the reference estimator  */
 
 /* Mirrors RendererParameters (pathtracer/src/renderer/RendererParameters.h:24-96).
  * omv_renderer_params_default() fills the reference defaults. */
@@ -231,6 +238,11 @@ int omv_scene_update_material_om_nk(omv_scene* s, int mat_id, const float n_rgb[
 int omv_scene_mesh_info(omv_scene* s, int mesh_id, int* vert_n, int* faces_n,
                         int* stride, int* mat_id);
 int omv_scene_get_vertices(omv_scene* s, int mesh_id, float* out, size_t cap_floats);
+/* Inspection (OMV_EXT_HDRI_NEE): the environment sampling tables built on
+ * the device at commit -- row table (H entries), per-row column tables and
+ * texel probabilities (W*H entries each); host buffers. */
+int omv_scene_env_tables(omv_scene* s, float* row_prob, int32_t* row_alias, float* col_prob, int32_t* col_alias,
+                         float* tex_pdf);
 /* BVH statistics: node count, max depth, leaf triangle records. */
 int omv_scene_bvh_info(omv_scene* s, int* nodes, int* max_depth, int* tri_records,
                        size_t* device_bytes);

@@ -23,6 +23,7 @@ LIB_PATH = _HERE / "libomv.so"
 OMV_OK = 0
 EXT_NONE = 0
 EXT_OM_ROUGH = 1
+EXT_HDRI_NEE = 2
 
 FC_SUBJECTS = ("pid", "gid", "mid", "rmp", "mmp", "md", "sg", "mn", "in")
 PBR_BASE_COLOR, PBR_METALLIC_ROUGHNESS, PBR_NORMAL, PBR_EMISSIVITY, PBR_OCCLUSION = range(5)
@@ -255,6 +256,8 @@ def _load():
         "omv_scene_mesh_info": [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
         "omv_scene_get_vertices": [vp, C.c_int, C.POINTER(C.c_float), C.c_size_t],
         "omv_scene_bvh_info": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_size_t)],
+        "omv_scene_env_tables": [vp, C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_float)],
         "omv_camera_create": [C.POINTER(vp), C.c_int, C.c_int], "omv_camera_destroy": [vp],
         "omv_camera_set_viewpoint": [vp] + [C.POINTER(C.c_float)] * 4,
         "omv_camera_adjust_to_resolution": [vp], "omv_camera_set_yfov": [vp, C.c_float],
@@ -356,6 +359,16 @@ class Scene(SceneApi):
         n, d, r, b = C.c_int(), C.c_int(), C.c_int(), C.c_size_t()
         self._check(self.lib.omv_scene_bvh_info(self.h, C.byref(n), C.byref(d), C.byref(r), C.byref(b)), "scene_bvh_info")
         return {"nodes": n.value, "depth": d.value, "tri_records": r.value, "device_bytes": b.value}
+
+    def env_tables(self, w: int, h: int):
+        """The environment sampling tables built on the device at commit
+        (OMV_EXT_HDRI_NEE): row prob/alias (h), column prob/alias and texel
+        probabilities (h, w)."""
+        rp, ra = np.zeros(h, np.float32), np.zeros(h, np.int32)
+        cp, ca, tp = np.zeros((h, w), np.float32), np.zeros((h, w), np.int32), np.zeros((h, w), np.float32)
+        self._check(self.lib.omv_scene_env_tables(self.h, _ptr(rp, C.c_float), _ptr(ra, C.c_int32), _ptr(cp, C.c_float),
+                                                  _ptr(ca, C.c_int32), _ptr(tp, C.c_float)), "scene_env_tables")
+        return rp, ra, cp, ca, tp
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -61,7 +61,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_adjoint_kernel(OmvSceneView S, 
         const uint64_t pix = s / (uint64_t)spp;
         const float A[3] = {dfilm[pix * 4 + 0] * sc, dfilm[pix * 4 + 1] * sc, dfilm[pix * 4 + 2] * sc};
         if (A[0] == 0.0f && A[1] == 0.0f && A[2] == 0.0f) continue;
-        const f3 C = pathSample<false>(S, cam, P, spp, s, L);
+        const f3 C = pathSample<false, false>(S, cam, P, spp, s, L);
         if (C.x == 0.0f && C.y == 0.0f && C.z == 0.0f) continue;
         // second replay: the camera loop of pathSample, same draws
         const int x = (int)(pix % (uint64_t)cam.width), y = (int)(pix / (uint64_t)cam.width);

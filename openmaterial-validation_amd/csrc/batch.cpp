@@ -116,7 +116,9 @@ int BatchRunner::prepare() {
     }
     totalTiles_ = t;
 
-    int occ = occupancyBlocksPerCU(3);
+    anyNee_ = false;
+    for (const Job& j : jobs_) anyNee_ = anyNee_ || ((j.p.extensions & OMV_EXT_HDRI_NEE) && j.scene->view().env_tex_pdf);
+    int occ = occupancyBlocksPerCU(anyNee_ ? 6 : 3);
     if (occ <= 0) occ = 1;
     const int perCu = tuning_.blocks_per_cu > 0 ? std::min(tuning_.blocks_per_cu, occ) : occ;
     blocks_ = std::max(1, cus_ * perCu);
@@ -183,7 +185,8 @@ int BatchRunner::prepareWavefront(const std::vector<int>& order) {
         const uint64_t chunkPix = std::max<uint64_t>(1, wfMaxSlots_ / (uint64_t)spp);
         const uint64_t slots = std::min(J.npix, chunkPix) * (uint64_t)spp;
         if (slots * (uint64_t)lc >= (1ull << 32)) { setError("batch: too many samples per pixel for one chunk"); return OMV_E_RANGE; }
-        arena = std::max(arena, wavefrontArenaBytes((uint32_t)slots, lvCap, lc, lights));
+        const bool nee = (J.p.extensions & OMV_EXT_HDRI_NEE) && J.scene->view().env_tex_pdf;
+        arena = std::max(arena, wavefrontArenaBytes((uint32_t)slots, lvCap, lc, lights, nee));
         rad = std::max(rad, (size_t)slots * 3 * sizeof(float));
         ncnt = std::max(ncnt, wavefrontCounterCount(J.p.camera_bounces, lvCap));
     }
@@ -255,7 +258,7 @@ int BatchRunner::launchWavefrontJobs() {
 int BatchRunner::launch() {
     if (tuning_.streams > 0) return launchWavefrontJobs();
     hipError_t e = launchBatch(dJobs_.ptr, (int)jobs_.size(), dTileEnd_.as<uint64_t>(), totalTiles_, tuning_.tile_waves,
-                               dCounters_.as<unsigned long long>(), dLv_.as<float>(), dStack_.as<uint32_t>(), blocks_, stream_);
+                               dCounters_.as<unsigned long long>(), dLv_.as<float>(), dStack_.as<uint32_t>(), blocks_, anyNee_, stream_);
     if (e != hipSuccess) return hipFail(e, "batch kernel launch");
     return OMV_OK;
 }
@@ -298,7 +301,9 @@ int BatchRunner::run() {
 int BatchRunner::autotune(int reps, omv_batch_tuning* best) {
     if (jobs_.empty()) { setError("batch: no jobs"); return OMV_E_STATE; }
     reps = std::max(1, reps);
-    const int occ = std::max(1, occupancyBlocksPerCU(3));
+    bool nee = false;
+    for (const Job& j : jobs_) nee = nee || ((j.p.extensions & OMV_EXT_HDRI_NEE) && j.scene->view().env_tex_pdf);
+    const int occ = std::max(1, occupancyBlocksPerCU(nee ? 6 : 3));
     const omv_batch_tuning start = tuning_;
     omv_batch_tuning bestT = start;
     double bestMs = 1e300;

@@ -25,7 +25,7 @@
 // ============================================================== kernels
 // Plain render: one sample per lane at a time over [sample0, sample0 + nsamples),
 // persistent grid; radiance per sample for the ordered film reduction.
-template <bool kLights>
+template <bool kLights, bool kNee>
 __global__ __launch_bounds__(OMV_BLOCK) void omv_bdpt_kernel(OmvSceneView S, omv_camera_desc cam, RP P, int spp,
                                                              uint64_t sample0, uint64_t nsamples, float* __restrict__ rad,
                                                              float* __restrict__ lv, uint32_t* __restrict__ gstack,
@@ -35,7 +35,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_bdpt_kernel(OmvSceneView S, omv
     Lane L;
     laneInit(L, gstack, lv, gtid, gthreads);
     for (uint64_t sl = gtid; sl < nsamples; sl += gthreads) {
-        f3 r = pathSample<kLights>(S, cam, P, spp, sample0 + sl, L);
+        f3 r = pathSample<kLights, kNee>(S, cam, P, spp, sample0 + sl, L);
         rad[sl * 3 + 0] = r.x;
         rad[sl * 3 + 1] = r.y;
         rad[sl * 3 + 2] = r.z;
@@ -59,6 +59,7 @@ struct OmvJobDev {
     float* film;
 };
 
+template <bool kNee>
 __global__ __launch_bounds__(OMV_BLOCK) void omv_batch_kernel(const OmvJobDev* __restrict__ jobs, int njobs,
                                                               const uint64_t* __restrict__ tileEnd, uint64_t totalTiles,
                                                               int tileWaves, unsigned long long* __restrict__ work,
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_batch_kernel(const OmvJobDev* _
             if (p >= J.npix) continue;
             float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
             for (int k = 0; k < J.spp; ++k) {
-                f3 r = pathSample<true>(J.S, J.cam, J.P, J.spp, p * (uint64_t)J.spp + (uint64_t)k, L);
+                f3 r = pathSample<true, kNee>(J.S, J.cam, J.P, J.spp, p * (uint64_t)J.spp + (uint64_t)k, L);
                 a0 = a0 + r.x; a1 = a1 + r.y; a2 = a2 + r.z;
             }
             float cnt = (float)J.spp;
@@ -210,12 +211,18 @@ hipError_t launchBdpt(const OmvSceneView& S, const omv_camera_desc& cam, const o
                       uint64_t nsamples, float* rad, float* lv, uint32_t* gstack, unsigned long long* counters, int blocks,
                       hipStream_t stream) {
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
-    if (S.num_lights > 0)
-        hipLaunchKernelGGL(omv_bdpt_kernel<true>, dim3(blocks), dim3(OMV_BLOCK), lds, stream, S, cam, toRP(p), p.samples_per_pixel,
-                           sample0, nsamples, rad, lv, gstack, counters);
+    RP P = toRP(p);
+    if (!S.env_tex_pdf) P.x.nee = false;   // no HDR environment: nothing to sample
+    const dim3 g(blocks), b(OMV_BLOCK);
+    const int spp = p.samples_per_pixel;
+    if (S.num_lights > 0 && P.x.nee)
+        hipLaunchKernelGGL((omv_bdpt_kernel<true, true>), g, b, lds, stream, S, cam, P, spp, sample0, nsamples, rad, lv, gstack, counters);
+    else if (S.num_lights > 0)
+        hipLaunchKernelGGL((omv_bdpt_kernel<true, false>), g, b, lds, stream, S, cam, P, spp, sample0, nsamples, rad, lv, gstack, counters);
+    else if (P.x.nee)
+        hipLaunchKernelGGL((omv_bdpt_kernel<false, true>), g, b, lds, stream, S, cam, P, spp, sample0, nsamples, rad, lv, gstack, counters);
     else
-        hipLaunchKernelGGL(omv_bdpt_kernel<false>, dim3(blocks), dim3(OMV_BLOCK), lds, stream, S, cam, toRP(p), p.samples_per_pixel,
-                           sample0, nsamples, rad, lv, gstack, counters);
+        hipLaunchKernelGGL((omv_bdpt_kernel<false, false>), g, b, lds, stream, S, cam, P, spp, sample0, nsamples, rad, lv, gstack, counters);
     return hipGetLastError();
 }
 
@@ -239,7 +246,9 @@ void packBatchJob(void* dst, const OmvSceneView& S, const omv_camera_desc& cam, 
                   uint64_t npix) {
     OmvJobDev j;
     std::memset(&j, 0, sizeof(j));
-    j.S = S; j.cam = cam; j.P = toRP(p); j.spp = p.samples_per_pixel; j.npix = npix; j.film = film;
+    j.S = S; j.cam = cam; j.P = toRP(p);
+    if (!S.env_tex_pdf) j.P.x.nee = false;
+    j.spp = p.samples_per_pixel; j.npix = npix; j.film = film;
     std::memcpy(dst, &j, sizeof(j));
 }
 
@@ -249,23 +258,33 @@ hipError_t launchFillU64(unsigned long long* p, int n, hipStream_t stream) {
 }
 
 hipError_t launchBatch(const void* jobs, int njobs, const uint64_t* tileEnd, uint64_t totalTiles, int tileWaves,
-                       unsigned long long* counters, float* lv, uint32_t* gstack, int blocks, hipStream_t stream) {
+                       unsigned long long* counters, float* lv, uint32_t* gstack, int blocks, bool nee, hipStream_t stream) {
     // counters[0..7]: statistics, counters[8]: work counter; cleared by a
     // kernel so that the sequence can be captured into a graph
     hipLaunchKernelGGL(omv_fill_u64, dim3(1), dim3(64), 0, stream, counters, 16);
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
-    hipLaunchKernelGGL(omv_batch_kernel, dim3(blocks), dim3(OMV_BLOCK), lds, stream, static_cast<const OmvJobDev*>(jobs), njobs,
-                       tileEnd, totalTiles, tileWaves, counters + 8, lv, gstack, counters);
+    if (nee)
+        hipLaunchKernelGGL(omv_batch_kernel<true>, dim3(blocks), dim3(OMV_BLOCK), lds, stream, static_cast<const OmvJobDev*>(jobs), njobs,
+                           tileEnd, totalTiles, tileWaves, counters + 8, lv, gstack, counters);
+    else
+        hipLaunchKernelGGL(omv_batch_kernel<false>, dim3(blocks), dim3(OMV_BLOCK), lds, stream, static_cast<const OmvJobDev*>(jobs), njobs,
+                           tileEnd, totalTiles, tileWaves, counters + 8, lv, gstack, counters);
     return hipGetLastError();
 }
 
 int occupancyBlocksPerCU(int which) {
     int n = 0;
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
-    hipError_t e = which == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<true>, OMV_BLOCK, lds)
-                   : which == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<false>, OMV_BLOCK, lds)
-                   : which == 3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_batch_kernel, OMV_BLOCK, lds)
-                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_fc_kernel, OMV_BLOCK, lds);
+    hipError_t e;
+    switch (which) {
+        case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<true, false>, OMV_BLOCK, lds); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<false, false>, OMV_BLOCK, lds); break;
+        case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_batch_kernel<false>, OMV_BLOCK, lds); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<true, true>, OMV_BLOCK, lds); break;
+        case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<false, true>, OMV_BLOCK, lds); break;
+        case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_batch_kernel<true>, OMV_BLOCK, lds); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_fc_kernel, OMV_BLOCK, lds); break;
+    }
     if (e != hipSuccess) n = 0;
     return n;
 }

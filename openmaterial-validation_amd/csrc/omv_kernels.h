@@ -30,7 +30,7 @@ size_t batchJobBytes();
 void packBatchJob(void* dst, const OmvSceneView& S, const omv_camera_desc& cam, const omv_renderer_params& p, float* film,
                   uint64_t npix);
 hipError_t launchBatch(const void* jobs, int njobs, const uint64_t* tileEnd, uint64_t totalTiles, int tileWaves,
-                       unsigned long long* counters, float* lv, uint32_t* gstack, int blocks, hipStream_t stream);
+                       unsigned long long* counters, float* lv, uint32_t* gstack, int blocks, bool nee, hipStream_t stream);
 // Wavefront render of one chunk of whole pixels (wavefront.hip).
 struct WfLaunch {
     OmvSceneView S;
@@ -39,7 +39,7 @@ struct WfLaunch {
     uint64_t sample0;          // first sample of the chunk
     uint32_t nslots;           // samples in the chunk
     int lvCap, lc;             // light vertices kept, connections per camera vertex
-    void* arena;               // wavefrontArenaBytes(nslots, lvCap, lc, lights)
+    void* arena;               // wavefrontArenaBytes(nslots, lvCap, lc, lights, nee)
     uint32_t* qcount;          // wavefrontCounterCount(B, lvCap) counters
     float* rad;                // 3 floats per sample of the chunk
     uint32_t* gstack;          // spill stacks of blocksTrace * OMV_BLOCK lanes
@@ -50,7 +50,7 @@ struct WfLaunch {
     int traceEvCap;            // events available from traceEv
     int* traceEvUsed;          // events recorded so far (in/out)
 };
-size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights);
+size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee);
 int wavefrontCounterCount(int B, int lvCap);
 void wavefrontBlocksPerCU(int out[4]);   // trace, shade, shade_light, gen
 hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream);
@@ -63,7 +63,11 @@ hipError_t launchAdjoint(const OmvSceneView& S, const omv_camera_desc& cam, cons
 // the 8-bit truncation of outF into outU8 when non-null; tmp holds w*h float4.
 hipError_t launchPost(const float* film, float* tmp, float* outF, uint8_t* outU8, int w, int h, float A, float gamma, bool denoise,
                       hipStream_t stream);
-// 0: sample kernel (scene with lights), 1: FC kernel, 2: sample kernel (no lights), 3: batch kernel
+// Environment alias tables (env.hip); scratch holds 4*W*H + 2*H + 2 words.
+hipError_t launchEnvBuild(const float* bg, int W, int H, float* rowProb, int* rowAlias, float* colProb, int* colAlias, float* texPdf,
+                          void* scratch, hipStream_t stream);
+// 0: sample kernel (scene with lights), 1: FC kernel, 2: sample kernel (no lights), 3: batch kernel;
+// 4, 5, 6: the sample kernels and the batch kernel with OMV_EXT_HDRI_NEE code
 int occupancyBlocksPerCU(int which);
 
 }  // namespace omv

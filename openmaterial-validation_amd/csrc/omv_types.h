@@ -107,6 +107,12 @@ struct OmvSceneView {
     float   bg_scale;
     float   light_max;
     float   min_density, max_density;
+    // environment importance sampling tables (HDR background), else null
+    const float*       env_row_prob;
+    const int32_t*     env_row_alias;
+    const float*       env_col_prob;
+    const int32_t*     env_col_alias;
+    const float*       env_tex_pdf;
 };
 
 #endif

@@ -117,7 +117,7 @@ struct Rng {
         return (float)(v >> 8) * (1.0f / 16777216.0f);
     }
 };
-enum { STREAM_CAMERA = 0, STREAM_LIGHT = 1 };
+enum { STREAM_CAMERA = 0, STREAM_LIGHT = 1, STREAM_NEE = 2 };   // NEE at camera vertex i: 2 + 16 * i
 
 // ----------------------------------------------------------------- warps
 // RandomSampler.h:66-103
@@ -661,7 +661,7 @@ DEV f3 omFresnelBrdf(const OmvMaterial& M, float fCos) {
     return mk3(o[0], o[1], o[2]);
 }
 // Estimator extensions (omv_renderer_params.extensions)
-struct Ext { bool omRough; float omAlpha; };
+struct Ext { bool omRough; float omAlpha; bool nee; };
 
 // OMV_EXT_OM_ROUGH: GGX rough conductor with the complex-IOR Fresnel as F.
 // Connection: f = F(|I.H|) D G2 / (4 V.N L.N), zero unless both cosines > 0.
@@ -1062,6 +1062,110 @@ DEV f3 background(const OmvSceneView& S, const f3& dir) {
     return mk3(val.x * S.bg_scale, val.y * S.bg_scale, val.z * S.bg_scale);
 }
 
+// ------------------------------------------------ OMV_EXT_HDRI_NEE
+// Next-event estimation toward the HDR environment at opaque PBR camera
+// vertices, combined with the escape of the BSDF-sampled direction by the
+// balance heuristic.  The environment sampler picks a row and a column from
+// the alias tables built at commit and a uniform point in the texel; its
+// solid-angle density is texPdf * W * H / (2 pi^2 sin(theta)).
+
+// Inverse of background()'s mapping for the point (ux, uy) of texel (row, col)
+DEV f3 envDirection(const OmvSceneView& S, int row, int col, float ux, float uy, float& sinT) {
+    const float u = ((float)col + ux) / (float)S.bg_w, v = ((float)row + uy) / (float)S.bg_h;
+    const float yang = (1.0f - v) * kPi;
+    const float rz = -cosf(yang);
+    const float xang = u * 2.0f * kPi;
+    const float r = sqrtf(mxf(1.0f - rz * rz, 0.0f));
+    sinT = r;
+    return mul33(S.bg_rot, mk3(r * cosf(xang), r * sinf(xang), rz));
+}
+// Solid-angle density of the environment sampler at dir
+DEV float envPdf(const OmvSceneView& S, const f3& dir) {
+    f3 rdir = mulT33(dir, S.bg_rot);
+    f3 rxy = normalize3(mk3(rdir.x, rdir.y, 0.0f));
+    float xang = acosf(clampr(rxy.x, -1.0f, 1.0f));
+    xang = rxy.y < 0.0f ? 2.0f * kPi - xang : xang;
+    const float z = clampr(rdir.z, -1.0f, 1.0f);
+    const float yang = asinf(z) + 0.5f * kPi;
+    const float u = xang / (2.0f * kPi), v = 1.0f - yang / kPi;
+    const int W = S.bg_w, H = S.bg_h;
+    const int col = min(max((int)(u * (float)W), 0), W - 1), row = min(max((int)(v * (float)H), 0), H - 1);
+    const float sinT = sqrtf(mxf(1.0f - z * z, 0.0f));
+    if (!(sinT > 0.0f)) return 0.0f;
+    return S.env_tex_pdf[(size_t)row * W + col] * (float)W * (float)H / (2.0f * kPi * kPi * sinT);
+}
+// Density of sampleIndirect's lobe mixture (BRDFModel.h:265-337) at an
+// opaque vertex: diffuse P LN / pi, reflection (1 - P) D NH / (4 LH)
+DEV float bsdfPdf(const Bsdf& b, const f3& O) {
+    const float VN = -dot3(b.I, b.N);
+    if (VN < 0.0f) return 0.0f;
+    const float P = 0.8f * fabsf(VN) + 0.1f;
+    const float LN = dot3(O, b.N);
+    if (!(LN > 0.0f)) return 0.0f;
+    float p = P * LN / kPi;
+    const f3 H = normalize3(sub3(O, b.I));
+    const float NH = dot3(b.N, H), LH = dot3(O, H);
+    if (NH > 0.0f && LH > 0.0f) p += (1.0f - P) * ggxPdf(NH, b.alpha) * NH / (4.0f * LH);
+    return p;
+}
+// The estimator's integrand per solid angle (sampled weight x density x
+// cosine, bsdfEvaluate's clamps included) of direction O
+DEV f3 bsdfEffCos(const Bsdf& b, const f3& O) {
+    const f3 H = normalize3(sub3(O, b.I));
+    const float VNc = clampr(fabsf(dot3(b.I, b.N)), 0.0001f, 1.0f);
+    const float LNc = clampr(fabsf(dot3(O, b.N)), 0.0001f, 1.0f);
+    const float LN = fabsf(dot3(O, b.N));
+    const float NH = fabsf(dot3(b.N, H)), VH = fabsf(dot3(b.I, H));
+    const float p5 = powf(clamp01(1.0f - VH), 5.0f);
+    const float D = ggxPdf(NH, b.alpha);
+    const float G = smithG2(smithG1(VNc, b.alpha), smithG1(LNc, b.alpha));
+    const float col[3] = {b.c0, b.c1, b.c2};
+    float out[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float F0 = lerpr(0.04f, col[c], b.metal);
+        const float F = lerpr(p5, 1.0f, F0);
+        const float fd = (1.0f - F) * ((1.0f - b.metal) * col[c]) / kPi;
+        out[c] = fd * LN * LN / LNc + F * G * D * LN / (4.0f * LNc * LNc);
+    }
+    return mk3(out[0], out[1], out[2]);
+}
+// Next-event estimate at camera vertex i (after its emission): the MIS
+// weight escW of an escape along the sampled direction outV, and, when the
+// estimate is non-zero, its contribution c and the visibility ray (org, w)
+// toward the environment (visible iff sceneIntersect finds no hit).
+DEV bool neePrepare(const OmvSceneView& S, const RP& P, const SP& sp, const BounceInfo& bi, const f3& ofPrev, const f3& outV, int i,
+                    uint64_t sample, float& escW, f3& c, f3& org, f3& w) {
+    escW = 1.0f;
+    if (!P.x.nee || !S.env_tex_pdf || bi.kind != 1) return false;
+    const Bsdf& b = bi.b;
+    if (b.transmissive || -dot3(b.I, b.N) < 0.0f || i >= P.camera_bounces || i + 1 > P.q) return false;
+    const float pbS = bsdfPdf(b, outV), peS = envPdf(S, outV);
+    if (pbS + peS > 0.0f) escW = pbS / (pbS + peS);
+    Rng ns; ns.init(P.seed, STREAM_NEE + 16u * (uint32_t)i, sample);
+    const float u0 = ns.next(), u1 = ns.next(), u2 = ns.next(), u3 = ns.next(), u4 = ns.next(), u5 = ns.next();
+    const int W = S.bg_w, H = S.bg_h;
+    int k = min((int)(u0 * (float)H), H - 1);
+    const int row = u1 < S.env_row_prob[k] ? k : S.env_row_alias[k];
+    k = min((int)(u2 * (float)W), W - 1);
+    const size_t rk = (size_t)row * W + k;
+    const int col = u3 < S.env_col_prob[rk] ? k : S.env_col_alias[rk];
+    float sinT;
+    w = envDirection(S, row, col, u4, u5, sinT);
+    if (!(sinT > 0.0f)) return false;
+    const float pe = S.env_tex_pdf[(size_t)row * W + col] * (float)W * (float)H / (2.0f * kPi * kPi * sinT);
+    const float pb = bsdfPdf(b, w);
+    if (!(pe > 0.0f) || !(dot3(w, b.N) > 0.0f)) return false;
+    const f3 f = bsdfEffCos(b, w);
+    const f3 bg = background(S, w);
+    const float inv = 1.0f / (pe + pb);
+    c = mk3(ofPrev.x * f.x * bg.x * inv, ofPrev.y * f.y * bg.y * inv, ofPrev.z * f.z * bg.z * inv);
+    if (c.x == 0.0f && c.y == 0.0f && c.z == 0.0f) return false;
+    const float delta = 3.0f * FLT_EPSILON * maxAbs(sp.pos);
+    org = mk3(sp.pos.x + delta * w.x, sp.pos.y + delta * w.y, sp.pos.z + delta * w.z);
+    return true;
+}
+
 // DistanceRandom::getValue (DistanceRandom.h:203-220)
 DEV int sampleLight(const OmvSceneView& S, float u) {
     const int n = S.num_lights;
@@ -1108,6 +1212,7 @@ inline RP toRP(const omv_renderer_params& p) {
     r.cut = p.ray_cut_pix_value; r.seed = p.seed;
     r.x.omRough = (p.extensions & OMV_EXT_OM_ROUGH) != 0;
     r.x.omAlpha = p.om_roughness * p.om_roughness;
+    r.x.nee = (p.extensions & OMV_EXT_HDRI_NEE) != 0;
     return r;
 }
 
@@ -1141,8 +1246,9 @@ DEV void laneInit(Lane& L, uint32_t* gstack, float* lv, uint32_t gtid, uint32_t 
 
 // One sample of BDPTRenderer::computePaths (BDPTRenderer.cpp:237-301):
 // sample s = pixel * spp + k.  kLights = false compiles no light subpath
-// and no connection code (scenes without point lights, BDPTRenderer.cpp:241-242).
-template <bool kLights>
+// and no connection code (scenes without point lights, BDPTRenderer.cpp:241-242);
+// kNee = false compiles no OMV_EXT_HDRI_NEE code.
+template <bool kLights, bool kNee = false>
 DEV f3 pathSample(const OmvSceneView& S, const omv_camera_desc& cam, const RP& P, int spp, uint64_t s, Lane& L) {
     const int q = P.q;
     const int lvCap = max(min(P.light_bounces + 1, max(q, 1)), 1);
@@ -1198,7 +1304,7 @@ DEV f3 pathSample(const OmvSceneView& S, const omv_camera_desc& cam, const RP& P
     // ---- camera subpath + connections
     f3 radiance = mk3(0.0f, 0.0f, 0.0f), conn = mk3(0.0f, 0.0f, 0.0f);
     f3 pPos = O, pDir = D, of = mk3(1.0f, 1.0f, 1.0f);
-    float tnear = 0.0f;
+    float tnear = 0.0f, escW = 1.0f;
     int n = 1;
     bool exitScene = false;
     for (int i = 1; i < P.camera_bounces + 1; ++i) {
@@ -1237,10 +1343,22 @@ DEV f3 pathSample(const OmvSceneView& S, const omv_camera_desc& cam, const RP& P
             }
         }
         f3 outRay, brdf, erad;
-        matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, true, cs);
+        BounceInfo bi;
+        matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, true, cs, kNee ? &bi : nullptr);
         float ct = fabsf(dot3(outRay, sp.n));
+        const f3 ofPrev = of;
         of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
         radiance = mk3(radiance.x + of.x * erad.x, radiance.y + of.y * erad.y, radiance.z + of.z * erad.z);
+        if (kNee && P.x.nee) {   // OMV_EXT_HDRI_NEE, after the vertex's emission
+            f3 c, org, w;
+            if (neePrepare(S, P, sp, bi, ofPrev, outRay, i, s, escW, c, org, w)) {
+                float tn = 0.0f;
+                Hit hs;
+                ++L.shadowRays;
+                if (!sceneIntersect(S, org, w, tn, FLT_MAX, hs, L.st, L.cnt, L.rays))
+                    radiance = mk3(radiance.x + c.x, radiance.y + c.y, radiance.z + c.z);
+            }
+        }
         ++n;
         pPos = sp.pos; pDir = outRay;
         // exact shortcuts, as in the oracle: zero throughput, or no emission past q
@@ -1250,6 +1368,7 @@ DEV f3 pathSample(const OmvSceneView& S, const omv_camera_desc& cam, const RP& P
     if (exitScene && S.bg_type != OMV_BG_NONE && n <= q) {
         f3 bg = background(S, pDir);
         bg = mk3(bg.x * of.x, bg.y * of.y, bg.z * of.z);
+        if (kNee && P.x.nee) bg = mk3(bg.x * escW, bg.y * escW, bg.z * escW);
         radiance = mk3(bg.x * 1.0f + radiance.x, bg.y * 1.0f + radiance.y, bg.z * 1.0f + radiance.z);
     }
     return mk3(conn.x * 1.0f + radiance.x, conn.y * 1.0f + radiance.y, conn.z * 1.0f + radiance.z);

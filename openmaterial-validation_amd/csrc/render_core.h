@@ -82,6 +82,7 @@ public:
     int updateTexture(int texId, const void* src);
     int updateMaterialOmNk(int matId, const float n[3], const float k[3]);
     int textureSize(int texId, int* w, int* h) const;
+    int envTables(float* rowProb, int* rowAlias, float* colProb, int* colAlias, float* texPdf) const;
     bool isOm(int matId) const { return matId >= 0 && matId < (int)mats_.size() && mats_[matId].type == OMV_MAT_OM; }
 
 private:
@@ -101,7 +102,7 @@ private:
     size_t deviceBytes_ = 0;
     std::vector<uint64_t> texOffsets_;   // byte offsets in the device texel pool
     OmvSceneView view_{};
-    DeviceBuffer dVtx_, dFaces_, dMeshes_, dInsts_, dMats_, dTexs_, dTexels_, dNodes_, dTris_, dLights_, dCdf_, dBg_;
+    DeviceBuffer dVtx_, dFaces_, dMeshes_, dInsts_, dMats_, dTexs_, dTexels_, dNodes_, dTris_, dLights_, dCdf_, dBg_, dEnv_, dEnvScratch_;
 };
 
 class PinholeCamera {
@@ -180,6 +181,7 @@ private:
     int launchWavefrontJobs();
     void dropGraph();
     std::vector<Job> jobs_;
+    bool anyNee_ = false;   // some job uses OMV_EXT_HDRI_NEE: the batch kernel with its code
     std::vector<int> order_;            // heaviest job first
     omv_batch_tuning tuning_{0, 1, 1, 0};
     struct StreamRes {

@@ -186,7 +186,9 @@ int BDPTRenderer::renderSamples(PinholeCamera& cam, double& kernelMs) {
     int blocks = 0;
     const uint64_t chunkPix = std::max<uint64_t>(1, kChunkSamples / (uint64_t)spp);
     const uint64_t firstChunk = std::min(npix, chunkPix) * (uint64_t)spp;
-    int rc = prepare(scene_->view().num_lights > 0 ? 0 : 2, firstChunk, blocks);
+    const bool nee = (p.extensions & OMV_EXT_HDRI_NEE) && scene_->view().env_tex_pdf;
+    const bool lights = scene_->view().num_lights > 0;
+    int rc = prepare(lights ? (nee ? 4 : 0) : (nee ? 5 : 2), firstChunk, blocks);
     if (rc) return rc;
     const uint64_t lanes = (uint64_t)blocks * OMV_BLOCK;
     const int lvCap = std::max(std::min(p.light_bounces + 1, std::max(p.max_path_length, 1)), 1);
@@ -228,7 +230,8 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     hipError_t e;
     if ((e = dStack_.reserve((uint64_t)blocksTrace * OMV_BLOCK * (OMV_STACK_TOTAL - OMV_STACK_LDS) * sizeof(uint32_t))) != hipSuccess)
         return hipFail(e, "stack spill");
-    if ((e = dArena_.reserve(wavefrontArenaBytes((uint32_t)slots, lvCap, lc, lights))) != hipSuccess) return hipFail(e, "wavefront state");
+    const bool nee = (p.extensions & OMV_EXT_HDRI_NEE) && S.env_tex_pdf;
+    if ((e = dArena_.reserve(wavefrontArenaBytes((uint32_t)slots, lvCap, lc, lights, nee))) != hipSuccess) return hipFail(e, "wavefront state");
     if ((e = dQcount_.reserve(wavefrontCounterCount(p.camera_bounces, lvCap) * sizeof(uint32_t))) != hipSuccess) return hipFail(e, "queue counters");
     if ((e = dRad_.reserve(slots * 3 * sizeof(float))) != hipSuccess) return hipFail(e, "sample buffer");
     WfLaunch L;
@@ -240,9 +243,9 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     L.blocksShade = cus_ * (lights ? wfPerCu_[1] : wfPerCu_[1]);
     L.blocksGen = cus_ * wfPerCu_[3];
     L.traceKeep = tuning_.trace_keep;
-    // an event pair around every traversal launch: (light + 2 camera rounds) per chunk
+    // an event pair around every traversal launch: (light + 3 camera rounds) per chunk
     const uint64_t chunks = (npix + chunkPix - 1) / chunkPix;
-    const size_t evNeed = (size_t)std::min<uint64_t>(1u << 14, chunks * 2 * (uint64_t)(lvCap + 2 * p.camera_bounces));
+    const size_t evNeed = (size_t)std::min<uint64_t>(1u << 14, chunks * 2 * (uint64_t)(lvCap + 3 * p.camera_bounces));
     while (traceEv_.size() < evNeed) {
         hipEvent_t ev;
         if ((e = hipEventCreate(&ev)) != hipSuccess) return hipFail(e, "hipEventCreate");
@@ -279,6 +282,7 @@ int BDPTRenderer::backward(PinholeCamera& cam, const float* dFilm, const omv_adj
     if (!scene_ || !scene_->committed()) { setError("backward: no committed scene"); return OMV_E_STATE; }
     const OmvSceneView& S = scene_->view();
     if (S.num_lights > 0) { setError("backward: scenes with point lights are not differentiated"); return OMV_E_STATE; }
+    if (params_.extensions & OMV_EXT_HDRI_NEE) { setError("backward: OMV_EXT_HDRI_NEE is not differentiated"); return OMV_E_STATE; }
     if (params_.extensions & OMV_EXT_OM_ROUGH) { setError("backward: the OM rough-conductor extension is not differentiated"); return OMV_E_STATE; }
     if (!dFilm) { setError("backward: null d_film"); return OMV_E_ARG; }
     if (t.n_textures < 0 || t.n_om_materials < 0 || (t.n_textures && (!t.texture_ids || !t.texture_grads)) ||

@@ -9,6 +9,7 @@
 #include <utility>
 
 #include "bvh.h"
+#include "omv_kernels.h"
 #include "render_core.h"
 
 namespace omv {
@@ -651,11 +652,28 @@ int RenderScene::commit(std::string& log) {
     v.light_max = cdf.empty() ? 0.0f : cdf.back();
     v.min_density = minD; v.max_density = maxD;
 
+    // environment importance sampling tables, built on the device
+    if (bgType_ == OMV_BG_HDR && bgW_ > 0 && bgH_ > 0) {
+        const size_t n = (size_t)bgW_ * bgH_, H = (size_t)bgH_;
+        if ((e = dEnv_.reserve((2 * H + 3 * n) * 4 + 256)) != hipSuccess) return hipFail(e, "environment tables");
+        if ((e = dEnvScratch_.reserve((4 * n + 2 * H + 2) * 4)) != hipSuccess) return hipFail(e, "environment scratch");
+        float* rp = dEnv_.as<float>();
+        int* ra = reinterpret_cast<int*>(rp + H);
+        float* cp = reinterpret_cast<float*>(ra + H);
+        int* ca = reinterpret_cast<int*>(cp + n);
+        float* tp = reinterpret_cast<float*>(ca + n);
+        if ((e = launchEnvBuild(dBg_.as<float>(), bgW_, bgH_, rp, ra, cp, ca, tp, dEnvScratch_.ptr, nullptr)) != hipSuccess ||
+            (e = hipDeviceSynchronize()) != hipSuccess)
+            return hipFail(e, "environment table build");
+        dEnvScratch_.release();
+        v.env_row_prob = rp; v.env_row_alias = ra; v.env_col_prob = cp; v.env_col_alias = ca; v.env_tex_pdf = tp;
+    }
+
     nodeCount_ = (int)nodes.size();
     bvhDepth_ = tl.depth + blasDepth;
     recordCount_ = (int)(tris.size() / 12);
     deviceBytes_ = dVtx_.bytes + dFaces_.bytes + dMeshes_.bytes + dInsts_.bytes + dMats_.bytes + dTexs_.bytes + dTexels_.bytes +
-                   dNodes_.bytes + dTris_.bytes + dLights_.bytes + dCdf_.bytes + dBg_.bytes;
+                   dNodes_.bytes + dTris_.bytes + dLights_.bytes + dCdf_.bytes + dBg_.bytes + dEnv_.bytes;
     committed_ = true;
     return OMV_OK;
 }
@@ -680,6 +698,18 @@ int RenderScene::updateMaterialOmNk(int matId, const float n[3], const float k[3
     m.has_base = 1;
     hipError_t e = hipMemcpy(dMats_.as<uint8_t>() + (size_t)matId * sizeof(OmvMaterial), &m, sizeof(OmvMaterial), hipMemcpyHostToDevice);
     if (e != hipSuccess) return hipFail(e, "update_material");
+    return OMV_OK;
+}
+
+int RenderScene::envTables(float* rowProb, int* rowAlias, float* colProb, int* colAlias, float* texPdf) const {
+    if (!committed_ || !view_.env_tex_pdf) { setError("env_tables: no environment tables"); return OMV_E_STATE; }
+    const size_t n = (size_t)bgW_ * bgH_, H = (size_t)bgH_;
+    hipError_t e = hipMemcpy(rowProb, view_.env_row_prob, H * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(rowAlias, view_.env_row_alias, H * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(colProb, view_.env_col_prob, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(colAlias, view_.env_col_alias, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(texPdf, view_.env_tex_pdf, n * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hipFail(e, "env_tables");
     return OMV_OK;
 }
 

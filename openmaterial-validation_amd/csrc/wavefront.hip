@@ -7,7 +7,9 @@
 //   gen             camera ray + light-subpath start of every slot
 //   light rounds    trace(light rays) -> shade_light   (vertex i = 1..)
 //   camera rounds   trace(camera rays) -> shade_cam    (vertex i = 1..B)
-//                   [ -> trace(shadow rays) -> connect  when there are lights ]
+//                   [ -> trace(shadow rays)  when there are lights ]
+//                   [ -> trace(environment rays)  with OMV_EXT_HDRI_NEE ]
+//                   [ -> connect  when either is there ]
 //
 // so each trace launch sees only rays that exist, whatever the lengths of
 // the paths around them, and the traversal kernel stays small (registers,
@@ -44,7 +46,7 @@ struct WfHits {
 };
 struct WfState {
     uint32_t n;        // slots
-    float4* thr;       // camera throughput xyz
+    float4* thr;       // camera throughput xyz, MIS weight of an escape (OMV_EXT_HDRI_NEE)
     float4* radc;      // camera radiance xyz, as_float(vertex count)
     float4* conn;      // connection sum xyz
     uint32_t* csn;     // camera stream draws consumed
@@ -57,6 +59,8 @@ struct WfState {
     float* lv;         // light vertices [vertex][6][slot]
     float* cc;         // staged connection contributions [l][3][slot]
     uint8_t* vis;      // [l][slot] 0 none/blocked, 1 pending, 2 visible
+    float4* neeC;      // staged next-event contribution xyz (OMV_EXT_HDRI_NEE)
+    uint8_t* nvis;     // [slot] its visibility: 0 none/blocked, 1 pending, 2 visible
     int lvCap, lc;     // light vertex capacity, connections per camera vertex
 };
 
@@ -117,7 +121,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_gen(OmvSceneView S, omv_came
         float u0 = cs.next(), u1 = cs.next();
         f3 O, D;
         cameraRay(cam, x, y, u0, u1, O, D);
-        W.thr[slot] = f4of(mk3(1.0f, 1.0f, 1.0f), 0.0f);
+        W.thr[slot] = f4of(mk3(1.0f, 1.0f, 1.0f), 1.0f);
         W.radc[slot] = f4of(mk3(0.0f, 0.0f, 0.0f), __int_as_float(1));
         W.conn[slot] = f4of(mk3(0.0f, 0.0f, 0.0f), 0.0f);
         W.csn[slot] = rngConsumed(cs);
@@ -164,7 +168,8 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_gen(OmvSceneView S, omv_came
 // ---------------------------------------------------------------- trace
 // sceneIntersect (BDPTRenderer.cpp:75-102) over a queue.  kKind 0: camera
 // rays, 1: light rays (closest hit + the re-trace tnear), 2: connection
-// rays (the visibility decision of isConnected, BDPTRenderer.cpp:221-234).
+// rays (the visibility decision of isConnected, BDPTRenderer.cpp:221-234),
+// 3: environment rays of OMV_EXT_HDRI_NEE (visible iff nothing is hit).
 template <int kKind>
 __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSceneView S, WfQueue q, WfShadowQ sq, const uint32_t* __restrict__ count,
                                                           WfHits H, uint8_t* __restrict__ vis, uint32_t nslots, int lc,
@@ -194,6 +199,12 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
             const uint32_t id = __float_as_uint(b.w);
             const uint32_t slot = id / (uint32_t)lc, l = id - slot * (uint32_t)lc;
             vis[(uint64_t)l * nslots + slot] = visible ? 2 : 0;
+        } else if (kKind == 3) {
+            const float4 o = q.o[e], d = q.d[e];
+            float tnear = o.w;
+            ++nShadow;
+            const bool hit = sceneIntersect(S, xyz(o), xyz(d), tnear, FLT_MAX, h, st, cnt, rays);
+            vis[__float_as_uint(d.w)] = hit ? 0 : 2;
         } else {
             const float4 o = q.o[e], d = q.d[e];
             float tnear = o.w;
@@ -253,7 +264,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_trace_pt(OmvSceneView S, WfQ
                     } else {
                         const float4 o = q.o[e], d = q.d[e];
                         org = xyz(o); dir = xyz(d); tn = o.w; t1 = FLT_MAX;
-                        if (kKind == 0) ++nCam; else ++nLight;
+                        if (kKind == 0) ++nCam; else if (kKind == 1) ++nLight; else ++nShadow;
                     }
                     itr = 10;
                     ++rays;
@@ -302,6 +313,8 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_trace_pt(OmvSceneView S, WfQ
                         const uint32_t id = __float_as_uint(sq.b[e].w);
                         const uint32_t slot = id / (uint32_t)lc, l = id - slot * (uint32_t)lc;
                         vis[(uint64_t)l * nslots + slot] = visible ? 2 : 0;
+                    } else if (kKind == 3) {
+                        vis[__float_as_uint(q.d[e].w)] = hit ? 0 : 2;
                     } else {
                         float4 r; r.x = T.h.t; r.y = T.h.u; r.z = T.h.v; r.w = T.tnear;
                         int2 id; id.x = hit ? T.h.inst : -1; id.y = T.h.prim | (T.h.tie ? (int)0x80000000 : 0);
@@ -384,12 +397,13 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_light(OmvSceneView S, 
 // ---------------------------------------------------------- shade camera
 // Camera vertex i (computePath with inv = true, BDPTRenderer.cpp:149-176),
 // the connections of that vertex staged for the visibility trace
-// (BDPTRenderer.cpp:266-297), and the escape term (:251-258).
-template <bool kLights>
+// (BDPTRenderer.cpp:266-297), the escape term (:251-258) and, with
+// OMV_EXT_HDRI_NEE, the next-event estimate staged for its visibility trace.
+template <bool kLights, bool kNee>
 __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_cam(OmvSceneView S, RP P, uint64_t sample0, int i, WfState W, WfQueue in,
                                                               const uint32_t* __restrict__ inCount, WfHits H, WfQueue out,
                                                               uint32_t* __restrict__ outCount, WfShadowQ sq, uint32_t* __restrict__ sqCount,
-                                                              float* __restrict__ rad) {
+                                                              WfQueue eq, uint32_t* __restrict__ eqCount, float* __restrict__ rad) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t total = *inCount;
@@ -403,8 +417,10 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_cam(OmvSceneView S, RP
         const float4 rc = W.radc[slot];
         f3 radiance = xyz(rc);
         int n = __float_as_int(rc.w);
-        f3 of = xyz(W.thr[slot]);
-        bool emit = false;
+        const float4 th = W.thr[slot];
+        f3 of = xyz(th);
+        bool emit = false, envRay = false;
+        f3 eOrg = mk3(0.0f, 0.0f, 0.0f), eDir = eOrg;
         f3 nPos = mk3(0.0f, 0.0f, 0.0f), nDir = nPos;
         float nTnear = 0.0f;
         if (id.x < 0) {
@@ -413,6 +429,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_cam(OmvSceneView S, RP
             if (S.bg_type != OMV_BG_NONE && n <= q) {
                 f3 bg = background(S, pDir);
                 bg = mk3(bg.x * of.x, bg.y * of.y, bg.z * of.z);
+                if (kNee) bg = mk3(bg.x * th.w, bg.y * th.w, bg.z * th.w);
                 radiance = mk3(bg.x * 1.0f + radiance.x, bg.y * 1.0f + radiance.y, bg.z * 1.0f + radiance.z);
             }
             const f3 conn = xyz(W.conn[slot]);
@@ -488,13 +505,22 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_cam(OmvSceneView S, RP
             Rng cs;
             rngRestore(cs, P.seed, STREAM_CAMERA, sample0 + slot, W.csn[slot]);
             f3 outRay, brdf, erad;
-            matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, true, cs);
+            BounceInfo bi;
+            matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, true, cs, kNee ? &bi : nullptr);
             float ct = fabsf(dot3(outRay, sp.n));
+            const f3 ofPrev = of;
             of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
             radiance = mk3(radiance.x + of.x * erad.x, radiance.y + of.y * erad.y, radiance.z + of.z * erad.z);
+            float escW = 1.0f;
+            if (kNee) {
+                f3 c;
+                envRay = neePrepare(S, P, sp, bi, ofPrev, outRay, i, sample0 + slot, escW, c, eOrg, eDir);
+                if (envRay) W.neeC[slot] = f4of(c, 0.0f);
+                W.nvis[slot] = envRay ? 1 : 0;
+            }
             ++n;
             W.csn[slot] = rngConsumed(cs);
-            W.thr[slot] = f4of(of, 0.0f);
+            W.thr[slot] = f4of(of, escW);
             W.radc[slot] = f4of(radiance, __int_as_float(n));
             // continue unless the loop ends (i == B) or an exact shortcut stops it
             const bool zero = of.x == 0.0f && of.y == 0.0f && of.z == 0.0f;
@@ -505,7 +531,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_cam(OmvSceneView S, RP
                 nPos = mk3(sp.pos.x + delta * outRay.x, sp.pos.y + delta * outRay.y, sp.pos.z + delta * outRay.z);
                 nDir = outRay;
                 nTnear = hr.w;
-            } else if (kLights) {
+            } else if (kLights || kNee) {
                 W.flags[slot] = 1u;   // written by the connect kernel of this round
             } else {
                 const f3 conn = xyz(W.conn[slot]);
@@ -520,14 +546,23 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_cam(OmvSceneView S, RP
             out.o[k] = f4of(nPos, nTnear);
             out.d[k] = f4of(nDir, __uint_as_float(slot));
         }
+        if (kNee) {
+            const uint32_t ke = waveAppend(eqCount, envRay);
+            if (envRay) {
+                eq.o[ke] = f4of(eOrg, 0.0f);
+                eq.d[ke] = f4of(eDir, __uint_as_float(slot));
+            }
+        }
     }
 }
 
 // -------------------------------------------------------------- connect
-// Adds the visible connections of this round in light-vertex order and
-// writes the sample of paths that ended this round.
+// Adds the visible connections of this round in light-vertex order and the
+// visible next-event estimate (after the vertex's emission, as the
+// per-sample function does), and writes the sample of paths that ended
+// this round.
 __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_connect(WfState W, WfQueue in, const uint32_t* __restrict__ inCount,
-                                                            float* __restrict__ rad) {
+                                                            float* __restrict__ rad, int lights, int nee) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t total = *inCount;
@@ -536,17 +571,25 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_connect(WfState W, WfQueue i
         const uint32_t slot = __float_as_uint(in.d[e].w);
         const uint32_t fl = W.flags[slot];
         if (fl == 2u) continue;
-        const uint32_t nc = W.nconn[slot];
         f3 conn = xyz(W.conn[slot]);
-        for (uint32_t l = 0; l < nc; ++l) {
-            if (W.vis[(uint64_t)l * N + slot] != 2) continue;
-            f3 lc = mk3(W.cc[((uint64_t)l * 3 + 0) * N + slot], W.cc[((uint64_t)l * 3 + 1) * N + slot],
-                        W.cc[((uint64_t)l * 3 + 2) * N + slot]);
-            conn = mk3(lc.x + conn.x, lc.y + conn.y, lc.z + conn.z);
+        if (lights) {
+            const uint32_t nc = W.nconn[slot];
+            for (uint32_t l = 0; l < nc; ++l) {
+                if (W.vis[(uint64_t)l * N + slot] != 2) continue;
+                f3 lc = mk3(W.cc[((uint64_t)l * 3 + 0) * N + slot], W.cc[((uint64_t)l * 3 + 1) * N + slot],
+                            W.cc[((uint64_t)l * 3 + 2) * N + slot]);
+                conn = mk3(lc.x + conn.x, lc.y + conn.y, lc.z + conn.z);
+            }
+            W.conn[slot] = f4of(conn, 0.0f);
         }
-        W.conn[slot] = f4of(conn, 0.0f);
+        float4 rc = W.radc[slot];
+        if (nee && W.nvis[slot] == 2) {
+            const f3 c = xyz(W.neeC[slot]);
+            rc.x = rc.x + c.x; rc.y = rc.y + c.y; rc.z = rc.z + c.z;
+            W.radc[slot] = rc;
+        }
         if (fl == 1u) {
-            const f3 radiance = xyz(W.radc[slot]);
+            const f3 radiance = xyz(rc);
             rad[(uint64_t)slot * 3 + 0] = conn.x * 1.0f + radiance.x;
             rad[(uint64_t)slot * 3 + 1] = conn.y * 1.0f + radiance.y;
             rad[(uint64_t)slot * 3 + 2] = conn.z * 1.0f + radiance.z;
@@ -577,7 +620,7 @@ constexpr size_t kAlign = 256;
 size_t up(size_t b) { return (b + kAlign - 1) & ~(kAlign - 1); }
 }  // namespace
 
-size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights) {
+size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee) {
     const size_t N = nslots;
     size_t b = 6 * up(N * 16)                        // three ray queues
                + up(N * 16) + up(N * 8)              // hits
@@ -585,15 +628,16 @@ size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights) {
     if (lights)
         b += up(N * 16) + 3 * up(N * 4) + up(N * (size_t)lvCap * 24) + up(N * (size_t)lc * 12) + up(N * (size_t)lc) +
              3 * up(N * (size_t)lc * 16);
+    if (nee) b += up(N * 16) + up(N) + 2 * up(N * 16);   // neeC, nvis, environment-ray queue
     return b;
 }
 
-int wavefrontCounterCount(int B, int lvCap) { return 2 * lvCap + 4 * B + 16; }
+int wavefrontCounterCount(int B, int lvCap) { return 2 * lvCap + 6 * B + 16; }
 
 void wavefrontBlocksPerCU(int out[4]) {
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
     out[0] = std::min(occ(omv_wf_trace<0>, lds), occ(omv_wf_trace_pt<2>, lds));
-    out[1] = occ(omv_wf_shade_cam<true>, 0);
+    out[1] = occ(omv_wf_shade_cam<true, true>, 0);
     out[2] = occ(omv_wf_shade_light, 0);
     out[3] = occ(omv_wf_gen<true>, 0);
 }
@@ -601,8 +645,10 @@ void wavefrontBlocksPerCU(int out[4]) {
 // One chunk of whole pixels: samples [sample0, sample0 + nslots).
 hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
     const OmvSceneView& S = L.S;
-    const RP P = toRP(L.p);
+    RP P = toRP(L.p);
     const bool lights = S.num_lights > 0;
+    if (!S.env_tex_pdf) P.x.nee = false;   // no HDR environment: nothing to sample
+    const bool nee = P.x.nee;
     const uint32_t n = L.nslots;
     const size_t N = n;
     uint8_t* cur = static_cast<uint8_t*>(L.arena);
@@ -635,15 +681,24 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
         sq.d = reinterpret_cast<float4*>(take(N * (size_t)L.lc * 16));
         sq.b = reinterpret_cast<float4*>(take(N * (size_t)L.lc * 16));
     }
+    WfQueue eq;
+    std::memset(&eq, 0, sizeof(eq));
+    if (nee) {
+        W.neeC = reinterpret_cast<float4*>(take(N * 16));
+        W.nvis = reinterpret_cast<uint8_t*>(take(N));
+        eq = queue();
+    }
     // queue counters: light round i reads lightCnt[i] (i = 1 .. lvCap-1) and
     // appends to lightCnt[i+1]; camera round i reads camCnt[i] (i = 1 .. B),
-    // appends to camCnt[i+1] and stages connection rays in shCnt[i]
+    // appends to camCnt[i+1], stages connection rays in shCnt[i] and
+    // environment rays in envCnt[i]
     const int B = L.p.camera_bounces;
     const int ncnt = wavefrontCounterCount(B, L.lvCap);
     uint32_t* lightCnt = L.qcount;
     uint32_t* camCnt = lightCnt + (L.lvCap + 2);
     uint32_t* shCnt = camCnt + (B + 2);
-    uint32_t* fetchCnt = shCnt + (B + 2);            // one per persistent traversal launch
+    uint32_t* envCnt = shCnt + (B + 2);
+    uint32_t* fetchCnt = envCnt + (B + 2);           // one per persistent traversal launch
     int nf = 0;
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
     const dim3 blk(OMV_BLOCK);
@@ -687,8 +742,12 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
                                L.counters);
         mark();
         if (lights) {
-            hipLaunchKernelGGL(omv_wf_shade_cam<true>, dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i, H, out,
-                               camCnt + i + 1, sq, shCnt + i, L.rad);
+            if (nee)
+                hipLaunchKernelGGL((omv_wf_shade_cam<true, true>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i, H,
+                                   out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad);
+            else
+                hipLaunchKernelGGL((omv_wf_shade_cam<true, false>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i,
+                                   H, out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad);
             mark();
             if (L.traceKeep > 0)
                 hipLaunchKernelGGL(omv_wf_trace_pt<2>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, shCnt + i, fetchCnt + (nf++),
@@ -697,11 +756,26 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
                 hipLaunchKernelGGL(omv_wf_trace<2>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, shCnt + i, H, W.vis, n, L.lc, L.gstack,
                                    L.counters);
             mark();
-            hipLaunchKernelGGL(omv_wf_connect, dim3(L.blocksShade), blk, 0, stream, W, in, camCnt + i, L.rad);
         } else {
-            hipLaunchKernelGGL(omv_wf_shade_cam<false>, dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i, H, out,
-                               camCnt + i + 1, sq, shCnt + i, L.rad);
+            if (nee)
+                hipLaunchKernelGGL((omv_wf_shade_cam<false, true>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i,
+                                   H, out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad);
+            else
+                hipLaunchKernelGGL((omv_wf_shade_cam<false, false>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i,
+                                   H, out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad);
         }
+        if (nee) {
+            mark();
+            if (L.traceKeep > 0)
+                hipLaunchKernelGGL(omv_wf_trace_pt<3>, dim3(L.blocksTrace), blk, lds, stream, S, eq, sq, envCnt + i, fetchCnt + (nf++),
+                                   L.traceKeep, H, W.nvis, n, L.lc, L.gstack, L.counters);
+            else
+                hipLaunchKernelGGL(omv_wf_trace<3>, dim3(L.blocksTrace), blk, lds, stream, S, eq, sq, envCnt + i, H, W.nvis, n, L.lc, L.gstack,
+                                   L.counters);
+            mark();
+        }
+        if (lights || nee)
+            hipLaunchKernelGGL(omv_wf_connect, dim3(L.blocksShade), blk, 0, stream, W, in, camCnt + i, L.rad, lights ? 1 : 0, nee ? 1 : 0);
         std::swap(in, out);
     }
     return hipGetLastError();

@@ -108,7 +108,7 @@ static void philox(const uint32_t in[4], const uint32_t key[2], uint32_t out[4])
 /* uniform float in [0,1) with 24 random bits */
 static inline float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
 
-enum { STREAM_CAMERA = 0, STREAM_LIGHT = 1 };
+enum { STREAM_CAMERA = 0, STREAM_LIGHT = 1, STREAM_NEE = 2 };  /* NEE: 2 + 16 * bounce */
 
 /* One sample's sub-stream: key (seed, 0), counter (call, stream, sample lo, hi). */
 struct Stream {
@@ -284,7 +284,72 @@ struct Scene {
     std::vector<Bvh> blas;
     Bvh tlas;
     std::vector<int> faceBase;
+    /* environment importance sampling (OMV_EXT_HDRI_NEE): a row table and one
+     * column table per row (Vose alias method), and the texel probabilities */
+    std::vector<float> envRowProb, envColProb, envTexPdf;
+    std::vector<int> envRowAlias, envColAlias;
 };
+
+/* ------------------------------------------ environment alias tables */
+/* Texel weight: Rec.709 luminance times sin(polar angle) at the row centre,
+ * the sine by Bhaskara I's rational form 16x(pi-x) / (5pi^2 - 4x(pi-x)) so
+ * that host and device build identical tables from + - * / alone. */
+static float envRowSine(int i, int H) {
+    const float pi = 3.14159265358979323846f;
+    const float x = ((float)i + 0.5f) * pi / (float)H;
+    const float a = x * (pi - x);
+    return 16.0f * a / (5.0f * pi * pi - 4.0f * a);
+}
+/* Vose's alias method over n weights, sequential, in the order the device
+ * kernel uses (sum in index order, LIFO work lists) */
+static void voseBuild(const float* w, int n, float* prob, int* alias, std::vector<float>& p, std::vector<int>& small,
+                      std::vector<int>& large) {
+    float S = 0.0f;
+    for (int k = 0; k < n; ++k) S = S + w[k];
+    if (!(S > 0.0f)) { for (int k = 0; k < n; ++k) { prob[k] = 1.0f; alias[k] = k; } return; }
+    const float scale = (float)n / S;
+    p.resize(n); small.resize(n); large.resize(n);
+    int ns = 0, nl = 0;
+    for (int k = 0; k < n; ++k) {
+        p[k] = w[k] * scale;
+        if (p[k] < 1.0f) small[ns++] = k; else large[nl++] = k;
+    }
+    while (ns > 0 && nl > 0) {
+        const int l = small[--ns], g = large[--nl];
+        prob[l] = p[l]; alias[l] = g;
+        p[g] = (p[g] + p[l]) - 1.0f;
+        if (p[g] < 1.0f) small[ns++] = g; else large[nl++] = g;
+    }
+    while (nl > 0) { const int g = large[--nl]; prob[g] = 1.0f; alias[g] = g; }
+    while (ns > 0) { const int l = small[--ns]; prob[l] = 1.0f; alias[l] = l; }
+}
+static void buildEnvTables(Scene& S) {
+    S.envRowProb.clear(); S.envColProb.clear(); S.envTexPdf.clear(); S.envRowAlias.clear(); S.envColAlias.clear();
+    if (S.bgType != 2 || S.bgW <= 0 || S.bgH <= 0) return;
+    const int W = S.bgW, H = S.bgH;
+    std::vector<float> w((size_t)W * H), rowW(H);
+    for (int i = 0; i < H; ++i) {
+        const float sn = envRowSine(i, H);
+        float rs = 0.0f;
+        for (int j = 0; j < W; ++j) {
+            const float* t = &S.bgRgb[((size_t)i * W + j) * 3];
+            const float lum = 0.2126f * t[0] + 0.7152f * t[1] + 0.0722f * t[2];
+            const float v = std::max(lum, 0.0f) * sn;
+            w[(size_t)i * W + j] = v;
+            rs = rs + v;
+        }
+        rowW[i] = rs;
+    }
+    float total = 0.0f;
+    for (int i = 0; i < H; ++i) total = total + rowW[i];
+    S.envRowProb.resize(H); S.envRowAlias.resize(H);
+    S.envColProb.resize((size_t)W * H); S.envColAlias.resize((size_t)W * H); S.envTexPdf.resize((size_t)W * H);
+    std::vector<float> p; std::vector<int> sm, lg;
+    voseBuild(rowW.data(), H, S.envRowProb.data(), S.envRowAlias.data(), p, sm, lg);
+    for (int i = 0; i < H; ++i)
+        voseBuild(&w[(size_t)i * W], W, &S.envColProb[(size_t)i * W], &S.envColAlias[(size_t)i * W], p, sm, lg);
+    for (size_t k = 0; k < w.size(); ++k) S.envTexPdf[k] = total > 0.0f ? w[k] / total : 0.0f;
+}
 
 /* ------------------------------------------------------- mesh commit */
 /* RenderMesh::computeNormals (RenderMesh.cpp:300-330) */
@@ -1206,9 +1271,11 @@ static V3<R> omFresnelBrdf(const Material& m, R fCos) {
 struct Ext {
     bool omRough = false;   // OMV_EXT_OM_ROUGH
     float omAlpha = 0.09f;  // GGX alpha = roughness^2
+    bool nee = false;       // OMV_EXT_HDRI_NEE
     explicit Ext(const omv_renderer_params& p) {
         omRough = (p.extensions & OMV_EXT_OM_ROUGH) != 0;
         omAlpha = p.om_roughness * p.om_roughness;
+        nee = (p.extensions & OMV_EXT_HDRI_NEE) != 0;
     }
 };
 
@@ -1407,9 +1474,121 @@ struct Tracer {
         switch (P.light_distance_attenuation) { case 1: return rD; case 2: return std::sqrt(rD); default: return d < attD ? R(1) : R(0); }
     }
 
+    /* ---- OMV_EXT_HDRI_NEE ------------------------------------------- */
+    R lastEscW = R(1);   /* MIS weight of an escape after the last camera vertex */
+
+    /* inverse of HDRBackground::getRadiance's mapping for a point (ux, uy) of
+     * texel (row, col); sinT = sine of the polar angle */
+    V3<R> envDirection(int row, int col, float ux, float uy, R& sinT) const {
+        const R pi = kPi<R>();
+        const R u = (R(col) + R(ux)) / R(S.bgW), v = (R(row) + R(uy)) / R(S.bgH);
+        const R yang = (R(1) - v) * pi;
+        const R rz = -std::cos(yang);
+        const R xang = u * R(2) * pi;
+        const R r = std::sqrt(mx(R(1) - rz * rz, R(0)));
+        sinT = r;
+        return mul33<R>(S.bgRot, mk(r * std::cos(xang), r * std::sin(xang), rz));
+    }
+    /* solid-angle density of the environment sampler at dir */
+    R envPdf(const V3<R>& dir) const {
+        if (S.envTexPdf.empty()) return R(0);
+        const R pi = kPi<R>();
+        V3<R> rdir = mulT33(dir, S.bgRot);
+        V3<R> rxy = normalize3(mk(rdir.x, rdir.y, R(0)));
+        R xang = std::acos(clampv(rxy.x, R(-1), R(1)));
+        xang = rxy.y < R(0) ? R(2) * pi - xang : xang;
+        const R z = clampv(rdir.z, R(-1), R(1));
+        const R yang = std::asin(z) + R(0.5f) * pi;
+        const R u = xang / (R(2) * pi), v = R(1) - yang / pi;
+        const int W = S.bgW, H = S.bgH;
+        const int col = std::min(std::max((int)(u * R(W)), 0), W - 1), row = std::min(std::max((int)(v * R(H)), 0), H - 1);
+        const R sinT = std::sqrt(mx(R(1) - z * z, R(0)));
+        if (!(sinT > R(0))) return R(0);
+        return R(S.envTexPdf[(size_t)row * W + col]) * R(W) * R(H) / (R(2) * pi * pi * sinT);
+    }
+    /* density of the reference's BSDF sampler (sampleIndirect's lobe mixture)
+     * of direction O at an opaque PBR vertex */
+    R bsdfPdf(const Bsdf<R>& b, const V3<R>& O) const {
+        const R VN = -dot3(b.I, b.N);
+        if (VN < R(0)) return R(0);
+        const R P = R(0.8f) * std::abs(VN) + R(0.1f);
+        const R LN = dot3(O, b.N);
+        if (!(LN > R(0))) return R(0);
+        R p = P * LN / kPi<R>();
+        const V3<R> H = normalize3(sub(O, b.I));
+        const R NH = dot3(b.N, H), LH = dot3(O, H);
+        if (NH > R(0) && LH > R(0)) p += (R(1) - P) * ggxPdf(NH, b.alpha) * NH / (R(4) * LH);
+        return p;
+    }
+    /* the reference estimator's integrand per solid angle (weight x density x
+     * cosine of sampleIndirect, clamps included) of direction O */
+    V3<R> bsdfEffCos(const Bsdf<R>& b, const V3<R>& O) const {
+        const V3<R> H = normalize3(sub(O, b.I));
+        const R VNc = clampv(std::abs(dot3(b.I, b.N)), R(0.0001f), R(1));
+        const R LNc = clampv(std::abs(dot3(O, b.N)), R(0.0001f), R(1));
+        const R LN = std::abs(dot3(O, b.N));
+        const R NH = std::abs(dot3(b.N, H)), VH = std::abs(dot3(b.I, H));
+        const R p5 = std::pow(clamp01(R(1) - VH), R(5));
+        const R D = ggxPdf(NH, b.alpha);
+        const R G = smithG2(smithG1(VNc, b.alpha), smithG1(LNc, b.alpha));
+        V3<R> out;
+        for (int c = 0; c < 3; ++c) {
+            const R F0 = lerpv(R(0.04f), b.color[c], b.metal);
+            const R F = lerpv(p5, R(1), F0);
+            const R fd = (R(1) - F) * ((R(1) - b.metal) * b.color[c]) / kPi<R>();
+            at(out, c) = fd * LN * LN / LNc + F * G * D * LN / (R(4) * LNc * LNc);
+        }
+        return out;
+    }
+    /* next-event estimate toward the environment at camera vertex i, and the
+     * MIS weight of the escape along the sampled direction outV */
+    V3<R> neeEstimate(const SurfacePoint<R>& sp, const BounceInfo<R>& bi, const V3<R>& ofPrev, const V3<R>& outV, int i,
+                      uint64_t sample) {
+        V3<R> z = mk(R(0), R(0), R(0));
+        lastEscW = R(1);
+        if (!X.nee || S.envTexPdf.empty() || bi.kind != 1) return z;
+        const Bsdf<R>& b = bi.b;
+        /* opaque, front-facing (sampleIndirect returns zero for V.N < 0), and
+         * an escape from this vertex must still be counted (n <= q) */
+        if (b.transmissive || -dot3(b.I, b.N) < R(0) || i >= P.camera_bounces || i + 1 > P.max_path_length) return z;
+        /* MIS weight of the escape along the BSDF-sampled direction */
+        const R pbS = bsdfPdf(b, outV), peS = envPdf(outV);
+        if (pbS + peS > R(0)) lastEscW = pbS / (pbS + peS);
+        /* environment sample: row, column (alias tables), point in the texel */
+        Stream ns(P.seed, STREAM_NEE + 16u * (uint32_t)i, sample);
+        const float u0 = ns.next(), u1 = ns.next(), u2 = ns.next(), u3 = ns.next(), u4 = ns.next(), u5 = ns.next();
+        const int W = S.bgW, H = S.bgH;
+        int k = std::min((int)(u0 * (float)H), H - 1);
+        const int row = u1 < S.envRowProb[k] ? k : S.envRowAlias[k];
+        k = std::min((int)(u2 * (float)W), W - 1);
+        const size_t rk = (size_t)row * W + k;
+        const int col = u3 < S.envColProb[rk] ? k : S.envColAlias[rk];
+        R sinT;
+        const V3<R> w = envDirection(row, col, u4, u5, sinT);
+        if (!(sinT > R(0))) return z;
+        const R pe = R(S.envTexPdf[(size_t)row * W + col]) * R(W) * R(H) / (R(2) * kPi<R>() * kPi<R>() * sinT);
+        const R pb = bsdfPdf(b, w);
+        if (!(pe > R(0)) || !(dot3(w, b.N) > R(0))) return z;
+        const V3<R> f = bsdfEffCos(b, w);
+        const V3<R> bg = background(w);
+        const R inv = R(1) / (pe + pb);
+        V3<R> c = mk(ofPrev.x * f.x * bg.x * inv, ofPrev.y * f.y * bg.y * inv, ofPrev.z * f.z * bg.z * inv);
+        if (c.x == R(0) && c.y == R(0) && c.z == R(0)) return z;
+        /* visibility toward the environment (sceneIntersect from the shifted vertex) */
+        const R delta = R(3) * feps<R>() * maxAbs(sp.position);
+        const V3<R> org = mk(sp.position.x + delta * w.x, sp.position.y + delta * w.y, sp.position.z + delta * w.z);
+        R tn = R(0);
+        SurfacePoint<R> hs;
+        ++shadowRays;
+        if (sceneIntersect(org, w, tn, std::numeric_limits<R>::max(), hs)) return z;
+        return c;
+    }
+
     /* computePath (BDPTRenderer.cpp:128-192).  maxN vertices incl. the start. */
+    uint64_t neeSample = 0;   /* sample index of the camera path being walked */
     bool computePath(Vtx<R>* Pv, int maxN, const V3<R>& O, const V3<R>& Rd, V3<R>& Rad, int& n, bool inv, R attD,
                      Uniforms& U) {
+        if (inv) lastEscW = R(1);
         Pv[0].sp = SurfacePoint<R>();
         Pv[0].pos = O; Pv[0].outRay = Rd; Pv[0].outRad = Rad; Pv[0].outFactor = mk(R(1), R(1), R(1));
         R rDterm = R(1);
@@ -1426,13 +1605,18 @@ struct Tracer {
             Pv[i].pos = Pv[i].sp.position;
             modifyFrame(S, Pv[i].sp);
             V3<R> brdf, erad;
-            matGetRayAndBrdf(S, X, Pv[i].sp, d0, Pv[i].outRay, brdf, erad, inv, U, E);
+            BounceInfo<R> bi;
+            matGetRayAndBrdf(S, X, Pv[i].sp, d0, Pv[i].outRay, brdf, erad, inv, U, E, &bi);
             R ct = std::abs(inv ? dot3(Pv[i].outRay, Pv[i].sp.normal) : dot3(d0, Pv[i].sp.normal));
             const V3<R>& of = Pv[i - 1].outFactor;
             Pv[i].outFactor = mk(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
             if (inv) {
                 Rad = mk(Rad.x + Pv[i].outFactor.x * erad.x, Rad.y + Pv[i].outFactor.y * erad.y, Rad.z + Pv[i].outFactor.z * erad.z);
                 Pv[i].outRad = mk(R(0), R(0), R(0));
+                if (X.nee) {   /* OMV_EXT_HDRI_NEE: after the vertex's emission */
+                    V3<R> c = neeEstimate(Pv[i].sp, bi, of, Pv[i].outRay, i, neeSample);
+                    Rad = mk(Rad.x + c.x, Rad.y + c.y, Rad.z + c.z);
+                }
             } else {
                 if (i == 1) rDterm = attenuation(distance3(Pv[1].pos, Pv[0].pos), attD);
                 Pv[i].outRad = mk(Pv[i].outFactor.x * Rad.x * rDterm, Pv[i].outFactor.y * Rad.y * rDterm, Pv[i].outFactor.z * Rad.z * rDterm);
@@ -1523,12 +1707,14 @@ struct Tracer {
         Uniforms U; U.s = &cs;
         int cpLen = 0;
         V3<R> radiance = mk(R(0), R(0), R(0));
+        neeSample = sample;
         bool cpExit = computePath(cp, P.camera_bounces + 1, O, Rd, radiance, cpLen, true, R(0), U);
         int totalPN = 0;
         if (cpExit && S.bgType != 0 && cpLen <= q) {
             V3<R> bg = background(cp[cpLen - 1].outRay);
             const V3<R>& kf = cp[cpLen - 1].outFactor;
             bg = mk(bg.x * kf.x, bg.y * kf.y, bg.z * kf.z);
+            if (X.nee) bg = mk(bg.x * lastEscW, bg.y * lastEscW, bg.z * lastEscW);   /* OMV_EXT_HDRI_NEE */
             radiance = mk(bg.x * R(1) + radiance.x, bg.y * R(1) + radiance.y, bg.z * R(1) + radiance.z);
             ++totalPN;
         } else if (intensity(radiance) > feps<R>()) ++totalPN;
@@ -2111,6 +2297,7 @@ int ref_scene_commit(ref_scene* r, char* log, size_t cap) {
     }
     buildBvh(S.tlas, ib);
     for (auto& p : S.tlas.prims) p = ids[p];
+    buildEnvTables(S);
     S.committed = true;
     return OMV_OK;
 }
@@ -2194,6 +2381,46 @@ int ref_adjoint(ref_scene* r, const omv_renderer_params* p, const omv_camera_des
 }
 
 void ref_adjoint_plant(int mask) { gAdjPlant = mask; }
+
+/* OMV_EXT_HDRI_NEE environment tables of a committed scene (H row entries,
+ * W*H column entries and texel probabilities); returns 0 without an HDR */
+int ref_env_tables(ref_scene* r, float* rowProb, int* rowAlias, float* colProb, int* colAlias, float* texPdf) {
+    if (!r || !r->s.committed || r->s.envRowProb.empty()) return 0;
+    const Scene& S = r->s;
+    std::memcpy(rowProb, S.envRowProb.data(), S.envRowProb.size() * 4);
+    std::memcpy(rowAlias, S.envRowAlias.data(), S.envRowAlias.size() * 4);
+    std::memcpy(colProb, S.envColProb.data(), S.envColProb.size() * 4);
+    std::memcpy(colAlias, S.envColAlias.data(), S.envColAlias.size() * 4);
+    std::memcpy(texPdf, S.envTexPdf.data(), S.envTexPdf.size() * 4);
+    return 1;
+}
+/* one environment sample from six uniforms: texel, direction, density */
+int ref_env_sample(ref_scene* r, const float u[6], double dir[3], double* pdf, int* row, int* col) {
+    if (!r || !r->s.committed || r->s.envRowProb.empty()) return OMV_E_STATE;
+    const Scene& S = r->s;
+    omv_renderer_params p;
+    ref_renderer_params_default(&p);
+    Tracer<double> T(S, p);
+    const int W = S.bgW, H = S.bgH;
+    int k = std::min((int)(u[0] * (float)H), H - 1);
+    *row = u[1] < S.envRowProb[k] ? k : S.envRowAlias[k];
+    k = std::min((int)(u[2] * (float)W), W - 1);
+    const size_t rk = (size_t)*row * W + k;
+    *col = u[3] < S.envColProb[rk] ? k : S.envColAlias[rk];
+    double sinT;
+    V3<double> w = T.envDirection(*row, *col, u[4], u[5], sinT);
+    dir[0] = w.x; dir[1] = w.y; dir[2] = w.z;
+    *pdf = sinT > 0 ? (double)S.envTexPdf[(size_t)*row * W + *col] * W * H / (2 * kPi<double>() * kPi<double>() * sinT) : 0.0;
+    return OMV_OK;
+}
+/* solid-angle density of the environment sampler at a direction */
+double ref_env_pdf(ref_scene* r, const double dir[3]) {
+    if (!r || !r->s.committed) return 0.0;
+    omv_renderer_params p;
+    ref_renderer_params_default(&p);
+    Tracer<double> T(r->s, p);
+    return T.envPdf(mk(dir[0], dir[1], dir[2]));
+}
 
 /* d(Rp, Rs)/d(n, k) in double: out = {dRp/dn, dRp/dk, dRs/dn, dRs/dk} */
 void ref_fresnel_grad(double n, double k, double c, double out[4]) {

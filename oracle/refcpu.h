@@ -70,6 +70,9 @@ void ref_fresnel(float n, float k, float cos_theta, float* rp, float* rs);
 void ref_fresnel64(double n, double k, double cos_theta, double out[2]);
 void ref_fresnel_grad(double n, double k, double cos_theta, double out[4]);
 void ref_median3(float* img, int w, int h);
+int  ref_env_tables(ref_scene* s, float* row_prob, int* row_alias, float* col_prob, int* col_alias, float* tex_pdf);
+int  ref_env_sample(ref_scene* s, const float u[6], double dir[3], double* pdf, int* row, int* col);
+double ref_env_pdf(ref_scene* s, const double dir[3]);
 void ref_to_u8(const float* img, size_t n, uint8_t* out);
 void ref_adjoint_plant(int mask);   /* test aid: plants a known mistake in the adjoint */
 int  ref_render_detached(ref_scene* sample, ref_scene* eval, const omv_renderer_params* p, const omv_camera_desc* cam,

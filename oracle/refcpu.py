@@ -95,6 +95,9 @@ def lib():
         "ref_fresnel_grad": ([C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)], None),
         "ref_adjoint_plant": ([C.c_int], None),
         "ref_median3": ([FP, C.c_int, C.c_int], None),
+        "ref_env_tables": ([vp, FP, C.POINTER(C.c_int), FP, C.POINTER(C.c_int), FP], C.c_int),
+        "ref_env_sample": ([vp, FP, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+        "ref_env_pdf": ([vp, C.POINTER(C.c_double)], C.c_double),
         "ref_to_u8": ([FP, C.c_size_t, C.POINTER(C.c_uint8)], None),
         "ref_render_detached": ([vp, vp, C.POINTER(RendererParams), C.POINTER(CameraDesc), C.c_int, C.c_int, C.POINTER(C.c_double)],
                                 C.c_int),
@@ -210,6 +213,25 @@ class Scene(SceneApi):
             raise RuntimeError(f"ref_adjoint failed {rc}")
         return ({t: g.reshape(tex_shapes[t]) for t, g in zip(textures, grads)},
                 {m: (omg[6 * i:6 * i + 3].copy(), omg[6 * i + 3:6 * i + 6].copy()) for i, m in enumerate(om_materials)})
+
+    def env_tables(self, w, h):
+        rp, cp, tp = np.zeros(h, np.float32), np.zeros(w * h, np.float32), np.zeros(w * h, np.float32)
+        ra, ca = np.zeros(h, np.int32), np.zeros(w * h, np.int32)
+        ok = self.lib.ref_env_tables(self.h, ptr(rp), ptr(ra, C.c_int), ptr(cp), ptr(ca, C.c_int), ptr(tp))
+        return (rp, ra, cp.reshape(h, w), ca.reshape(h, w), tp.reshape(h, w)) if ok else None
+
+    def env_sample(self, u):
+        uu = f32(u, (6,))
+        d = (C.c_double * 3)()
+        pdf, row, col = C.c_double(), C.c_int(), C.c_int()
+        rc = self.lib.ref_env_sample(self.h, ptr(uu), d, C.byref(pdf), C.byref(row), C.byref(col))
+        if rc != 0:
+            raise RuntimeError("no environment tables")
+        return np.array(d[:]), pdf.value, row.value, col.value
+
+    def env_pdf(self, d):
+        dd = (C.c_double * 3)(*[float(x) for x in d])
+        return self.lib.ref_env_pdf(self.h, dd)
 
     def closest_hit(self, org, d, tnear=0.0, tfar=3.4e38, bvh=True):
         o, dd = f32(org, (3,)), f32(d, (3,))

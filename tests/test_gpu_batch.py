@@ -112,7 +112,8 @@ def test_batch_autotune(pkg, wl):
     plain = _plain_films(pkg, jobs, scenes)
     b = _batch(pkg, jobs, scenes)
     best = b.autotune(reps=1)
-    assert best["blocks_per_cu"] >= 1 and best["tile_waves"] in (1, 2, 4)
+    # either a multi-stream wavefront configuration or a per-pixel kernel one
+    assert best["streams"] in (1, 2, 4, 8) or (best["blocks_per_cu"] >= 1 and best["tile_waves"] in (1, 2, 4)), best
     b.run()
     for j in range(len(jobs)):
         assert np.array_equal(b.film(j), plain[j])

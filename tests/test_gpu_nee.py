@@ -1,0 +1,143 @@
+"""OMV_EXT_HDRI_NEE on the GPU against the oracle.
+
+Bars:
+  * the environment alias tables the device builds at commit equal the
+    oracle's bit for bit (same sequential Vose construction, + - * / only);
+  * radiance with NEE: the parity bar of the plain estimator (per-pixel
+    relative error below 1e-3 for >= 99% of pixels, means within 1e-3);
+  * the wavefront and per-sample schedules give the same film bit for bit
+    and the same ray counts, with and without point lights;
+  * a batch job with NEE equals the plain render of that job;
+  * backward() refuses NEE (its terms are not differentiated).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _env_desc(wl, lights=False, w=64, h=32, sun=200.0, masked=False):
+    desc, view = wl.mixed_test_scene(hdr=True, lights=lights, masked=masked)
+    sky = wl.hdri(w, h, seed=9, sun_factor=sun)
+    desc.background = ("hdr", sky, wl.HDR_ROTATION, wl.HDR_SCALE)
+    return desc, view, sky
+
+
+@pytest.mark.parametrize("w,h", [(64, 32), (256, 128), (37, 19)])
+def test_env_tables_bitexact(pkg, wl, ref, w, h):
+    desc, _, _ = _env_desc(wl, w=w, h=h)
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    rs = ref.Scene()
+    desc.apply(rs, ref.pbr_desc)
+    got = s.env_tables(w, h)
+    want = rs.env_tables(w, h)
+    for name, a, b in zip(("row_prob", "row_alias", "col_prob", "col_alias", "tex_pdf"), got, want):
+        assert np.array_equal(np.asarray(a).reshape(-1), np.asarray(b).reshape(-1)), name
+
+
+def test_env_tables_absent_without_hdr(pkg, wl):
+    desc, _ = wl.mixed_test_scene(hdr=False, lights=True)
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    with pytest.raises(RuntimeError):
+        s.env_tables(4, 4)
+
+
+def _cam(pkg, view):
+    return pkg.Camera(view.width, view.height, view.position, view.forward, view.up, view.yfov)
+
+
+PARAMS = dict(samples_per_pixel=8, camera_bounces=4, light_bounces=3, max_path_length=5, seed=11, extensions=2)
+
+
+@pytest.mark.parametrize("lights", [False, True])
+def test_nee_radiance_matches_oracle(pkg, wl, ref, lights):
+    desc, view, _ = _env_desc(wl, lights=lights, masked=True)
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = _cam(pkg, view)
+    r = pkg.Renderer(pkg.default_params(**PARAMS))
+    r.set_scene(s)
+    img = r.render(cam)
+    rs = ref.Scene()
+    desc.apply(rs, ref.pbr_desc)
+    rimg, rst = rs.render(ref.default_params(**PARAMS), cam.desc(), threads=8)
+    a, b = img[..., :3].astype(np.float64), rimg[..., :3].astype(np.float64)
+    assert np.all(np.isfinite(a))
+    err = np.abs(a - b) / np.maximum(np.abs(b), 1e-3)
+    frac = float(np.mean(err.max(axis=-1) < 1e-3))
+    assert frac >= 0.99, (frac, err.max())
+    assert abs(a.mean() - b.mean()) <= 1e-3 * max(abs(b.mean()), 1e-6)
+    st = r.stats()
+    # environment rays are counted as shadow rays, as in the oracle
+    assert abs(st["shadow_rays"] - rst["shadow_rays"]) <= 0.002 * rst["shadow_rays"] + 4
+    # NEE changes the image (the estimator is different, its mean is not)
+    plain = pkg.Renderer(pkg.default_params(**dict(PARAMS, extensions=0)))
+    plain.set_scene(s)
+    assert not np.array_equal(plain.render(cam), img)
+
+
+@pytest.mark.parametrize("lights", [False, True])
+def test_nee_schedules_agree(pkg, wl, lights):
+    desc, view, _ = _env_desc(wl, lights=lights, masked=True)
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = _cam(pkg, view)
+    out = []
+    for sched in (pkg.SCHEDULE_WAVEFRONT, pkg.SCHEDULE_PER_SAMPLE):
+        r = pkg.Renderer(pkg.default_params(**PARAMS))
+        r.set_scene(s)
+        r.set_schedule(sched)
+        out.append((r.render(cam), r.stats()))
+    (a, sa), (b, sb) = out
+    assert np.array_equal(a, b), np.argwhere(a != b)[:5].tolist()
+    for k in ("samples", "rays", "camera_rays", "light_rays", "shadow_rays", "nodes_visited", "tris_tested"):
+        assert sa[k] == sb[k], (k, sa[k], sb[k])
+
+
+def test_nee_multi_chunk(pkg, wl):
+    desc, view, _ = _env_desc(wl, lights=False)
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = _cam(pkg, view)
+    r = pkg.Renderer(pkg.default_params(**PARAMS))
+    r.set_scene(s)
+    whole = r.render(cam)
+    r.set_tuning(chunk_samples=20001)
+    assert np.array_equal(r.render(cam), whole)
+
+
+@pytest.mark.parametrize("streams", [0, 2])
+def test_nee_batch_equals_plain(pkg, wl, streams):
+    desc, view, _ = _env_desc(wl, lights=False)
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = _cam(pkg, view)
+    plain_p = dict(PARAMS, extensions=0, seed=5)
+    films = []
+    for p in (PARAMS, plain_p):
+        r = pkg.Renderer(pkg.default_params(**p))
+        r.set_scene(s)
+        films.append(r.render(cam))
+    b = pkg.Batch()
+    b.add_job(s, cam, pkg.default_params(**PARAMS))
+    b.add_job(s, cam, pkg.default_params(**plain_p))
+    b.set_tuning(streams=streams)
+    b.run()
+    assert np.array_equal(b.film(0), films[0])
+    assert np.array_equal(b.film(1), films[1])
+
+
+def test_backward_rejects_nee(pkg, wl):
+    desc, view, _ = _env_desc(wl, lights=False)
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    r = pkg.Renderer(pkg.default_params(samples_per_pixel=1, extensions=2))
+    r.set_scene(s)
+    cam = pkg.Camera(8, 8, view.position, view.forward, view.up, view.yfov)
+    d = torch.zeros((8, 8, 4), device="cuda")
+    with pytest.raises(RuntimeError, match="NEE"):
+        r.backward(cam, d.data_ptr())
