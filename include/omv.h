@@ -65,6 +65,14 @@ extern "C" {
                                     term by the balance heuristic against the
                                     reference's own BSDF sampling density; same
                                     expected image as the reference estimator  */
+#define OMV_EXT_LIGHT_SPLAT 4u   /* light-tracer (t = 1) splats: every stored
+                                    light-subpath vertex is connected to the
+                                    pinhole and its particle contribution
+                                    added, with a visibility ray, to the pixel
+                                    it projects to (float atomics); the film is
+                                    the reference estimate plus the splat image
+                                    over all light paths (W*H*spp); scenes
+                                    without point lights are unaffected       */
 
 /* Mirrors RendererParameters (pathtracer/src/renderer/RendererParameters.h:24-96).
  * omv_renderer_params_default() fills the reference defaults. */

@@ -24,6 +24,7 @@ OMV_OK = 0
 EXT_NONE = 0
 EXT_OM_ROUGH = 1
 EXT_HDRI_NEE = 2
+EXT_LIGHT_SPLAT = 4
 
 FC_SUBJECTS = ("pid", "gid", "mid", "rmp", "mmp", "md", "sg", "mn", "in")
 PBR_BASE_COLOR, PBR_METALLIC_ROUGHNESS, PBR_NORMAL, PBR_EMISSIVITY, PBR_OCCLUSION = range(5)

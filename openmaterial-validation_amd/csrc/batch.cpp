@@ -48,6 +48,10 @@ int BatchRunner::addJob(const RenderScene* s, const omv_camera_desc& cam, const 
     if (!s || !s->committed()) { setError("batch: job scene is not committed"); return OMV_E_STATE; }
     if (p.samples_per_pixel < 0 || p.camera_bounces < 0 || p.light_bounces < 0) { setError("batch: negative parameter"); return OMV_E_ARG; }
     if (cam.width <= 0 || cam.height <= 0) { setError("batch: empty camera"); return OMV_E_ARG; }
+    if ((p.extensions & OMV_EXT_LIGHT_SPLAT) && s->view().num_lights > 0) {
+        setError("batch: OMV_EXT_LIGHT_SPLAT jobs are rendered with omv_renderer_render");
+        return OMV_E_ARG;
+    }
     Job j;
     j.scene = s;
     j.cam = cam;
@@ -186,7 +190,7 @@ int BatchRunner::prepareWavefront(const std::vector<int>& order) {
         const uint64_t slots = std::min(J.npix, chunkPix) * (uint64_t)spp;
         if (slots * (uint64_t)lc >= (1ull << 32)) { setError("batch: too many samples per pixel for one chunk"); return OMV_E_RANGE; }
         const bool nee = (J.p.extensions & OMV_EXT_HDRI_NEE) && J.scene->view().env_tex_pdf;
-        arena = std::max(arena, wavefrontArenaBytes((uint32_t)slots, lvCap, lc, lights, nee));
+        arena = std::max(arena, wavefrontArenaBytes((uint32_t)slots, lvCap, lc, lights, nee, false));
         rad = std::max(rad, (size_t)slots * 3 * sizeof(float));
         ncnt = std::max(ncnt, wavefrontCounterCount(J.p.camera_bounces, lvCap));
     }

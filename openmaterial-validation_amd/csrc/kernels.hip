@@ -25,17 +25,18 @@
 // ============================================================== kernels
 // Plain render: one sample per lane at a time over [sample0, sample0 + nsamples),
 // persistent grid; radiance per sample for the ordered film reduction.
-template <bool kLights, bool kNee>
+template <bool kLights, bool kNee, bool kSplat>
 __global__ __launch_bounds__(OMV_BLOCK) void omv_bdpt_kernel(OmvSceneView S, omv_camera_desc cam, RP P, int spp,
                                                              uint64_t sample0, uint64_t nsamples, float* __restrict__ rad,
                                                              float* __restrict__ lv, uint32_t* __restrict__ gstack,
-                                                             unsigned long long* __restrict__ counters) {
+                                                             unsigned long long* __restrict__ counters, float* __restrict__ splat) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Lane L;
     laneInit(L, gstack, lv, gtid, gthreads);
+    L.splat = splat;
     for (uint64_t sl = gtid; sl < nsamples; sl += gthreads) {
-        f3 r = pathSample<kLights, kNee>(S, cam, P, spp, sample0 + sl, L);
+        f3 r = pathSample<kLights, kNee, kSplat>(S, cam, P, spp, sample0 + sl, L);
         rad[sl * 3 + 0] = r.x;
         rad[sl * 3 + 1] = r.y;
         rad[sl * 3 + 2] = r.z;
@@ -116,6 +117,17 @@ __global__ void omv_film_kernel(const float* __restrict__ rad, int spp, uint64_t
     float sc = cnt < 0.5f ? 1.0f : 1.0f / cnt;
     float4 o; o.x = sc * a0; o.y = sc * a1; o.z = sc * a2; o.w = 1.0f;
     reinterpret_cast<float4*>(film)[pix0 + p] = o;
+}
+
+// OMV_EXT_LIGHT_SPLAT: film += splat sums / (number of light paths).
+__global__ void omv_splat_add_kernel(const float* __restrict__ splat, uint64_t npix, float inv, float* __restrict__ film) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    float4 o = reinterpret_cast<float4*>(film)[p];
+    o.x = o.x + splat[p * 3 + 0] * inv;
+    o.y = o.y + splat[p * 3 + 1] * inv;
+    o.z = o.z + splat[p * 3 + 2] * inv;
+    reinterpret_cast<float4*>(film)[p] = o;
 }
 
 // False-colour pass: one camera bounce per pixel (main.cpp:261-270 set-up).
@@ -209,20 +221,33 @@ namespace omv {
 
 hipError_t launchBdpt(const OmvSceneView& S, const omv_camera_desc& cam, const omv_renderer_params& p, uint64_t sample0,
                       uint64_t nsamples, float* rad, float* lv, uint32_t* gstack, unsigned long long* counters, int blocks,
-                      hipStream_t stream) {
+                      float* splat, hipStream_t stream) {
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
     RP P = toRP(p);
     if (!S.env_tex_pdf) P.x.nee = false;   // no HDR environment: nothing to sample
+    if (!splat || S.num_lights <= 0) P.x.splat = false;
     const dim3 g(blocks), b(OMV_BLOCK);
     const int spp = p.samples_per_pixel;
-    if (S.num_lights > 0 && P.x.nee)
-        hipLaunchKernelGGL((omv_bdpt_kernel<true, true>), g, b, lds, stream, S, cam, P, spp, sample0, nsamples, rad, lv, gstack, counters);
-    else if (S.num_lights > 0)
-        hipLaunchKernelGGL((omv_bdpt_kernel<true, false>), g, b, lds, stream, S, cam, P, spp, sample0, nsamples, rad, lv, gstack, counters);
-    else if (P.x.nee)
-        hipLaunchKernelGGL((omv_bdpt_kernel<false, true>), g, b, lds, stream, S, cam, P, spp, sample0, nsamples, rad, lv, gstack, counters);
-    else
-        hipLaunchKernelGGL((omv_bdpt_kernel<false, false>), g, b, lds, stream, S, cam, P, spp, sample0, nsamples, rad, lv, gstack, counters);
+    if (S.num_lights > 0) {
+        if (P.x.splat && P.x.nee)
+            hipLaunchKernelGGL((omv_bdpt_kernel<true, true, true>), g, b, lds, stream, S, cam, P, spp, sample0, nsamples, rad, lv, gstack,
+                               counters, splat);
+        else if (P.x.splat)
+            hipLaunchKernelGGL((omv_bdpt_kernel<true, false, true>), g, b, lds, stream, S, cam, P, spp, sample0, nsamples, rad, lv, gstack,
+                               counters, splat);
+        else if (P.x.nee)
+            hipLaunchKernelGGL((omv_bdpt_kernel<true, true, false>), g, b, lds, stream, S, cam, P, spp, sample0, nsamples, rad, lv, gstack,
+                               counters, splat);
+        else
+            hipLaunchKernelGGL((omv_bdpt_kernel<true, false, false>), g, b, lds, stream, S, cam, P, spp, sample0, nsamples, rad, lv, gstack,
+                               counters, splat);
+    } else if (P.x.nee) {
+        hipLaunchKernelGGL((omv_bdpt_kernel<false, true, false>), g, b, lds, stream, S, cam, P, spp, sample0, nsamples, rad, lv, gstack,
+                           counters, splat);
+    } else {
+        hipLaunchKernelGGL((omv_bdpt_kernel<false, false, false>), g, b, lds, stream, S, cam, P, spp, sample0, nsamples, rad, lv, gstack,
+                           counters, splat);
+    }
     return hipGetLastError();
 }
 
@@ -230,6 +255,12 @@ hipError_t launchFilm(const float* rad, int spp, uint64_t pix0, uint64_t npix, f
     const int tb = 256;
     const uint64_t blocks = (npix + tb - 1) / tb;
     hipLaunchKernelGGL(omv_film_kernel, dim3((unsigned)blocks), dim3(tb), 0, stream, rad, spp, pix0, npix, film);
+    return hipGetLastError();
+}
+
+hipError_t launchSplatAdd(const float* splat, uint64_t npix, float inv, float* film, hipStream_t stream) {
+    const int tb = 256;
+    hipLaunchKernelGGL(omv_splat_add_kernel, dim3((unsigned)((npix + tb - 1) / tb)), dim3(tb), 0, stream, splat, npix, inv, film);
     return hipGetLastError();
 }
 
@@ -277,12 +308,14 @@ int occupancyBlocksPerCU(int which) {
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
     hipError_t e;
     switch (which) {
-        case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<true, false>, OMV_BLOCK, lds); break;
-        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<false, false>, OMV_BLOCK, lds); break;
+        case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<true, false, false>, OMV_BLOCK, lds); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<false, false, false>, OMV_BLOCK, lds); break;
         case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_batch_kernel<false>, OMV_BLOCK, lds); break;
-        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<true, true>, OMV_BLOCK, lds); break;
-        case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<false, true>, OMV_BLOCK, lds); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<true, true, false>, OMV_BLOCK, lds); break;
+        case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<false, true, false>, OMV_BLOCK, lds); break;
         case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_batch_kernel<true>, OMV_BLOCK, lds); break;
+        case 7: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<true, false, true>, OMV_BLOCK, lds); break;
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_bdpt_kernel<true, true, true>, OMV_BLOCK, lds); break;
         default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, omv_fc_kernel, OMV_BLOCK, lds); break;
     }
     if (e != hipSuccess) n = 0;

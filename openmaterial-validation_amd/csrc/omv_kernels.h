@@ -15,9 +15,11 @@ namespace omv {
 // Sample kernel over samples [sample0, sample0 + nsamples) of a chunk that
 // covers whole pixels; rad receives 3 floats per sample (chunk-relative).
 hipError_t launchBdpt(const OmvSceneView& S, const omv_camera_desc& cam, const omv_renderer_params& p, uint64_t sample0,
-                      uint64_t nsamples, float* rad, float* lv, uint32_t* gstack, unsigned long long* counters, int blocks,
+                      uint64_t nsamples, float* rad, float* lv, uint32_t* gstack, unsigned long long* counters, int blocks, float* splat,
                       hipStream_t stream);
 // Ordered per-pixel reduction + PinholeCamera::stop for pixels [pix0, pix0 + npix).
+// OMV_EXT_LIGHT_SPLAT: film (float4 per pixel) += splat (3 floats per pixel) * inv
+hipError_t launchSplatAdd(const float* splat, uint64_t npix, float inv, float* film, hipStream_t stream);
 hipError_t launchFilm(const float* rad, int spp, uint64_t pix0, uint64_t npix, float* film, hipStream_t stream);
 hipError_t launchFc(const OmvSceneView& S, const omv_camera_desc& cam, uint32_t seed, float* film, int32_t* ids, uint8_t* tie,
                     uint32_t* gstack, unsigned long long* counters, int blocks, hipStream_t stream);
@@ -39,7 +41,7 @@ struct WfLaunch {
     uint64_t sample0;          // first sample of the chunk
     uint32_t nslots;           // samples in the chunk
     int lvCap, lc;             // light vertices kept, connections per camera vertex
-    void* arena;               // wavefrontArenaBytes(nslots, lvCap, lc, lights, nee)
+    void* arena;               // wavefrontArenaBytes(nslots, lvCap, lc, lights, nee, splat)
     uint32_t* qcount;          // wavefrontCounterCount(B, lvCap) counters
     float* rad;                // 3 floats per sample of the chunk
     uint32_t* gstack;          // spill stacks of blocksTrace * OMV_BLOCK lanes
@@ -49,8 +51,9 @@ struct WfLaunch {
     hipEvent_t* traceEv;       // optional: an event pair around every traversal launch
     int traceEvCap;            // events available from traceEv
     int* traceEvUsed;          // events recorded so far (in/out)
+    float* splat;              // OMV_EXT_LIGHT_SPLAT sums (3 floats per pixel), or null
 };
-size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee);
+size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat);
 int wavefrontCounterCount(int B, int lvCap);
 void wavefrontBlocksPerCU(int out[4]);   // trace, shade, shade_light, gen
 hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream);

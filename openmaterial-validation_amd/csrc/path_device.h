@@ -661,7 +661,7 @@ DEV f3 omFresnelBrdf(const OmvMaterial& M, float fCos) {
     return mk3(o[0], o[1], o[2]);
 }
 // Estimator extensions (omv_renderer_params.extensions)
-struct Ext { bool omRough; float omAlpha; bool nee; };
+struct Ext { bool omRough; float omAlpha; bool nee, splat; };
 
 // OMV_EXT_OM_ROUGH: GGX rough conductor with the complex-IOR Fresnel as F.
 // Connection: f = F(|I.H|) D G2 / (4 V.N L.N), zero unless both cosines > 0.
@@ -1166,6 +1166,78 @@ DEV bool neePrepare(const OmvSceneView& S, const RP& P, const SP& sp, const Boun
     return true;
 }
 
+// ---------------------------------------------- OMV_EXT_LIGHT_SPLAT
+// Light-tracer (t = 1) splats.  Particle weight of a light subpath at its
+// first hit: beta_1 = I * lightScale / p(w) * att(r1) * r1^2 with p = 2/pi,
+// the density of uniformSphere's + octant; beta_{i+1} = beta_i * w_i *
+// |w_out . n_i| (the camera subpath's throughput update).
+DEV f3 splatBeta1(const RP& P, const f3& intensity, float att, float r1) {
+    const float k1 = P.light_scale * (kPi * 0.5f) * att * r1 * r1;
+    return mk3(intensity.x * k1, intensity.y * k1, intensity.z * k1);
+}
+// Pinhole connection of light vertex sp (arrived along inV) with weight
+// beta: the pixel it projects to, its contribution
+//   beta * getBrdf(inV, w_c) * |w_c . n| * f^2 / (d^2 cos^3(theta))
+// and the visibility ray (from the vertex toward the pinhole, tnear = the
+// isConnected offset, tfar = d; visible iff nothing is hit).
+DEV bool splatPrepare(const OmvSceneView& S, const Ext& X, const omv_camera_desc& c, const SP& sp, const f3& inV, const f3& beta,
+                      f3& ctb, uint32_t& pix, f3& wc, float& d, float& tn) {
+    const f3 C = ld3(c.position);
+    const f3 y = sp.pos;
+    const f3 v = sub3(C, y);
+    d = length3(v);
+    if (!(d > 0.0f)) return false;
+    wc = mk3(v.x / d, v.y / d, v.z / d);
+    const f3 l = mulT33(sub3(y, C), c.rotation);
+    if (!(l.z > 0.0f)) return false;
+    const float px = c.focus * l.x / l.z + c.half_w, py = c.focus * l.y / l.z + c.half_h;
+    if (!(px >= 0.0f && px < (float)c.width && py >= 0.0f && py < (float)c.height)) return false;
+    const int ix = min((int)px, c.width - 1), iy = min((int)py, c.height - 1);
+    const float cosT = l.z / length3(l);
+    const f3 f = matGetBrdf(S, X, sp, inV, wc);
+    const float k = c.focus * c.focus * fabsf(dot3(wc, sp.n)) / (d * d * cosT * cosT * cosT);
+    ctb = mk3(beta.x * f.x * k, beta.y * f.y * k, beta.z * f.z * k);
+    if (ctb.x == 0.0f && ctb.y == 0.0f && ctb.z == 0.0f) return false;
+    if (!(isfinite(ctb.x) && isfinite(ctb.y) && isfinite(ctb.z))) return false;
+    pix = (uint32_t)iy * (uint32_t)c.width + (uint32_t)ix;
+    tn = 2.0f * 3.0f * FLT_EPSILON * maxAbs(y);
+    return true;
+}
+// Film accumulation of splats with wave-aggregated atomics: the active lanes
+// of the wave that add to the same pixel are summed first and one lane adds
+// the sum; lanes alone on their pixel add in one vector atomic.  Call with
+// every active lane (want = false for lanes with nothing to add).
+DEV void splatAdd(float* film, bool want, uint32_t pix, const f3& v) {
+    const int lane = threadIdx.x & 63;
+    uint64_t pend = __ballot(want);
+    bool solo = false;
+    while (pend) {
+        const int leader = __ffsll((unsigned long long)pend) - 1;
+        const uint32_t pl = __shfl(pix, leader, 64);
+        const uint64_t m = __ballot(want && pix == pl) & pend;
+        if (m == (1ull << leader)) {
+            solo = solo || lane == leader;
+        } else {
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            for (uint64_t mm = m; mm; mm &= mm - 1ull) {
+                const int j = __ffsll((unsigned long long)mm) - 1;
+                sx += __shfl(v.x, j, 64); sy += __shfl(v.y, j, 64); sz += __shfl(v.z, j, 64);
+            }
+            if (lane == leader) {
+                atomicAdd(film + (size_t)pl * 3 + 0, sx);
+                atomicAdd(film + (size_t)pl * 3 + 1, sy);
+                atomicAdd(film + (size_t)pl * 3 + 2, sz);
+            }
+        }
+        pend &= ~m;
+    }
+    if (solo) {
+        atomicAdd(film + (size_t)pix * 3 + 0, v.x);
+        atomicAdd(film + (size_t)pix * 3 + 1, v.y);
+        atomicAdd(film + (size_t)pix * 3 + 2, v.z);
+    }
+}
+
 // DistanceRandom::getValue (DistanceRandom.h:203-220)
 DEV int sampleLight(const OmvSceneView& S, float u) {
     const int n = S.num_lights;
@@ -1213,6 +1285,7 @@ inline RP toRP(const omv_renderer_params& p) {
     r.x.omRough = (p.extensions & OMV_EXT_OM_ROUGH) != 0;
     r.x.omAlpha = p.om_roughness * p.om_roughness;
     r.x.nee = (p.extensions & OMV_EXT_HDRI_NEE) != 0;
+    r.x.splat = (p.extensions & OMV_EXT_LIGHT_SPLAT) != 0;
     return r;
 }
 
@@ -1230,6 +1303,7 @@ struct Lane {
     Stack st;
     Counters cnt;
     uint32_t rays, camRays, lightRays, shadowRays;
+    float* splat;   // OMV_EXT_LIGHT_SPLAT film (W*H*3 sums), or null
     float* lv;
     uint64_t lvStride;
     DEV float& lvAt(int vtx, int c) { return lv[((uint64_t)vtx * 6 + c) * lvStride]; }
@@ -1241,14 +1315,16 @@ DEV void laneInit(Lane& L, uint32_t* gstack, float* lv, uint32_t gtid, uint32_t 
     L.st.sp = 0;
     L.cnt.nodes = 0; L.cnt.tris = 0; L.cnt.overflow = 0;
     L.rays = L.camRays = L.lightRays = L.shadowRays = 0;
+    L.splat = nullptr;
     L.lv = lv + gtid; L.lvStride = gthreads;
 }
 
 // One sample of BDPTRenderer::computePaths (BDPTRenderer.cpp:237-301):
 // sample s = pixel * spp + k.  kLights = false compiles no light subpath
 // and no connection code (scenes without point lights, BDPTRenderer.cpp:241-242);
-// kNee = false compiles no OMV_EXT_HDRI_NEE code.
-template <bool kLights, bool kNee = false>
+// kNee = false compiles no OMV_EXT_HDRI_NEE code, kSplat = false no
+// OMV_EXT_LIGHT_SPLAT code.
+template <bool kLights, bool kNee = false, bool kSplat = false>
 DEV f3 pathSample(const OmvSceneView& S, const omv_camera_desc& cam, const RP& P, int spp, uint64_t s, Lane& L) {
     const int q = P.q;
     const int lvCap = max(min(P.light_bounces + 1, max(q, 1)), 1);
@@ -1273,7 +1349,7 @@ DEV f3 pathSample(const OmvSceneView& S, const omv_camera_desc& cam, const RP& P
         // vertex 0: the light itself
         L.lvAt(0, 0) = lightPos.x; L.lvAt(0, 1) = lightPos.y; L.lvAt(0, 2) = lightPos.z;
         L.lvAt(0, 3) = lRad.x; L.lvAt(0, 4) = lRad.y; L.lvAt(0, 5) = lRad.z;
-        f3 pPos = lightPos, pDir = dir, of = mk3(1.0f, 1.0f, 1.0f);
+        f3 pPos = lightPos, pDir = dir, of = mk3(1.0f, 1.0f, 1.0f), beta = of;
         float tnear = 0.0f, rD = 1.0f;
         int n = 1;
         // exact shortcut: light vertices with index >= max_path_length are never connected
@@ -1291,6 +1367,21 @@ DEV f3 pathSample(const OmvSceneView& S, const omv_camera_desc& cam, const RP& P
             float ct = fabsf(dot3(pDir, sp.n));
             of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
             if (i == 1) rD = attenuation(P, length3(sub3(sp.pos, lightPos)), lightRange);
+            if (kSplat && P.x.splat) {   // OMV_EXT_LIGHT_SPLAT: every hit of the walk, before the hard cut
+                if (i == 1) beta = splatBeta1(P, lightInt, rD, length3(sub3(sp.pos, lightPos)));
+                f3 ctb, wc;
+                uint32_t pix = 0;
+                float d, tn;
+                bool vis = false;
+                if (splatPrepare(S, P.x, cam, sp, pDir, beta, ctb, pix, wc, d, tn)) {
+                    Hit hs;
+                    ++L.shadowRays;
+                    vis = !sceneIntersect(S, sp.pos, wc, tn, d, hs, L.st, L.cnt, L.rays);
+                }
+                splatAdd(L.splat, vis, pix, ctb);
+                const float co = fabsf(dot3(outRay, sp.n));
+                beta = mk3(beta.x * brdf.x * co, beta.y * brdf.y * co, beta.z * brdf.z * co);
+            }
             f3 outRad = mk3(of.x * lRad.x * rD, of.y * lRad.y * rD, of.z * lRad.z * rD);
             if (intensity(outRad) < P.cut) break;
             L.lvAt(i, 0) = sp.pos.x; L.lvAt(i, 1) = sp.pos.y; L.lvAt(i, 2) = sp.pos.z;

@@ -146,7 +146,8 @@ private:
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
     int cus_ = 0;
-    DeviceBuffer dRad_, dLv_, dStack_, dCounters_, dFilm_, dIds_, dTie_, dArena_, dQcount_, dDfilm_, dAdjMaps_, dPost_;
+    DeviceBuffer dRad_, dLv_, dStack_, dCounters_, dFilm_, dIds_, dTie_, dArena_, dQcount_, dDfilm_, dAdjMaps_, dPost_, dSplat_;
+    float* splat_ = nullptr;   // OMV_EXT_LIGHT_SPLAT sums of the current render, or null
     const PinholeCamera* filmOf_ = nullptr;   // camera whose film dFilm_ holds
     std::vector<hipEvent_t> traceEv_;
 };

@@ -132,11 +132,21 @@ int BDPTRenderer::render(PinholeCamera& cam) {
     if ((e = hipMemsetAsync(dCounters_.ptr, 0, 8 * sizeof(unsigned long long), stream_)) != hipSuccess) return hipFail(e, "memset");
     // PinholeCamera::init (PinholeCamera.cpp:65-68)
     if ((e = hipMemsetAsync(dFilm_.ptr, 0, npix * 4 * sizeof(float), stream_)) != hipSuccess) return hipFail(e, "memset film");
+    // OMV_EXT_LIGHT_SPLAT: splat sums over the whole render, added to the film at the end
+    splat_ = nullptr;
+    if ((p.extensions & OMV_EXT_LIGHT_SPLAT) && scene_->view().num_lights > 0 && spp > 0) {
+        if ((e = dSplat_.reserve(npix * 3 * sizeof(float))) != hipSuccess) return hipFail(e, "splat film");
+        if ((e = hipMemsetAsync(dSplat_.ptr, 0, npix * 3 * sizeof(float), stream_)) != hipSuccess) return hipFail(e, "memset splat");
+        splat_ = dSplat_.as<float>();
+    }
     (void)hipEventRecord(ev_[0], stream_);
     double kernelMs = 0.0;
     if (spp > 0) {
         rc = schedule_ == OMV_SCHEDULE_WAVEFRONT ? renderWavefront(cam, kernelMs) : renderSamples(cam, kernelMs);
         if (rc) return rc;
+        if (splat_ && (e = launchSplatAdd(splat_, npix, (float)(1.0 / ((double)npix * (double)spp)), dFilm_.as<float>(), stream_)) !=
+                          hipSuccess)
+            return hipFail(e, "splat kernel launch");
     } else {
         // PinholeCamera::stop with a zero count leaves rgb = 0 and alpha = 1
         std::vector<float> z(npix * 4, 0.0f);
@@ -188,7 +198,7 @@ int BDPTRenderer::renderSamples(PinholeCamera& cam, double& kernelMs) {
     const uint64_t firstChunk = std::min(npix, chunkPix) * (uint64_t)spp;
     const bool nee = (p.extensions & OMV_EXT_HDRI_NEE) && scene_->view().env_tex_pdf;
     const bool lights = scene_->view().num_lights > 0;
-    int rc = prepare(lights ? (nee ? 4 : 0) : (nee ? 5 : 2), firstChunk, blocks);
+    int rc = prepare(lights ? (splat_ ? (nee ? 8 : 7) : (nee ? 4 : 0)) : (nee ? 5 : 2), firstChunk, blocks);
     if (rc) return rc;
     const uint64_t lanes = (uint64_t)blocks * OMV_BLOCK;
     const int lvCap = std::max(std::min(p.light_bounces + 1, std::max(p.max_path_length, 1)), 1);
@@ -199,7 +209,7 @@ int BDPTRenderer::renderSamples(PinholeCamera& cam, double& kernelMs) {
         const uint64_t np = std::min(chunkPix, npix - pix0);
         (void)hipEventRecord(ev_[2], stream_);
         if ((e = launchBdpt(scene_->view(), cam.desc, p, pix0 * (uint64_t)spp, np * (uint64_t)spp, dRad_.as<float>(), dLv_.as<float>(),
-                            dStack_.as<uint32_t>(), dCounters_.as<unsigned long long>(), blocks, stream_)) != hipSuccess)
+                            dStack_.as<uint32_t>(), dCounters_.as<unsigned long long>(), blocks, splat_, stream_)) != hipSuccess)
             return hipFail(e, "sample kernel launch");
         (void)hipEventRecord(ev_[3], stream_);
         if ((e = launchFilm(dRad_.as<float>(), spp, pix0, np, dFilm_.as<float>(), stream_)) != hipSuccess) return hipFail(e, "film kernel launch");
@@ -231,7 +241,8 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     if ((e = dStack_.reserve((uint64_t)blocksTrace * OMV_BLOCK * (OMV_STACK_TOTAL - OMV_STACK_LDS) * sizeof(uint32_t))) != hipSuccess)
         return hipFail(e, "stack spill");
     const bool nee = (p.extensions & OMV_EXT_HDRI_NEE) && S.env_tex_pdf;
-    if ((e = dArena_.reserve(wavefrontArenaBytes((uint32_t)slots, lvCap, lc, lights, nee))) != hipSuccess) return hipFail(e, "wavefront state");
+    if ((e = dArena_.reserve(wavefrontArenaBytes((uint32_t)slots, lvCap, lc, lights, nee, splat_ != nullptr))) != hipSuccess)
+        return hipFail(e, "wavefront state");
     if ((e = dQcount_.reserve(wavefrontCounterCount(p.camera_bounces, lvCap) * sizeof(uint32_t))) != hipSuccess) return hipFail(e, "queue counters");
     if ((e = dRad_.reserve(slots * 3 * sizeof(float))) != hipSuccess) return hipFail(e, "sample buffer");
     WfLaunch L;
@@ -243,9 +254,10 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     L.blocksShade = cus_ * (lights ? wfPerCu_[1] : wfPerCu_[1]);
     L.blocksGen = cus_ * wfPerCu_[3];
     L.traceKeep = tuning_.trace_keep;
-    // an event pair around every traversal launch: (light + 3 camera rounds) per chunk
+    L.splat = splat_;
+    // an event pair around every traversal launch: (2 light + 3 camera rounds) per chunk
     const uint64_t chunks = (npix + chunkPix - 1) / chunkPix;
-    const size_t evNeed = (size_t)std::min<uint64_t>(1u << 14, chunks * 2 * (uint64_t)(lvCap + 3 * p.camera_bounces));
+    const size_t evNeed = (size_t)std::min<uint64_t>(1u << 14, chunks * 2 * (uint64_t)(2 * lvCap + 3 * p.camera_bounces));
     while (traceEv_.size() < evNeed) {
         hipEvent_t ev;
         if ((e = hipEventCreate(&ev)) != hipSuccess) return hipFail(e, "hipEventCreate");

@@ -59,6 +59,7 @@ struct WfState {
     float* lv;         // light vertices [vertex][6][slot]
     float* cc;         // staged connection contributions [l][3][slot]
     uint8_t* vis;      // [l][slot] 0 none/blocked, 1 pending, 2 visible
+    float4* lbeta;     // light-tracer particle weight for the next vertex (OMV_EXT_LIGHT_SPLAT)
     float4* neeC;      // staged next-event contribution xyz (OMV_EXT_HDRI_NEE)
     uint8_t* nvis;     // [slot] its visibility: 0 none/blocked, 1 pending, 2 visible
     int lvCap, lc;     // light vertex capacity, connections per camera vertex
@@ -169,11 +170,14 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_gen(OmvSceneView S, omv_came
 // sceneIntersect (BDPTRenderer.cpp:75-102) over a queue.  kKind 0: camera
 // rays, 1: light rays (closest hit + the re-trace tnear), 2: connection
 // rays (the visibility decision of isConnected, BDPTRenderer.cpp:221-234),
-// 3: environment rays of OMV_EXT_HDRI_NEE (visible iff nothing is hit).
+// 3: environment rays of OMV_EXT_HDRI_NEE (visible iff nothing is hit),
+// 4: pinhole rays of OMV_EXT_LIGHT_SPLAT (visible iff nothing is hit; the
+//    visible contributions are added to the splat film).
 template <int kKind>
 __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSceneView S, WfQueue q, WfShadowQ sq, const uint32_t* __restrict__ count,
                                                           WfHits H, uint8_t* __restrict__ vis, uint32_t nslots, int lc,
-                                                          uint32_t* __restrict__ gstack, unsigned long long* __restrict__ counters) {
+                                                          uint32_t* __restrict__ gstack, unsigned long long* __restrict__ counters,
+                                                          float* __restrict__ splat) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
@@ -205,6 +209,12 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
             ++nShadow;
             const bool hit = sceneIntersect(S, xyz(o), xyz(d), tnear, FLT_MAX, h, st, cnt, rays);
             vis[__float_as_uint(d.w)] = hit ? 0 : 2;
+        } else if (kKind == 4) {
+            const float4 a = sq.o[e], dd = sq.d[e], b = sq.b[e];
+            float tnear = a.w;
+            ++nShadow;
+            const bool hit = sceneIntersect(S, xyz(a), xyz(dd), tnear, dd.w, h, st, cnt, rays);
+            splatAdd(splat, !hit, __float_as_uint(b.w), xyz(b));
         } else {
             const float4 o = q.o[e], d = q.d[e];
             float tnear = o.w;
@@ -228,7 +238,7 @@ template <int kKind>
 __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_trace_pt(OmvSceneView S, WfQueue q, WfShadowQ sq, const uint32_t* __restrict__ count,
                                                              uint32_t* __restrict__ fetch, int keep, WfHits H, uint8_t* __restrict__ vis,
                                                              uint32_t nslots, int lc, uint32_t* __restrict__ gstack,
-                                                             unsigned long long* __restrict__ counters) {
+                                                             unsigned long long* __restrict__ counters, float* __restrict__ splat) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -257,7 +267,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_trace_pt(OmvSceneView S, WfQ
                 if (k < total) {
                     e = k;
                     float tn;
-                    if (kKind == 2) {
+                    if (kKind == 2 || kKind == 4) {
                         const float4 a = sq.o[e], dd = sq.d[e], b = sq.b[e];
                         org = xyz(a); dir = xyz(dd); B = xyz(b); tn = a.w; t1 = dd.w;
                         ++nShadow;
@@ -315,6 +325,8 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_trace_pt(OmvSceneView S, WfQ
                         vis[(uint64_t)l * nslots + slot] = visible ? 2 : 0;
                     } else if (kKind == 3) {
                         vis[__float_as_uint(q.d[e].w)] = hit ? 0 : 2;
+                    } else if (kKind == 4) {
+                        splatAdd(splat, !hit, __float_as_uint(sq.b[e].w), B);
                     } else {
                         float4 r; r.x = T.h.t; r.y = T.h.u; r.z = T.h.v; r.w = T.tnear;
                         int2 id; id.x = hit ? T.h.inst : -1; id.y = T.h.prim | (T.h.tie ? (int)0x80000000 : 0);
@@ -331,10 +343,13 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_trace_pt(OmvSceneView S, WfQ
 }
 
 // ---------------------------------------------------------- shade light
-// Light-subpath vertex i (computePath with inv = false, BDPTRenderer.cpp:149-188).
-__global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_light(OmvSceneView S, RP P, uint64_t sample0, int i, WfState W, WfQueue in,
-                                                                const uint32_t* __restrict__ inCount, WfHits H, WfQueue out,
-                                                                uint32_t* __restrict__ outCount) {
+// Light-subpath vertex i (computePath with inv = false, BDPTRenderer.cpp:149-188)
+// and, with OMV_EXT_LIGHT_SPLAT (kSplat), its pinhole connection staged for
+// the visibility trace.
+template <bool kSplat>
+__global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_light(OmvSceneView S, omv_camera_desc cam, RP P, uint64_t sample0, int i, WfState W,
+                                                                WfQueue in, const uint32_t* __restrict__ inCount, WfHits H, WfQueue out,
+                                                                uint32_t* __restrict__ outCount, WfShadowQ sq, uint32_t* __restrict__ sqCount) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t total = *inCount;
@@ -343,9 +358,10 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_light(OmvSceneView S, 
         const float4 dd = in.d[e];
         const uint32_t slot = __float_as_uint(dd.w);
         const int2 id = H.id[e];
-        bool emit = false;
-        f3 nPos = mk3(0.0f, 0.0f, 0.0f), nDir = nPos;
-        float nTnear = 0.0f;
+        bool emit = false, spl = false;
+        f3 nPos = mk3(0.0f, 0.0f, 0.0f), nDir = nPos, sPos = nPos, sDir = nPos, ctb = nPos;
+        float nTnear = 0.0f, sTn = 0.0f, sD = 0.0f;
+        uint32_t pix = 0;
         if (id.x >= 0) {
             const float4 hr = H.h[e];
             Hit h; h.t = hr.x; h.u = hr.y; h.v = hr.z; h.inst = id.x; h.prim = id.y & 0x7FFFFFFF; h.tie = id.y < 0;
@@ -366,6 +382,13 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_light(OmvSceneView S, 
             float ct = fabsf(dot3(pDir, sp.n));
             of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
             if (i == 1) rD = attenuation(P, length3(sub3(sp.pos, lightPos)), Lt.range);
+            if (kSplat) {   // every hit of the walk, before the hard cut
+                f3 beta = i == 1 ? splatBeta1(P, ld3(Lt.intensity), rD, length3(sub3(sp.pos, lightPos))) : xyz(W.lbeta[slot]);
+                spl = splatPrepare(S, P.x, cam, sp, pDir, beta, ctb, pix, sDir, sD, sTn);
+                sPos = sp.pos;
+                const float co = fabsf(dot3(outRay, sp.n));
+                W.lbeta[slot] = f4of(mk3(beta.x * brdf.x * co, beta.y * brdf.y * co, beta.z * brdf.z * co), 0.0f);
+            }
             f3 outRad = mk3(of.x * lRad.x * rD, of.y * lRad.y * rD, of.z * lRad.z * rD);
             if (!(intensity(outRad) < P.cut)) {
                 W.lv[((uint64_t)i * 6 + 0) * N + slot] = sp.pos.x;
@@ -390,6 +413,14 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_light(OmvSceneView S, 
         if (emit) {
             out.o[k] = f4of(nPos, nTnear);
             out.d[k] = f4of(nDir, __uint_as_float(slot));
+        }
+        if (kSplat) {
+            const uint32_t ks = waveAppend(sqCount, spl);
+            if (spl) {
+                sq.o[ks] = f4of(sPos, sTn);
+                sq.d[ks] = f4of(sDir, sD);
+                sq.b[ks] = f4of(ctb, __uint_as_float(pix));
+            }
         }
     }
 }
@@ -620,7 +651,7 @@ constexpr size_t kAlign = 256;
 size_t up(size_t b) { return (b + kAlign - 1) & ~(kAlign - 1); }
 }  // namespace
 
-size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee) {
+size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat) {
     const size_t N = nslots;
     size_t b = 6 * up(N * 16)                        // three ray queues
                + up(N * 16) + up(N * 8)              // hits
@@ -629,16 +660,17 @@ size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool
         b += up(N * 16) + 3 * up(N * 4) + up(N * (size_t)lvCap * 24) + up(N * (size_t)lc * 12) + up(N * (size_t)lc) +
              3 * up(N * (size_t)lc * 16);
     if (nee) b += up(N * 16) + up(N) + 2 * up(N * 16);   // neeC, nvis, environment-ray queue
+    if (lights && splat) b += up(N * 16);                 // lbeta
     return b;
 }
 
-int wavefrontCounterCount(int B, int lvCap) { return 2 * lvCap + 6 * B + 16; }
+int wavefrontCounterCount(int B, int lvCap) { return 4 * lvCap + 6 * B + 16; }
 
 void wavefrontBlocksPerCU(int out[4]) {
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
     out[0] = std::min(occ(omv_wf_trace<0>, lds), occ(omv_wf_trace_pt<2>, lds));
     out[1] = occ(omv_wf_shade_cam<true, true>, 0);
-    out[2] = occ(omv_wf_shade_light, 0);
+    out[2] = occ(omv_wf_shade_light<true>, 0);
     out[3] = occ(omv_wf_gen<true>, 0);
 }
 
@@ -649,6 +681,8 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
     const bool lights = S.num_lights > 0;
     if (!S.env_tex_pdf) P.x.nee = false;   // no HDR environment: nothing to sample
     const bool nee = P.x.nee;
+    if (!lights || !L.splat) P.x.splat = false;
+    const bool splat = P.x.splat;
     const uint32_t n = L.nslots;
     const size_t N = n;
     uint8_t* cur = static_cast<uint8_t*>(L.arena);
@@ -681,6 +715,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
         sq.d = reinterpret_cast<float4*>(take(N * (size_t)L.lc * 16));
         sq.b = reinterpret_cast<float4*>(take(N * (size_t)L.lc * 16));
     }
+    if (splat) W.lbeta = reinterpret_cast<float4*>(take(N * 16));
     WfQueue eq;
     std::memset(&eq, 0, sizeof(eq));
     if (nee) {
@@ -691,14 +726,16 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
     // queue counters: light round i reads lightCnt[i] (i = 1 .. lvCap-1) and
     // appends to lightCnt[i+1]; camera round i reads camCnt[i] (i = 1 .. B),
     // appends to camCnt[i+1], stages connection rays in shCnt[i] and
-    // environment rays in envCnt[i]
+    // environment rays in envCnt[i]; light round i stages pinhole rays in
+    // splCnt[i] (the connection-ray queue is free during the light rounds)
     const int B = L.p.camera_bounces;
     const int ncnt = wavefrontCounterCount(B, L.lvCap);
     uint32_t* lightCnt = L.qcount;
     uint32_t* camCnt = lightCnt + (L.lvCap + 2);
     uint32_t* shCnt = camCnt + (B + 2);
     uint32_t* envCnt = shCnt + (B + 2);
-    uint32_t* fetchCnt = envCnt + (B + 2);           // one per persistent traversal launch
+    uint32_t* splCnt = envCnt + (B + 2);
+    uint32_t* fetchCnt = splCnt + (L.lvCap + 2);     // one per persistent traversal launch
     int nf = 0;
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
     const dim3 blk(OMV_BLOCK);
@@ -721,13 +758,26 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
             mark();
             if (L.traceKeep > 0)
                 hipLaunchKernelGGL(omv_wf_trace_pt<1>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, lightCnt + i, fetchCnt + (nf++),
-                                   L.traceKeep, H, W.vis, n, L.lc, L.gstack, L.counters);
+                                   L.traceKeep, H, W.vis, n, L.lc, L.gstack, L.counters, L.splat);
             else
                 hipLaunchKernelGGL(omv_wf_trace<1>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, lightCnt + i, H, W.vis, n, L.lc, L.gstack,
-                                   L.counters);
+                                   L.counters, L.splat);
             mark();
-            hipLaunchKernelGGL(omv_wf_shade_light, dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, lightCnt + i, H, out,
-                               lightCnt + i + 1);
+            if (splat) {
+                hipLaunchKernelGGL(omv_wf_shade_light<true>, dim3(L.blocksShade), blk, 0, stream, S, L.cam, P, L.sample0, i, W, in, lightCnt + i,
+                                   H, out, lightCnt + i + 1, sq, splCnt + i);
+                mark();
+                if (L.traceKeep > 0)
+                    hipLaunchKernelGGL(omv_wf_trace_pt<4>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, splCnt + i, fetchCnt + (nf++),
+                                       L.traceKeep, H, W.vis, n, L.lc, L.gstack, L.counters, L.splat);
+                else
+                    hipLaunchKernelGGL(omv_wf_trace<4>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, splCnt + i, H, W.vis, n, L.lc, L.gstack,
+                                       L.counters, L.splat);
+                mark();
+            } else {
+                hipLaunchKernelGGL(omv_wf_shade_light<false>, dim3(L.blocksShade), blk, 0, stream, S, L.cam, P, L.sample0, i, W, in, lightCnt + i,
+                                   H, out, lightCnt + i + 1, sq, splCnt + i);
+            }
             std::swap(in, out);
         }
     }
@@ -736,10 +786,10 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
         mark();
         if (L.traceKeep > 0)
             hipLaunchKernelGGL(omv_wf_trace_pt<0>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, camCnt + i, fetchCnt + (nf++), L.traceKeep,
-                               H, W.vis, n, L.lc, L.gstack, L.counters);
+                               H, W.vis, n, L.lc, L.gstack, L.counters, L.splat);
         else
             hipLaunchKernelGGL(omv_wf_trace<0>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, camCnt + i, H, W.vis, n, L.lc, L.gstack,
-                               L.counters);
+                               L.counters, L.splat);
         mark();
         if (lights) {
             if (nee)
@@ -751,10 +801,10 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
             mark();
             if (L.traceKeep > 0)
                 hipLaunchKernelGGL(omv_wf_trace_pt<2>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, shCnt + i, fetchCnt + (nf++),
-                                   L.traceKeep, H, W.vis, n, L.lc, L.gstack, L.counters);
+                                   L.traceKeep, H, W.vis, n, L.lc, L.gstack, L.counters, L.splat);
             else
                 hipLaunchKernelGGL(omv_wf_trace<2>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, shCnt + i, H, W.vis, n, L.lc, L.gstack,
-                                   L.counters);
+                                   L.counters, L.splat);
             mark();
         } else {
             if (nee)
@@ -768,10 +818,10 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
             mark();
             if (L.traceKeep > 0)
                 hipLaunchKernelGGL(omv_wf_trace_pt<3>, dim3(L.blocksTrace), blk, lds, stream, S, eq, sq, envCnt + i, fetchCnt + (nf++),
-                                   L.traceKeep, H, W.nvis, n, L.lc, L.gstack, L.counters);
+                                   L.traceKeep, H, W.nvis, n, L.lc, L.gstack, L.counters, L.splat);
             else
                 hipLaunchKernelGGL(omv_wf_trace<3>, dim3(L.blocksTrace), blk, lds, stream, S, eq, sq, envCnt + i, H, W.nvis, n, L.lc, L.gstack,
-                                   L.counters);
+                                   L.counters, L.splat);
             mark();
         }
         if (lights || nee)

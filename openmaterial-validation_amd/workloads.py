@@ -522,3 +522,63 @@ def mixed_test_scene(hdr=True, lights=True, tex_f32=False, masked=False, missing
         d.background = ("hdr", hdri(64, 32, seed=9, sun_factor=50.0), HDR_ROTATION, HDR_SCALE)
     view = View(64, 48, (0.3, 0.9, 4.5), (-0.05, -0.25, -1.0), yfov=55.0)
     return d, view
+
+
+# ------------------------------------------------- light-tracer splat checks
+def _placed(t, flip=False):
+    """Translation by t; flip = rotation by pi about x (a +y-facing plane() then faces -y)."""
+    M = np.eye(4, dtype=np.float32)
+    if flip:
+        M[1, 1] = -1.0
+        M[2, 2] = -1.0
+    M[:3, 3] = t
+    return M
+
+
+def light_splat_scenes(width=32, height=32):
+    """Configurations in which the light-tracer splat image (OMV_EXT_LIGHT_SPLAT)
+    has a known expectation, all with inverse-square attenuation and every
+    lit point inside the point light's emission octant (+x, +y, +z):
+
+    * "direct": a point light under a diffuse PBR ceiling, the camera looking
+      up at it.  Light vertex 1 splats estimate the reference estimator's own
+      image of the same scene with B = 1, q = 2 (its only non-zero terms are
+      the direct connections).
+    * "mirror": the light reaches a diffuse floor only through a near-perfect
+      OpenMATERIAL mirror (k = 1e4, F = 1 - O(1e-4)); light vertex 2 splats
+      estimate the direct image of the mirror-image light (the "mirror_ref"
+      scene: the floor and the light reflected in the mirror plane).
+
+    Returns {name: (desc, view, params)}; params without the extension bit."""
+    diffuse = ("pbr", dict(base_color=(0.8, 0.7, 0.6, 1.0), metallic=0.0, roughness=1.0))
+    light = ((0.0, 0.0, 0.0), (50.0, 50.0, 50.0), -1.0)
+    out = {}
+    # direct: ceiling y = 2 (facing -y) over x, z in [0.5, 4.5]
+    d = SceneDesc()
+    d.materials.append(diffuse)
+    pv, pf, puv = plane(4, 4, 2.0, 0.0)
+    d.meshes.append({"vertices": pv, "faces": pf, "mat_id": 0, "uvs": [puv]})
+    d.instances.append((_placed((2.5, 2.0, 2.5), flip=True), 0))
+    d.lights.append(light)
+    view = View(width, height, (2.5, 0.5, 2.5), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0), 90.0)
+    out["direct"] = (d, view, dict(camera_bounces=1, light_bounces=1, max_path_length=2, light_distance_attenuation=2))
+    # mirror: mirror y = 2 (facing -y) over [0, 6]^2, floor y = 1 (facing +y) over [2.5, 6]^2
+    floor_v, floor_f, floor_uv = plane(4, 4, 1.75, 0.0)
+    d = SceneDesc()
+    d.materials.append(diffuse)
+    d.materials.append(("om_nk", np.ones(3, np.float32), np.full(3, 1e4, np.float32)))
+    mv, mf, muv = plane(4, 4, 3.0, 0.0)
+    d.meshes.append({"vertices": floor_v, "faces": floor_f, "mat_id": 0, "uvs": [floor_uv]})
+    d.meshes.append({"vertices": mv, "faces": mf, "mat_id": 1, "uvs": [muv]})
+    d.instances.append((_placed((4.25, 1.0, 4.25)), 0))
+    d.instances.append((_placed((3.0, 2.0, 3.0), flip=True), 1))
+    d.lights.append(light)
+    view = View(width, height, (4.25, 1.8, 4.25), (0.0, -1.0, 0.0), (0.0, 0.0, 1.0), 90.0)
+    out["mirror"] = (d, view, dict(camera_bounces=1, light_bounces=2, max_path_length=3, light_distance_attenuation=2))
+    r = SceneDesc()
+    r.materials.append(diffuse)
+    r.meshes.append({"vertices": floor_v, "faces": floor_f, "mat_id": 0, "uvs": [floor_uv]})
+    r.instances.append((_placed((4.25, 1.0, 4.25)), 0))
+    r.lights.append(((0.0, 4.0, 0.0), (50.0, 50.0, 50.0), -1.0))
+    out["mirror_ref"] = (r, view, dict(camera_bounces=1, light_bounces=1, max_path_length=2, light_distance_attenuation=2))
+    return out

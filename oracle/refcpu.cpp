@@ -1072,6 +1072,7 @@ static R dlnSmithG1(R c, R alpha) {
 /* Planted mistakes for the tests that show the adjoint checks have teeth:
  * bit 0 drops d ln D / d alpha, bit 1 drops the base-colour term of F. */
 static int gAdjPlant = 0;
+static int gSplatPlant = 0;   /* test aid: 1 drops a cos(theta) of the pinhole Jacobian, 2 the 1/p of the emission octant */
 
 /* d ln f_c / d(color_c, metal, alpha) for the three lobes; 0 where f_c == 0 */
 template <class R>
@@ -1272,10 +1273,12 @@ struct Ext {
     bool omRough = false;   // OMV_EXT_OM_ROUGH
     float omAlpha = 0.09f;  // GGX alpha = roughness^2
     bool nee = false;       // OMV_EXT_HDRI_NEE
+    bool splat = false;     // OMV_EXT_LIGHT_SPLAT
     explicit Ext(const omv_renderer_params& p) {
         omRough = (p.extensions & OMV_EXT_OM_ROUGH) != 0;
         omAlpha = p.om_roughness * p.om_roughness;
         nee = (p.extensions & OMV_EXT_HDRI_NEE) != 0;
+        splat = (p.extensions & OMV_EXT_LIGHT_SPLAT) != 0;
     }
 };
 
@@ -1429,6 +1432,7 @@ static void matGetRayAndBrdf(const Scene& S, const Ext& X, const SurfacePoint<R>
 template <class R>
 struct Vtx {
     V3<R> pos, outRay, outRad, outFactor;
+    V3<R> w;   /* sampled weight of the bounce (getRayAndBrdf's brdf) */
     SurfacePoint<R> sp;
 };
 
@@ -1584,6 +1588,65 @@ struct Tracer {
         return c;
     }
 
+    /* ---- OMV_EXT_LIGHT_SPLAT ---------------------------------------- */
+    const omv_camera_desc* cam = nullptr;   /* the pinhole the splats go to */
+    double* splat = nullptr;                /* W*H*3 splat sums (unnormalised) */
+
+    const Light* splatLight = nullptr;      /* the light of the subpath being walked */
+    V3<R> splatBeta;                        /* particle weight at the current light vertex */
+
+    /* Light-tracer (t = 1) splat of light-subpath vertex i (every hit of the
+     * walk, before the hard cut decides whether the walk goes on).  Particle
+     * weight: beta_1 = I * (lightScale * pi/2 * att(r1) * r1^2) at the first
+     * hit, pi/2 being 1 / the density of uniformSphere's + octant (so att =
+     * 1/d^2 gives the physical particle weight and any attenuation mode
+     * reproduces the reference's direct connection in expectation);
+     * beta_{i+1} = beta_i * w_i * |w_out . n_i| (the camera subpath's
+     * throughput update).  Vertex i projects through the pinhole to pixel p;
+     * its contribution
+     *   beta_i * getBrdf(w_in, w_c) * |w_c . n_i| * f^2 / (d^2 cos^3(theta))
+     * (f: focus in pixels, theta: angle to the optical axis, d: distance to
+     * the pinhole) is added to p if the segment to the pinhole is unoccluded;
+     * the film adds sum / (W*H*spp). */
+    void splatVertex(const Vtx<R>* Pv, int i) {
+        const omv_camera_desc& c = *cam;
+        if (i == 1) {
+            const R r1 = distance3(Pv[1].pos, Pv[0].pos);
+            const R k1 = R(P.light_scale_value) * ((gSplatPlant & 2) ? R(1) : kPi<R>() * R(0.5f)) * attenuation(r1, R(splatLight->range)) * r1 * r1;
+            splatBeta = mk(R(splatLight->intensity[0]) * k1, R(splatLight->intensity[1]) * k1, R(splatLight->intensity[2]) * k1);
+        } else {
+            const R ct = std::abs(dot3(Pv[i - 1].outRay, Pv[i - 1].sp.normal));
+            const V3<R>& w = Pv[i - 1].w;
+            splatBeta = mk(splatBeta.x * w.x * ct, splatBeta.y * w.y * ct, splatBeta.z * w.z * ct);
+        }
+        const V3<R>& beta = splatBeta;
+        const V3<R> C = ld3<R>(c.position);
+        const V3<R>& y = Pv[i].pos;
+        const V3<R> v = sub(C, y);
+        const R d = length3(v);
+        if (!(d > R(0))) return;
+        const V3<R> wc = mk(v.x / d, v.y / d, v.z / d);
+        const V3<R> l = mulT33(sub(y, C), c.rotation);
+        if (!(l.z > R(0))) return;
+        const R px = R(c.focus) * l.x / l.z + R(c.half_w), py = R(c.focus) * l.y / l.z + R(c.half_h);
+        if (!(px >= R(0) && px < R(c.width) && py >= R(0) && py < R(c.height))) return;
+        const int ix = std::min((int)px, c.width - 1), iy = std::min((int)py, c.height - 1);
+        const R cosT = l.z / length3(l);
+        const V3<R> f = matGetBrdf(S, X, Pv[i].sp, Pv[i - 1].outRay, wc);
+        const R k = R(c.focus) * R(c.focus) * std::abs(dot3(wc, Pv[i].sp.normal)) / (d * d * cosT * cosT * ((gSplatPlant & 1) ? R(1) : cosT));
+        const V3<R> ctb = mk(beta.x * f.x * k, beta.y * f.y * k, beta.z * f.z * k);
+        if (ctb.x == R(0) && ctb.y == R(0) && ctb.z == R(0)) return;
+        if (!(std::isfinite((double)ctb.x) && std::isfinite((double)ctb.y) && std::isfinite((double)ctb.z))) return;
+        /* visibility of the pinhole: ray from the vertex, tnear = the
+         * isConnected offset, tfar = d; visible iff nothing is hit */
+        R tn = R(2) * R(3) * feps<R>() * maxAbs(y);
+        SurfacePoint<R> hs;
+        ++shadowRays;
+        if (sceneIntersect(y, wc, tn, d, hs)) return;
+        double* o = splat + ((size_t)iy * c.width + ix) * 3;
+        o[0] += (double)ctb.x; o[1] += (double)ctb.y; o[2] += (double)ctb.z;
+    }
+
     /* computePath (BDPTRenderer.cpp:128-192).  maxN vertices incl. the start. */
     uint64_t neeSample = 0;   /* sample index of the camera path being walked */
     bool computePath(Vtx<R>* Pv, int maxN, const V3<R>& O, const V3<R>& Rd, V3<R>& Rad, int& n, bool inv, R attD,
@@ -1607,6 +1670,7 @@ struct Tracer {
             V3<R> brdf, erad;
             BounceInfo<R> bi;
             matGetRayAndBrdf(S, X, Pv[i].sp, d0, Pv[i].outRay, brdf, erad, inv, U, E, &bi);
+            Pv[i].w = brdf;
             R ct = std::abs(inv ? dot3(Pv[i].outRay, Pv[i].sp.normal) : dot3(d0, Pv[i].sp.normal));
             const V3<R>& of = Pv[i - 1].outFactor;
             Pv[i].outFactor = mk(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
@@ -1619,6 +1683,7 @@ struct Tracer {
                 }
             } else {
                 if (i == 1) rDterm = attenuation(distance3(Pv[1].pos, Pv[0].pos), attD);
+                if (X.splat && splat && cam) splatVertex(Pv, i);   /* OMV_EXT_LIGHT_SPLAT, before the hard cut */
                 Pv[i].outRad = mk(Pv[i].outFactor.x * Rad.x * rDterm, Pv[i].outFactor.y * Rad.y * rDterm, Pv[i].outFactor.z * Rad.z * rDterm);
                 if (intensity(Pv[i].outRad) < R(P.ray_cut_pix_value)) break;
             }
@@ -1701,6 +1766,7 @@ struct Tracer {
              * never connected (i + j <= q with j >= 1) */
             int maxN = std::min(P.light_bounces + 1, std::max(q, 1));
             V3<R> rad = lightRad;
+            splatLight = &L;
             computePath(lp, maxN, ld3<R>(L.pos), dir, rad, lpLen, false, R(L.range), U);
         }
         /* the camera sub-stream continues after the two pixel-jitter draws */
@@ -1929,9 +1995,15 @@ static void renderRows(const Scene& S, const omv_renderer_params& P, const omv_c
     const int rEnd = std::min(H, row0 + rows);
     std::atomic<int> next(row0);
     std::atomic<uint64_t> rays(0), cr(0), lr(0), sr(0), nodes(0), tris(0);
+    /* OMV_EXT_LIGHT_SPLAT: per-thread splat sums, added to the film below */
+    const bool splat = (P.extensions & OMV_EXT_LIGHT_SPLAT) && !S.lights.empty() && (film || film64);
+    std::mutex splatMu;
+    std::vector<double> splatSum(splat ? (size_t)W * H * 3 : 0, 0.0);
     auto worker = [&]() {
         Tracer<R> T(S, P);
         T.E = E;
+        std::vector<double> mySplat(splat ? (size_t)W * H * 3 : 0, 0.0);
+        if (splat) { T.cam = &cam; T.splat = mySplat.data(); }
         std::vector<Vtx<R>> cp(P.camera_bounces + 2), lp(P.light_bounces + 2);
         int row;
         while ((row = next.fetch_add(1)) < rEnd) {
@@ -1963,12 +2035,25 @@ static void renderRows(const Scene& S, const omv_renderer_params& P, const omv_c
             }
         }
         rays += T.rays; cr += T.camRays; lr += T.lightRays; sr += T.shadowRays; nodes += T.cnt.nodes; tris += T.cnt.tris;
+        if (splat) {
+            std::lock_guard<std::mutex> g(splatMu);
+            for (size_t k = 0; k < mySplat.size(); ++k) splatSum[k] += mySplat[k];
+        }
     };
     int nt = threads > 0 ? threads : (P.number_of_cores > 0 ? P.number_of_cores : 1);
     nt = std::max(1, std::min(nt, std::max(1, rEnd - row0)));
     std::vector<std::thread> th;
     for (int i = 0; i < nt; ++i) th.emplace_back(worker);
     for (auto& t : th) t.join();
+    if (splat) {
+        /* film = camera estimate + splat sums over all W*H*spp light paths */
+        const double inv = 1.0 / ((double)W * H * std::max(spp, 1));
+        for (size_t p = 0; p < (size_t)W * H; ++p)
+            for (int c = 0; c < 3; ++c) {
+                if (film) film[p * 4 + c] = film[p * 4 + c] + (float)(splatSum[p * 3 + c] * inv);
+                if (film64) film64[p * 4 + c] = film64[p * 4 + c] + splatSum[p * 3 + c] * inv;
+            }
+    }
     if (st) {
         std::memset(st, 0, sizeof(*st));
         st->samples = (uint64_t)(rEnd - row0) * W * spp;
@@ -2381,6 +2466,7 @@ int ref_adjoint(ref_scene* r, const omv_renderer_params* p, const omv_camera_des
 }
 
 void ref_adjoint_plant(int mask) { gAdjPlant = mask; }
+void ref_splat_plant(int mask) { gSplatPlant = mask; }
 
 /* OMV_EXT_HDRI_NEE environment tables of a committed scene (H row entries,
  * W*H column entries and texel probabilities); returns 0 without an HDR */

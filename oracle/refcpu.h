@@ -75,6 +75,7 @@ int  ref_env_sample(ref_scene* s, const float u[6], double dir[3], double* pdf, 
 double ref_env_pdf(ref_scene* s, const double dir[3]);
 void ref_to_u8(const float* img, size_t n, uint8_t* out);
 void ref_adjoint_plant(int mask);   /* test aid: plants a known mistake in the adjoint */
+void ref_splat_plant(int mask);     /* test aid: plants a known mistake in the light-tracer splats */
 int  ref_render_detached(ref_scene* sample, ref_scene* eval, const omv_renderer_params* p, const omv_camera_desc* cam,
                          int fp64, int threads, double* film);
 int  ref_adjoint(ref_scene* s, const omv_renderer_params* p, const omv_camera_desc* cam, int fp64, int threads,

@@ -94,6 +94,7 @@ def lib():
         "ref_fresnel64": ([C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)], None),
         "ref_fresnel_grad": ([C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)], None),
         "ref_adjoint_plant": ([C.c_int], None),
+        "ref_splat_plant": ([C.c_int], None),
         "ref_median3": ([FP, C.c_int, C.c_int], None),
         "ref_env_tables": ([vp, FP, C.POINTER(C.c_int), FP, C.POINTER(C.c_int), FP], C.c_int),
         "ref_env_sample": ([vp, FP, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
