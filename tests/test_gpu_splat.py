@@ -80,8 +80,15 @@ def test_splat_expectation_high_spp(pkg, wl, name, tot, blk):
 PARAMS = dict(camera_bounces=4, light_bounces=4, max_path_length=5)
 
 
-def test_splat_schedules_and_chunks_agree(pkg, wl):
-    desc, view = wl.mixed_test_scene(hdr=True, lights=True, masked=True)
+@pytest.mark.parametrize("name", ["mirror", "mixed"])
+def test_splat_schedules_and_chunks_agree(pkg, wl, name):
+    # the mirror scene: deep light walks (mirror <-> floor) that splat at
+    # every vertex; the mixed scene: every material, masked re-traces
+    if name == "mirror":
+        desc, view, _ = wl.light_splat_scenes()["mirror"]
+    else:
+        desc, view = wl.mixed_test_scene(hdr=True, lights=True, masked=True)
+        desc.lights.append(((-2.5, -0.9, -2.0), (30.0, 30.0, 30.0), -1.0))   # its + octant faces the objects
     a, sa = _gpu(pkg, desc, view, PARAMS, 4, 8, schedule=pkg.SCHEDULE_WAVEFRONT)
     b, sb = _gpu(pkg, desc, view, PARAMS, 4, 8, schedule=pkg.SCHEDULE_PER_SAMPLE)
     c, sc_ = _gpu(pkg, desc, view, PARAMS, 4, 8, tuning=dict(chunk_samples=5001))
