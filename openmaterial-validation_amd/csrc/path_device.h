@@ -878,21 +878,34 @@ DEV void loadNode(const OmvNode4* nodes, uint32_t idx, OmvNode4& nd) {
     for (int i = 0; i < 8; ++i) dst[i] = src[i];
 }
 
-// push the hit children nearest-last so the nearest is popped first
-DEV void pushSorted(Stack& st, Counters& c, uint32_t e[4], float t[4], bool h[4]) {
+// push the hit children farthest first; the nearest is not pushed but
+// returned in nxt (hasNext), so the next step needs no stack round trip
+DEV void pushSorted(Stack& st, Counters& c, uint32_t e[4], float t[4], bool h[4], uint32_t& nxt, bool& hasNext) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) if (!h[i]) t[i] = FLT_MAX;
 #define OMV_CSWAP(a, b) if (t[b] > t[a]) { float tt = t[a]; t[a] = t[b]; t[b] = tt; uint32_t ee = e[a]; e[a] = e[b]; e[b] = ee; bool hh = h[a]; h[a] = h[b]; h[b] = hh; }
     // descending by t (farthest first)
     OMV_CSWAP(0, 1) OMV_CSWAP(2, 3) OMV_CSWAP(0, 2) OMV_CSWAP(1, 3) OMV_CSWAP(1, 2)
 #undef OMV_CSWAP
+    bool have = false;
+    uint32_t prev = 0u;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) if (h[i]) st.push(e[i], c);
+    for (int i = 0; i < 4; ++i) {
+        if (h[i]) {
+            if (have) st.push(prev, c);
+            prev = e[i];
+            have = true;
+        }
+    }
+    nxt = prev;
+    hasNext = have;
 }
 
 // Closest hit of a world-space ray over the two-level BVH, as a resumable
-// traversal: start() pushes the TLAS root, every step() consumes one stack
-// entry (TLAS/BLAS node, leaf, instance entry or instance end).  The result
+// traversal: start() sets the TLAS root as the next entry, every step()
+// consumes one entry (TLAS/BLAS node, leaf, instance entry or instance end):
+// the nearest hit child of the last node when there is one (kept in a
+// register), else the top of the stack; more() says whether entries remain.  The result
 // does not depend on the visiting order: boxes are tested conservatively
 // against the current closest t, triangles accept t in (tnear, t_closest],
 // and offer() keeps the (t, instance, primitive) minimum.
@@ -903,8 +916,12 @@ struct Trav {
     float tnear, tfar;
     int curInst;
     bool found;
+    uint32_t nxt;    // next entry when hasNext
+    bool hasNext;
+    DEV bool more(const Stack& st) const { return hasNext || st.sp > 0; }
     DEV bool start(const OmvSceneView& S, const f3& o, const f3& d, float tn, float tf, Stack& st, Counters& c) {
         found = false;
+        hasNext = false;
         h.tie = false; h.t = tf; h.u = 0.0f; h.v = 0.0f; h.inst = -1; h.prim = 0;
         tnear = tn; tfar = tf;
         st.sp = 0;
@@ -913,11 +930,14 @@ struct Trav {
         wo = o; wd = d;
         widir = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
         curInst = -1;
-        st.push((E_TLAS_NODE << E_KIND_SHIFT) | (uint32_t)S.tlas_root, c);
+        nxt = (E_TLAS_NODE << E_KIND_SHIFT) | (uint32_t)S.tlas_root;
+        hasNext = true;
         return true;
     }
     DEV void step(const OmvSceneView& S, Stack& st, Counters& c) {
-        uint32_t e = st.pop();
+        uint32_t e;
+        if (hasNext) { e = nxt; hasNext = false; }
+        else e = st.pop();
         if (e == E_END) { curInst = -1; return; }
         const uint32_t kind = e >> E_KIND_SHIFT;
         const uint32_t idx = e & 0x3FFFFFFFu;
@@ -944,7 +964,8 @@ struct Trav {
             if (!rayPrep(oo, od, op)) return;
             curInst = inst;
             st.push(E_END, c);
-            st.push((E_BLAS_NODE << E_KIND_SHIFT) | (uint32_t)S.meshes[in.mesh_id].blas_root, c);
+            nxt = (E_BLAS_NODE << E_KIND_SHIFT) | (uint32_t)S.meshes[in.mesh_id].blas_root;
+            hasNext = true;
             return;
         }
         // internal node (TLAS uses the world ray, BLAS the object ray)
@@ -962,7 +983,7 @@ struct Trav {
             else if (tl) ent[k] = (E_INST << E_KIND_SHIFT) | (uint32_t)(-(ch + 1));
             else ent[k] = (E_BLAS_LEAF << E_KIND_SHIFT) | ((uint32_t)(-(ch + 1)) << 3) | ((nd.count[k] - 1u) & 7u);
         }
-        pushSorted(st, c, ent, tent, hit);
+        pushSorted(st, c, ent, tent, hit, nxt, hasNext);
     }
 };
 
@@ -970,7 +991,7 @@ DEV bool traceClosest(const OmvSceneView& S, const f3& o, const f3& d, float tne
     Trav T;
     const bool ok = T.start(S, o, d, tnear, tfar, st, c);
     if (ok)
-        while (st.sp > 0) T.step(S, st, c);
+        while (T.more(st)) T.step(S, st, c);
     h = T.h;
     return ok && T.found;
 }

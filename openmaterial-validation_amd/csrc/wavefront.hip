@@ -288,8 +288,8 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_trace_pt(OmvSceneView S, WfQ
         if (__ballot(busy) == 0ull) break;
         // traverse until too few lanes of the wave are busy
         for (;;) {
-            if (busy && st.sp > 0) T.step(S, st, cnt);
-            if (busy && st.sp == 0) {
+            if (busy && T.more(st)) T.step(S, st, cnt);
+            if (busy && !T.more(st)) {
                 // one traceClosest call finished: sceneIntersect's masked re-trace loop
                 bool done = true, hit = T.found;
                 if (hit) {
