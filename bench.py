@@ -289,9 +289,16 @@ def bench_material_fit(pkg, wl, args):
 
 
 def fd_check(pkg, wl, count, dt, di, view, params, ids, T):
-    """GPU adjoint vs central differences (h = 1e-3) of the FP64 oracle's
-    detached functional, on `count` texels drawn with seed 5002 among those the
-    loss sees, and on the OM (n, k)."""
+    """GPU adjoint vs central differences (h = 1e-3) of the oracle's detached
+    functional, on `count` texels drawn with seed 5002 among those the loss
+    sees, and on the OM (n, k).  Two comparisons:
+      * "float": d loss / d film from the float oracle's image, differences of
+        the float oracle's functional (the device's arithmetic), so the only
+        differences left are the rare samples whose paths diverge between
+        device and host transcendentals;
+      * "fp64": d loss / d film from the GPU image, differences of the FP64
+        oracle's functional (float and double paths also diverge where a
+        direction's rounding flips a branch)."""
     import copy
     import torch
     ref = ge.load_oracle()
@@ -302,61 +309,75 @@ def fd_check(pkg, wl, count, dt, di, view, params, ids, T):
     r = pkg.Renderer(pkg.default_params(**params))
     r.set_scene(s)
     img = r.render(cam)
-    dfilm = torch.from_numpy(np.ascontiguousarray(2.0 * (img - T) * np.array([1, 1, 1, 0], np.float32))).to(dev)
-    tex = di.textures[0][1].shape[0]
-    gb = torch.zeros((tex, tex, 4), device=dev)
-    gmr = torch.zeros((tex, tex, 4), device=dev)
-    gom = torch.zeros(6, device=dev)
-    r.backward(cam, dfilm.data_ptr(), textures=[(0, gb.data_ptr()), (1, gmr.data_ptr())], om_materials=[1], om_grads_ptr=gom.data_ptr())
-    g = {0: gb.cpu().numpy(), 1: gmr.cpu().numpy()}
-    gom = gom.cpu().numpy()
     rs = ref.Scene()
     di.apply(rs, ref.pbr_desc)
     p = ref.default_params(**params)
     cd = cam.desc()
     threads = min(16, os.cpu_count() or 1)
+    img32, _ = rs.render(p, cd, threads=threads)
+    tex = di.textures[0][1].shape[0]
+    mask = np.array([1, 1, 1, 0], np.float32)
 
-    def loss_of(d):
+    def adjoint(base):
+        dfilm = torch.from_numpy(np.ascontiguousarray(2.0 * (base - T) * mask)).to(dev)
+        gb = torch.zeros((tex, tex, 4), device=dev)
+        gmr = torch.zeros((tex, tex, 4), device=dev)
+        gom = torch.zeros(6, device=dev)
+        r.backward(cam, dfilm.data_ptr(), textures=[(0, gb.data_ptr()), (1, gmr.data_ptr())], om_materials=[1],
+                   om_grads_ptr=gom.data_ptr())
+        return {0: gb.cpu().numpy(), 1: gmr.cpu().numpy()}, gom.cpu().numpy()
+
+    grads = {"float": adjoint(img32), "fp64": adjoint(img)}
+
+    def loss_of(d, fp64):
         e = ref.Scene()
         d.apply(e, ref.pbr_desc)
-        out = rs.render_detached(e, p, cd, fp64=True, threads=threads)
-        return float(((out[..., :3] - T[..., :3]) ** 2).sum())
+        out = rs.render_detached(e, p, cd, fp64=fp64, threads=threads)
+        return float(((out[..., :3].astype(np.float64) - T[..., :3]) ** 2).sum())
 
     rng = np.random.default_rng(5002)
+    g0 = grads["float"][0]
     cands = [(t, c) for t, cs in ((0, (0, 1, 2)), (1, (1, 2))) for c in cs]
     picks = []
     while len(picks) < count:
         t, c = cands[int(rng.integers(len(cands)))]
-        nz = np.argwhere(np.abs(g[t][..., c]) > 1e-3 * np.abs(g[t][..., c]).max())
+        nz = np.argwhere(np.abs(g0[t][..., c]) > 1e-3 * np.abs(g0[t][..., c]).max())
         y, x = nz[int(rng.integers(len(nz)))]
         picks.append((t, int(y), int(x), c))
-    rel = []
-    for t, y, x, c in picks:
-        vals = []
-        for sgn in (1.0, -1.0):
-            d = copy.deepcopy(di)
-            a = d.textures[t][1]
-            a[y, x, c] = np.float32(a[y, x, c] + np.float32(sgn * 1e-3))
-            vals += [float(a[y, x, c]), loss_of(d)]
-        fd = (vals[1] - vals[3]) / (vals[0] - vals[2])
-        rel.append(abs(g[t][y, x, c] - fd) / max(abs(fd), 1e-12))
-        print(f"# fd texel {t},{y},{x},{c}: adjoint {g[t][y, x, c]:.6g} fd {fd:.6g}", file=sys.stderr, flush=True)
-    om_rel = []
-    for j in range(6):
-        vals = []
-        for sgn in (1.0, -1.0):
-            d = copy.deepcopy(di)
-            nn, kk = np.array(d.materials[1][1], np.float32), np.array(d.materials[1][2], np.float32)
-            arr = nn if j < 3 else kk
-            arr[j % 3] = np.float32(arr[j % 3] + np.float32(sgn * 1e-3))
-            d.materials[1] = ("om_nk", nn, kk)
-            vals += [float(arr[j % 3]), loss_of(d)]
-        fd = (vals[1] - vals[3]) / (vals[0] - vals[2])
-        om_rel.append(abs(gom[j] - fd) / max(abs(fd), 1e-12))
-        print(f"# fd om {j}: adjoint {gom[j]:.6g} fd {fd:.6g}", file=sys.stderr, flush=True)
-    rel = np.array(rel)
-    return {"texels": count, "seed": 5002, "h": 1e-3, "median_rel": float(np.median(rel)), "p90_rel": float(np.quantile(rel, 0.9)),
-            "max_rel": float(rel.max()), "within_1e-3": float(np.mean(rel < 1e-3)), "om_max_rel": float(max(om_rel))}
+
+    def mutators():
+        for t, y, x, c in picks:
+            def mut(d, dh, t=t, y=y, x=x, c=c):
+                a = d.textures[t][1]
+                a[y, x, c] = np.float32(a[y, x, c] + np.float32(dh))
+                return float(a[y, x, c])
+            yield ("tex", t, y, x, c), mut
+        for j in range(6):
+            def mut(d, dh, j=j):
+                nn, kk = np.array(d.materials[1][1], np.float32), np.array(d.materials[1][2], np.float32)
+                arr = nn if j < 3 else kk
+                arr[j % 3] = np.float32(arr[j % 3] + np.float32(dh))
+                d.materials[1] = ("om_nk", nn, kk)
+                return float(arr[j % 3])
+            yield ("om", j), mut
+
+    out = {"texels": count, "seed": 5002, "h": 1e-3}
+    for mode, fp64 in (("float", False), ("fp64", True)):
+        g, gom = grads[mode]
+        rel, om_rel = [], []
+        for key, mut in mutators():
+            vals = []
+            for sgn in (1.0, -1.0):
+                d = copy.deepcopy(di)
+                vals += [mut(d, sgn * 1e-3), loss_of(d, fp64)]
+            fd = (vals[1] - vals[3]) / (vals[0] - vals[2])
+            adj = g[key[1]][key[2], key[3], key[4]] if key[0] == "tex" else gom[key[1]]
+            (rel if key[0] == "tex" else om_rel).append(abs(adj - fd) / max(abs(fd), 1e-12))
+            print(f"# fd {mode} {key}: adjoint {adj:.6g} fd {fd:.6g}", file=sys.stderr, flush=True)
+        rel = np.array(rel)
+        out[mode] = {"median_rel": float(np.median(rel)), "p90_rel": float(np.quantile(rel, 0.9)), "max_rel": float(rel.max()),
+                     "within_1e-3": float(np.mean(rel < 1e-3)), "om_max_rel": float(max(om_rel))}
+    return out
 
 
 def main():

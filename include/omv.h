@@ -308,6 +308,10 @@ typedef struct omv_render_tuning {
                                  (0 = occupancy limit)                          */
     int chunk_samples;        /* samples per wavefront chunk, rounded down to
                                  whole pixels (0 = default: 2^25)               */
+    int traversal;            /* 0 = by scene (wave-cooperative up to 4
+                                 instances), 1 = entry by entry, 2 = wave-
+                                 cooperative: node entries until every lane
+                                 of the wave holds a leaf, then the leaves    */
 } omv_render_tuning;
 int omv_renderer_set_tuning(omv_renderer* r, const omv_render_tuning* t);
 int omv_renderer_get_tuning(const omv_renderer* r, omv_render_tuning* t);

@@ -55,7 +55,7 @@ struct WfLaunch {
 };
 size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat);
 int wavefrontCounterCount(int B, int lvCap);
-void wavefrontBlocksPerCU(int out[4]);   // trace, shade, shade_light, gen
+void wavefrontBlocksPerCU(int out[5]);   // trace, shade, shade_light, gen, persistent trace
 hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream);
 // Path-replay adjoint over samples [sample0, sample0 + nsamples) (adjoint.hip).
 int adjointBlocksPerCU();

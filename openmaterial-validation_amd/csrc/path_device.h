@@ -789,6 +789,10 @@ struct Stack {
         --sp;
         return sp < OMV_STACK_LDS ? lds[sp * ldsStride] : glb[(sp - OMV_STACK_LDS) * glbStride];
     }
+    DEV uint32_t top() const {
+        const int t = sp - 1;
+        return t < OMV_STACK_LDS ? lds[t * ldsStride] : glb[(t - OMV_STACK_LDS) * glbStride];
+    }
 };
 
 enum : uint32_t {
@@ -934,25 +938,34 @@ struct Trav {
         hasNext = true;
         return true;
     }
-    DEV void step(const OmvSceneView& S, Stack& st, Counters& c) {
-        uint32_t e;
-        if (hasNext) { e = nxt; hasNext = false; }
-        else e = st.pop();
+    DEV uint32_t take(Stack& st) {
+        if (hasNext) { hasNext = false; return nxt; }
+        return st.pop();
+    }
+    DEV static bool isLeaf(uint32_t e) { return e != E_END && (e >> E_KIND_SHIFT) == E_BLAS_LEAF; }
+    // kind of the next entry without consuming it (more() must hold)
+    DEV bool nextIsBlasNode(const Stack& st) const {
+        const uint32_t e = hasNext ? nxt : st.top();
+        return e != E_END && (e >> E_KIND_SHIFT) == E_BLAS_NODE;
+    }
+    // triangles of a BLAS leaf entry (object ray of the current instance)
+    DEV void leaf(const OmvSceneView& S, uint32_t e, Counters& c) {
+        const uint32_t idx = e & 0x3FFFFFFFu;
+        const uint32_t first = idx >> 3, cnt = (idx & 7u) + 1u;
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const float4* rec = reinterpret_cast<const float4*>(S.tris) + 3u * (first + k);
+            float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+            ++c.tris;
+            float t, u, v;
+            if (triIntersect(op, r0, r1, r2, tnear, found ? h.t : tfar, t, u, v)) offer(h, found, t, u, v, curInst, __float_as_int(r0.w));
+        }
+    }
+    // any other entry: instance end, instance entry, TLAS or BLAS node
+    DEV void visit(const OmvSceneView& S, uint32_t e, Stack& st, Counters& c) {
         if (e == E_END) { curInst = -1; return; }
         const uint32_t kind = e >> E_KIND_SHIFT;
         const uint32_t idx = e & 0x3FFFFFFFu;
         const float tb = found ? h.t : tfar;
-        if (kind == E_BLAS_LEAF) {
-            const uint32_t first = idx >> 3, cnt = (idx & 7u) + 1u;
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const float4* rec = reinterpret_cast<const float4*>(S.tris) + 3u * (first + k);
-                float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-                ++c.tris;
-                float t, u, v;
-                if (triIntersect(op, r0, r1, r2, tnear, found ? h.t : tfar, t, u, v)) offer(h, found, t, u, v, curInst, __float_as_int(r0.w));
-            }
-            return;
-        }
         if (kind == E_INST) {
             const int inst = (int)(idx & 0x1FFFFFFFu);
             const OmvInstance& in = S.insts[inst];
@@ -985,13 +998,45 @@ struct Trav {
         }
         pushSorted(st, c, ent, tent, hit, nxt, hasNext);
     }
+    // one entry, whatever its kind (the persistent traversal's unit of work)
+    DEV void step(const OmvSceneView& S, Stack& st, Counters& c) {
+        const uint32_t e = take(st);
+        if (isLeaf(e)) leaf(S, e, c);
+        else visit(S, e, st, c);
+    }
+    // The whole traversal, wave-cooperative (Aila & Laine 2009, speculative
+    // while-while): node entries are taken until every active lane of the
+    // wave holds a leaf or has nothing left -- a lane that already holds one
+    // goes on speculatively through BLAS nodes of the same instance -- and
+    // then the lanes test their leaves together, so node and triangle work do
+    // not alternate under divergence.  Postponing a leaf only loosens culling
+    // for a while; the closest hit does not depend on the order.
+    DEV void run(const OmvSceneView& S, Stack& st, Counters& c) {
+        bool hasPend = false;
+        uint32_t pend = 0u;
+        while (more(st) || hasPend) {
+            for (;;) {
+                bool go = !hasPend && more(st);
+                if (!go && hasPend && more(st)) go = nextIsBlasNode(st);
+                if (go) {
+                    const uint32_t e = take(st);
+                    if (isLeaf(e)) { pend = e; hasPend = true; }
+                    else visit(S, e, st, c);
+                }
+                if (__ballot(!hasPend && more(st)) == 0ull) break;
+            }
+            if (hasPend) { leaf(S, pend, c); hasPend = false; }
+        }
+    }
 };
 
 DEV bool traceClosest(const OmvSceneView& S, const f3& o, const f3& d, float tnear, float tfar, Hit& h, Stack& st, Counters& c) {
     Trav T;
     const bool ok = T.start(S, o, d, tnear, tfar, st, c);
-    if (ok)
-        while (T.more(st)) T.step(S, st, c);
+    if (ok) {
+        if (S.trav_wave) T.run(S, st, c);
+        else while (T.more(st)) T.step(S, st, c);
+    }
     h = T.h;
     return ok && T.found;
 }

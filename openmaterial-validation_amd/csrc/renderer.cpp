@@ -82,12 +82,18 @@ int BDPTRenderer::setSchedule(int s) {
 }
 
 int BDPTRenderer::setTuning(const omv_render_tuning& t) {
-    if (t.trace_keep < 0 || t.trace_keep > 64 || t.trace_blocks_per_cu < 0 || t.chunk_samples < 0) {
+    if (t.trace_keep < 0 || t.trace_keep > 64 || t.trace_blocks_per_cu < 0 || t.chunk_samples < 0 || t.traversal < 0 || t.traversal > 2) {
         setError("set_tuning: value out of range");
         return OMV_E_RANGE;
     }
     tuning_ = t;
     return OMV_OK;
+}
+
+OmvSceneView BDPTRenderer::viewForLaunch() const {
+    OmvSceneView v = scene_->view();
+    if (tuning_.traversal) v.trav_wave = tuning_.traversal == 2 ? 1 : 0;
+    return v;
 }
 
 int BDPTRenderer::initStream() {
@@ -208,7 +214,7 @@ int BDPTRenderer::renderSamples(PinholeCamera& cam, double& kernelMs) {
     for (uint64_t pix0 = 0; pix0 < npix; pix0 += chunkPix) {
         const uint64_t np = std::min(chunkPix, npix - pix0);
         (void)hipEventRecord(ev_[2], stream_);
-        if ((e = launchBdpt(scene_->view(), cam.desc, p, pix0 * (uint64_t)spp, np * (uint64_t)spp, dRad_.as<float>(), dLv_.as<float>(),
+        if ((e = launchBdpt(viewForLaunch(), cam.desc, p, pix0 * (uint64_t)spp, np * (uint64_t)spp, dRad_.as<float>(), dLv_.as<float>(),
                             dStack_.as<uint32_t>(), dCounters_.as<unsigned long long>(), blocks, splat_, stream_)) != hipSuccess)
             return hipFail(e, "sample kernel launch");
         (void)hipEventRecord(ev_[3], stream_);
@@ -225,7 +231,7 @@ int BDPTRenderer::renderSamples(PinholeCamera& cam, double& kernelMs) {
 // (wavefront.hip), then the ordered film reduction for that chunk.
 int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     const omv_renderer_params& p = params_;
-    const OmvSceneView& S = scene_->view();
+    const OmvSceneView S = viewForLaunch();
     const bool lights = S.num_lights > 0;
     const uint64_t npix = (uint64_t)cam.desc.width * cam.desc.height;
     const int spp = p.samples_per_pixel;
@@ -236,7 +242,10 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     const uint64_t slots = std::min(npix, chunkPix) * (uint64_t)spp;
     if (slots * (uint64_t)lc >= (1ull << 32)) { setError("render: too many samples per pixel for one chunk"); return OMV_E_RANGE; }
     if (!wfPerCu_[0]) wavefrontBlocksPerCU(wfPerCu_);
-    const int blocksTrace = cus_ * (tuning_.trace_blocks_per_cu > 0 ? std::min(tuning_.trace_blocks_per_cu, wfPerCu_[0]) : wfPerCu_[0]);
+    // resident traversal workgroups per CU: the occupancy limit of the traversal
+    // kernel in use (the persistent form holds more registers)
+    const int traceOcc = tuning_.trace_keep > 0 ? wfPerCu_[4] : wfPerCu_[0];
+    const int blocksTrace = cus_ * (tuning_.trace_blocks_per_cu > 0 ? std::min(tuning_.trace_blocks_per_cu, traceOcc) : traceOcc);
     hipError_t e;
     if ((e = dStack_.reserve((uint64_t)blocksTrace * OMV_BLOCK * (OMV_STACK_TOTAL - OMV_STACK_LDS) * sizeof(uint32_t))) != hipSuccess)
         return hipFail(e, "stack spill");

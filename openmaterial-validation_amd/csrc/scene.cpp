@@ -645,6 +645,9 @@ int RenderScene::commit(std::string& log) {
     v.total_faces = faceBase;
     v.fc_subject = fcSubject_;
     v.any_emissive = anyEmissive;
+    // wave-cooperative traversal pays off when rays stay inside one BLAS; with
+    // many instances lanes holding a leaf stall at instance boundaries
+    v.trav_wave = insts_.size() <= 4 ? 1 : 0;
     v.bg_type = bgType_; v.bg_w = bgW_; v.bg_h = bgH_;
     for (int c = 0; c < 3; ++c) v.bg_const[c] = bgConst_[c];
     std::memcpy(v.bg_rot, bgRot_, sizeof(bgRot_));

@@ -24,9 +24,11 @@
 
 #include "path_device.h"
 
-// Traversal kernels are held to 5 waves per SIMD (<= 102 VGPRs, no spill):
-// more resident rays hide more of the dependent node-fetch latency.
-#define OMV_TRACE_WAVES 5
+// Traversal kernels are held to 4 waves per SIMD (<= 128 VGPRs, no spill):
+// measured on the headline and the 2M-triangle instanced scene, 4 resident
+// waves per SIMD trace faster than 2, 3 or 5 (5 thrash the L2 with node
+// and triangle lines).
+#define OMV_TRACE_WAVES 4
 
 namespace {
 
@@ -666,9 +668,15 @@ size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool
 
 int wavefrontCounterCount(int B, int lvCap) { return 4 * lvCap + 6 * B + 16; }
 
-void wavefrontBlocksPerCU(int out[4]) {
+void wavefrontBlocksPerCU(int out[5]) {
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
-    out[0] = std::min(occ(omv_wf_trace<0>, lds), occ(omv_wf_trace_pt<2>, lds));
+    int t = occ(omv_wf_trace<0>, lds);
+    for (int k : {occ(omv_wf_trace<1>, lds), occ(omv_wf_trace<2>, lds), occ(omv_wf_trace<3>, lds), occ(omv_wf_trace<4>, lds)}) t = std::min(t, k);
+    int tp = occ(omv_wf_trace_pt<0>, lds);
+    for (int k : {occ(omv_wf_trace_pt<1>, lds), occ(omv_wf_trace_pt<2>, lds), occ(omv_wf_trace_pt<3>, lds), occ(omv_wf_trace_pt<4>, lds)})
+        tp = std::min(tp, k);
+    out[0] = t;
+    out[4] = tp;
     out[1] = occ(omv_wf_shade_cam<true, true>, 0);
     out[2] = occ(omv_wf_shade_light<true>, 0);
     out[3] = occ(omv_wf_gen<true>, 0);

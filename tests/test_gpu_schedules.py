@@ -33,8 +33,12 @@ def _same(out):
     (a, sa), (b, sb) = out
     assert np.all(np.isfinite(a))
     assert np.array_equal(a, b), f"films differ at {np.argwhere(a != b)[:5].tolist()}"
-    for k in ("samples", "rays", "camera_rays", "light_rays", "shadow_rays", "nodes_visited", "tris_tested"):
+    for k in ("samples", "rays", "camera_rays", "light_rays", "shadow_rays"):
         assert sa[k] == sb[k], (k, sa[k], sb[k])
+    # node / triangle counts depend on which rays share a wave (the traversal
+    # postpones leaves wave-cooperatively), the closest hits do not
+    for k in ("nodes_visited", "tris_tested"):
+        assert abs(sa[k] - sb[k]) <= 0.1 * sb[k], (k, sa[k], sb[k])
     assert sa["stack_overflows"] == 0
 
 
@@ -100,7 +104,8 @@ def test_wavefront_multi_chunk(pkg, wl, lights):
 
 
 @pytest.mark.parametrize("tuning", [dict(trace_keep=1), dict(trace_keep=32), dict(trace_keep=64),
-                                    dict(trace_keep=48, trace_blocks_per_cu=1, chunk_samples=1000)])
+                                    dict(trace_keep=48, trace_blocks_per_cu=1, chunk_samples=1000),
+                                    dict(traversal=1), dict(traversal=2), dict(traversal=2, trace_blocks_per_cu=3)])
 def test_wavefront_tuning_agrees(pkg, wl, tuning):
     """Launch tuning (persistent traversal with refill, fewer resident blocks,
     small chunks) changes the schedule only: the film and the counts equal the
