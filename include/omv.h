@@ -312,6 +312,10 @@ typedef struct omv_render_tuning {
                                  instances), 1 = entry by entry, 2 = wave-
                                  cooperative: node entries until every lane
                                  of the wave holds a leaf, then the leaves    */
+    int sort_rays;            /* secondary camera rays reordered by
+                                 (direction octant, origin cell of a 16^3
+                                 grid over the scene) before each traversal:
+                                 0 = default, 1 = never, 2 = always          */
 } omv_render_tuning;
 int omv_renderer_set_tuning(omv_renderer* r, const omv_render_tuning* t);
 int omv_renderer_get_tuning(const omv_renderer* r, omv_render_tuning* t);

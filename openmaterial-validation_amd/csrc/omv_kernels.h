@@ -41,7 +41,7 @@ struct WfLaunch {
     uint64_t sample0;          // first sample of the chunk
     uint32_t nslots;           // samples in the chunk
     int lvCap, lc;             // light vertices kept, connections per camera vertex
-    void* arena;               // wavefrontArenaBytes(nslots, lvCap, lc, lights, nee, splat)
+    void* arena;               // wavefrontArenaBytes(nslots, lvCap, lc, lights, nee, splat, sortRays)
     uint32_t* qcount;          // wavefrontCounterCount(B, lvCap) counters
     float* rad;                // 3 floats per sample of the chunk
     uint32_t* gstack;          // spill stacks of blocksTrace * OMV_BLOCK lanes
@@ -52,8 +52,9 @@ struct WfLaunch {
     int traceEvCap;            // events available from traceEv
     int* traceEvUsed;          // events recorded so far (in/out)
     float* splat;              // OMV_EXT_LIGHT_SPLAT sums (3 floats per pixel), or null
+    bool sortRays;             // reorder secondary camera rays before each traversal
 };
-size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat);
+size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat, bool sortRays);
 int wavefrontCounterCount(int B, int lvCap);
 void wavefrontBlocksPerCU(int out[5]);   // trace, shade, shade_light, gen, persistent trace
 hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream);

@@ -101,6 +101,7 @@ struct OmvSceneView {
     int32_t total_faces;
     int32_t fc_subject;
     int32_t any_emissive;
+    float   scene_lo[3], scene_inv[3];   // world box of the instances, 1 / its extent (0 if flat)
     int32_t trav_wave;    // traversal order: 1 wave-cooperative (leaves postponed until every
                           // lane of the wave holds one), 0 entry by entry; the hits do not depend on it
     int32_t bg_type, bg_w, bg_h;

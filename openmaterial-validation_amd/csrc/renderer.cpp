@@ -82,7 +82,8 @@ int BDPTRenderer::setSchedule(int s) {
 }
 
 int BDPTRenderer::setTuning(const omv_render_tuning& t) {
-    if (t.trace_keep < 0 || t.trace_keep > 64 || t.trace_blocks_per_cu < 0 || t.chunk_samples < 0 || t.traversal < 0 || t.traversal > 2) {
+    if (t.trace_keep < 0 || t.trace_keep > 64 || t.trace_blocks_per_cu < 0 || t.chunk_samples < 0 || t.traversal < 0 || t.traversal > 2 ||
+        t.sort_rays < 0 || t.sort_rays > 2) {
         setError("set_tuning: value out of range");
         return OMV_E_RANGE;
     }
@@ -250,7 +251,8 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     if ((e = dStack_.reserve((uint64_t)blocksTrace * OMV_BLOCK * (OMV_STACK_TOTAL - OMV_STACK_LDS) * sizeof(uint32_t))) != hipSuccess)
         return hipFail(e, "stack spill");
     const bool nee = (p.extensions & OMV_EXT_HDRI_NEE) && S.env_tex_pdf;
-    if ((e = dArena_.reserve(wavefrontArenaBytes((uint32_t)slots, lvCap, lc, lights, nee, splat_ != nullptr))) != hipSuccess)
+    if ((e = dArena_.reserve(wavefrontArenaBytes((uint32_t)slots, lvCap, lc, lights, nee, splat_ != nullptr, tuning_.sort_rays == 2))) !=
+        hipSuccess)
         return hipFail(e, "wavefront state");
     if ((e = dQcount_.reserve(wavefrontCounterCount(p.camera_bounces, lvCap) * sizeof(uint32_t))) != hipSuccess) return hipFail(e, "queue counters");
     if ((e = dRad_.reserve(slots * 3 * sizeof(float))) != hipSuccess) return hipFail(e, "sample buffer");
@@ -264,6 +266,7 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     L.blocksGen = cus_ * wfPerCu_[3];
     L.traceKeep = tuning_.trace_keep;
     L.splat = splat_;
+    L.sortRays = tuning_.sort_rays == 2;
     // an event pair around every traversal launch: (2 light + 3 camera rounds) per chunk
     const uint64_t chunks = (npix + chunkPix - 1) / chunkPix;
     const size_t evNeed = (size_t)std::min<uint64_t>(1u << 14, chunks * 2 * (uint64_t)(2 * lvCap + 3 * p.camera_bounces));

@@ -572,6 +572,9 @@ int RenderScene::commit(std::string& log) {
         iids.push_back((int)i);
     }
     Bvh4Result tl = buildBvh4(iboxes, 1, 0, nodes);
+    float sceneLo[3] = {INFINITY, INFINITY, INFINITY}, sceneHi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (const BuildBox& b : iboxes)
+        for (int a = 0; a < 3; ++a) { sceneLo[a] = std::min(sceneLo[a], b.lo[a]); sceneHi[a] = std::max(sceneHi[a], b.hi[a]); }
     // TLAS leaves must name instances: the builder wrote -(order index + 1)
     {
         std::vector<int> stack = {tl.root};
@@ -645,6 +648,11 @@ int RenderScene::commit(std::string& log) {
     v.total_faces = faceBase;
     v.fc_subject = fcSubject_;
     v.any_emissive = anyEmissive;
+    for (int a = 0; a < 3; ++a) {
+        const float ext = sceneHi[a] - sceneLo[a];
+        v.scene_lo[a] = std::isfinite(sceneLo[a]) ? sceneLo[a] : 0.0f;
+        v.scene_inv[a] = (std::isfinite(ext) && ext > 0.0f) ? 1.0f / ext : 0.0f;
+    }
     // wave-cooperative traversal pays off when rays stay inside one BLAS; with
     // many instances lanes holding a leaf stall at instance boundaries
     v.trav_wave = insts_.size() <= 4 ? 1 : 0;

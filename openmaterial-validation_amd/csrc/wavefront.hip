@@ -634,6 +634,72 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_connect(WfState W, WfQueue i
 // Paths that end when the camera loop runs out (i == B) are written by
 // shade/connect above; nothing is left over after the last round.
 
+// ----------------------------------------------------------- ray sorting
+// A round's secondary camera rays reordered by (direction octant, cell of a
+// 16^3 grid over the scene box, Morton order) with a counting sort, so that
+// a traversal wave holds rays that start close together and head the same
+// way.  Order within a bucket is arbitrary; every ray's result is its own.
+#define OMV_SORT_BUCKETS 32768
+DEV uint32_t spread4(uint32_t v) {   // 4 bits -> every third bit
+    v &= 0xFu;
+    v = (v | (v << 4)) & 0x0C3u;
+    v = (v | (v << 2)) & 0x249u;
+    return v;
+}
+DEV uint32_t rayKey(const OmvSceneView& S, const float4& o, const float4& d) {
+    const uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+    const float p[3] = {o.x, o.y, o.z};
+    uint32_t c[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int q = (int)((p[a] - S.scene_lo[a]) * S.scene_inv[a] * 16.0f);
+        c[a] = (uint32_t)min(max(q, 0), 15);
+    }
+    return (oct << 12) | (spread4(c[0]) << 2) | (spread4(c[1]) << 1) | spread4(c[2]);
+}
+__global__ __launch_bounds__(OMV_BLOCK) void omv_wf_sort_keys(OmvSceneView S, WfQueue q, const uint32_t* __restrict__ count,
+                                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ hist) {
+    const uint32_t gthreads = gridDim.x * blockDim.x;
+    const uint32_t total = *count;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gthreads) {
+        const uint32_t k = rayKey(S, q.o[e], q.d[e]);
+        keys[e] = k;
+        atomicAdd(hist + k, 1u);
+    }
+}
+// exclusive offsets of the bucket counts, in place: one block of 1024 lanes,
+// 32 consecutive buckets each
+__global__ __launch_bounds__(1024) void omv_wf_sort_scan(uint32_t* __restrict__ hist) {
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x;
+    uint32_t v[OMV_SORT_BUCKETS / 1024];
+    uint32_t s = 0u;
+#pragma unroll
+    for (int k = 0; k < OMV_SORT_BUCKETS / 1024; ++k) { v[k] = hist[t * (OMV_SORT_BUCKETS / 1024) + k]; s += v[k]; }
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t x = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    uint32_t base = part[t] - s;
+#pragma unroll
+    for (int k = 0; k < OMV_SORT_BUCKETS / 1024; ++k) { hist[t * (OMV_SORT_BUCKETS / 1024) + k] = base; base += v[k]; }
+}
+__global__ __launch_bounds__(OMV_BLOCK) void omv_wf_sort_scatter(WfQueue q, const uint32_t* __restrict__ count,
+                                                                 const uint32_t* __restrict__ keys, uint32_t* __restrict__ offs,
+                                                                 WfQueue out) {
+    const uint32_t gthreads = gridDim.x * blockDim.x;
+    const uint32_t total = *count;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gthreads) {
+        const uint32_t pos = atomicAdd(offs + keys[e], 1u);
+        out.o[pos] = q.o[e];
+        out.d[pos] = q.d[e];
+    }
+}
+
 __global__ void omv_wf_fill_u32(uint32_t* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
@@ -653,7 +719,7 @@ constexpr size_t kAlign = 256;
 size_t up(size_t b) { return (b + kAlign - 1) & ~(kAlign - 1); }
 }  // namespace
 
-size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat) {
+size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat, bool sortRays) {
     const size_t N = nslots;
     size_t b = 6 * up(N * 16)                        // three ray queues
                + up(N * 16) + up(N * 8)              // hits
@@ -663,6 +729,7 @@ size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool
              3 * up(N * (size_t)lc * 16);
     if (nee) b += up(N * 16) + up(N) + 2 * up(N * 16);   // neeC, nvis, environment-ray queue
     if (lights && splat) b += up(N * 16);                 // lbeta
+    if (sortRays) b += up(N * 4) + up(OMV_SORT_BUCKETS * 4);   // keys, bucket counts / offsets
     return b;
 }
 
@@ -724,6 +791,12 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
         sq.b = reinterpret_cast<float4*>(take(N * (size_t)L.lc * 16));
     }
     if (splat) W.lbeta = reinterpret_cast<float4*>(take(N * 16));
+    uint32_t* sortKeys = nullptr;
+    uint32_t* sortHist = nullptr;
+    if (L.sortRays) {
+        sortKeys = reinterpret_cast<uint32_t*>(take(N * 4));
+        sortHist = reinterpret_cast<uint32_t*>(take(OMV_SORT_BUCKETS * 4));
+    }
     WfQueue eq;
     std::memset(&eq, 0, sizeof(eq));
     if (nee) {
@@ -789,7 +862,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
             std::swap(in, out);
         }
     }
-    WfQueue in = qa, out = qb;
+    WfQueue in = qa, out = qb, spare = qc;   // qc is free once the light rounds are done
     for (int i = 1; i <= B; ++i) {
         mark();
         if (L.traceKeep > 0)
@@ -834,7 +907,19 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
         }
         if (lights || nee)
             hipLaunchKernelGGL(omv_wf_connect, dim3(L.blocksShade), blk, 0, stream, W, in, camCnt + i, L.rad, lights ? 1 : 0, nee ? 1 : 0);
-        std::swap(in, out);
+        if (L.sortRays && i < B) {
+            // reorder the next round's rays (secondary rays) into the spare queue
+            hipLaunchKernelGGL(omv_wf_fill_u32, dim3(OMV_SORT_BUCKETS / 256), dim3(256), 0, stream, sortHist, OMV_SORT_BUCKETS);
+            hipLaunchKernelGGL(omv_wf_sort_keys, dim3(L.blocksShade), blk, 0, stream, S, out, camCnt + i + 1, sortKeys, sortHist);
+            hipLaunchKernelGGL(omv_wf_sort_scan, dim3(1), dim3(1024), 0, stream, sortHist);
+            hipLaunchKernelGGL(omv_wf_sort_scatter, dim3(L.blocksShade), blk, 0, stream, out, camCnt + i + 1, sortKeys, sortHist, spare);
+            const WfQueue freed = in;
+            in = spare;
+            spare = out;
+            out = freed;
+        } else {
+            std::swap(in, out);
+        }
     }
     return hipGetLastError();
 }

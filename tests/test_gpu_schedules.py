@@ -105,7 +105,8 @@ def test_wavefront_multi_chunk(pkg, wl, lights):
 
 @pytest.mark.parametrize("tuning", [dict(trace_keep=1), dict(trace_keep=32), dict(trace_keep=64),
                                     dict(trace_keep=48, trace_blocks_per_cu=1, chunk_samples=1000),
-                                    dict(traversal=1), dict(traversal=2), dict(traversal=2, trace_blocks_per_cu=3)])
+                                    dict(traversal=1), dict(traversal=2), dict(traversal=2, trace_blocks_per_cu=3),
+                                    dict(sort_rays=2), dict(sort_rays=2, chunk_samples=1000, traversal=1)])
 def test_wavefront_tuning_agrees(pkg, wl, tuning):
     """Launch tuning (persistent traversal with refill, fewer resident blocks,
     small chunks) changes the schedule only: the film and the counts equal the
