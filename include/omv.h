@@ -266,6 +266,11 @@ int omv_camera_adjust_to_resolution(omv_camera* c);               /* :156-160 */
 int omv_camera_set_yfov(omv_camera* c, float yfov_deg);           /* :167-174 */
 int omv_camera_set_real_height(omv_camera* c, float height, float aspect);
 int omv_camera_set_focus(omv_camera* c, float focus);
+/* PinholeCamera::loadProperties (PinholeCamera.cpp:115-148): the "pinhole"
+ * object of a camera JSON file, parsed by the caller; NULL = key absent
+ * (aspect and y-fov then 0, focus unchanged).  half_h = focus *
+ * tan(y-fov / 2), half_w = aspect * half_h, no clamping. */
+int omv_camera_load_properties(omv_camera* c, const float* aspect, const float* focus, const float* yfov);
 int omv_camera_get_desc(const omv_camera* c, omv_camera_desc* out);
 /* getImpression (:110-113) for one pixel, and the whole normalised film
  * (W*H*4 floats, rgb + 1) after a render. */

@@ -159,18 +159,25 @@ class SceneApi:
     def allocate(self, meshes, instances, materials, textures, lights):
         self._check(self._fn("scene_allocate")(self.h, meshes, instances, materials, textures, lights), "scene_allocate")
 
-    def set_mesh(self, mesh_id, vertices, faces, mat_id, normals=None, tangents=None, uvs=(), tex_n=None):
+    def set_mesh(self, mesh_id, vertices, faces, mat_id, normals=None, tangents=None, uvs=(), tex_n=None, tangent_stride=4):
+        """RenderScene::setMesh.  tangents are 4 floats per vertex read at
+        tangent_stride floats apart (main.cpp:386 passes glTF VEC4 tangents
+        with stride 3, which this reproduces when asked)."""
         v = _f32(vertices, (-1, 3))
         f = np.ascontiguousarray(np.asarray(faces, np.uint32).reshape(-1, 3))
         n = _f32(normals, (-1, 3)) if normals is not None else None
-        t = _f32(tangents, (-1, 4)) if tangents is not None else None
+        t = None
+        if tangents is not None:
+            t = _f32(tangents).reshape(-1)
+            if len(v) and t.size < tangent_stride * (len(v) - 1) + 4:
+                raise ValueError(f"set_mesh: {t.size} tangent floats for {len(v)} vertices at stride {tangent_stride}")
         uvl = [(_f32(u, (-1, 2)) if u is not None else None) for u in uvs]
         texn = len(uvl) if tex_n is None else tex_n
         arr = (C.POINTER(C.c_float) * max(1, texn))()
         for i in range(texn):
             arr[i] = _ptr(uvl[i], C.c_float) if i < len(uvl) and uvl[i] is not None else None
         rc = self._fn("scene_set_mesh")(self.h, mesh_id, len(v), len(f), texn, mat_id, _ptr(f, C.c_uint32), _ptr(v, C.c_float), 3,
-                                         _ptr(n, C.c_float), 3, _ptr(t, C.c_float), 4, arr, 2)
+                                         _ptr(n, C.c_float), 3, _ptr(t, C.c_float), int(tangent_stride), arr, 2)
         self._check(rc, "scene_set_mesh")
 
     def set_instance(self, inst_id, trans, mesh_id):
@@ -264,6 +271,7 @@ def _load():
         "omv_camera_set_viewpoint": [vp] + [C.POINTER(C.c_float)] * 4,
         "omv_camera_adjust_to_resolution": [vp], "omv_camera_set_yfov": [vp, C.c_float],
         "omv_camera_set_real_height": [vp, C.c_float, C.c_float], "omv_camera_set_focus": [vp, C.c_float],
+        "omv_camera_load_properties": [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)],
         "omv_camera_get_desc": [vp, C.POINTER(CameraDesc)],
         "omv_camera_get_impression": [vp, C.c_int, C.c_int, C.POINTER(C.c_float)],
         "omv_camera_get_film": [vp, C.POINTER(C.c_float), C.c_size_t],
@@ -412,6 +420,12 @@ class Camera:
         d = CameraDesc()
         self._check(self.lib.omv_camera_get_desc(self.h, C.byref(d)))
         return d
+
+    def load_properties(self, aspect=None, focus=None, yfov=None):
+        """PinholeCamera::loadProperties with a camera file's values
+        (assets.read_camera_properties); None = key absent."""
+        vals = [None if v is None else C.c_float(float(v)) for v in (aspect, focus, yfov)]
+        self._check(self.lib.omv_camera_load_properties(self.h, *[C.byref(v) if v is not None else None for v in vals]))
 
     def film(self):
         out = np.zeros(self.width * self.height * 4, np.float32)

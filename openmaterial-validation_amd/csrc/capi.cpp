@@ -151,6 +151,11 @@ int omv_camera_adjust_to_resolution(omv_camera* c) { OMV_NULLCHK(c); c->c.adjust
 int omv_camera_set_yfov(omv_camera* c, float fov) { OMV_NULLCHK(c); return c->c.setYFoV(fov); }
 int omv_camera_set_real_height(omv_camera* c, float h, float a) { OMV_NULLCHK(c); c->c.setRealHeight(h, a); return OMV_OK; }
 int omv_camera_set_focus(omv_camera* c, float f) { OMV_NULLCHK(c); c->c.desc.focus = f; return OMV_OK; }
+int omv_camera_load_properties(omv_camera* c, const float* aspect, const float* focus, const float* yfov) {
+    OMV_NULLCHK(c);
+    c->c.loadProperties(aspect, focus, yfov);
+    return OMV_OK;
+}
 int omv_camera_get_desc(const omv_camera* c, omv_camera_desc* out) {
     OMV_NULLCHK(c); OMV_NULLCHK(out);
     *out = c->c.desc;

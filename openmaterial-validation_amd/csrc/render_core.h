@@ -112,6 +112,7 @@ public:
     void adjustToResolution();
     int setYFoV(float fov);
     void setRealHeight(float height, float aspect);
+    void loadProperties(const float* aspect, const float* focus, const float* yfov);
     omv_camera_desc desc;
     std::vector<float> film;   // W*H*4 after a render
 };

@@ -62,6 +62,17 @@ int PinholeCamera::setYFoV(float fov) {      // PinholeCamera.cpp:167-174
     return OMV_E_ARG;
 }
 
+// PinholeCamera::loadProperties (PinholeCamera.cpp:115-148) with the camera
+// file's "pinhole" values already parsed; a null pointer is a key the file
+// does not have (aspect and y-fov then count as 0, the focus stays).
+void PinholeCamera::loadProperties(const float* aspect, const float* focus, const float* yfov) {
+    const float a = aspect ? *aspect : 0.0f;
+    const float fov = yfov ? *yfov : 0.0f;
+    if (focus) desc.focus = *focus;
+    desc.half_h = desc.focus * std::tan(0.5f * fov * 3.14159265358979323846f / 180.0f);
+    desc.half_w = a * desc.half_h;
+}
+
 void PinholeCamera::setRealHeight(float height, float aspect) {   // PinholeCamera.cpp:150-154
     desc.half_h = std::max(0.001f, height);
     desc.half_w = std::max(0.1f, aspect) * desc.half_h;

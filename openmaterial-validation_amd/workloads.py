@@ -46,7 +46,10 @@ class SceneDesc:
     def apply(self, scene, pbr_desc):
         """Drive RenderScene set-up in the reference main's order (main.cpp:314-416)."""
         scene.allocate(len(self.meshes), len(self.instances), len(self.materials), len(self.textures), len(self.lights))
-        for i, (kind, img) in enumerate(self.textures):
+        for i, tex in enumerate(self.textures):
+            if tex is None:       # slot allocated, never set
+                continue
+            kind, img = tex
             if kind == "rgba8":
                 scene.set_texture(i, img)
             else:
@@ -61,7 +64,8 @@ class SceneDesc:
         for i, (pos, inten, rng) in enumerate(self.lights):
             scene.set_light_point(i, pos, inten, rng)
         for i, m in enumerate(self.meshes):
-            scene.set_mesh(i, m["vertices"], m["faces"], m["mat_id"], m.get("normals"), m.get("tangents"), m.get("uvs", ()))
+            scene.set_mesh(i, m["vertices"], m["faces"], m["mat_id"], m.get("normals"), m.get("tangents"), m.get("uvs", ()),
+                           m.get("tex_n"), m.get("tangent_stride", 4))
         for i, (tm, mesh) in enumerate(self.instances):
             scene.set_instance(i, tm, mesh)
         if self.background is not None:

@@ -32,3 +32,14 @@ for t in "${TUS[@]}"; do SRCS+=("$SRC/$t"); done
 g++ -std=c++11 -O0 -ffp-contract=off -ffunction-sections -Wl,--gc-sections -DDOCTEST_CONFIG_DISABLE "${INC[@]}" \
     "$HERE/golden_harness.cpp" "$HERE/golden_stb.cpp" "${SRCS[@]}" -o "$OUT/golden_harness"
 echo "built $OUT/golden_harness"
+
+# The glTF / OpenMATERIAL asset loader (AssetGeometry and what it pulls in);
+# the reference's external/tiny_gltf.cpp is its tinygltf + stb_image TU.
+ATUS=(assets/AssetGeometry.cpp assets/ImageData.cpp assets/Mesh.cpp assets/AssetDefs.cpp
+      assets/materials/AssetMaterial.cpp assets/materials/PbrMaterial.cpp assets/materials/AssetMaterialIor.cpp
+      utilities/ReferenceLink.cpp utilities/Utils.cpp utilities/Random.cpp lights/PointLight.cpp)
+ASRCS=("$REF/external/tiny_gltf.cpp")
+for t in "${ATUS[@]}"; do ASRCS+=("$SRC/$t"); done
+g++ -std=c++11 -O1 -ffp-contract=off -DDOCTEST_CONFIG_DISABLE -Dcimg_display=0 "${INC[@]}" -I"$REF/external/CImg" \
+    "$HERE/asset_harness.cpp" "${ASRCS[@]}" -lpthread -o "$OUT/asset_harness"
+echo "built $OUT/asset_harness"
