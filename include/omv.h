@@ -46,6 +46,7 @@ extern "C" {
 #define OMV_E_SCENE    -5   /* commit failed: nothing renderable           */
 #define OMV_E_NOMEM    -6
 #define OMV_E_INTERNAL -7
+#define OMV_E_IO       -8   /* file could not be opened or written         */
 
 /* ------------------------------------------------------- parameter POD */
 /* Estimator extensions.  0 = the reference estimator exactly
@@ -332,6 +333,19 @@ int omv_renderer_get_tuning(const omv_renderer* r, omv_render_tuning* t);
  * saveImage (main.cpp:88-93).  out_f32 (W*H*4 floats) and out_u8 (W*H*4
  * bytes) are host or device pointers, either may be NULL. */
 int omv_renderer_post(omv_renderer* r, omv_camera* c, float gamma, int denoise, float* out_f32, uint8_t* out_u8);
+
+/* saveImage (main.cpp:86-120) with ImageSaver::flipVerticallyOnWrite(true)
+ * (main.cpp:149): rgba is the W*H*4 gamma-corrected float image (host
+ * memory, values in [0, 1]); ".png", ".bmp", ".tga", ".jpg" (quality 100)
+ * write the 8-bit truncation 255*f, ".pfm" the floats.  The bytes equal
+ * stb_image_write's (ImageSaver.h:23-97).  Another extension is OMV_E_ARG
+ * ("This image extension is not supported"), a file error OMV_E_IO. */
+int omv_save_image(const char* filename, const float* rgba, int width, int height);
+/* The bytes omv_save_image would write for extension ext ("png", "bmp",
+ * "tga", "jpg", "pfm"), into out (capacity cap); *len receives the size.
+ * With out == NULL or cap too small only *len is set (OMV_E_RANGE when
+ * out != NULL and cap < *len). */
+int omv_encode_image(const char* ext, const float* rgba, int width, int height, uint8_t* out, size_t cap, size_t* len);
 
 /* ---------------------------------------------------------- adjoint */
 /* Gradient targets of omv_renderer_backward.  All pointers are device

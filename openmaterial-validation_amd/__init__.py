@@ -271,6 +271,9 @@ def _load():
         "omv_camera_set_viewpoint": [vp] + [C.POINTER(C.c_float)] * 4,
         "omv_camera_adjust_to_resolution": [vp], "omv_camera_set_yfov": [vp, C.c_float],
         "omv_camera_set_real_height": [vp, C.c_float, C.c_float], "omv_camera_set_focus": [vp, C.c_float],
+        "omv_save_image": [C.c_char_p, C.POINTER(C.c_float), C.c_int, C.c_int],
+        "omv_encode_image": [C.c_char_p, C.POINTER(C.c_float), C.c_int, C.c_int, C.POINTER(C.c_uint8), C.c_size_t,
+                             C.POINTER(C.c_size_t)],
         "omv_camera_load_properties": [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)],
         "omv_camera_get_desc": [vp, C.POINTER(CameraDesc)],
         "omv_camera_get_impression": [vp, C.c_int, C.c_int, C.POINTER(C.c_float)],
@@ -588,3 +591,29 @@ class Batch:
         if getattr(self, "h", None):
             self.lib.omv_batch_destroy(self.h)
             self.h = None
+
+
+def save_image(filename: str, rgba) -> None:
+    """saveImage of the reference main (main.cpp:86-120): the (H, W, 4)
+    gamma-corrected float image as .png/.bmp/.tga/.jpg (8-bit, 255*f
+    truncated) or .pfm, rows written bottom-up, bytes as stb_image_write's."""
+    a = _f32(rgba)
+    h, w = a.shape[:2]
+    rc = lib().omv_save_image(str(filename).encode(), _ptr(a, C.c_float), int(w), int(h))
+    if rc != 0:
+        raise RuntimeError(f"omv_save_image failed ({rc}): {lib().omv_last_error().decode()}")
+
+
+def encode_image(ext: str, rgba) -> bytes:
+    """The bytes save_image writes for file extension ``ext``."""
+    a = _f32(rgba)
+    h, w = a.shape[:2]
+    n = C.c_size_t()
+    rc = lib().omv_encode_image(ext.encode(), _ptr(a, C.c_float), int(w), int(h), None, 0, C.byref(n))
+    if rc != 0:
+        raise RuntimeError(f"omv_encode_image failed ({rc}): {lib().omv_last_error().decode()}")
+    out = (C.c_uint8 * n.value)()
+    rc = lib().omv_encode_image(ext.encode(), _ptr(a, C.c_float), int(w), int(h), out, n.value, C.byref(n))
+    if rc != 0:
+        raise RuntimeError(f"omv_encode_image failed ({rc}): {lib().omv_last_error().decode()}")
+    return bytes(out)

@@ -1378,6 +1378,42 @@ def load_scene(path: str, lights: bool = False, material_priority: str = "om", r
     return asset, desc, view_points(asset, T, lightbox, automatic_centering), T
 
 
+def _insert_before_ext(path: str, text: str) -> str:
+    """std::string::insert(find_last_of('.'), text) as main.cpp:495 / 517
+    does; a name without '.' gets the text appended (npos insert throws in
+    the reference)."""
+    k = path.rfind(".")
+    return path + text if k < 0 else path[:k] + text + path[k:]
+
+
+def render_view_points(pkg, scene, view_points, width: int, height: int, params, output: str, lightbox: bool = False,
+                       camera_properties=None, gamma: float = 0.5, denoise: bool = False):
+    """main.cpp:480-521 for a committed ``scene``: one PinholeCamera
+    (camera-file properties unless light box, else adjustToResolution +
+    setYFoV(60)), then per view point: render, gammaCorrection (+ median
+    denoiser, written as *_denoised) on the device, saveImage.  Light-box
+    outputs are numbered *_0 .. *_7.  Returns the files written."""
+    r = pkg.Renderer(params)
+    r.set_scene(scene)
+    cam = pkg.Camera(width, height)
+    if not lightbox and camera_properties is not None:
+        cam.load_properties(*camera_properties)
+    written = []
+    for i, vp in enumerate(view_points):
+        cam.set_viewpoint(vp.position, vp.left, vp.up, vp.forward)
+        out = _insert_before_ext(output, f"_{i}") if lightbox else output
+        r.render(cam)
+        f, _ = r.post(cam, gamma=gamma, denoise=False)
+        pkg.save_image(out, f)
+        written.append(out)
+        if denoise:
+            fd, _ = r.post(cam, gamma=gamma, denoise=True)
+            d = _insert_before_ext(out, "_denoised")
+            pkg.save_image(d, fd)
+            written.append(d)
+    return written
+
+
 # ------------------------------------------------------------------ HDR background file (HDRBackground's stbi_loadf)
 class _Bytes:
     """Byte reader that yields 0 past the end, as stb_image's stbi__get8."""

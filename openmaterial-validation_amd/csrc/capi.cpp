@@ -294,4 +294,23 @@ int omv_batch_get_stats(const omv_batch* b, omv_render_stats* out) {
     return OMV_OK;
 }
 
+// saveImage (main.cpp:86-120)
+int omv_save_image(const char* filename, const float* rgba, int width, int height) {
+    OMV_NULLCHK(filename); OMV_NULLCHK(rgba);
+    if (width <= 0 || height <= 0) { omv::setError("omv_save_image: empty image"); return OMV_E_ARG; }
+    return omv::saveImage(filename, rgba, width, height);
+}
+int omv_encode_image(const char* ext, const float* rgba, int width, int height, uint8_t* out, size_t cap, size_t* len) {
+    OMV_NULLCHK(ext); OMV_NULLCHK(rgba); OMV_NULLCHK(len);
+    if (width <= 0 || height <= 0) { omv::setError("omv_encode_image: empty image"); return OMV_E_ARG; }
+    std::vector<uint8_t> bytes;
+    const int rc = omv::encodeImage(ext, rgba, width, height, bytes);
+    if (rc != OMV_OK) return rc;
+    *len = bytes.size();
+    if (!out) return OMV_OK;
+    if (cap < bytes.size()) { omv::setError("omv_encode_image: buffer too small"); return OMV_E_RANGE; }
+    std::memcpy(out, bytes.data(), bytes.size());
+    return OMV_OK;
+}
+
 }  // extern "C"

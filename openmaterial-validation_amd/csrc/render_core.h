@@ -211,6 +211,9 @@ private:
 // thread-local last error
 void setError(const std::string& msg);
 const char* lastError();
+// image_io.cpp: saveImage of the reference main (main.cpp:86-120)
+int encodeImage(const std::string& ext, const float* rgba, int w, int h, std::vector<uint8_t>& out);
+int saveImage(const char* filename, const float* rgba, int w, int h);
 
 }  // namespace omv
 #endif

@@ -18,12 +18,17 @@
  * Usage: asset_harness OUT.json LIGHTS(0|1) PRIORITY(om|pbr) FILE...
  *        asset_harness OUT.json hdr FILE...   (stbi_loadf of each file, the
  *        decoder HDRBackground uses, HdrBackground.cpp:38-47)
+ *        asset_harness OUT.json write DIR NAME W H...   (ImageSaver with
+ *        flipVerticallyOnWrite(true), as main.cpp:149 sets it: DIR/NAME.rgba8
+ *        (W*H*4 bytes) written as NAME.png/.bmp/.tga/.jpg (quality 100) and
+ *        DIR/NAME.f32 (W*H*4 floats) as NAME.pfm, ImageSaver.h:23-97)
  * Buffers are summarised by (length, checksum): the checksum is the sum over
  * the little-endian 32-bit words w_i of the raw bytes of w_i * (2 i + 1),
  * modulo 2^64 (zero-padded to a whole word).
  */
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <string>
 #include <vector>
@@ -34,6 +39,7 @@
 #include "materials/AssetMaterialIor.h"
 #include "materials/PbrMaterial.h"
 #include "stb_image.h"
+#include "ImageSaver.h"
 
 namespace {
 
@@ -295,7 +301,51 @@ int dumpHdr(int argc, char** argv) {
     return 0;
 }
 
+std::vector<char> readFile(const std::string& path) {
+    std::vector<char> d;
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return d;
+    char buf[65536];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) d.insert(d.end(), buf, buf + n);
+    std::fclose(f);
+    return d;
+}
+
+int writeImages(int argc, char** argv) {
+    ImageSaver::flipVerticallyOnWrite(true);
+    const std::string dir = argv[3];
+    std::fprintf(OUT, "[");
+    for (int a = 4; a + 2 < argc; a += 3) {
+        const std::string name = argv[a];
+        const int w = std::atoi(argv[a + 1]), h = std::atoi(argv[a + 2]);
+        std::vector<char> px = readFile(dir + "/" + name + ".rgba8"), fl = readFile(dir + "/" + name + ".f32");
+        if ((int)px.size() != w * h * 4 || (int)fl.size() != w * h * 16) return 3;
+        const std::string base = dir + "/" + name;
+        ImageSaver::savePng((base + ".png").c_str(), w, h, 4, px.data());
+        ImageSaver::saveBmp((base + ".bmp").c_str(), w, h, 4, px.data());
+        ImageSaver::saveTga((base + ".tga").c_str(), w, h, 4, px.data());
+        ImageSaver::saveJpg((base + ".jpg").c_str(), w, h, 4, px.data(), 100);
+        ImageSaver::savePfm((base + ".pfm").c_str(), w, h, reinterpret_cast<const float*>(fl.data()));
+        std::fprintf(OUT, "%s\n{\"name\":", a > 4 ? "," : "");
+        str(name);
+        std::fprintf(OUT, ",\"w\":%d,\"h\":%d", w, h);
+        for (const char* ext : {"png", "bmp", "tga", "jpg", "pfm"}) {
+            std::vector<char> b = readFile(base + "." + ext);
+            std::fprintf(OUT, ",\"%s\":[%zu,\"%016llx\"]", ext, b.size(), (unsigned long long)checksum(b.data(), b.size()));
+        }
+        std::fprintf(OUT, "}");
+    }
+    std::fprintf(OUT, "]\n");
+    std::fclose(OUT);
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc >= 4 && std::string(argv[2]) == "write") {
+        OUT = std::fopen(argv[1], "w");
+        return OUT ? writeImages(argc, argv) : 2;
+    }
     if (argc >= 3 && std::string(argv[2]) == "hdr") {
         OUT = std::fopen(argv[1], "w");
         return OUT ? dumpHdr(argc, argv) : 2;

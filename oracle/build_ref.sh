@@ -37,7 +37,7 @@ echo "built $OUT/golden_harness"
 # the reference's external/tiny_gltf.cpp is its tinygltf + stb_image TU.
 ATUS=(assets/AssetGeometry.cpp assets/ImageData.cpp assets/Mesh.cpp assets/AssetDefs.cpp
       assets/materials/AssetMaterial.cpp assets/materials/PbrMaterial.cpp assets/materials/AssetMaterialIor.cpp
-      utilities/ReferenceLink.cpp utilities/Utils.cpp utilities/Random.cpp lights/PointLight.cpp)
+      utilities/ReferenceLink.cpp utilities/Utils.cpp utilities/Random.cpp utilities/ImageSaver.cpp lights/PointLight.cpp)
 ASRCS=("$REF/external/tiny_gltf.cpp")
 for t in "${ATUS[@]}"; do ASRCS+=("$SRC/$t"); done
 g++ -std=c++11 -O1 -ffp-contract=off -DDOCTEST_CONFIG_DISABLE -Dcimg_display=0 "${INC[@]}" -I"$REF/external/CImg" \

@@ -93,3 +93,26 @@ def test_loaded_scene_ids_bitexact(A, pkg, ref, name, subject, monkeypatch):
     rfilm, rids, rtie = rs.render_fc(ref.default_params(seed=7), cam.desc())
     assert (ids[0] >= 0).mean() > 0.04 and len(np.unique(ids[1][ids[1] >= 0])) >= 3   # several meshes in view
     assert np.array_equal(ids, rids) and np.array_equal(tie, rtie) and np.array_equal(film, rfilm)
+
+
+def test_view_points_to_files(A, pkg, tmp_path, monkeypatch):
+    """main's render loop for a light box: 8 numbered PNGs (+ denoised),
+    each decoding to the device post pass's 8-bit image, rows bottom-up."""
+    from PIL import Image
+    monkeypatch.chdir(ASSETS)
+    asset, desc, vps, _ = A.load_scene("objects/cubes_pbr.gltf", lightbox=True, hdr_file="synthetic/sky_rle_wide.hdr")
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    params = pkg.default_params(**PARAMS)
+    files = A.render_view_points(pkg, s, vps, 48, 36, params, str(tmp_path / "cube.png"), lightbox=True, denoise=True)
+    assert [pathlib.Path(f).name for f in files[:4]] == ["cube_0.png", "cube_0_denoised.png", "cube_1.png", "cube_1_denoised.png"]
+    assert len(files) == 16 and all(pathlib.Path(f).stat().st_size > 0 for f in files)
+    r = pkg.Renderer(params)
+    r.set_scene(s)
+    cam = pkg.Camera(48, 36)
+    vp = vps[7]
+    cam.set_viewpoint(vp.position, vp.left, vp.up, vp.forward)
+    r.render(cam)
+    _, u8 = r.post(cam, gamma=0.5, denoise=True)
+    got = np.asarray(Image.open(files[15]).convert("RGBA"))
+    assert np.array_equal(got, u8[::-1])
