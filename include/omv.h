@@ -156,6 +156,12 @@ typedef struct omv_ior_table {
 #define OMV_FC_SG     7   /* "sg"  surface gradient      */
 #define OMV_FC_MN     8   /* "mn"  material name         */
 #define OMV_FC_IN     9   /* "in"  inverted normals      */
+/* Instance-aware subjects (no reference counterpart: the reference's ids
+ * are per mesh, so the instances of one mesh share them,
+ * BDPTRenderer.cpp:64): */
+#define OMV_FC_IID   10   /* "iid"  instance id                            */
+#define OMV_FC_IPID  11   /* "ipid" primitive id counted over instances:
+                             the instance's first face + the face index    */
 
 /* Render statistics. */
 typedef struct omv_render_stats {
@@ -293,6 +299,16 @@ int omv_renderer_render(omv_renderer* r, omv_camera* c);
  * material), instance id; -1 where the ray escapes.  tie (may be NULL) gets 1
  * where the closest hit had an exact-t tie with a different triangle. */
 int omv_renderer_render_fc(omv_renderer* r, omv_camera* c, int32_t* ids, uint8_t* tie);
+/* The colour table the enumerated subjects (pid, gid, mid, iid, ipid) paint
+ * with: EnumeratedFCSubject's construction (FCSubject.h:153-167) -- size
+ * distinct colours 0xRRGGBB in [1, 0xFFFFFF], ascending, then shuffled --
+ * with the draws and the shuffle taken from a generator seeded by
+ * (seed, subject) instead of the reference's unseeded one, so that a table,
+ * and with it an FC image, is reproducible.  render_fc uses the renderer's
+ * params.seed.  Category sizes: pid = all mesh faces, gid = meshes, mid =
+ * allocated materials, iid = instances, ipid = faces over all instances.
+ * size > 0xFFFFFF is OMV_E_RANGE. */
+int omv_fc_color_table(uint32_t seed, int subject, uint32_t size, uint32_t* out);
 int omv_renderer_get_stats(const omv_renderer* r, omv_render_stats* out);
 /* How omv_renderer_render schedules the work on the GPU (no reference
  * counterpart; both give the same film bit for bit):

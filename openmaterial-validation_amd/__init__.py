@@ -272,6 +272,7 @@ def _load():
         "omv_camera_adjust_to_resolution": [vp], "omv_camera_set_yfov": [vp, C.c_float],
         "omv_camera_set_real_height": [vp, C.c_float, C.c_float], "omv_camera_set_focus": [vp, C.c_float],
         "omv_save_image": [C.c_char_p, C.POINTER(C.c_float), C.c_int, C.c_int],
+        "omv_fc_color_table": [C.c_uint32, C.c_int, C.c_uint32, C.POINTER(C.c_uint32)],
         "omv_encode_image": [C.c_char_p, C.POINTER(C.c_float), C.c_int, C.c_int, C.POINTER(C.c_uint8), C.c_size_t,
                              C.POINTER(C.c_size_t)],
         "omv_camera_load_properties": [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)],
@@ -617,3 +618,13 @@ def encode_image(ext: str, rgba) -> bytes:
     if rc != 0:
         raise RuntimeError(f"omv_encode_image failed ({rc}): {lib().omv_last_error().decode()}")
     return bytes(out)
+
+
+def fc_color_table(seed: int, subject: int, size: int) -> np.ndarray:
+    """The seeded colour table of an enumerated false-colour subject
+    (omv_fc_color_table): size distinct 0xRRGGBB values."""
+    out = np.zeros(max(1, size), np.uint32)
+    rc = lib().omv_fc_color_table(C.c_uint32(seed), int(subject), C.c_uint32(size), _ptr(out, C.c_uint32))
+    if rc != 0:
+        raise RuntimeError(f"omv_fc_color_table failed ({rc}): {lib().omv_last_error().decode()}")
+    return out[:size]

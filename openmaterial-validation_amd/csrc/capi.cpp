@@ -3,6 +3,7 @@
 #include <new>
 #include <string>
 
+#include "fc_table.h"
 #include "render_core.h"
 
 struct omv_scene { omv::RenderScene s; };
@@ -291,6 +292,14 @@ int omv_batch_get_film(const omv_batch* b, int job_id, float* out, size_t cap) {
 int omv_batch_get_stats(const omv_batch* b, omv_render_stats* out) {
     OMV_NULLCHK(b); OMV_NULLCHK(out);
     *out = b->b.stats();
+    return OMV_OK;
+}
+
+int omv_fc_color_table(uint32_t seed, int subject, uint32_t size, uint32_t* out) {
+    if (size && !out) { omv::setError("omv_fc_color_table: null argument"); return OMV_E_ARG; }
+    std::vector<uint32_t> t;
+    if (!omv::fcColorTable(seed, subject, size, t)) { omv::setError("omv_fc_color_table: more than 0xFFFFFF colours"); return OMV_E_RANGE; }
+    if (size) std::memcpy(out, t.data(), size * sizeof(uint32_t));
     return OMV_OK;
 }
 

@@ -172,12 +172,15 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_fc_kernel(OmvSceneView S, omv_c
             }
             const int sub = S.fc_subject;
             const int globalFace = sp.face + mesh.face_base;
-            if (sub == OMV_FC_PID || sub == OMV_FC_GID || sub == OMV_FC_MID) {
-                uint32_t id, size;
-                if (sub == OMV_FC_PID) { id = (uint32_t)globalFace; size = (uint32_t)S.total_faces; }
-                else if (sub == OMV_FC_GID) { id = (uint32_t)sp.mesh; size = (uint32_t)S.num_meshes; }
-                else { id = (uint32_t)mesh.mat_id; size = (uint32_t)S.num_mats; }
-                uint32_t cc = id < size ? (((id + 1u) * 0x9E3779B1u) & 0xFFFFFFu) : 0u;
+            if (sub == OMV_FC_PID || sub == OMV_FC_GID || sub == OMV_FC_MID || sub == OMV_FC_IID || sub == OMV_FC_IPID) {
+                // EnumeratedFCSubject::paint (FCSubject.h:172-177): the id's table colour, black past the table
+                uint32_t id;
+                if (sub == OMV_FC_PID) id = (uint32_t)globalFace;
+                else if (sub == OMV_FC_GID) id = (uint32_t)sp.mesh;
+                else if (sub == OMV_FC_MID) id = (uint32_t)mesh.mat_id;
+                else if (sub == OMV_FC_IID) id = (uint32_t)sp.inst;
+                else id = S.fc_inst_base[sp.inst] + (uint32_t)sp.face;
+                const uint32_t cc = id < S.fc_table_size ? S.fc_table[id] : 0u;
                 float r = (float)((cc >> 16) & 0xFF) / 255.0f, g = (float)((cc >> 8) & 0xFF) / 255.0f, b = (float)(cc & 0xFF) / 255.0f;
                 if (sub == OMV_FC_MID) col = missing ? mk3(1000.0f, 0.0f, 1000.0f) : mk3(0.0f, g, b);
                 else col = mk3(r, g, b);

@@ -116,6 +116,11 @@ struct OmvSceneView {
     const float*       env_col_prob;
     const int32_t*     env_col_alias;
     const float*       env_tex_pdf;
+    // false-colour table of an enumerated subject (pid, gid, mid, iid, ipid)
+    // and, for ipid, the first instance-face index of every instance
+    const uint32_t*    fc_table;
+    const uint32_t*    fc_inst_base;
+    uint32_t           fc_table_size;
 };
 
 #endif

@@ -84,6 +84,10 @@ public:
     int textureSize(int texId, int* w, int* h) const;
     int envTables(float* rowProb, int* rowAlias, float* colProb, int* colAlias, float* texPdf) const;
     bool isOm(int matId) const { return matId >= 0 && matId < (int)mats_.size() && mats_[matId].type == OMV_MAT_OM; }
+    int fcSubject() const { return fcSubject_; }
+    // category size of an enumerated false-colour subject (0 for the others)
+    // and, for ipid, the first instance-face index of every instance
+    uint64_t fcCategory(int subject, std::vector<uint32_t>& instBase) const;
 
 private:
     std::vector<HostMesh> meshes_;
@@ -149,6 +153,10 @@ private:
     hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
     int cus_ = 0;
     DeviceBuffer dRad_, dLv_, dStack_, dCounters_, dFilm_, dIds_, dTie_, dArena_, dQcount_, dDfilm_, dAdjMaps_, dPost_, dSplat_;
+    DeviceBuffer dFcTable_, dFcInstBase_;
+    int fcTableSubject_ = -1;          // the table in dFcTable_ is for (subject, seed, size)
+    uint32_t fcTableSeed_ = 0;
+    uint64_t fcTableSize_ = 0;
     float* splat_ = nullptr;   // OMV_EXT_LIGHT_SPLAT sums of the current render, or null
     const PinholeCamera* filmOf_ = nullptr;   // camera whose film dFilm_ holds
     std::vector<hipEvent_t> traceEv_;

@@ -7,6 +7,7 @@
 #include <cstring>
 
 #include "omv_kernels.h"
+#include "fc_table.h"
 #include "render_core.h"
 
 namespace omv {
@@ -396,9 +397,31 @@ int BDPTRenderer::renderFc(PinholeCamera& cam, int32_t* ids, uint8_t* tie) {
     if ((e = dFilm_.reserve(npix * 4 * sizeof(float))) != hipSuccess) return hipFail(e, "film");
     if ((e = dIds_.reserve(npix * 4 * sizeof(int32_t))) != hipSuccess) return hipFail(e, "ids");
     if ((e = dTie_.reserve(npix)) != hipSuccess) return hipFail(e, "tie");
+    // colour table of an enumerated subject (and the instance-face bases of ipid)
+    OmvSceneView view = scene_->view();
+    const int sub = scene_->fcSubject();
+    std::vector<uint32_t> instBase;
+    const uint64_t catSize = scene_->fcCategory(sub, instBase);
+    if (sub == OMV_FC_PID || sub == OMV_FC_GID || sub == OMV_FC_MID || sub == OMV_FC_IID || sub == OMV_FC_IPID) {
+        if (catSize > 0xFFFFFFu) { setError("render_fc: more ids than there are 24-bit colours"); return OMV_E_RANGE; }
+        if (sub != fcTableSubject_ || params_.seed != fcTableSeed_ || catSize != fcTableSize_) {
+            std::vector<uint32_t> table;
+            fcColorTable(params_.seed, sub, (uint32_t)catSize, table);
+            if (table.empty()) table.push_back(0);
+            if ((e = dFcTable_.upload(table.data(), table.size() * sizeof(uint32_t))) != hipSuccess) return hipFail(e, "fc table");
+            fcTableSubject_ = sub; fcTableSeed_ = params_.seed; fcTableSize_ = catSize;
+        }
+        view.fc_table = dFcTable_.as<uint32_t>();
+        view.fc_table_size = (uint32_t)catSize;
+        if (sub == OMV_FC_IPID) {
+            if (instBase.empty()) instBase.push_back(0);
+            if ((e = dFcInstBase_.upload(instBase.data(), instBase.size() * sizeof(uint32_t))) != hipSuccess) return hipFail(e, "fc instance bases");
+            view.fc_inst_base = dFcInstBase_.as<uint32_t>();
+        }
+    }
     if ((e = hipMemsetAsync(dCounters_.ptr, 0, 8 * sizeof(unsigned long long), stream_)) != hipSuccess) return hipFail(e, "memset");
     (void)hipEventRecord(ev_[0], stream_);
-    if ((e = launchFc(scene_->view(), cam.desc, params_.seed, dFilm_.as<float>(), dIds_.as<int32_t>(), dTie_.as<uint8_t>(),
+    if ((e = launchFc(view, cam.desc, params_.seed, dFilm_.as<float>(), dIds_.as<int32_t>(), dTie_.as<uint8_t>(),
                       dStack_.as<uint32_t>(), dCounters_.as<unsigned long long>(), blocks, stream_)) != hipSuccess)
         return hipFail(e, "fc kernel launch");
     (void)hipEventRecord(ev_[1], stream_);

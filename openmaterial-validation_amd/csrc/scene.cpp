@@ -408,11 +408,38 @@ int RenderScene::setBackgroundConstant(const float rgb[3]) {
 }
 
 int RenderScene::setFalseColorSubject(const char* name) {
-    static const char* names[] = {"", "pid", "gid", "mid", "rmp", "mmp", "md", "sg", "mn", "in"};
+    static const char* names[] = {"", "pid", "gid", "mid", "rmp", "mmp", "md", "sg", "mn", "in", "iid", "ipid"};
     fcSubject_ = OMV_FC_NONE;   // unknown names get the empty subject (FCHandler.cpp:87-95)
-    for (int i = 1; i < 10; ++i) if (std::strcmp(name, names[i]) == 0) fcSubject_ = i;
+    for (int i = 1; i < 12; ++i) if (std::strcmp(name, names[i]) == 0) fcSubject_ = i;
     committed_ = false;
     return OMV_OK;
+}
+
+// Category sizes of RenderScene::setFalseColorSubject (RenderScene.cpp:
+// 130-146): pid = the faces of all meshes, gid = meshes, mid = allocated
+// materials; iid = instances and ipid = faces over all instances (the
+// instance-aware subjects).
+uint64_t RenderScene::fcCategory(int subject, std::vector<uint32_t>& instBase) const {
+    instBase.clear();
+    switch (subject) {
+        case OMV_FC_PID: {
+            uint64_t n = 0;
+            for (const HostMesh& m : meshes_) n += (uint64_t)m.fn;
+            return n;
+        }
+        case OMV_FC_GID: return meshes_.size();
+        case OMV_FC_MID: return mats_.size() - 1;
+        case OMV_FC_IID: return insts_.size();
+        case OMV_FC_IPID: {
+            uint64_t n = 0;
+            for (const HostInstance& in : insts_) {
+                instBase.push_back((uint32_t)std::min<uint64_t>(n, 0xFFFFFFFFu));
+                if (in.meshId >= 0 && in.meshId < (int)meshes_.size() && meshes_[in.meshId].valid) n += (uint64_t)meshes_[in.meshId].fn;
+            }
+            return n;
+        }
+        default: return 0;
+    }
 }
 
 // RenderScene::commitScene (RenderScene.cpp:155-223)

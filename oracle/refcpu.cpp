@@ -83,6 +83,7 @@
 #include <string>
 #include <mutex>
 #include <thread>
+#include <set>
 #include <vector>
 
 namespace refcpu {
@@ -1445,6 +1446,8 @@ struct Tracer {
     uint64_t rays = 0, camRays = 0, lightRays = 0, shadowRays = 0;
     Counters cnt;
     Tracer(const Scene& s, const omv_renderer_params& p) : S(s), P(p), X(p) {}
+    /* false-colour table of an enumerated subject, first instance-face index per instance (ipid) */
+    std::vector<uint32_t> fcTable, fcInstBase;
 
     /* sceneIntersect (BDPTRenderer.cpp:75-102); tnear persists in the caller */
     bool sceneIntersect(const V3<R>& org, const V3<R>& dir, R& tnear, R tfar, SurfacePoint<R>& sp) {
@@ -1909,8 +1912,9 @@ struct Tracer {
             a = clampv(a, R(0), R(1));
             return mk(lerpv(R(1), hi.x, a), lerpv(R(1), hi.y, a), lerpv(R(1), hi.z, a));
         };
-        auto enumColor = [&](uint32_t id, uint32_t size, bool midStyle) {
-            uint32_t c = id < size ? (((id + 1u) * 0x9E3779B1u) & 0xFFFFFFu) : 0u;
+        auto enumColor = [&](uint32_t id, uint32_t /*size*/, bool midStyle) {
+            /* EnumeratedFCSubject::paint (FCSubject.h:172-177) */
+            uint32_t c = id < fcTable.size() ? fcTable[id] : 0u;
             R r = R((float)((c >> 16) & 0xFF) / 255.0f), g = R((float)((c >> 8) & 0xFF) / 255.0f), b = R((float)(c & 0xFF) / 255.0f);
             return midStyle ? mk(R(0), g, b) : mk(r, g, b);
         };
@@ -1926,6 +1930,8 @@ struct Tracer {
         switch (sub) {
             case OMV_FC_PID: col = enumColor((uint32_t)sp.globalFace, (uint32_t)S.faceBase.back(), false); break;
             case OMV_FC_GID: col = enumColor((uint32_t)sp.mesh, (uint32_t)S.meshes.size(), false); break;
+            case OMV_FC_IID: col = enumColor((uint32_t)sp.inst, (uint32_t)S.instances.size(), false); break;
+            case OMV_FC_IPID: col = enumColor(fcInstBase[sp.inst] + (uint32_t)sp.face, 0u, false); break;
             case OMV_FC_MID:
                 if (isMissing) col = mk(R(1000), R(0), R(1000));
                 else col = enumColor((uint32_t)mesh.matId, (uint32_t)S.materials.size(), true);
@@ -2270,9 +2276,9 @@ int ref_scene_set_background_constant(ref_scene* r, const float rgb[3]) {
 
 int ref_scene_set_false_color_subject(ref_scene* r, const char* name) {
     if (!r || !name) return OMV_E_ARG;
-    static const char* names[] = {"", "pid", "gid", "mid", "rmp", "mmp", "md", "sg", "mn", "in"};
+    static const char* names[] = {"", "pid", "gid", "mid", "rmp", "mmp", "md", "sg", "mn", "in", "iid", "ipid"};
     r->s.fcSubject = OMV_FC_NONE;
-    for (int i = 1; i < 10; ++i) if (std::strcmp(name, names[i]) == 0) r->s.fcSubject = i;
+    for (int i = 1; i < 12; ++i) if (std::strcmp(name, names[i]) == 0) r->s.fcSubject = i;
     return OMV_OK;
 }
 
@@ -2523,12 +2529,70 @@ int ref_render_samples(ref_scene* r, const omv_renderer_params* p, const omv_cam
     return OMV_OK;
 }
 
+/* Colour table of an enumerated false-colour subject: EnumeratedFCSubject's
+ * construction (FCSubject.h:153-167) -- `size` distinct colours drawn
+ * uniformly from [1, 0xFFFFFF] into a std::set (ascending), then shuffled --
+ * with the reference's unseeded draws and std::random_shuffle replaced by one
+ * splitmix64 sequence (Steele, Lea, Flood 2014) started at
+ * (seed << 8 | subject): v = 1 + x mod 0xFFFFFF until size distinct values,
+ * then for i = size down to 2: swap(t[i-1], t[x mod i]). */
+static bool oracleFcTable(uint32_t seed, int subject, uint64_t size, std::vector<uint32_t>& out) {
+    out.clear();
+    if (size > 0xFFFFFFu) return false;
+    uint64_t st = ((uint64_t)seed << 8) | (uint64_t)(subject & 0xFF);
+    auto next = [&st]() {
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    std::set<uint32_t> colours;
+    while (colours.size() < size) colours.insert(1u + (uint32_t)(next() % 0xFFFFFFull));
+    out.assign(colours.begin(), colours.end());
+    for (uint64_t i = size; i > 1; --i) std::swap(out[i - 1], out[next() % i]);
+    return true;
+}
+
+/* category sizes of RenderScene::setFalseColorSubject (RenderScene.cpp:130-146)
+ * plus the instance-aware iid (instances) and ipid (faces over instances) */
+static uint64_t oracleFcCategory(const Scene& S, std::vector<uint32_t>& instBase) {
+    instBase.clear();
+    switch (S.fcSubject) {
+        case OMV_FC_PID: return (uint64_t)S.faceBase.back();
+        case OMV_FC_GID: return S.meshes.size();
+        case OMV_FC_MID: return S.materials.size() - 1;
+        case OMV_FC_IID: return S.instances.size();
+        case OMV_FC_IPID: {
+            uint64_t n = 0;
+            for (const Instance& in : S.instances) {
+                instBase.push_back((uint32_t)n);
+                if (in.meshId >= 0 && in.meshId < (int)S.meshes.size()) n += (uint64_t)S.meshes[in.meshId].fn;
+            }
+            return n;
+        }
+        default: return 0;
+    }
+}
+
+int ref_fc_color_table(uint32_t seed, int subject, uint32_t size, uint32_t* out) {
+    if (size && !out) return OMV_E_ARG;
+    std::vector<uint32_t> t;
+    if (!oracleFcTable(seed, subject, size, t)) return OMV_E_RANGE;
+    for (uint32_t i = 0; i < size; ++i) out[i] = t[i];
+    return OMV_OK;
+}
+
 /* FC mode (main.cpp:261-270): spp = 1, one camera bounce, no lights, no background */
 int ref_render_fc(ref_scene* r, const omv_renderer_params* p, const omv_camera_desc* cam, float* film, int32_t* ids, uint8_t* tie) {
     if (!r || !p || !cam || !r->s.committed) return OMV_E_ARG;
     const Scene& S = r->s;
     const int W = cam->width, H = cam->height;
     Tracer<float> T(S, *p);
+    {
+        const uint64_t n = oracleFcCategory(S, T.fcInstBase);
+        if (!oracleFcTable(p->seed, S.fcSubject, n, T.fcTable)) return OMV_E_RANGE;
+    }
     std::vector<Vtx<float>> cp(3);
     for (int y = 0; y < H; ++y)
         for (int x = 0; x < W; ++x) {

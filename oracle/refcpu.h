@@ -54,6 +54,7 @@ int  ref_render(ref_scene* s, const omv_renderer_params* p, const omv_camera_des
 int  ref_render_samples(ref_scene* s, const omv_renderer_params* p, const omv_camera_desc* cam, int fp64,
                         int row0, int rows, int threads, float* rad);
 /* False-colour pass, see omv_renderer_render_fc. */
+int  ref_fc_color_table(uint32_t seed, int subject, uint32_t size, uint32_t* out);
 int  ref_render_fc(ref_scene* s, const omv_renderer_params* p, const omv_camera_desc* cam,
                    float* film, int32_t* ids, uint8_t* tie);
 

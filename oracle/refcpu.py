@@ -71,6 +71,7 @@ def lib():
         "ref_render": ([vp, C.POINTER(RendererParams), C.POINTER(CameraDesc), C.c_int, C.c_int, C.c_int, C.c_int, FP,
                         C.POINTER(RenderStats)], C.c_int),
         "ref_render_samples": ([vp, C.POINTER(RendererParams), C.POINTER(CameraDesc), C.c_int, C.c_int, C.c_int, C.c_int, FP], C.c_int),
+        "ref_fc_color_table": ([C.c_uint32, C.c_int, C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
         "ref_render_fc": ([vp, C.POINTER(RendererParams), C.POINTER(CameraDesc), FP, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)], C.c_int),
         "ref_closest_hit": ([vp, FP, FP, C.c_float, C.c_float, C.c_int, FP, FP, FP, C.POINTER(C.c_int), C.POINTER(C.c_int),
                              C.POINTER(C.c_int), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),

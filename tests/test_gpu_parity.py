@@ -40,7 +40,7 @@ def _radiance_close(img, rimg, frac_min=0.99, rel=1e-3):
     return frac
 
 
-@pytest.mark.parametrize("subject", ["pid", "gid", "mid", "mn", "in", "rmp", "mmp", "md", "sg"])
+@pytest.mark.parametrize("subject", ["pid", "gid", "mid", "mn", "in", "rmp", "mmp", "md", "sg", "iid", "ipid"])
 def test_fc_pass_bitexact(pkg, wl, ref, subject):
     desc, view = wl.mixed_test_scene(hdr=False, lights=False, masked=True)
     view.width, view.height = 96, 72
@@ -61,6 +61,50 @@ def test_fc_pass_bitexact(pkg, wl, ref, subject):
         np.testing.assert_allclose(film, rfilm, atol=1e-5, rtol=0)
     else:
         assert np.array_equal(film, rfilm)
+
+
+@pytest.mark.parametrize("subject", ["pid", "ipid", "iid"])
+def test_fc_instance_aware_colours(pkg, wl, ref, subject):
+    """pid paints every instance of a mesh alike (per-mesh face ids, as the
+    reference); ipid numbers faces over instances and iid paints instances:
+    the colour at every hit pixel is the seeded table entry of that id."""
+    desc, view, _ = wl.large_instanced(width=160, height=90, count=24, tris_per=2000)
+    desc.instances = [(tm, m % 3) for tm, m in desc.instances]     # 24 instances of 3 meshes
+    desc.fc_subject = subject
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = pkg.Camera(view.width, view.height, view.position, view.forward, view.up, view.yfov)
+    r = pkg.Renderer(pkg.default_params(seed=12))
+    r.set_scene(s)
+    film, ids, _ = r.render_fc(cam)
+    pid, mesh, inst = ids[0], ids[1], ids[3]
+    hit = pid >= 0
+    fn = np.array([len(m["faces"]) for m in desc.meshes])
+    mesh_base = np.concatenate([[0], np.cumsum(fn)])
+    inst_fn = np.array([fn[m] for _, m in desc.instances])
+    inst_base = np.concatenate([[0], np.cumsum(inst_fn)])
+    if subject == "pid":
+        key, size = pid, int(fn.sum())
+    elif subject == "iid":
+        key, size = inst, len(desc.instances)
+    else:
+        key, size = inst_base[np.maximum(inst, 0)] + (pid - mesh_base[np.maximum(mesh, 0)]), int(inst_fn.sum())
+    table = pkg.fc_color_table(12, {"pid": 1, "iid": 10, "ipid": 11}[subject], size)
+    c = table[key[hit]]
+    want = np.stack([(c >> 16) & 255, (c >> 8) & 255, c & 255], -1).astype(np.float32) / np.float32(255.0)
+    assert np.array_equal(film[..., :3][hit], want)
+    assert np.all(film[..., :3][~hit] == 0)
+    assert np.unique(inst[hit]).size >= 4 and np.unique(mesh[hit]).size == 3   # instances share meshes in view
+    if subject == "ipid":   # the same face of two instances of one mesh: one pid, two ipid colours
+        pairs = {}
+        for p_, i_, c_ in zip(pid[hit], inst[hit], c):
+            pairs.setdefault(p_, {})[i_] = c_
+        shared = [v for v in pairs.values() if len(v) > 1]
+        assert shared and all(len(set(v.values())) == len(v) for v in shared)
+    rs = ref.Scene()
+    desc.apply(rs, ref.pbr_desc)
+    rfilm, rids, _ = rs.render_fc(ref.default_params(seed=12), cam.desc())
+    assert np.array_equal(film, rfilm) and np.array_equal(ids, rids)
 
 
 def test_fc_ids_large_instanced(pkg, wl, ref):

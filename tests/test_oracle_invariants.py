@@ -273,11 +273,15 @@ def test_fc_global_face_ids(ref, wl):
     assert np.array_equal(inst_mesh[ids[3][hit]], mesh)
     mats = np.array([m["mat_id"] if m["mat_id"] < len(desc.materials) else -1 for m in desc.meshes])
     assert np.array_equal(mats[mesh], ids[2][hit])
-    # the pid colour is the project's fixed table ((id + 1) * 0x9E3779B1) & 0xFFFFFF
-    y, x = np.argwhere(hit)[0]
-    cid = ((int(ids[0][y, x]) + 1) * 0x9E3779B1) & 0xFFFFFF
-    exp = f32([((cid >> 16) & 255) / 255.0, ((cid >> 8) & 255) / 255.0, (cid & 255) / 255.0])
-    assert np.array_equal(film[y, x, :3], exp)
+    # the pid colour is the seeded colour table's entry of the id (params.seed)
+    import ctypes as C
+    n = int(base[-1])
+    table = np.zeros(n, np.uint32)
+    assert ref.lib().ref_fc_color_table(ref.default_params().seed, 1, n, table.ctypes.data_as(C.POINTER(C.c_uint32))) == 0
+    for y, x in np.argwhere(hit)[:50]:
+        cid = int(table[ids[0][y, x]])
+        exp = f32([((cid >> 16) & 255) / 255.0, ((cid >> 8) & 255) / 255.0, (cid & 255) / 255.0])
+        assert np.array_equal(film[y, x, :3], exp)
 
 
 # ------------------------------------------------ estimator closed forms
