@@ -133,8 +133,12 @@ def measured_traffic(workload):
     return None
 
 
+BUILD_MODE = 0   # set from --build
+
+
 def scene_of(pkg, desc):
     s = pkg.Scene()
+    s.set_build_mode(BUILD_MODE)
     desc.apply(s, pkg.pbr_desc)
     return s
 
@@ -194,6 +198,8 @@ def bench_batched(pkg, wl, args):
     print(f"# sequential {len(jobs)} renders in {seq:.2f}s", file=sys.stderr, flush=True)
     jobs_s = len(jobs) * args.steps / el
     return {
+        "commit_s": round(commit_s, 3), "build": "device LBVH" if BUILD_MODE else "host SAH",
+        "jobs_per_s_incl_commit": round(len(jobs) / (commit_s + el / args.steps), 3),
         "metric": "batched validation jobs/s on 1 GPU (vs sequential plain launches)", "value": round(jobs_s, 3),
         "unit": "jobs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -397,9 +403,13 @@ def main():
                     help="equal-RMSE check of GPU and CPU images against the FP64 oracle at 16x spp on ROWS rows")
     ap.add_argument("--fd", type=int, default=0, help="material_fit: finite-difference check on this many texels")
     ap.add_argument("--autotune", action="store_true", help="batched_lightbox: run the autotuner instead of the fixed tuning")
+    ap.add_argument("--build", default="host", choices=["host", "device"],
+                    help="per-mesh BVH build at commit: host binned SAH (default) or device linear BVH")
     ap.add_argument("--estimator", default="reference", choices=["reference", "nee"],
                     help="hdri_torus_knot: the reference estimator, or the HDRI next-event-estimation extension")
     args = ap.parse_args()
+    global BUILD_MODE
+    BUILD_MODE = 1 if args.build == "device" else 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (no multi-GPU path in this build)", file=sys.stderr)
         sys.exit(2)
@@ -434,6 +444,8 @@ def main():
         cfg = {"workload": "large_instanced", "triangles": desc.triangles(), "resolution": [1920, 1080], "spp": 16, "depth": 8,
                "estimator": "reference"}
 
+    if BUILD_MODE:
+        cfg["bvh_build"] = "device LBVH"
     t0 = time.perf_counter()
     scene = scene_of(pkg, desc)
     commit_s = time.perf_counter() - t0

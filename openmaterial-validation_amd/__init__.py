@@ -24,6 +24,7 @@ OMV_OK = 0
 EXT_NONE = 0
 EXT_OM_ROUGH = 1
 EXT_HDRI_NEE = 2
+BUILD_HOST_SAH, BUILD_DEVICE_LBVH = 0, 1
 EXT_LIGHT_SPLAT = 4
 
 FC_SUBJECTS = ("pid", "gid", "mid", "rmp", "mmp", "md", "sg", "mn", "in")
@@ -222,6 +223,10 @@ class SceneApi:
         a = _f32(rgb, (3,))
         self._check(self._fn("scene_set_background_constant")(self.h, _ptr(a, C.c_float)), "scene_set_background_constant")
 
+    def set_build_mode(self, mode: int):
+        """omv_scene_set_build_mode: BUILD_HOST_SAH (default) or BUILD_DEVICE_LBVH."""
+        self._check(self._fn("scene_set_build_mode")(self.h, int(mode)), "scene_set_build_mode")
+
     def set_false_color_subject(self, name):
         self._check(self._fn("scene_set_false_color_subject")(self.h, name.encode()), "scene_set_false_color_subject")
 
@@ -260,7 +265,7 @@ def _load():
         "omv_scene_set_light_point": [vp, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float],
         "omv_scene_set_background_hdr": [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float],
         "omv_scene_set_background_constant": [vp, C.POINTER(C.c_float)],
-        "omv_scene_set_false_color_subject": [vp, C.c_char_p],
+        "omv_scene_set_false_color_subject": [vp, C.c_char_p], "omv_scene_set_build_mode": [vp, C.c_int],
         "omv_scene_commit": [vp, C.c_char_p, C.c_size_t],
         "omv_scene_mesh_info": [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
         "omv_scene_get_vertices": [vp, C.c_int, C.POINTER(C.c_float), C.c_size_t],

@@ -113,6 +113,10 @@ int omv_scene_set_false_color_subject(omv_scene* s, const char* name) {
     OMV_NULLCHK(s); OMV_NULLCHK(name);
     return s->s.setFalseColorSubject(name);
 }
+int omv_scene_set_build_mode(omv_scene* s, int mode) {
+    OMV_NULLCHK(s);
+    return s->s.setBuildMode(mode);
+}
 int omv_scene_commit(omv_scene* s, char* log, size_t log_cap) {
     OMV_NULLCHK(s);
     std::string lg;

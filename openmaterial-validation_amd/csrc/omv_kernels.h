@@ -68,6 +68,12 @@ hipError_t launchAdjoint(const OmvSceneView& S, const omv_camera_desc& cam, cons
 hipError_t launchPost(const float* film, float* tmp, float* outF, uint8_t* outU8, int w, int h, float A, float gamma, bool denoise,
                       hipStream_t stream);
 // Environment alias tables (env.hip); scratch holds 4*W*H + 2*H + 2 words.
+// bvh_device.hip: one mesh's BLAS built on the device (linear BVH collapsed
+// into OmvNode4) into nodes[nodeBase ...] and records[12 * recBase ...]
+size_t blasDeviceScratchBytes(uint32_t n);
+hipError_t buildBlasDevice(const float* vtx, uint32_t vtxOff, uint32_t stride, const uint32_t* faces, uint32_t faceOff, uint32_t n,
+                           int leafMax, int recBase, int nodeBase, uint32_t nodeCap, OmvNode4* nodes, float* records, void* scratch,
+                           hipStream_t stream, int* root, int* nodeCount, int* depth);
 hipError_t launchEnvBuild(const float* bg, int W, int H, float* rowProb, int* rowAlias, float* colProb, int* colAlias, float* texPdf,
                           void* scratch, hipStream_t stream);
 // 0: sample kernel (scene with lights), 1: FC kernel, 2: sample kernel (no lights), 3: batch kernel;

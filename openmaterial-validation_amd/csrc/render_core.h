@@ -71,6 +71,7 @@ public:
     int setBackgroundHdr(int w, int h, const float* rgb, const float rot[9], float scale);
     int setBackgroundConstant(const float rgb[3]);
     int setFalseColorSubject(const char* name);
+    int setBuildMode(int mode);
     int commit(std::string& log);
 
     bool committed() const { return committed_; }
@@ -101,6 +102,7 @@ private:
     float bgRot_[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     float bgScale_ = 1.0f, bgConst_[3] = {100.0f, 100.0f, 100.0f};
     int fcSubject_ = OMV_FC_NONE;
+    int buildMode_ = OMV_BUILD_HOST_SAH;
     bool committed_ = false;
     int nodeCount_ = 0, bvhDepth_ = 0, recordCount_ = 0, stackBound_ = 0;
     size_t deviceBytes_ = 0;

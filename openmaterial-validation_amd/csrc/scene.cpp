@@ -415,6 +415,13 @@ int RenderScene::setFalseColorSubject(const char* name) {
     return OMV_OK;
 }
 
+int RenderScene::setBuildMode(int mode) {
+    if (mode != OMV_BUILD_HOST_SAH && mode != OMV_BUILD_DEVICE_LBVH) { setError("set_build_mode: unknown mode"); return OMV_E_ARG; }
+    buildMode_ = mode;
+    committed_ = false;
+    return OMV_OK;
+}
+
 // Category sizes of RenderScene::setFalseColorSubject (RenderScene.cpp:
 // 130-146): pid = the faces of all meshes, gid = meshes, mid = allocated
 // materials; iid = instances and ipid = faces over all instances (the
@@ -517,6 +524,9 @@ int RenderScene::commit(std::string& log) {
     std::vector<OmvNode4> nodes;
     std::vector<float> tris;
     int faceBase = 0, blasDepth = 0;
+    const bool deviceBuild = buildMode_ == OMV_BUILD_DEVICE_LBVH;
+    std::vector<int> meshRec(meshes_.size(), -1);
+    int recTotal = 0;
     for (size_t i = 0; i < meshes_.size(); ++i) {
         const HostMesh& m = meshes_[i];
         OmvMesh& d = dmeshes[i];
@@ -534,6 +544,11 @@ int RenderScene::commit(std::string& log) {
         if (!m.valid) continue;
         vtx.insert(vtx.end(), m.V.begin(), m.V.end());
         faces.insert(faces.end(), m.F.begin(), m.F.end());
+        if (deviceBuild) {   // built on the GPU below, records in mesh order
+            meshRec[i] = recTotal;
+            recTotal += m.fn;
+            continue;
+        }
         std::vector<BuildBox> boxes(m.fn);
         for (int f = 0; f < m.fn; ++f) {
             BuildBox& b = boxes[f];
@@ -558,7 +573,40 @@ int RenderScene::commit(std::string& log) {
             }
         }
     }
-    if (tris.size() / 12 >= (1u << 27)) { setError("commit: more than 2^27 triangles"); return OMV_E_SCENE; }
+    if ((deviceBuild ? (size_t)recTotal : tris.size() / 12) >= (1u << 27)) { setError("commit: more than 2^27 triangles"); return OMV_E_SCENE; }
+    // device BLAS build: vertex and face pools first, then one build per mesh
+    std::vector<OmvNode4> blasRoots(meshes_.size());
+    int deviceNodes = 0;
+    if (deviceBuild) {
+        hipError_t e;
+        if ((e = dVtx_.upload(vtx.data(), vtx.size() * 4)) != hipSuccess) return hipFail(e, "upload vertices");
+        if ((e = dFaces_.upload(faces.data(), faces.size() * 4)) != hipSuccess) return hipFail(e, "upload faces");
+        size_t nodeCap = 0, scratch = 0;
+        for (size_t i = 0; i < meshes_.size(); ++i)
+            if (meshRec[i] >= 0) {
+                nodeCap += (size_t)meshes_[i].fn;
+                scratch = std::max(scratch, blasDeviceScratchBytes((uint32_t)meshes_[i].fn));
+            }
+        const size_t tlasCap = std::max<size_t>(1, insts_.size());
+        if ((e = dNodes_.reserve((nodeCap + tlasCap) * sizeof(OmvNode4))) != hipSuccess) return hipFail(e, "BVH nodes");
+        if ((e = dTris_.reserve((size_t)recTotal * 12 * 4)) != hipSuccess) return hipFail(e, "triangle records");
+        DeviceBuffer work;
+        if ((e = work.reserve(scratch)) != hipSuccess) return hipFail(e, "BVH build scratch");
+        for (size_t i = 0; i < meshes_.size(); ++i) {
+            if (meshRec[i] < 0) continue;
+            const HostMesh& m = meshes_[i];
+            int root = -1, count = 0, depth = 0;
+            e = buildBlasDevice(dVtx_.as<float>(), dmeshes[i].vtx_offset, dmeshes[i].stride, dFaces_.as<uint32_t>(), dmeshes[i].face_offset,
+                                (uint32_t)m.fn, OMV_BVH_LEAF_MAX, meshRec[i], deviceNodes, (uint32_t)m.fn, dNodes_.as<OmvNode4>(),
+                                dTris_.as<float>(), work.ptr, nullptr, &root, &count, &depth);
+            if (e != hipSuccess) return hipFail(e, "device BVH build");
+            dmeshes[i].blas_root = root;
+            deviceNodes += count;
+            blasDepth = std::max(blasDepth, depth);
+            if ((e = hipMemcpy(&blasRoots[i], dNodes_.as<OmvNode4>() + root, sizeof(OmvNode4), hipMemcpyDeviceToHost)) != hipSuccess)
+                return hipFail(e, "BVH root readback");
+        }
+    }
     if (insts_.size() >= (1u << 29)) { setError("commit: too many instances"); return OMV_E_SCENE; }
     // TLAS over world-space instance boxes (transformed BLAS root boxes, rounded outward)
     std::vector<BuildBox> iboxes;
@@ -573,7 +621,7 @@ int RenderScene::commit(std::string& log) {
         std::memcpy(d.nrm, in.nrm, sizeof(d.nrm));
         d.mesh_id = in.meshId;
         if (in.meshId < 0) continue;
-        const OmvNode4& root = nodes[dmeshes[in.meshId].blas_root];
+        const OmvNode4& root = deviceBuild ? blasRoots[in.meshId] : nodes[dmeshes[in.meshId].blas_root];
         float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (int k = 0; k < 4; ++k) {
             if (root.child[k] == OMV_EMPTY_CHILD) continue;
@@ -616,6 +664,15 @@ int RenderScene::commit(std::string& log) {
             }
         }
     }
+    // device build: the TLAS (built above from index 0) goes after the BLAS nodes
+    if (deviceBuild) {
+        for (OmvNode4& nd : nodes)
+            for (int k = 0; k < 4; ++k)
+                if (nd.child[k] != OMV_EMPTY_CHILD && nd.child[k] >= 0) nd.child[k] += deviceNodes;
+        tl.root += deviceNodes;
+        hipError_t e = hipMemcpy(dNodes_.as<OmvNode4>() + deviceNodes, nodes.data(), nodes.size() * sizeof(OmvNode4), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hipFail(e, "upload TLAS");
+    }
     // traversal stack bound: 3 siblings per BVH4 level plus the instance markers
     stackBound_ = 3 * (tl.depth + blasDepth) + 4;
     if (stackBound_ > OMV_STACK_TOTAL) {
@@ -650,15 +707,19 @@ int RenderScene::commit(std::string& log) {
     texOffsets_.resize(dtex.size());
     for (size_t i = 0; i < dtex.size(); ++i) texOffsets_[i] = dtex[i].offset;
     hipError_t e;
-    if ((e = dVtx_.upload(vtx.data(), vtx.size() * 4)) != hipSuccess) return hipFail(e, "upload vertices");
-    if ((e = dFaces_.upload(faces.data(), faces.size() * 4)) != hipSuccess) return hipFail(e, "upload faces");
+    if (!deviceBuild) {
+        if ((e = dVtx_.upload(vtx.data(), vtx.size() * 4)) != hipSuccess) return hipFail(e, "upload vertices");
+        if ((e = dFaces_.upload(faces.data(), faces.size() * 4)) != hipSuccess) return hipFail(e, "upload faces");
+    }
     if ((e = dMeshes_.upload(dmeshes.data(), dmeshes.size() * sizeof(OmvMesh))) != hipSuccess) return hipFail(e, "upload meshes");
     if ((e = dInsts_.upload(dinsts.data(), dinsts.size() * sizeof(OmvInstance))) != hipSuccess) return hipFail(e, "upload instances");
     if ((e = dMats_.upload(mats_.data(), mats_.size() * sizeof(OmvMaterial))) != hipSuccess) return hipFail(e, "upload materials");
     if ((e = dTexs_.upload(dtex.data(), dtex.size() * sizeof(OmvTexture))) != hipSuccess) return hipFail(e, "upload textures");
     if ((e = dTexels_.upload(texels.data(), texels.size())) != hipSuccess) return hipFail(e, "upload texels");
-    if ((e = dNodes_.upload(nodes.data(), nodes.size() * sizeof(OmvNode4))) != hipSuccess) return hipFail(e, "upload nodes");
-    if ((e = dTris_.upload(tris.data(), tris.size() * 4)) != hipSuccess) return hipFail(e, "upload triangles");
+    if (!deviceBuild) {
+        if ((e = dNodes_.upload(nodes.data(), nodes.size() * sizeof(OmvNode4))) != hipSuccess) return hipFail(e, "upload nodes");
+        if ((e = dTris_.upload(tris.data(), tris.size() * 4)) != hipSuccess) return hipFail(e, "upload triangles");
+    }
     if ((e = dLights_.upload(lights_.data(), lights_.size() * sizeof(OmvLight))) != hipSuccess) return hipFail(e, "upload lights");
     if ((e = dCdf_.upload(cdf.data(), cdf.size() * 4)) != hipSuccess) return hipFail(e, "upload light cdf");
     if ((e = dBg_.upload(bgRgb_.data(), bgRgb_.size() * 4)) != hipSuccess) return hipFail(e, "upload background");
@@ -707,9 +768,9 @@ int RenderScene::commit(std::string& log) {
         v.env_row_prob = rp; v.env_row_alias = ra; v.env_col_prob = cp; v.env_col_alias = ca; v.env_tex_pdf = tp;
     }
 
-    nodeCount_ = (int)nodes.size();
+    nodeCount_ = (int)nodes.size() + deviceNodes;
     bvhDepth_ = tl.depth + blasDepth;
-    recordCount_ = (int)(tris.size() / 12);
+    recordCount_ = deviceBuild ? recTotal : (int)(tris.size() / 12);
     deviceBytes_ = dVtx_.bytes + dFaces_.bytes + dMeshes_.bytes + dInsts_.bytes + dMats_.bytes + dTexs_.bytes + dTexels_.bytes +
                    dNodes_.bytes + dTris_.bytes + dLights_.bytes + dCdf_.bytes + dBg_.bytes + dEnv_.bytes;
     committed_ = true;
