@@ -29,6 +29,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "device_sort.h"
 #include "omv_kernels.h"
 
 namespace {
@@ -119,50 +120,6 @@ __global__ void lbMorton(const float* __restrict__ cen, const float* __restrict_
     }
     key[f] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
     val[f] = f;
-}
-
-// 2b. radix sort: digit counts per chunk, digit-major [digit][chunk]
-__global__ void lbHist(const uint32_t* __restrict__ key, uint32_t n, int shift, uint32_t nchunks, uint32_t* __restrict__ hist) {
-    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= nchunks) return;
-    const uint32_t end = min(n, (c + 1) * (uint32_t)kChunk);
-    for (uint32_t i = c * kChunk; i < end; ++i) {
-        const uint32_t d = (key[i] >> shift) & 0xFFu;
-        hist[(size_t)d * nchunks + c] += 1u;
-    }
-}
-
-// exclusive scan of m counts in one block (each lane scans a contiguous span)
-__global__ void lbScan(uint32_t* __restrict__ a, uint32_t m) {
-    __shared__ uint32_t sums[kBlock];
-    const uint32_t per = (m + kBlock - 1) / kBlock;
-    const uint32_t b = threadIdx.x * per, e = min(m, b + per);
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += a[i];
-    sums[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (int t = 0; t < kBlock; ++t) { const uint32_t x = sums[t]; sums[t] = acc; acc += x; }
-    }
-    __syncthreads();
-    uint32_t acc = sums[threadIdx.x];
-    for (uint32_t i = b; i < e; ++i) { const uint32_t x = a[i]; a[i] = acc; acc += x; }
-}
-
-// stable scatter: each lane walks its chunk in order
-__global__ void lbScatter(const uint32_t* __restrict__ key, const uint32_t* __restrict__ val, uint32_t n, int shift, uint32_t nchunks,
-                          uint32_t* __restrict__ offs, uint32_t* __restrict__ keyOut, uint32_t* __restrict__ valOut) {
-    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= nchunks) return;
-    const uint32_t end = min(n, (c + 1) * (uint32_t)kChunk);
-    for (uint32_t i = c * kChunk; i < end; ++i) {
-        const uint32_t d = (key[i] >> shift) & 0xFFu;
-        const uint32_t o = offs[(size_t)d * nchunks + c];
-        offs[(size_t)d * nchunks + c] = o + 1u;
-        keyOut[o] = key[i];
-        valOut[o] = val[i];
-    }
 }
 
 // 3. radix tree; codes extended by the sorted position make them unique
@@ -374,16 +331,9 @@ hipError_t buildBlasDevice(const float* vtx, uint32_t vtxOff, uint32_t stride, c
     lbBoxes<<<nb, kBlock, 0, stream>>>(vtx, vtxOff, stride, faces, faceOff, n, triBox, cen, part);
     lbBounds<<<1, kBlock, 0, stream>>>(part, nb, bounds);
     lbMorton<<<nb, kBlock, 0, stream>>>(cen, bounds, n, keyA, valA);
-    uint32_t* kIn = keyA; uint32_t* vIn = valA; uint32_t* kOut = keyB; uint32_t* vOut = valB;
-    for (int shift = 0; shift < 32; shift += 8) {
-        if ((e = hipMemsetAsync(hist, 0, (size_t)256 * nc * 4, stream)) != hipSuccess) return e;
-        lbHist<<<blocks(nc), kBlock, 0, stream>>>(kIn, n, shift, nc, hist);
-        lbScan<<<1, kBlock, 0, stream>>>(hist, 256u * nc);
-        lbScatter<<<blocks(nc), kBlock, 0, stream>>>(kIn, vIn, n, shift, nc, hist, kOut, vOut);
-        std::swap(kIn, kOut);
-        std::swap(vIn, vOut);
-    }
-    // after four passes the sorted keys are back in keyA / valA; the root has no parent
+    if ((e = sortPairs32(keyA, valA, keyB, valB, hist, n, stream)) != hipSuccess) return e;
+    uint32_t* kIn = keyA; uint32_t* vIn = valA;
+    // the root has no parent
     if ((e = hipMemsetAsync(parentNode, 0xFF, (size_t)n * 4, stream)) != hipSuccess) return e;
     if (n > 1) lbTree<<<blocks(n - 1), kBlock, 0, stream>>>(kIn, (int)n, left, right, first, last, parentNode, parentLeaf);
     if ((e = hipMemsetAsync(arrived, 0, (size_t)n * 4, stream)) != hipSuccess) return e;

@@ -74,6 +74,10 @@ size_t blasDeviceScratchBytes(uint32_t n);
 hipError_t buildBlasDevice(const float* vtx, uint32_t vtxOff, uint32_t stride, const uint32_t* faces, uint32_t faceOff, uint32_t n,
                            int leafMax, int recBase, int nodeBase, uint32_t nodeCap, OmvNode4* nodes, float* records, void* scratch,
                            hipStream_t stream, int* root, int* nodeCount, int* depth);
+// mesh_device.hip: commitMesh's normals / tangents of one mesh on the device
+size_t meshDeviceScratchBytes(uint32_t vn, uint32_t fn);
+hipError_t preprocessMeshDevice(float* vertices, uint32_t stride, uint32_t vn, const uint32_t* faces, uint32_t fn, bool normals,
+                                bool tangents, int uvOff, void* scratch, hipStream_t stream, uint32_t* badIndices);
 hipError_t launchEnvBuild(const float* bg, int W, int H, float* rowProb, int* rowAlias, float* colProb, int* colAlias, float* texPdf,
                           void* scratch, hipStream_t stream);
 // 0: sample kernel (scene with lights), 1: FC kernel, 2: sample kernel (no lights), 3: batch kernel;

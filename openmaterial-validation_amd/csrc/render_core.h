@@ -103,6 +103,8 @@ private:
     float bgScale_ = 1.0f, bgConst_[3] = {100.0f, 100.0f, 100.0f};
     int fcSubject_ = OMV_FC_NONE;
     int buildMode_ = OMV_BUILD_HOST_SAH;
+    std::vector<uint32_t> devVtxOffset_;   // device build: meshes whose normals / tangents live only on the device
+    std::vector<char> devPrepared_;
     bool committed_ = false;
     int nodeCount_ = 0, bvhDepth_ = 0, recordCount_ = 0, stackBound_ = 0;
     size_t deviceBytes_ = 0;

@@ -452,6 +452,10 @@ uint64_t RenderScene::fcCategory(int subject, std::vector<uint32_t>& instBase) c
 // RenderScene::commitScene (RenderScene.cpp:155-223)
 int RenderScene::commit(std::string& log) {
     const int M = (int)mats_.size();
+    std::vector<char> prepNormals(meshes_.size(), 0), prepTangents(meshes_.size(), 0);
+    std::vector<int> prepUv(meshes_.size(), -1);
+    devPrepared_.assign(meshes_.size(), 0);
+    devVtxOffset_.assign(meshes_.size(), 0);
     for (int i = 0; i < (int)meshes_.size(); ++i) {
         HostMesh& m = meshes_[i];
         int normCh = -1;
@@ -473,6 +477,14 @@ int RenderScene::commit(std::string& log) {
             const int keep = m.matId;
             m = HostMesh();
             m.matId = keep;
+            continue;
+        }
+        if (buildMode_ == OMV_BUILD_DEVICE_LBVH) {   // normals / tangents on the device after the upload
+            const int slot = (normCh >= 0 && normCh < OMV_MAX_UV_CHANNELS && m.texMap[normCh] >= 0) ? m.texMap[normCh] : -1;
+            prepNormals[i] = !m.normalDef;
+            prepTangents[i] = !m.tangentDef;
+            prepUv[i] = (slot >= 0 && m.texDef[slot]) ? m.texOffs[slot] : -1;
+            m.normalDef = m.tangentDef = true;
             continue;
         }
         if (!m.normalDef) computeNormals(m);
@@ -590,8 +602,23 @@ int RenderScene::commit(std::string& log) {
         const size_t tlasCap = std::max<size_t>(1, insts_.size());
         if ((e = dNodes_.reserve((nodeCap + tlasCap) * sizeof(OmvNode4))) != hipSuccess) return hipFail(e, "BVH nodes");
         if ((e = dTris_.reserve((size_t)recTotal * 12 * 4)) != hipSuccess) return hipFail(e, "triangle records");
+        for (size_t i = 0; i < meshes_.size(); ++i)
+            if (meshRec[i] >= 0 && (prepNormals[i] || prepTangents[i]))
+                scratch = std::max(scratch, meshDeviceScratchBytes((uint32_t)meshes_[i].vn, (uint32_t)meshes_[i].fn));
         DeviceBuffer work;
         if ((e = work.reserve(scratch)) != hipSuccess) return hipFail(e, "BVH build scratch");
+        for (size_t i = 0; i < meshes_.size(); ++i) {   // commitMesh's normals / tangents
+            if (meshRec[i] < 0 || !(prepNormals[i] || prepTangents[i])) continue;
+            const HostMesh& m = meshes_[i];
+            uint32_t bad = 0;
+            e = preprocessMeshDevice(dVtx_.as<float>() + dmeshes[i].vtx_offset, dmeshes[i].stride, (uint32_t)m.vn,
+                                     dFaces_.as<uint32_t>() + dmeshes[i].face_offset, (uint32_t)m.fn, prepNormals[i] != 0,
+                                     prepTangents[i] != 0, prepUv[i], work.ptr, nullptr, &bad);
+            if (e != hipSuccess) return hipFail(e, "device mesh preprocessing");
+            if (bad) { setError("commit: face index out of range"); return OMV_E_INTERNAL; }
+            devPrepared_[i] = 1;
+            devVtxOffset_[i] = dmeshes[i].vtx_offset;
+        }
         for (size_t i = 0; i < meshes_.size(); ++i) {
             if (meshRec[i] < 0) continue;
             const HostMesh& m = meshes_[i];
@@ -833,6 +860,11 @@ int RenderScene::vertices(int id, float* out, size_t cap) const {
     if (id < 0 || id >= (int)meshes_.size()) { setError("vertices: id out of range"); return OMV_E_RANGE; }
     const HostMesh& m = meshes_[id];
     if (!out || cap < m.V.size()) { setError("vertices: buffer too small"); return OMV_E_ARG; }
+    if (committed_ && id < (int)devPrepared_.size() && devPrepared_[id]) {   // normals / tangents were made on the device
+        hipError_t e = hipMemcpy(out, dVtx_.as<float>() + devVtxOffset_[id], m.V.size() * sizeof(float), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { setError(std::string("vertices: ") + hipGetErrorString(e)); return OMV_E_DEVICE; }
+        return OMV_OK;
+    }
     std::memcpy(out, m.V.data(), m.V.size() * sizeof(float));
     return OMV_OK;
 }

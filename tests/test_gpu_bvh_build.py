@@ -132,3 +132,48 @@ def test_device_build_commit_time(pkg, wl):
         times[mode] = time.perf_counter() - t
     print(f"commit {len(f)} triangles: host SAH {times[0]:.3f} s, device LBVH {times[1]:.3f} s", flush=True)
     assert times[1] < times[0]
+
+
+def _prep_scene(wl, kind):
+    """Meshes that exercise commitMesh's normal / tangent paths."""
+    d = wl.SceneDesc()
+    d.materials.append(("pbr", dict(base_color=(0.8, 0.6, 0.4, 1.0), metallic=0.0, roughness=0.5)))
+    d.materials.append(("pbr", dict(base_color=(0.5, 0.5, 0.5, 1.0), metallic=0.2, roughness=0.4, textures={2: 0}, channels={2: 1})))
+    d.textures.append(("rgba8", np.full((4, 4, 4), 128, np.uint8)))
+    v, f, uv = wl.torus_knot(segments=300, tube=40)
+    if kind == "axis":            # no normals, no uv: computed normals, axis tangents
+        d.meshes.append({"vertices": v, "faces": f, "mat_id": 0})
+    elif kind == "uv":            # normal map on uv channel 1: uv tangents
+        d.meshes.append({"vertices": v, "faces": f, "mat_id": 1, "uvs": (uv, uv * 0.5 + 0.25)})
+    elif kind == "normals_given":
+        n = np.random.default_rng(3).normal(0, 1, v.shape).astype(np.float32)
+        d.meshes.append({"vertices": v, "faces": f, "mat_id": 1, "normals": n, "uvs": (uv, uv[::-1].copy())})
+    else:                         # degenerate faces, a vertex twice in a face, unused vertices
+        ff = f.copy()
+        ff[::7, 1] = ff[::7, 0]
+        ff[::11, 2] = ff[::11, 1]
+        vv = np.concatenate([v, np.zeros((5, 3), np.float32)])
+        uu = np.concatenate([uv, np.zeros((5, 2), np.float32)])
+        uu[::13] = uu[::13] * 0.0
+        d.meshes.append({"vertices": vv, "faces": ff, "mat_id": 1, "uvs": (uu, uu)})
+    d.instances.append((wl.trs((0.0, 0.0, -3.0)), 0))
+    d.background = ("const", (0.3, 0.3, 0.3))
+    return d
+
+
+@pytest.mark.parametrize("kind", ["axis", "uv", "normals_given", "degenerate"])
+def test_device_mesh_preprocessing_bitexact(pkg, wl, kind):
+    """commitMesh's normals and tangents made on the device equal the host's
+    bit for bit (per-vertex sums in face order)."""
+    desc = _prep_scene(wl, kind)
+    print(f"mesh preprocessing case {kind}", flush=True)
+    out = []
+    for mode in (pkg.BUILD_HOST_SAH, pkg.BUILD_DEVICE_LBVH):
+        s = pkg.Scene()
+        s.set_build_mode(mode)
+        desc.apply(s, pkg.pbr_desc)
+        out.append(s.mesh_vertices(0))
+    (a, ma), (b, mb) = out
+    assert ma == mb
+    assert np.array_equal(a, b), np.argwhere(a != b)[:5].tolist()
+    assert np.all(np.isfinite(a))
