@@ -352,6 +352,15 @@ typedef struct omv_render_tuning {
                                  (direction octant, origin cell of a 16^3
                                  grid over the scene) before each traversal:
                                  0 = default, 1 = never, 2 = always          */
+    int chunk_streams;        /* renders of several chunks: 2 (= default 0):
+                                 chunks alternate between two streams, so one
+                                 chunk's late bounces (few rays, bound by
+                                 traversal latency) run beside the next
+                                 chunk's first bounces; 1: chunks one after
+                                 another on one stream                        */
+    int chunk_stagger;        /* with two streams: a chunk starts once the
+                                 chunk before it has finished this many camera
+                                 rounds (0 = default: 1)                      */
 } omv_render_tuning;
 int omv_renderer_set_tuning(omv_renderer* r, const omv_render_tuning* t);
 int omv_renderer_get_tuning(const omv_renderer* r, omv_render_tuning* t);

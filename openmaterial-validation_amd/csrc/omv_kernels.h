@@ -53,6 +53,8 @@ struct WfLaunch {
     int* traceEvUsed;          // events recorded so far (in/out)
     float* splat;              // OMV_EXT_LIGHT_SPLAT sums (3 floats per pixel), or null
     bool sortRays;             // reorder secondary camera rays before each traversal
+    hipEvent_t roundEv;        // optional: recorded once camera round roundEvAt is
+    int roundEvAt;             // done (or at the end when there are fewer rounds)
 };
 size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat, bool sortRays);
 int wavefrontCounterCount(int B, int lvCap);

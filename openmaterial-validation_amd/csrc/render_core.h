@@ -148,7 +148,7 @@ private:
     int renderWavefront(PinholeCamera& cam, double& kernelMs);
     omv_renderer_params params_;
     int schedule_ = OMV_SCHEDULE_WAVEFRONT;
-    omv_render_tuning tuning_{0, 0, 0, 0, 0};
+    omv_render_tuning tuning_{0, 0, 0, 0, 0, 0, 0};
     OmvSceneView viewForLaunch() const;   // the scene view with the tuning's traversal order
     int wfPerCu_[5] = {0, 0, 0, 0, 0};
     RenderScene* scene_ = nullptr;
@@ -164,6 +164,12 @@ private:
     float* splat_ = nullptr;   // OMV_EXT_LIGHT_SPLAT sums of the current render, or null
     const PinholeCamera* filmOf_ = nullptr;   // camera whose film dFilm_ holds
     std::vector<hipEvent_t> traceEv_;
+    // the second chunk stream (chunk_streams 2): its stream, its own wavefront
+    // state, queue counters, sample buffer and spill stacks, and one event per
+    // stream marking where the next chunk may start
+    hipStream_t stream2_ = nullptr;
+    hipEvent_t chunkEv_[3] = {nullptr, nullptr, nullptr};
+    DeviceBuffer dArena2_, dQcount2_, dRad2_, dStack2_;
 };
 
 // Many BDPTRenderer::render jobs in one persistent launch, replayed from a

@@ -84,6 +84,8 @@ BDPTRenderer::BDPTRenderer() { omv_renderer_params_default(&params_); }
 BDPTRenderer::~BDPTRenderer() {
     for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
     for (auto& e : traceEv_) (void)hipEventDestroy(e);
+    for (auto& e : chunkEv_) if (e) (void)hipEventDestroy(e);
+    if (stream2_) (void)hipStreamDestroy(stream2_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -95,7 +97,7 @@ int BDPTRenderer::setSchedule(int s) {
 
 int BDPTRenderer::setTuning(const omv_render_tuning& t) {
     if (t.trace_keep < 0 || t.trace_keep > 64 || t.trace_blocks_per_cu < 0 || t.chunk_samples < 0 || t.traversal < 0 || t.traversal > 2 ||
-        t.sort_rays < 0 || t.sort_rays > 2) {
+        t.sort_rays < 0 || t.sort_rays > 2 || t.chunk_streams < 0 || t.chunk_streams > 2 || t.chunk_stagger < 0) {
         setError("set_tuning: value out of range");
         return OMV_E_RANGE;
     }
@@ -289,13 +291,45 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     }
     int evUsed = 0;
     L.traceEv = traceEv_.data(); L.traceEvCap = (int)(evNeed & ~(size_t)1); L.traceEvUsed = &evUsed;
+    // Two chunk streams (the default when there are several chunks): chunks
+    // alternate between stream_ and stream2_, each with its own state, and
+    // chunk k + 1 starts once chunk k has finished chunk_stagger camera rounds.
+    // Chunks write disjoint film rows and their own sample buffers, so the
+    // film is the same as with one stream.
+    const int nstreams = tuning_.chunk_streams != 1 && chunks > 1 ? 2 : 1;
+    hipStream_t st[2] = {stream_, stream_};
+    WfLaunch Ls[2] = {L, L};
+    float* rad[2] = {dRad_.as<float>(), dRad_.as<float>()};
+    if (nstreams == 2) {
+        if (!stream2_ && (e = hipStreamCreate(&stream2_)) != hipSuccess) return hipFail(e, "hipStreamCreate");
+        for (auto& ev : chunkEv_)
+            if (!ev && (e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return hipFail(e, "hipEventCreate");
+        if ((e = dStack2_.reserve(dStack_.bytes)) != hipSuccess) return hipFail(e, "stack spill");
+        if ((e = dArena2_.reserve(dArena_.bytes)) != hipSuccess) return hipFail(e, "wavefront state");
+        if ((e = dQcount2_.reserve(dQcount_.bytes)) != hipSuccess) return hipFail(e, "queue counters");
+        if ((e = dRad2_.reserve(slots * 3 * sizeof(float))) != hipSuccess) return hipFail(e, "sample buffer");
+        st[1] = stream2_;
+        Ls[1].arena = dArena2_.ptr; Ls[1].qcount = dQcount2_.as<uint32_t>(); Ls[1].rad = rad[1] = dRad2_.as<float>();
+        Ls[1].gstack = dStack2_.as<uint32_t>();
+        const int stagger = tuning_.chunk_stagger > 0 ? tuning_.chunk_stagger : 1;
+        for (int k = 0; k < 2; ++k) { Ls[k].roundEv = chunkEv_[k]; Ls[k].roundEvAt = stagger; }
+    }
     (void)hipEventRecord(ev_[2], stream_);
-    for (uint64_t pix0 = 0; pix0 < npix; pix0 += chunkPix) {
+    uint64_t k = 0;
+    for (uint64_t pix0 = 0; pix0 < npix; pix0 += chunkPix, ++k) {
+        const int j = (int)(k % (uint64_t)nstreams);
         const uint64_t np = std::min(chunkPix, npix - pix0);
-        L.sample0 = pix0 * (uint64_t)spp;
-        L.nslots = (uint32_t)(np * (uint64_t)spp);
-        if ((e = launchWavefront(L, stream_)) != hipSuccess) return hipFail(e, "wavefront launch");
-        if ((e = launchFilm(dRad_.as<float>(), spp, pix0, np, dFilm_.as<float>(), stream_)) != hipSuccess) return hipFail(e, "film kernel launch");
+        WfLaunch& Lk = Ls[j];
+        Lk.sample0 = pix0 * (uint64_t)spp;
+        Lk.nslots = (uint32_t)(np * (uint64_t)spp);
+        if (nstreams == 2 && k > 0 && (e = hipStreamWaitEvent(st[j], chunkEv_[1 - j], 0)) != hipSuccess) return hipFail(e, "hipStreamWaitEvent");
+        if ((e = launchWavefront(Lk, st[j])) != hipSuccess) return hipFail(e, "wavefront launch");
+        if ((e = launchFilm(rad[j], spp, pix0, np, dFilm_.as<float>(), st[j])) != hipSuccess) return hipFail(e, "film kernel launch");
+    }
+    if (nstreams == 2) {
+        // stream_ goes on (splat sum, read-back) once stream2_ has finished
+        (void)hipEventRecord(chunkEv_[2], stream2_);
+        if ((e = hipStreamWaitEvent(stream_, chunkEv_[2], 0)) != hipSuccess) return hipFail(e, "hipStreamWaitEvent");
     }
     (void)hipEventRecord(ev_[3], stream_);
     if ((e = hipEventSynchronize(ev_[3])) != hipSuccess) return hipFail(e, "wavefront kernels");

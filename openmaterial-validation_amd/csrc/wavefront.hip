@@ -920,7 +920,9 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
         } else {
             std::swap(in, out);
         }
+        if (L.roundEv && i == std::min(L.roundEvAt, B)) (void)hipEventRecord(L.roundEv, stream);
     }
+    if (L.roundEv && B < 1) (void)hipEventRecord(L.roundEv, stream);
     return hipGetLastError();
 }
 

@@ -82,10 +82,12 @@ def test_schedules_agree_instanced(pkg, wl):
     _same(_both(pkg, desc, view, **params))
 
 
+@pytest.mark.parametrize("streams", [1, 2])
 @pytest.mark.parametrize("lights", [False, True])
-def test_wavefront_multi_chunk(pkg, wl, lights):
+def test_wavefront_multi_chunk(pkg, wl, lights, streams):
     """Several wavefront chunks (chunk_samples not a multiple of spp): chunk
-    boundaries fall on whole pixels and the film is unchanged."""
+    boundaries fall on whole pixels and the film is unchanged, also with the
+    chunks alternating between two streams."""
     desc, view = wl.mixed_test_scene(hdr=True, lights=lights)
     view.width, view.height = 200, 130
     params = dict(samples_per_pixel=5, camera_bounces=3, light_bounces=2, max_path_length=4, seed=8)
@@ -98,7 +100,7 @@ def test_wavefront_multi_chunk(pkg, wl, lights):
         r.set_scene(s)
         r.set_schedule(sched)
         if sched == pkg.SCHEDULE_WAVEFRONT:
-            r.set_tuning(chunk_samples=20001)
+            r.set_tuning(chunk_samples=20001, chunk_streams=streams)
         out.append((r.render(cam), r.stats()))
     _same(out)
 
@@ -106,7 +108,9 @@ def test_wavefront_multi_chunk(pkg, wl, lights):
 @pytest.mark.parametrize("tuning", [dict(trace_keep=1), dict(trace_keep=32), dict(trace_keep=64),
                                     dict(trace_keep=48, trace_blocks_per_cu=1, chunk_samples=1000),
                                     dict(traversal=1), dict(traversal=2), dict(traversal=2, trace_blocks_per_cu=3),
-                                    dict(sort_rays=2), dict(sort_rays=2, chunk_samples=1000, traversal=1)])
+                                    dict(sort_rays=2), dict(sort_rays=2, chunk_samples=1000, traversal=1),
+                                    dict(chunk_streams=2, chunk_samples=1000), dict(chunk_streams=2, chunk_samples=777, chunk_stagger=1, trace_keep=32),
+                                    dict(chunk_streams=2, chunk_samples=1000, chunk_stagger=9, sort_rays=2)])
 def test_wavefront_tuning_agrees(pkg, wl, tuning):
     """Launch tuning (persistent traversal with refill, fewer resident blocks,
     small chunks) changes the schedule only: the film and the counts equal the

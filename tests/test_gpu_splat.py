@@ -91,7 +91,7 @@ def test_splat_schedules_and_chunks_agree(pkg, wl, name):
         desc.lights.append(((-2.5, -0.9, -2.0), (30.0, 30.0, 30.0), -1.0))   # its + octant faces the objects
     a, sa = _gpu(pkg, desc, view, PARAMS, 4, 8, schedule=pkg.SCHEDULE_WAVEFRONT)
     b, sb = _gpu(pkg, desc, view, PARAMS, 4, 8, schedule=pkg.SCHEDULE_PER_SAMPLE)
-    c, sc_ = _gpu(pkg, desc, view, PARAMS, 4, 8, tuning=dict(chunk_samples=5001))
+    c, sc_ = _gpu(pkg, desc, view, PARAMS, 4, 8, tuning=dict(chunk_samples=5001, chunk_streams=2))
     plain, _ = _gpu(pkg, desc, view, PARAMS, 0, 8)
     assert np.all(np.isfinite(a))
     assert not np.allclose(a, plain)   # the splats land on the image
