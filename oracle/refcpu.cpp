@@ -41,10 +41,18 @@
  *                                (unit normals, tangent orthogonality,
  *                                handedness sign) in tests.
  *   SurfacePoint::applyTextureNormal  SurfacePoint.cpp:19-34.  Pinned:
- *                                invariants (orthonormal frame, idempotence).
+ *                                closed forms for rotated frames of both
+ *                                handednesses and tilts about T and B;
+ *                                orthonormality; applied once per point.
  *   OpenMaterial ............... OpenMaterial.cpp:66-232.  Unbuildable TU.
  *                                Pinned: mirror closed form (throughput = F),
- *                                0.1 correction geometry, gate.
+ *                                gate; defineNextDirection by the mirror law,
+ *                                the 0.1 lift along the face normal (grazing,
+ *                                tilted shading normal, normal matrix of a
+ *                                non-uniform scale, reversed winding).
+ *   normal-map texel step ...... PhysicallyBasedMaterial.cpp:158-164.
+ *                                Pinned: the glTF 2.0 normalTexture rule and
+ *                                the blue floor.
  *   PhysicallyBasedMaterial .... PhysicallyBasedMaterial.cpp:27-316.
  *                                Unbuildable TU.  Pinned: evaluateDirect/
  *                                sampleIndirect golden vectors (the material
@@ -1189,16 +1197,24 @@ static void pbrFinish(const omv_pbr_desc& d, Bsdf<R>& b, const R omr[3]) {
     b.alpha = std::pow(b.rough, R(2));
 }
 
+/* tangent-space normal of a normal-map texel (PhysicallyBasedMaterial.cpp:158-164):
+ * blue lifted above 0.5, 2 * rgb - 1, x and y scaled, normalized */
+template <class R>
+static V3<R> texelNormal(const R texel[4], R scale) {
+    R tn[3] = {texel[0], texel[1], texel[2]};
+    if (tn[2] <= R(0.5f)) tn[2] = R(0.5f) + feps<R>();
+    for (int i = 0; i < 3; ++i) tn[i] = R(2) * tn[i] + R(-1);
+    for (int i = 0; i < 2; ++i) tn[i] = scale * tn[i];
+    return normalize3(mk(tn[0], tn[1], tn[2]));
+}
+
 /* PhysicallyBasedMaterial::modifyFrame (PhysicallyBasedMaterial.cpp:147-167) */
 template <class R>
 static void pbrModifyFrame(const Scene& S, SurfacePoint<R>& sp, const omv_pbr_desc& d) {
     R tn[4] = {R(0), R(0), R(1), R(0)};
     if (pbrTex(S, sp, d, OMV_PBR_NORMAL, tn)) {
-        if (tn[2] <= R(0.5f)) tn[2] = R(0.5f) + feps<R>();
-        for (int i = 0; i < 3; ++i) tn[i] = R(2) * tn[i] + R(-1);
-        for (int i = 0; i < 2; ++i) tn[i] = R(d.normal_scale) * tn[i];
-        V3<R> n = normalize3(mk(tn[0], tn[1], tn[2]));
-        tn[0] = n.x; tn[1] = n.y; tn[2] = n.z;
+        applyTextureNormal(sp, texelNormal(tn, R(d.normal_scale)));
+        return;
     }
     applyTextureNormal(sp, mk(tn[0], tn[1], tn[2]));
 }
@@ -1238,25 +1254,33 @@ static bool isMasked(const Scene& S, const SurfacePoint<R>& sp) {
 
 /* geometric normal of the hit face through the normal matrix (OpenMaterial.cpp:76-86) */
 template <class R>
+static V3<R> faceNormalOf(const V3<R>& P0, const V3<R>& P1, const V3<R>& P2, const float* nm) {
+    V3<R> fn = cross3(sub(P1, P0), sub(P2, P0));
+    return normalize3(mul33(nm, fn));
+}
+template <class R>
 static V3<R> faceNormal(const Scene& S, const SurfacePoint<R>& sp) {
     const Mesh& m = S.meshes[sp.mesh];
     const uint32_t* f = &m.F[(size_t)3 * sp.face];
-    V3<R> P0 = ld3<R>(m.vtx(f[0])), P1 = ld3<R>(m.vtx(f[1])), P2 = ld3<R>(m.vtx(f[2]));
-    V3<R> fn = cross3(sub(P1, P0), sub(P2, P0));
-    return normalize3(mul33(S.instances[sp.inst].nm, fn));
+    return faceNormalOf(ld3<R>(m.vtx(f[0])), ld3<R>(m.vtx(f[1])), ld3<R>(m.vtx(f[2])), S.instances[sp.inst].nm);
 }
 
-/* OpenMaterial::defineNextDirection (OpenMaterial.cpp:72-93) */
+/* OpenMaterial::defineNextDirection (OpenMaterial.cpp:72-93): mirror
+ * direction about the shading normal, lifted to 0.1 above the geometric face
+ * normal tfn */
 template <class R>
-static V3<R> omNextDirection(const Scene& S, const V3<R>& inV, const SurfacePoint<R>& sp) {
-    V3<R> out = reflect3(inV, sp.normal);
-    V3<R> tfn = faceNormal(S, sp);
+static V3<R> omNextDirectionOf(const V3<R>& inV, const V3<R>& normal, const V3<R>& tfn) {
+    V3<R> out = reflect3(inV, normal);
     R corr = dot3(out, tfn);
     if (corr < R(0.1f)) {
         R k = R(0.1f) - corr;
         out = normalize3(mk(tfn.x * k + out.x, tfn.y * k + out.y, tfn.z * k + out.z));
     }
     return out;
+}
+template <class R>
+static V3<R> omNextDirection(const Scene& S, const V3<R>& inV, const SurfacePoint<R>& sp) {
+    return omNextDirectionOf(inV, sp.normal, faceNormal(S, sp));
 }
 template <class R>
 static V3<R> omFresnelBrdf(const Material& m, R fCos) {
@@ -2725,6 +2749,36 @@ int ref_get_ior(int nt, const omv_ior_table* t, float T, float wl, float* n, flo
     std::vector<IorTable> tabs = makeTables(nt, t);
     return getIor(tabs, T, wl, *n, *k);
 }
+/* SurfacePoint::applyTextureNormal on the frame f = (tangent, binormal,
+ * normal, handedness) applied `calls` times with the same tn; out = the
+ * tangent, binormal and normal after it */
+void ref_apply_texture_normal(const float* f, const float* tn, int calls, float* out) {
+    SurfacePoint<float> sp;
+    sp.tangent = mk(f[0], f[1], f[2]);
+    sp.binormal = mk(f[3], f[4], f[5]);
+    sp.normal = mk(f[6], f[7], f[8]);
+    sp.tw = f[9];
+    for (int i = 0; i < calls; ++i) applyTextureNormal(sp, mk(tn[0], tn[1], tn[2]));
+    const V3<float> v[3] = {sp.tangent, sp.binormal, sp.normal};
+    for (int i = 0; i < 3; ++i) { out[3 * i] = v[i].x; out[3 * i + 1] = v[i].y; out[3 * i + 2] = v[i].z; }
+}
+
+/* the normal-map texel (RGBA) to tangent-space normal step of
+ * PhysicallyBasedMaterial::modifyFrame */
+void ref_texel_normal(const float* texel, float scale, float* out) {
+    const V3<float> n = texelNormal(texel, scale);
+    out[0] = n.x; out[1] = n.y; out[2] = n.z;
+}
+
+/* OpenMaterial::defineNextDirection for a hit on the object-space triangle
+ * tri (3 x 3 floats) of an instance with normal matrix nm (3 x 3, row-major)
+ * and shading normal `normal` */
+void ref_om_next_direction(const float* inV, const float* normal, const float* tri, const float* nm, float* out) {
+    const V3<float> tfn = faceNormalOf(mk(tri[0], tri[1], tri[2]), mk(tri[3], tri[4], tri[5]), mk(tri[6], tri[7], tri[8]), nm);
+    const V3<float> o = omNextDirectionOf(mk(inV[0], inV[1], inV[2]), mk(normal[0], normal[1], normal[2]), tfn);
+    out[0] = o.x; out[1] = o.y; out[2] = o.z;
+}
+
 void ref_gamma_correction(float* img, int w, int h, float A, float gamma) {
     for (int i = 0; i < w * h; ++i) {
         float* p = img + (size_t)i * 4;

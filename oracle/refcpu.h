@@ -103,6 +103,9 @@ int  ref_distance_random(int n, const float* weights, float u, float* pdf);
 int  ref_get_ior(int n_tables, const omv_ior_table* tables, float temperature, float wavelength,
                  float* n, float* k);
 /* tonemapping gammaCorrection (ToneMapping.cpp:24-54), in place on W*H*4. */
+void ref_apply_texture_normal(const float* frame, const float* tn, int calls, float* out);
+void ref_texel_normal(const float* texel, float scale, float* out);
+void ref_om_next_direction(const float* inV, const float* normal, const float* tri, const float* nm, float* out);
 void ref_gamma_correction(float* img, int w, int h, float A, float gamma);
 /* Reference float helpers exposed for invariant tests. */
 void ref_compute_camera(int w, int h, float yfov_deg, float* half_w, float* half_h, float* focus);

@@ -68,6 +68,9 @@ def lib():
         "ref_scene_mesh_info": ([vp, C.c_int] + [C.POINTER(C.c_int)] * 4, C.c_int),
         "ref_scene_get_vertices": ([vp, C.c_int, FP, C.c_size_t], C.c_int),
         "ref_scene_get_om_nk": ([vp, C.c_int, FP, FP], C.c_int),
+        "ref_apply_texture_normal": ([FP, FP, C.c_int, FP], None),
+        "ref_texel_normal": ([FP, C.c_float, FP], None),
+        "ref_om_next_direction": ([FP, FP, FP, FP, FP], None),
         "ref_render": ([vp, C.POINTER(RendererParams), C.POINTER(CameraDesc), C.c_int, C.c_int, C.c_int, C.c_int, FP,
                         C.POINTER(RenderStats)], C.c_int),
         "ref_render_samples": ([vp, C.POINTER(RendererParams), C.POINTER(CameraDesc), C.c_int, C.c_int, C.c_int, C.c_int, FP], C.c_int),
