@@ -107,12 +107,47 @@ __global__ void omv_fill_u64(unsigned long long* p, int n) {
 }
 
 // Film: PinholeCamera::init/hit/stop (PinholeCamera.cpp:65-108) for whole pixels.
-__global__ void omv_film_kernel(const float* __restrict__ rad, int spp, uint64_t pix0, uint64_t npix, float* __restrict__ film) {
-    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npix) return;
+// A block reduces kFilmPix pixels.  Their samples are staged through LDS in
+// rounds of kFilmK samples per pixel -- each pixel's round is one contiguous
+// run of 3 * kFilmK floats, loaded by the lanes of one wave in one access --
+// and each thread then adds its own pixel's staged samples in sample order, so
+// the sums are the sequential ones.  Rows of the stage are 3 * kFilmK + 1
+// floats apart (odd), so the threads' reads fall in distinct banks.
+constexpr int kFilmPix = 256, kFilmK = 16, kFilmRow = 3 * kFilmK + 1;
+__global__ __launch_bounds__(kFilmPix) void omv_film_kernel(const float* __restrict__ rad, int spp, uint64_t pix0, uint64_t npix,
+                                                            float* __restrict__ film) {
+    __shared__ float stage[kFilmPix * kFilmRow];
+    const uint64_t b0 = (uint64_t)blockIdx.x * kFilmPix;
+    const int np = (int)min<uint64_t>((uint64_t)kFilmPix, npix - b0);
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-    const float* r = rad + p * (uint64_t)spp * 3;
-    for (int k = 0; k < spp; ++k) { a0 = a0 + r[3 * k]; a1 = a1 + r[3 * k + 1]; a2 = a2 + r[3 * k + 2]; }
+    for (int k0 = 0; k0 < spp; k0 += kFilmK) {
+        const int per = 3 * min(kFilmK, spp - k0);   // floats per pixel in this round (<= 48 < 64 lanes)
+        __syncthreads();                              // the previous round is consumed
+        if (lane < per) {
+            // 16 pixel runs in flight per wave before they are stored
+            for (int q0 = wave; q0 < np; q0 += 16 * (kFilmPix / 64)) {
+                float v[16];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int q = q0 + i * (kFilmPix / 64);
+                    v[i] = q < np ? rad[((b0 + (uint64_t)q) * (uint64_t)spp + (uint64_t)k0) * 3 + (uint64_t)lane] : 0.0f;
+                }
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int q = q0 + i * (kFilmPix / 64);
+                    if (q < np) stage[q * kFilmRow + lane] = v[i];
+                }
+            }
+        }
+        __syncthreads();
+        if (t < np) {
+            const float* r = stage + t * kFilmRow;
+            for (int k = 0; 3 * k < per; ++k) { a0 = a0 + r[3 * k]; a1 = a1 + r[3 * k + 1]; a2 = a2 + r[3 * k + 2]; }
+        }
+    }
+    if (t >= np) return;
+    const uint64_t p = b0 + (uint64_t)t;
     float cnt = (float)spp;
     float sc = cnt < 0.5f ? 1.0f : 1.0f / cnt;
     float4 o; o.x = sc * a0; o.y = sc * a1; o.z = sc * a2; o.w = 1.0f;
@@ -255,9 +290,9 @@ hipError_t launchBdpt(const OmvSceneView& S, const omv_camera_desc& cam, const o
 }
 
 hipError_t launchFilm(const float* rad, int spp, uint64_t pix0, uint64_t npix, float* film, hipStream_t stream) {
-    const int tb = 256;
-    const uint64_t blocks = (npix + tb - 1) / tb;
-    hipLaunchKernelGGL(omv_film_kernel, dim3((unsigned)blocks), dim3(tb), 0, stream, rad, spp, pix0, npix, film);
+    if (npix == 0) return hipSuccess;
+    const uint64_t blocks = (npix + kFilmPix - 1) / kFilmPix;
+    hipLaunchKernelGGL(omv_film_kernel, dim3((unsigned)blocks), dim3(kFilmPix), 0, stream, rad, spp, pix0, npix, film);
     return hipGetLastError();
 }
 
