@@ -32,8 +32,10 @@ def _env_desc(wl, lights=False, w=64, h=32, sun=200.0, masked=False):
     return desc, view, sky
 
 
-@pytest.mark.parametrize("w,h", [(64, 32), (256, 128), (37, 19)])
+@pytest.mark.parametrize("w,h", [(64, 32), (256, 128), (37, 19), (2048, 1024), (4000, 6), (5, 3500), (1, 1)])
 def test_env_tables_bitexact(pkg, wl, ref, w, h):
+    """Widths and heights on both sides of the one-wave table limit (3264)."""
+    print(f"env tables {w}x{h}", flush=True)
     desc, _, _ = _env_desc(wl, w=w, h=h)
     s = pkg.Scene()
     desc.apply(s, pkg.pbr_desc)
@@ -43,6 +45,20 @@ def test_env_tables_bitexact(pkg, wl, ref, w, h):
     want = rs.env_tables(w, h)
     for name, a, b in zip(("row_prob", "row_alias", "col_prob", "col_alias", "tex_pdf"), got, want):
         assert np.array_equal(np.asarray(a).reshape(-1), np.asarray(b).reshape(-1)), name
+
+
+def test_env_tables_black_sky(pkg, wl, ref):
+    """All-zero weights: every table entry has probability 1 and itself as alias."""
+    desc, _, sky = _env_desc(wl, w=40, h=20)
+    desc.background = ("hdr", np.zeros_like(sky), wl.HDR_ROTATION, wl.HDR_SCALE)
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    rs = ref.Scene()
+    desc.apply(rs, ref.pbr_desc)
+    got, want = s.env_tables(40, 20), rs.env_tables(40, 20)
+    for a, b in zip(got, want):
+        assert np.array_equal(np.asarray(a).reshape(-1), np.asarray(b).reshape(-1))
+    assert np.all(np.asarray(got[0]) == 1.0) and np.array_equal(np.asarray(got[1]).reshape(-1), np.arange(20))
 
 
 def test_env_tables_absent_without_hdr(pkg, wl):
