@@ -399,8 +399,9 @@ def main():
                     choices=["hdri_torus_knot", "plain_bdpt_icosphere", "plain_bdpt_icosphere_om_rough", "large_instanced",
                              "batched_lightbox", "material_fit"])
     ap.add_argument("--fit-steps", type=int, default=20, help="material_fit: Adam steps after the timed region")
-    ap.add_argument("--rmse", type=int, default=0, metavar="ROWS",
-                    help="equal-RMSE check of GPU and CPU images against the FP64 oracle at 16x spp on ROWS rows")
+    ap.add_argument("--rmse", type=int, default=16, metavar="ROWS",
+                    help="equal-RMSE check of GPU and CPU images against the FP64 oracle at 16x spp on ROWS rows "
+                         "(0: off; skipped with --no-cpu)")
     ap.add_argument("--fd", type=int, default=0, help="material_fit: finite-difference check on this many texels")
     ap.add_argument("--autotune", action="store_true", help="batched_lightbox: run the autotuner instead of the fixed tuning")
     ap.add_argument("--build", default="host", choices=["host", "device"],
@@ -515,7 +516,7 @@ def main():
     }
     if cpu is not None:
         line["gpu_over_cpu"] = round(paths_per_s / cpu["paths_per_s"], 1)
-    if args.rmse > 0:
+    if args.rmse > 0 and not args.no_cpu:
         ref = ge.load_oracle()
         plain = None
         if params.get("extensions", 0) & 2:   # NEE estimates the same image: score the reference estimator beside it

@@ -437,7 +437,7 @@ class Camera:
         self._check(self.lib.omv_camera_load_properties(self.h, *[C.byref(v) if v is not None else None for v in vals]))
 
     def film(self):
-        out = np.zeros(self.width * self.height * 4, np.float32)
+        out = np.empty(self.width * self.height * 4, np.float32)
         self._check(self.lib.omv_camera_get_film(self.h, _ptr(out, C.c_float), out.size))
         return out.reshape(self.height, self.width, 4)
 

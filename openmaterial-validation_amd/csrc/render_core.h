@@ -11,6 +11,7 @@
 #include <stdint.h>
 
 #include <deque>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -113,6 +114,26 @@ private:
     DeviceBuffer dVtx_, dFaces_, dMeshes_, dInsts_, dMats_, dTexs_, dTexels_, dNodes_, dTris_, dLights_, dCdf_, dBg_, dEnv_, dEnvScratch_;
 };
 
+// Page-locked host memory (hipHostMalloc) for films, so a film is read back
+// from the device in one DMA without a staging copy.
+template <class T>
+struct PinnedAllocator {
+    using value_type = T;
+    PinnedAllocator() = default;
+    template <class U>
+    PinnedAllocator(const PinnedAllocator<U>&) {}
+    T* allocate(size_t n) {
+        void* p = nullptr;
+        if (hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault) != hipSuccess || !p) throw std::bad_alloc();
+        return static_cast<T*>(p);
+    }
+    void deallocate(T* p, size_t) { (void)hipHostFree(p); }
+    template <class U>
+    bool operator==(const PinnedAllocator<U>&) const { return true; }
+    template <class U>
+    bool operator!=(const PinnedAllocator<U>&) const { return false; }
+};
+
 class PinholeCamera {
 public:
     PinholeCamera(int w, int h);
@@ -122,7 +143,7 @@ public:
     void setRealHeight(float height, float aspect);
     void loadProperties(const float* aspect, const float* focus, const float* yfov);
     omv_camera_desc desc;
-    std::vector<float> film;   // W*H*4 after a render
+    std::vector<float, PinnedAllocator<float>> film;   // W*H*4 after a render
 };
 
 class BDPTRenderer {
@@ -146,6 +167,9 @@ private:
     int initStream();
     int renderSamples(PinholeCamera& cam, double& kernelMs);
     int renderWavefront(PinholeCamera& cam, double& kernelMs);
+    void collectWavefrontTiming(double& kernelMs);
+    int wfEvUsed_ = 0;
+    bool wfTimingPending_ = false;
     omv_renderer_params params_;
     int schedule_ = OMV_SCHEDULE_WAVEFRONT;
     omv_render_tuning tuning_{0, 0, 0, 0, 0, 0, 0};
@@ -154,7 +178,9 @@ private:
     RenderScene* scene_ = nullptr;
     omv_render_stats stats_{};
     hipStream_t stream_ = nullptr;
-    hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // pinned host staging for the counter read-back (hipHostMalloc)
+    void* hostStage_ = nullptr;
     int cus_ = 0;
     DeviceBuffer dRad_, dLv_, dStack_, dCounters_, dFilm_, dIds_, dTie_, dArena_, dQcount_, dDfilm_, dAdjMaps_, dPost_, dSplat_;
     DeviceBuffer dFcTable_, dFcInstBase_;

@@ -85,6 +85,7 @@ BDPTRenderer::~BDPTRenderer() {
     for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
     for (auto& e : traceEv_) (void)hipEventDestroy(e);
     for (auto& e : chunkEv_) if (e) (void)hipEventDestroy(e);
+    if (hostStage_) (void)hipHostFree(hostStage_);
     if (stream2_) (void)hipStreamDestroy(stream2_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -144,11 +145,21 @@ int BDPTRenderer::render(PinholeCamera& cam) {
     const uint64_t W = cam.desc.width, H = cam.desc.height, npix = W * H;
     const int spp = p.samples_per_pixel;
     std::memset(&stats_, 0, sizeof(stats_));
-    cam.film.assign(npix * 4, 0.0f);
+    filmOf_ = nullptr;
+    try {
+        cam.film.resize(npix * 4);   // written from the device below
+    } catch (const std::bad_alloc&) {
+        setError("render: cannot allocate the host film");
+        return OMV_E_NOMEM;
+    }
     int rc = initStream();
     if (rc) return rc;
     hipError_t e;
     if ((e = dFilm_.reserve(npix * 4 * sizeof(float))) != hipSuccess) return hipFail(e, "film");
+    if (!hostStage_ && (e = hipHostMalloc(&hostStage_, 8 * sizeof(unsigned long long), hipHostMallocDefault)) != hipSuccess) {
+        hostStage_ = nullptr;
+        return hipFail(e, "counter staging");
+    }
     if ((e = dCounters_.reserve(8 * sizeof(unsigned long long))) != hipSuccess) return hipFail(e, "counters");
     if ((e = hipMemsetAsync(dCounters_.ptr, 0, 8 * sizeof(unsigned long long), stream_)) != hipSuccess) return hipFail(e, "memset");
     // PinholeCamera::init (PinholeCamera.cpp:65-68)
@@ -162,6 +173,7 @@ int BDPTRenderer::render(PinholeCamera& cam) {
     }
     (void)hipEventRecord(ev_[0], stream_);
     double kernelMs = 0.0;
+    wfTimingPending_ = false;
     if (spp > 0) {
         rc = schedule_ == OMV_SCHEDULE_WAVEFRONT ? renderWavefront(cam, kernelMs) : renderSamples(cam, kernelMs);
         if (rc) return rc;
@@ -175,12 +187,19 @@ int BDPTRenderer::render(PinholeCamera& cam) {
         if ((e = hipMemcpyAsync(dFilm_.ptr, z.data(), z.size() * 4, hipMemcpyHostToDevice, stream_)) != hipSuccess) return hipFail(e, "film");
     }
     (void)hipEventRecord(ev_[1], stream_);
-    if ((e = hipEventSynchronize(ev_[1])) != hipSuccess) return hipFail(e, "render");
+    // film (pinned) and counters on the same stream, one wait
+    unsigned long long* hostCnt = static_cast<unsigned long long*>(hostStage_);
+    if ((e = hipMemcpyAsync(cam.film.data(), dFilm_.ptr, npix * 4 * sizeof(float), hipMemcpyDeviceToHost, stream_)) != hipSuccess)
+        return hipFail(e, "film readback");
+    if ((e = hipMemcpyAsync(hostCnt, dCounters_.ptr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_)) != hipSuccess)
+        return hipFail(e, "counter readback");
+    (void)hipEventRecord(ev_[4], stream_);
+    if ((e = hipEventSynchronize(ev_[4])) != hipSuccess) return hipFail(e, "render");
     float total = 0.0f;
     (void)hipEventElapsedTime(&total, ev_[0], ev_[1]);
-    if ((e = hipMemcpy(cam.film.data(), dFilm_.ptr, npix * 4 * sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess) return hipFail(e, "film readback");
-    unsigned long long c[8] = {0};
-    if ((e = hipMemcpy(c, dCounters_.ptr, sizeof(c), hipMemcpyDeviceToHost)) != hipSuccess) return hipFail(e, "counter readback");
+    if (wfTimingPending_) collectWavefrontTiming(kernelMs);
+    unsigned long long c[8];
+    std::memcpy(c, hostCnt, sizeof(c));
     stats_.samples = npix * (uint64_t)spp;
     stats_.rays = c[0]; stats_.camera_rays = c[1]; stats_.light_rays = c[2]; stats_.shadow_rays = c[3];
     stats_.nodes_visited = c[4]; stats_.tris_tested = c[5]; stats_.stack_overflows = c[6];
@@ -332,19 +351,28 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
         if ((e = hipStreamWaitEvent(stream_, chunkEv_[2], 0)) != hipSuccess) return hipFail(e, "hipStreamWaitEvent");
     }
     (void)hipEventRecord(ev_[3], stream_);
-    if ((e = hipEventSynchronize(ev_[3])) != hipSuccess) return hipFail(e, "wavefront kernels");
+    // kernel and traversal times are read once render() has waited for the stream
+    wfEvUsed_ = evUsed;
+    wfTimingPending_ = true;
+    (void)kernelMs;
+    return OMV_OK;
+}
+
+// Times of the last wavefront render's kernels (ev_[2] .. ev_[3]) and of its
+// traversal launches; the stream has been waited for.
+void BDPTRenderer::collectWavefrontTiming(double& kernelMs) {
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, ev_[2], ev_[3]);
     kernelMs += ms;
     double tr = 0.0;
-    for (int k = 0; k + 1 < evUsed; k += 2) {
+    for (int k = 0; k + 1 < wfEvUsed_; k += 2) {
         float t = 0.0f;
         (void)hipEventElapsedTime(&t, traceEv_[k], traceEv_[k + 1]);
         tr += t;
     }
     stats_.trace_ms = tr;
-    stats_.trace_launches = (uint64_t)(evUsed / 2);
-    return OMV_OK;
+    stats_.trace_launches = (uint64_t)(wfEvUsed_ / 2);
+    wfTimingPending_ = false;
 }
 
 // Path-replay adjoint (adjoint.hip) of the last render's estimator.
@@ -423,7 +451,12 @@ int BDPTRenderer::renderFc(PinholeCamera& cam, int32_t* ids, uint8_t* tie) {
     if (!scene_ || !scene_->committed()) { setError("render_fc: no committed scene"); return OMV_E_STATE; }
     const uint64_t npix = (uint64_t)cam.desc.width * cam.desc.height;
     std::memset(&stats_, 0, sizeof(stats_));
-    cam.film.assign(npix * 4, 0.0f);
+    try {
+        cam.film.assign(npix * 4, 0.0f);
+    } catch (const std::bad_alloc&) {
+        setError("render_fc: cannot allocate the host film");
+        return OMV_E_NOMEM;
+    }
     int blocks = 0;
     int rc = prepare(1, npix, blocks);
     if (rc) return rc;
