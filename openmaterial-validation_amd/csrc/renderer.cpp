@@ -341,9 +341,16 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
         WfLaunch& Lk = Ls[j];
         Lk.sample0 = pix0 * (uint64_t)spp;
         Lk.nslots = (uint32_t)(np * (uint64_t)spp);
-        if (nstreams == 2 && k > 0 && (e = hipStreamWaitEvent(st[j], chunkEv_[1 - j], 0)) != hipSuccess) return hipFail(e, "hipStreamWaitEvent");
-        if ((e = launchWavefront(Lk, st[j])) != hipSuccess) return hipFail(e, "wavefront launch");
-        if ((e = launchFilm(rad[j], spp, pix0, np, dFilm_.as<float>(), st[j])) != hipSuccess) return hipFail(e, "film kernel launch");
+        const char* what = nullptr;
+        if (nstreams == 2 && k > 0 && (e = hipStreamWaitEvent(st[j], chunkEv_[1 - j], 0)) != hipSuccess) what = "hipStreamWaitEvent";
+        else if ((e = launchWavefront(Lk, st[j])) != hipSuccess) what = "wavefront launch";
+        else if ((e = launchFilm(rad[j], spp, pix0, np, dFilm_.as<float>(), st[j])) != hipSuccess) what = "film kernel launch";
+        if (what) {
+            // both streams drained before the buffers can be used again
+            if (nstreams == 2) (void)hipStreamSynchronize(stream2_);
+            (void)hipStreamSynchronize(stream_);
+            return hipFail(e, what);
+        }
     }
     if (nstreams == 2) {
         // stream_ goes on (splat sum, read-back) once stream2_ has finished
