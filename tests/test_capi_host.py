@@ -171,3 +171,28 @@ def test_schedule_switch(pkg):
     assert L.omv_renderer_set_schedule(r, pkg.SCHEDULE_PER_SAMPLE) == 0
     assert L.omv_renderer_get_schedule(r, C.byref(v)) == 0 and v.value == pkg.SCHEDULE_PER_SAMPLE
     assert L.omv_renderer_destroy(r) == 0
+
+
+def test_tuning_round_trip_and_ranges(pkg):
+    """omv_renderer_set_tuning keeps what it accepts and rejects values out of
+    range (chunk streams beyond two, a negative stagger, an unknown traversal
+    or sort mode) without changing the tuning it holds."""
+    L = pkg.lib()
+    r = C.c_void_p()
+    assert L.omv_renderer_create(C.byref(r)) == 0
+    t = pkg.RenderTuning()
+    assert L.omv_renderer_get_tuning(r, C.byref(t)) == 0
+    assert (t.chunk_streams, t.chunk_stagger, t.chunk_samples) == (0, 0, 0)
+    t.chunk_streams, t.chunk_stagger, t.chunk_samples, t.trace_keep = 2, 3, 1000, 32
+    assert L.omv_renderer_set_tuning(r, C.byref(t)) == 0
+    got = pkg.RenderTuning()
+    assert L.omv_renderer_get_tuning(r, C.byref(got)) == 0
+    assert (got.chunk_streams, got.chunk_stagger, got.chunk_samples, got.trace_keep) == (2, 3, 1000, 32)
+    for field, bad in (("chunk_streams", 3), ("chunk_stagger", -1), ("traversal", 3), ("sort_rays", 3), ("trace_keep", 65)):
+        b = pkg.RenderTuning()
+        assert L.omv_renderer_get_tuning(r, C.byref(b)) == 0
+        setattr(b, field, bad)
+        assert L.omv_renderer_set_tuning(r, C.byref(b)) == -2, field   # OMV_E_RANGE
+        assert L.omv_renderer_get_tuning(r, C.byref(got)) == 0
+        assert (got.chunk_streams, got.chunk_stagger) == (2, 3)
+    assert L.omv_renderer_destroy(r) == 0
