@@ -1,22 +1,28 @@
 """Benchmark: BDPT paths/s and Mrays/s of the render core on one MI355X.
 
-Workload (one "step" = one full render):
-  HDRI-lit torus knot: 100,000 triangles, glTF metallic-roughness with
-  1024^2 roughness/metallic maps (RGBA8, the reference's texture format),
-  2048x1024 fp32 HDRI (log-normal sky + 0.5 deg sun), 512x512, 64 spp,
-  depth 8 (B = b = q = 8), Philox seed 0, reference estimator semantics.
-  Scene commit (mesh preprocessing, BVH build, upload) is outside the timed
-  region, as the reference times BDPTRenderer::render only (main.cpp:499-507).
+Default workload (one "step" = one full render), large_instanced:
+  200 instances of their own randomly displaced ellipsoid / box meshes of
+  ~10k triangles (2.0M unique triangles), 8 materials (OpenMATERIAL complex
+  IOR mirrors and glTF metallic-roughness, one with an RGBA8 metallic-
+  roughness map), four point lights plus a 2048x1024 fp32 HDRI (log-normal
+  sky + 0.5 deg sun), 1920x1080, 16 spp, depth 8 (B = b = q = 8), Philox
+  seed 0, reference estimator semantics.  Scene commit (mesh preprocessing,
+  BVH build, upload) is outside the timed region, as the reference times
+  BDPTRenderer::render only (main.cpp:499-507).
 
 Prints one JSON line.  value = paths/s over the timed steps (inputs resident
-in HBM); roofline = the path kernel's algorithmic bytes (rays x B_ray from the
-oracle's canonical binary SAH BVH over the same rays) per launch / its mean
-HIP-event duration, against 8 TB/s; cpu_baseline = the CPU oracle (float,
-reference estimator, same Philox streams) on whole-image passes of the same
-workload, all threads it was given (at most 16).
+in HBM); roofline = the traversal kernel's algorithmic bytes (rays x B_ray
+from the oracle's canonical binary SAH BVH over the same rays) per render /
+the summed HIP-event duration of its launches, against 8 TB/s, with the HBM
+bytes the counters measured beside it; cpu_baseline = the CPU oracle (float,
+reference estimator, same Philox streams) on a band of rows of the same
+workload: median of 3 runs on the host's CPU share, plus one 1-thread run;
+equal_rmse = GPU and CPU images scored against the FP64 oracle at 16x spp.
 
+--workload hdri_torus_knot: 100k-triangle torus knot, 512x512, 64 spp.
 --workload batched_lightbox prints the batched form's line instead (jobs/s of
 256 lightbox jobs in one launch, beside sequential plain launches).
+--workload material_fit prints the adjoint form's line.
 --copy-probe GIB only runs a plain copy, the calibration kernel for the
 FETCH_SIZE / WRITE_SIZE counters behind roofline.traffic.
 """
@@ -36,40 +42,97 @@ HBM_PEAK_GBS = 8000.0
 B_RAY_IN, B_RAY_OUT, B_NODE, B_TRI = 32, 16, 64, 48
 
 
+def host_cpu():
+    """The host's CPUs as this process sees them: nproc, the affinity mask,
+    the cgroup CPU quota, the CPU model, and the thread count the baseline
+    uses = the CPUs this process may run on (affinity, capped by the cgroup
+    quota and by OMP_NUM_THREADS, the host's declared CPU share)."""
+    nproc = os.cpu_count() or 1
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    share = aff
+    if quota is not None:
+        share = min(share, max(1, int(quota)))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        share = min(share, int(omp))
+    return {"nproc": nproc, "affinity": aff, "cgroup_quota_cpus": quota, "omp_num_threads": omp, "model": model,
+            "threads": share}
+
+
+def _cpu_pass(ref, rs, params, cam_desc, view, rows, threads, bands=8):
+    """One CPU run: `bands` bands of `rows` whole rows each, evenly spaced
+    from the top of the image to the bottom, at the workload's spp and seed
+    (the same samples the GPU renders for those rows).  Returns (seconds,
+    summed stats)."""
+    p = ref.default_params(**params)
+    bands = max(1, min(bands, view.height // max(rows, 1)))
+    tot = {"samples": 0, "rays": 0, "nodes_visited": 0, "tris_tested": 0}
+    dt = 0.0
+    for b in range(bands):
+        row0 = min(view.height - rows, (2 * b + 1) * view.height // (2 * bands) - rows // 2)
+        t0 = time.perf_counter()
+        _, st = rs.render(p, cam_desc, row0=max(0, row0), rows=rows, threads=threads)
+        dt += time.perf_counter() - t0
+        for k in tot:
+            tot[k] += st[k]
+    return dt, tot, bands
+
+
 def cpu_baseline(ref, desc, view, params, cam_desc, budget_s, threads):
-    """Times the oracle on bands of whole rows of the same workload, top to
-    bottom and then again with the next seed, until about budget_s of CPU
-    work; returns paths/s and the per-ray counts of the canonical binary BVH."""
+    """The oracle (float, same estimator and Philox streams) on the host's
+    CPUs, the reference's row scheduler (an atomic row counter over
+    std::thread workers, BDPTRenderer.cpp:349-393).  Bounded sample: 8 bands
+    of whole rows spread evenly over the image, the band height sized from a
+    calibration pass so that one run takes about budget_s / 4 (at least 4
+    rows per thread in a band); that sample is rendered 3 times on `threads`
+    threads (median reported) and 8 bands holding 1/threads as many rows once
+    on 1 thread.  Also returns the per-ray
+    counts of the oracle's canonical binary BVH over the measured rays."""
     rs = ref.Scene()
     desc.apply(rs, ref.pbr_desc)
-    rows, row0, seed = 1, 0, params.get("seed", 0)
-    tot_t, tot_samples, tot_rays, tot_nodes, tot_tris, tot_rows = 0.0, 0, 0, 0, 0, 0
-    while True:
-        p = ref.default_params(**dict(params, seed=seed))
-        t0 = time.perf_counter()
-        _, st = rs.render(p, cam_desc, row0=row0, rows=rows, threads=threads)
-        dt = time.perf_counter() - t0
-        tot_t += dt
-        tot_samples += st["samples"]
-        tot_rays += st["rays"]
-        tot_nodes += st["nodes_visited"]
-        tot_tris += st["tris_tested"]
-        tot_rows += rows
-        row0 += rows
-        if row0 >= view.height:
-            row0, seed = 0, seed + 1
-        if tot_t >= budget_s:
-            break
-        want = int(rows * max(1.5, (budget_s - tot_t) / max(dt, 1e-3)))
-        rows = max(1, min(view.height - row0, want))
+    # calibration: one band of 4 rows per thread through the middle (the row
+    # scheduler needs several rows per thread to keep every thread busy)
+    cal = min(view.height, 4 * threads)
+    dt, _, _ = _cpu_pass(ref, rs, params, cam_desc, view, cal, threads, bands=1)
+    per_run = budget_s / 4.0
+    total_rows = max(1, int(per_run / max(dt / cal, 1e-6)))
+    rows = max(min(view.height, 4 * threads), total_rows // 8)
+    rows = min(rows, view.height // 8 if view.height >= 64 else view.height)
+    runs, tot = [], {"samples": 0, "rays": 0, "nodes_visited": 0, "tris_tested": 0}
+    for _ in range(3):
+        dt, st, nb = _cpu_pass(ref, rs, params, cam_desc, view, rows, threads)
+        runs.append(st["samples"] / dt)
+        for k in tot:
+            tot[k] += st[k]
+    rows1 = max(1, int(round(nb * rows / max(threads, 1) / 8)))   # about the same seconds on one thread
+    dt1, st1, nb1 = _cpu_pass(ref, rs, params, cam_desc, view, rows1, 1)
     return {
-        "paths_per_s": tot_samples / tot_t,
-        "seconds": tot_t,
-        "rows": tot_rows,
-        "samples": tot_samples,
-        "rays_per_path": tot_rays / max(tot_samples, 1),
-        "nodes_per_ray": tot_nodes / max(tot_rays, 1),
-        "tris_per_ray": tot_tris / max(tot_rays, 1),
+        "paths_per_s": float(np.median(runs)),
+        "runs_paths_per_s": [round(x, 1) for x in runs],
+        "bands": [nb, rows],
+        "samples": int(tot["samples"] // 3),
+        "seconds": round(float(sum(tot["samples"] / 3 / x for x in runs)), 2),
+        "one_thread_paths_per_s": st1["samples"] / dt1,
+        "one_thread_bands": [nb1, rows1],
+        "one_thread_seconds": round(dt1, 2),
+        "rays_per_path": tot["rays"] / max(tot["samples"], 1),
+        "nodes_per_ray": tot["nodes_visited"] / max(tot["rays"], 1),
+        "tris_per_ray": tot["tris_tested"] / max(tot["rays"], 1),
     }
 
 
@@ -120,17 +183,19 @@ def copy_probe(gib):
 
 
 def measured_traffic(workload):
-    """HBM bytes per launch of the path kernel from the committed rocprofv3
-    --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy),
-    or None when there is none for this workload."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"bdpt_kernel_{workload}_pmc.txt")
+    """HBM bytes per launch of the traversal kernel (the launches of one
+    render) from the committed rocprofv3 --pmc summary of this workload
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy) and the file
+    it came from, or (None, None) when there is none."""
+    rel = os.path.join("profiles", f"trace_kernel_{workload}_pmc.txt")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), rel)
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1])
+                return float(line.split(":", 1)[1]), rel
     except OSError:
         pass
-    return None
+    return None, None
 
 
 BUILD_MODE = 0   # set from --build
@@ -319,7 +384,7 @@ def fd_check(pkg, wl, count, dt, di, view, params, ids, T):
     di.apply(rs, ref.pbr_desc)
     p = ref.default_params(**params)
     cd = cam.desc()
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_cpu()["threads"]
     img32, _ = rs.render(p, cd, threads=threads)
     tex = di.textures[0][1].shape[0]
     mask = np.array([1, 1, 1, 0], np.float32)
@@ -391,11 +456,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=16.0,
+                    help="CPU baseline budget: 3 runs of about a quarter of it on all threads + one 1-thread run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB GiB (counter calibration)")
-    ap.add_argument("--workload", default="hdri_torus_knot",
+    ap.add_argument("--workload", default="large_instanced",
                     choices=["hdri_torus_knot", "plain_bdpt_icosphere", "plain_bdpt_icosphere_om_rough", "large_instanced",
                              "batched_lightbox", "material_fit"])
     ap.add_argument("--fit-steps", type=int, default=20, help="material_fit: Adam steps after the timed region")
@@ -483,12 +549,13 @@ def main():
 
     cpu = None
     b_ray = None
-    threads = min(16, os.cpu_count() or 1)
+    hc = host_cpu()
+    threads = hc["threads"]
     if not args.no_cpu:
         ref = ge.load_oracle()
         cpu = cpu_baseline(ref, desc, view, params, cam.desc(), args.cpu_seconds, threads)
         b_ray = B_RAY_IN + B_RAY_OUT + B_NODE * cpu["nodes_per_ray"] + B_TRI * cpu["tris_per_ray"]
-        print(f"# cpu baseline {cpu}", file=sys.stderr, flush=True)
+        print(f"# cpu baseline {cpu} on {hc}", file=sys.stderr, flush=True)
 
     roofline = None
     if b_ray is not None:
@@ -496,12 +563,27 @@ def main():
         # the per-sample schedule has a single path kernel
         dom_ms = t_ms if t_ms > 0 else k_ms
         achieved = rays_per_launch * b_ray / (dom_ms * 1e-3) / 1e9
-        traffic = measured_traffic(args.workload)
+        traffic, traffic_src = measured_traffic(args.workload)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "achieved_model": "cold-cache algorithmic bytes: rays x B_ray, B_ray = 32 + 16 + 64 x nodes + 48 x tris "
+                                      "of the oracle's canonical binary SAH BVH over the CPU sample's rays",
                     "kernel": "omv_wf_trace (all traversal launches of one render)" if t_ms > 0 else "omv_bdpt_kernel",
                     "b_ray": round(b_ray, 1), "rays_per_render": int(rays_per_launch), "kernel_ms": round(dom_ms, 3),
                     "render_kernels_ms": round(k_ms, 3)}
+        # the same model over the device's own BVH4 (128-byte nodes, 48-byte
+        # triangle records) and its measured visit counts
+        b_dev = B_RAY_IN + B_RAY_OUT + 128.0 * nodes / max(rays, 1) + B_TRI * tris / max(rays, 1)
+        roofline["b_ray_device_bvh4"] = round(b_dev, 1)
+        roofline["achieved_device_bvh4"] = round(rays_per_launch * b_dev / (dom_ms * 1e-3) / 1e9, 1)
+        if traffic:
+            # what the memory system actually moved (rocprofv3 counters): the
+            # traversal's working set sits in L2 and the Infinity Cache, so this
+            # rate, not the model's, says how far the kernel is from HBM-bound
+            roofline["traffic_source"] = traffic_src
+            roofline["measured_hbm_gbs"] = round(traffic / (dom_ms * 1e-3) / 1e9, 1)
+            roofline["measured_hbm_frac"] = round(traffic / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+            roofline["limiter"] = "dependent node-fetch latency and lane divergence (measured HBM rate well below peak)"
     line = {
         "metric": "BDPT paths/s (Mrays/s beside it) on 1 GPU", "value": round(paths_per_s, 1), "unit": "paths/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3),
@@ -510,12 +592,19 @@ def main():
         "roofline": roofline,
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["paths_per_s"], 1), "unit": "paths/s", "cores": threads, "kind": "port",
-            "sample": f"{cpu['rows']} rows x {view.width} px x {params['samples_per_pixel']} spp ({cpu['samples']} paths, whole-image "
-                      f"passes with successive seeds) of the same workload in {cpu['seconds']:.1f}s, float oracle, {cfg['estimator']} "
-                      f"estimator"},
+            "sample": f"{cpu['bands'][0]} bands of {cpu['bands'][1]} rows spread over the image x {view.width} px x "
+                      f"{params['samples_per_pixel']} spp ({cpu['samples']} paths) of the same workload, median of 3 runs on {threads} threads "
+                      f"({cpu['seconds']:.1f}s in all); float oracle, {cfg['estimator']} estimator, same Philox streams; "
+                      f"reference-semantics CPU restatement (Embree unavailable offline)",
+            "runs": cpu["runs_paths_per_s"], "one_thread": round(cpu["one_thread_paths_per_s"], 1),
+            "one_thread_sample": f"{cpu['one_thread_bands'][0]} bands of {cpu['one_thread_bands'][1]} rows, {cpu['one_thread_seconds']}s",
+            "nproc": hc["nproc"], "affinity_cpus": hc["affinity"], "cgroup_quota_cpus": hc["cgroup_quota_cpus"],
+            "omp_num_threads": hc["omp_num_threads"], "cpu_model": hc["model"]},
     }
     if cpu is not None:
         line["gpu_over_cpu"] = round(paths_per_s / cpu["paths_per_s"], 1)
+        # upper bound on an all-core CPU: the 1-thread rate scaled linearly to nproc
+        line["gpu_over_cpu_nproc_linear"] = round(paths_per_s / (cpu["one_thread_paths_per_s"] * hc["nproc"]), 2)
     if args.rmse > 0 and not args.no_cpu:
         ref = ge.load_oracle()
         plain = None
