@@ -386,6 +386,29 @@ int omv_save_image(const char* filename, const float* rgba, int width, int heigh
  * out != NULL and cap < *len). */
 int omv_encode_image(const char* ext, const float* rgba, int width, int height, uint8_t* out, size_t cap, size_t* len);
 
+/* Allocation-free render.  No counterpart in the reference: this is for
+ * callers that cannot allocate device memory while they render -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Renders the renderer's committed scene with its parameters from camera c
+ * into film_dev (device pointer, W*H*4 floats: the normalised film
+ * omv_renderer_render gives, bit for bit) on `stream` (a hipStream_t; NULL =
+ * the null stream), as one persistent per-pixel kernel.  It calls no
+ * hipMalloc / hipFree: its only device memory besides the scene and film_dev
+ * is a fixed array of OMV_NOALLOC_SCRATCH_BYTES per GPU (counters, traversal
+ * spill stacks, light vertices of the resident lanes), which each GPU the
+ * library runs on holds until the process ends.  Every call on a GPU shares
+ * that GPU's array, so calls that overlap on one GPU race.  stats == NULL:
+ * the call only enqueues its work (no wait, usable under capture); else it
+ * waits for the stream and fills samples, ray and traversal counts (the
+ * times are 0).  Limits: min(light_bounces + 1, max_path_length) light
+ * vertices kept <= OMV_NOALLOC_MAX_LIGHT_VERTICES (else OMV_E_RANGE);
+ * OMV_EXT_LIGHT_SPLAT with point lights needs a W*H buffer of sums and is
+ * refused (OMV_E_STATE). */
+#define OMV_NOALLOC_SCRATCH_BYTES      (64u << 20)
+#define OMV_NOALLOC_MIN_BLOCKS         256   /* blocks of 128 lanes the scratch must hold */
+#define OMV_NOALLOC_MAX_LIGHT_VERTICES 69
+int omv_renderer_render_noalloc(omv_renderer* r, const omv_camera* c, float* film_dev, void* stream, omv_render_stats* stats);
+
 /* ---------------------------------------------------------- adjoint */
 /* Gradient targets of omv_renderer_backward.  All pointers are device
  * pointers; gradients are ADDED to them.  texture_grads[i] holds w*h*4

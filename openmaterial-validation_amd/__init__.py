@@ -286,7 +286,7 @@ def _load():
         "omv_camera_get_film": [vp, C.POINTER(C.c_float), C.c_size_t],
         "omv_renderer_create": [C.POINTER(vp)], "omv_renderer_destroy": [vp],
         "omv_renderer_set_parameters": [vp, C.POINTER(RendererParams)], "omv_renderer_set_scene": [vp, vp],
-        "omv_renderer_render": [vp, vp], "omv_renderer_render_fc": [vp, vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)],
+        "omv_renderer_render": [vp, vp], "omv_renderer_render_noalloc": [vp, vp, C.c_void_p, C.c_void_p, C.c_void_p], "omv_renderer_render_fc": [vp, vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)],
         "omv_renderer_get_stats": [vp, C.POINTER(RenderStats)],
         "omv_renderer_backward": [vp, vp, C.c_void_p, C.POINTER(AdjointTargets)],
         "omv_renderer_post": [vp, vp, C.c_float, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_uint8)],
@@ -500,6 +500,15 @@ class Renderer:
     def render(self, cam: Camera):
         self._check(self.lib.omv_renderer_render(self.h, cam.h))
         return cam.film()
+
+    def render_noalloc(self, cam: Camera, film_ptr: int, stream: int = 0, stats: bool = True):
+        """Allocation-free render into a caller-owned device film (film_ptr =
+        device address of H*W*4 floats) on `stream` (0 = the null stream).
+        stats=True waits and returns the counters; False only enqueues."""
+        st = RenderStats() if stats else None
+        self._check(self.lib.omv_renderer_render_noalloc(self.h, cam.h, C.c_void_p(film_ptr), C.c_void_p(stream or None),
+                                                         C.byref(st) if st is not None else None))
+        return {k: getattr(st, k) for k, _ in RenderStats._fields_} if st is not None else None
 
     def backward(self, cam: Camera, d_film_ptr: int, textures=(), om_materials=(), om_grads_ptr: int = 0):
         """Path-replay adjoint of the last render: d_film_ptr = device address of

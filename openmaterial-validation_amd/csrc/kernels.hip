@@ -86,15 +86,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_batch_kernel(const OmvJobDev* _
         for (int w = 0; w < tileWaves; ++w) {
             const uint64_t p = pix0 + (uint64_t)w * 64ull + (uint64_t)lane;
             if (p >= J.npix) continue;
-            float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-            for (int k = 0; k < J.spp; ++k) {
-                f3 r = pathSample<true, kNee>(J.S, J.cam, J.P, J.spp, p * (uint64_t)J.spp + (uint64_t)k, L);
-                a0 = a0 + r.x; a1 = a1 + r.y; a2 = a2 + r.z;
-            }
-            float cnt = (float)J.spp;
-            float sc = cnt < 0.5f ? 1.0f : 1.0f / cnt;
-            float4 o; o.x = sc * a0; o.y = sc * a1; o.z = sc * a2; o.w = 1.0f;
-            reinterpret_cast<float4*>(J.film)[p] = o;
+            reinterpret_cast<float4*>(J.film)[p] = renderPixel<true, kNee>(J.S, J.cam, J.P, J.spp, p, L);
         }
     }
     flushCounters(counters, L.rays, L.camRays, L.lightRays, L.shadowRays, L.cnt);

@@ -33,6 +33,11 @@ void packBatchJob(void* dst, const OmvSceneView& S, const omv_camera_desc& cam, 
                   uint64_t npix);
 hipError_t launchBatch(const void* jobs, int njobs, const uint64_t* tileEnd, uint64_t totalTiles, int tileWaves,
                        unsigned long long* counters, float* lv, uint32_t* gstack, int blocks, bool nee, hipStream_t stream);
+// Allocation-free render (noalloc.hip): one persistent per-pixel kernel over
+// the file-scope scratch array; statsOut (may be null: no wait) receives the
+// 8 counters after the stream has finished.  On failure err / what say why.
+int renderNoalloc(const OmvSceneView& S, const omv_camera_desc& cam, const omv_renderer_params& p, float* film, hipStream_t stream,
+                  unsigned long long statsOut[8], hipError_t& err, const char*& what);
 // Wavefront render of one chunk of whole pixels (wavefront.hip).
 struct WfLaunch {
     OmvSceneView S;

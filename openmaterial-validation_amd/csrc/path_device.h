@@ -1531,6 +1531,23 @@ DEV f3 pathSample(const OmvSceneView& S, const omv_camera_desc& cam, const RP& P
     return mk3(conn.x * 1.0f + radiance.x, conn.y * 1.0f + radiance.y, conn.z * 1.0f + radiance.z);
 }
 
+// One pixel of the normalised film: its spp samples summed in sample order
+// (PinholeCamera::hit, PinholeCamera.cpp:97-108), then PinholeCamera::stop
+// (rgb / count when count >= 0.5, alpha 1).  The sums and the scale are the
+// film kernel's, so the pixel equals the per-sample schedule's bit for bit.
+template <bool kLights, bool kNee = false>
+DEV float4 renderPixel(const OmvSceneView& S, const omv_camera_desc& cam, const RP& P, int spp, uint64_t p, Lane& L) {
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    for (int k = 0; k < spp; ++k) {
+        f3 r = pathSample<kLights, kNee>(S, cam, P, spp, p * (uint64_t)spp + (uint64_t)k, L);
+        a0 = a0 + r.x; a1 = a1 + r.y; a2 = a2 + r.z;
+    }
+    float cnt = (float)spp;
+    float sc = cnt < 0.5f ? 1.0f : 1.0f / cnt;
+    float4 o; o.x = sc * a0; o.y = sc * a1; o.z = sc * a2; o.w = 1.0f;
+    return o;
+}
+
 }  // namespace
 
 #endif

@@ -158,6 +158,7 @@ public:
     const omv_render_tuning& tuning() const { return tuning_; }
     int render(PinholeCamera& cam);
     int renderFc(PinholeCamera& cam, int32_t* ids, uint8_t* tie);
+    int renderNoalloc(const PinholeCamera& cam, float* film, hipStream_t stream, omv_render_stats* st);
     int backward(PinholeCamera& cam, const float* dFilm, const omv_adjoint_targets& t);
     int post(PinholeCamera& cam, float gamma, bool denoise, float* outF, uint8_t* outU8);
     const omv_render_stats& stats() const { return stats_; }

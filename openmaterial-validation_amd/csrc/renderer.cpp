@@ -210,6 +210,37 @@ int BDPTRenderer::render(PinholeCamera& cam) {
     return OMV_OK;
 }
 
+// Allocation-free render: validates on the host, then one persistent
+// per-pixel launch over the fixed scratch array (noalloc.hip).  Touches none
+// of this renderer's buffers, streams or events.
+int BDPTRenderer::renderNoalloc(const PinholeCamera& cam, float* film, hipStream_t stream, omv_render_stats* st) {
+    if (!scene_ || !scene_->committed()) { setError("render_noalloc: no committed scene"); return OMV_E_STATE; }
+    if (!film) { setError("render_noalloc: null film"); return OMV_E_ARG; }
+    const omv_renderer_params& p = params_;
+    if (p.samples_per_pixel < 0 || p.camera_bounces < 0 || p.light_bounces < 0) { setError("render_noalloc: negative parameter"); return OMV_E_ARG; }
+    const OmvSceneView S = viewForLaunch();
+    if ((p.extensions & OMV_EXT_LIGHT_SPLAT) && S.num_lights > 0) {
+        setError("render_noalloc: OMV_EXT_LIGHT_SPLAT needs a W*H buffer of splat sums; use omv_renderer_render");
+        return OMV_E_STATE;
+    }
+    unsigned long long c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    hipError_t e = hipSuccess;
+    const char* what = nullptr;
+    int rc = omv::renderNoalloc(S, cam.desc, p, film, stream, st ? c : nullptr, e, what);
+    if (rc) {
+        setError(std::string(what ? what : "render_noalloc") + (e != hipSuccess ? std::string(": ") + hipGetErrorString(e) : std::string()));
+        return rc;
+    }
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->samples = (uint64_t)cam.desc.width * (uint64_t)cam.desc.height * (uint64_t)std::max(p.samples_per_pixel, 0);
+        st->rays = c[0]; st->camera_rays = c[1]; st->light_rays = c[2]; st->shadow_rays = c[3];
+        st->nodes_visited = c[4]; st->tris_tested = c[5]; st->stack_overflows = c[6];
+        if (st->stack_overflows) { setError("render_noalloc: traversal stack overflow"); return OMV_E_INTERNAL; }
+    }
+    return OMV_OK;
+}
+
 int BDPTRenderer::post(PinholeCamera& cam, float gamma, bool denoise, float* outF, uint8_t* outU8) {
     if (filmOf_ != &cam || !dFilm_.ptr) { setError("post: this camera has no film from this renderer"); return OMV_E_STATE; }
     const uint64_t npix = (uint64_t)cam.desc.width * cam.desc.height;
