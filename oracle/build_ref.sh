@@ -33,6 +33,12 @@ g++ -std=c++11 -O0 -ffp-contract=off -ffunction-sections -Wl,--gc-sections -DDOC
     "$HERE/golden_harness.cpp" "$HERE/golden_stb.cpp" "${SRCS[@]}" -o "$OUT/golden_harness"
 echo "built $OUT/golden_harness"
 
+# The false-colour subjects (validation/FCSubject.h, header-only paints)
+# with FC_VALIDATION, the reference's own validation build flag.
+g++ -std=c++11 -O0 -ffp-contract=off -DFC_VALIDATION -DDOCTEST_CONFIG_DISABLE "${INC[@]}" \
+    "$HERE/fc_harness.cpp" "$SRC/utilities/Random.cpp" -o "$OUT/fc_harness"
+echo "built $OUT/fc_harness"
+
 # The glTF / OpenMATERIAL asset loader (AssetGeometry and what it pulls in);
 # the reference's external/tiny_gltf.cpp is its tinygltf + stb_image TU.
 ATUS=(assets/AssetGeometry.cpp assets/ImageData.cpp assets/Mesh.cpp assets/AssetDefs.cpp

@@ -64,12 +64,25 @@
  *                                background -> every pixel = background;
  *                                mirror sphere -> 100*F(cos); one lit PBR
  *                                plane -> evaluateDirect * geometry by hand).
+ *                                isConnected (:208-235): an occluder between
+ *                                the light and a lit floor -> 0; a second
+ *                                floor just beyond the lit point, inside
+ *                                tfar -> film unchanged.
+ *   FC paint rules ............. FCSubject.h:153-482.  Pinned: golden
+ *                                vectors from the reference's own subject
+ *                                classes (FC_VALIDATION build of the header)
+ *                                and its unit tests FCSubject.cpp:189-315.
+ *   FC density / gradient ...... RenderMesh.cpp:197-212, BDPTRenderer.cpp:
+ *                                69-71.  Pinned: closed forms on triangles of
+ *                                known area and known vertex-normal tilt.
  *   Embree rtcIntersect1 ....... replaced by the watertight ray/triangle test
  *                                of Woop, Benthin & Wald (JCGT 2013) with a
  *                                deterministic (t, instance, primitive)
  *                                tie-break.  Embree's own edge behaviour is
  *                                parity unpinned (no reference test exists).
- *                                Pinned: brute force == BVH closest hit.
+ *                                Pinned: brute force == BVH closest hit;
+ *                                (t, u, v) against a float64 intersection;
+ *                                no gaps on shared edges and vertices.
  *
  * Two shortcuts are exact (they change no result; see the comments where
  * they sit): light subpaths stop after max_path_length-1 bounces and camera
@@ -202,6 +215,10 @@ template <class R> static inline V3<R> mul34(const float* A, const V3<R>& v) {  
     return mk(R(A[0]) * v.x + R(A[1]) * v.y + R(A[2]) * v.z + R(A[3]),
               R(A[4]) * v.x + R(A[5]) * v.y + R(A[6]) * v.z + R(A[7]),
               R(A[8]) * v.x + R(A[9]) * v.y + R(A[10]) * v.z + R(A[11]));
+}
+template <class R> static inline V3<R> mul33tm(const float* A, const V3<R>& v) {  // the 3x3 part of a row-major 3x4
+    return mk(R(A[0]) * v.x + R(A[1]) * v.y + R(A[2]) * v.z, R(A[4]) * v.x + R(A[5]) * v.y + R(A[6]) * v.z,
+              R(A[8]) * v.x + R(A[9]) * v.y + R(A[10]) * v.z);
 }
 template <class R> static inline V3<R> mulT33(const V3<R>& v, const float* A) {  // v * A (row vector)
     return mk(R(A[0]) * v.x + R(A[3]) * v.y + R(A[6]) * v.z, R(A[1]) * v.x + R(A[4]) * v.y + R(A[7]) * v.z,
@@ -643,6 +660,57 @@ static void affineInverse(const float* M, double* inv) {
     }
 }
 
+/* Planted mistakes for the pin tests (test infrastructure; 0 = none).  Each
+ * bit puts one plausible slip into the oracle so that a test can show the
+ * check it runs fails on it:
+ *   1  rmp / mmp paint toward blue / green (the targets swapped)
+ *   2  md: linear density normalisation d / max instead of the log one
+ *   4  watertight test returns (v, u) instead of (u, v)
+ *   8  watertight test returns t * (1 + 2^-10)
+ *  16  isConnected: visible whatever the closest hit
+ *  32  isConnected: any hit beyond B (up to tfar) occludes
+ *  64  sg: vertex normals by the position matrix instead of the normal one */
+static int gPinPlant = 0;
+
+/* False-colour paint rules (FCSubject.h:153-482).  The argument is what a
+ * material's FC branch hands to the subject: a clamped float for the
+ * map / density / gradient subjects (GenericFCArgument clamps it to [0, 1]),
+ * an id into the colour table for the enumerated ones, the material kind for
+ * mn, a flag for in. */
+template <class R> static V3<R> fcPaintClamped(int sub, R a) {
+    a = clampv(a, R(0), R(1));
+    V3<R> hi;
+    switch (sub) {
+        case OMV_FC_RMP: hi = (gPinPlant & 1) ? mk(R(0), R(0), R(1)) : mk(R(0), R(1), R(0)); break;   // RoughnessCftFC: green
+        case OMV_FC_MMP: hi = (gPinPlant & 1) ? mk(R(0), R(1), R(0)) : mk(R(0), R(0), R(1)); break;   // MetallicCftFC: blue
+        case OMV_FC_MD: hi = mk(R(1), R(0), R(0)); break;                                             // MeshDensityFC: red
+        case OMV_FC_SG: return mk(R(1) - a, R(1) - a, R(1) - a);                                        // SurfaceGradientFC
+        default: return mk(R(0), R(0), R(0));
+    }
+    /* lerp(lower, upper, t) = (upper - lower) * t + lower from white (UtilityFunctions.h:459-469) */
+    return mk(lerpv(R(1), hi.x, a), lerpv(R(1), hi.y, a), lerpv(R(1), hi.z, a));
+}
+/* EnumeratedFCSubject::paint (FCSubject.h:172-177): the id's table colour,
+ * black past the table; MaterialIDFC::paint (:234-245) drops red. */
+template <class R> static V3<R> fcPaintEnumerated(uint32_t id, const uint32_t* table, uint32_t size, bool midStyle) {
+    uint32_t c = id < size ? table[id] : 0u;
+    R r = R((float)((c >> 16) & 0xFF) / 255.0f), g = R((float)((c >> 8) & 0xFF) / 255.0f), b = R((float)(c & 0xFF) / 255.0f);
+    return midStyle ? mk(R(0), g, b) : mk(r, g, b);
+}
+/* MaterialNameFC::paint (FCSubject.h:412-418) with the dictionary of
+ * RenderScene::setFalseColorSubject (RenderScene.cpp:142): "om" green,
+ * "pbr" blue; anything else (the missing material passes "mm") pink. */
+template <class R> static V3<R> fcPaintMaterialName(int matType) {
+    if (matType == MAT_OM) return mk(R(0), R(1), R(0));
+    if (matType == MAT_PBR) return mk(R(0), R(0), R(1));
+    return mk(R(1000), R(0), R(1000));
+}
+/* InvertedNormalFC::paint (FCSubject.h:472-476) */
+template <class R> static V3<R> fcPaintInverted(bool inv) {
+    R f = inv ? R(1) : R(0);
+    return mk(f, R(1) - f, R(0));
+}
+
 /* ---------------------------------------------- watertight ray/triangle */
 template <class R> struct RayPre { int kx, ky, kz; R Sx, Sy, Sz; V3<R> o, d; };
 
@@ -691,6 +759,8 @@ static bool triIntersect(const RayPre<R>& p, const float* v0, const float* v1, c
     R tt = T * rcp;
     if (!(tt > tnear && tt <= tfar)) return false;
     t = tt; u = V * rcp; v = W * rcp;
+    if (gPinPlant & 4) std::swap(u, v);
+    if (gPinPlant & 8) t = t * (R(1) + R(1.0 / 1024.0));
     return true;
 }
 
@@ -839,12 +909,13 @@ static void computeSurfacePoint(const Scene& S, const Hit<R>& h, SurfacePoint<R>
         tfn = normalize3(tfn);
         R ag = R(0);
         for (int k = 0; k < 3; ++k) {
-            V3<R> tvn = normalizeIfNeeded3(mul33(in.nm, ld3<R>(v[k] + 3)));
+            V3<R> tvn = normalizeIfNeeded3((gPinPlant & 64) ? mul33tm(in.tm, ld3<R>(v[k] + 3)) : mul33(in.nm, ld3<R>(v[k] + 3)));
             ag = ag + std::sqrt((R(1) - clampv(dot3(tvn, tfn), R(-1), R(1))) / R(2));
         }
         sp.gradient = std::pow(clampv(ag / R(3), R(1e-6f), R(1)), R(0.286f));
         const R mnD = R(S.minDensity), mxD = R(S.maxDensity);
-        sp.density = clamp01(R(1) - std::log(mxD / clampv(sp.density, mnD, mxD)) / std::log(mxD / mnD));
+        if (gPinPlant & 2) sp.density = clamp01(clampv(sp.density, mnD, mxD) / mxD);
+        else sp.density = clamp01(R(1) - std::log(mxD / clampv(sp.density, mnD, mxD)) / std::log(mxD / mnD));
     }
 }
 
@@ -1739,10 +1810,16 @@ struct Tracer {
         SurfacePoint<R> sp;
         ++shadowRays;
         if (!sceneIntersect(org, dir, tnear, tfar, sp)) return true;
+        if (gPinPlant & 16) return true;
         V3<R> dd = sub(sp.position, B);
         R sd = dot3(dd, dd);
         maxsz = maxAbs(B);
         safeD = R(10) * feps<R>() * maxsz;
+        if ((gPinPlant & 32) && sd <= safeD * safeD) {
+            SurfacePoint<R> sp2;   // the slip: any further hit up to tfar counts as an occluder
+            R tn2 = len + safeD;
+            if (sceneIntersect(org, dir, tn2, tfar, sp2)) return false;
+        }
         return sd <= safeD * safeD;
     }
 
@@ -1932,16 +2009,8 @@ struct Tracer {
         const Mesh& mesh = S.meshes[sp.mesh];
         const Material& m = S.materials[mesh.matId];
         const int sub = S.fcSubject;
-        auto lerpCol = [&](const V3<R>& hi, R a) {
-            a = clampv(a, R(0), R(1));
-            return mk(lerpv(R(1), hi.x, a), lerpv(R(1), hi.y, a), lerpv(R(1), hi.z, a));
-        };
-        auto enumColor = [&](uint32_t id, uint32_t /*size*/, bool midStyle) {
-            /* EnumeratedFCSubject::paint (FCSubject.h:172-177) */
-            uint32_t c = id < fcTable.size() ? fcTable[id] : 0u;
-            R r = R((float)((c >> 16) & 0xFF) / 255.0f), g = R((float)((c >> 8) & 0xFF) / 255.0f), b = R((float)(c & 0xFF) / 255.0f);
-            return midStyle ? mk(R(0), g, b) : mk(r, g, b);
-        };
+        const uint32_t* tab = fcTable.data();
+        const uint32_t tsz = (uint32_t)fcTable.size();
         bool isMissing = (m.type == MAT_DIFFUSE);
         V3<R> inDir = inV;
         if (isMissing) {
@@ -1952,35 +2021,33 @@ struct Tracer {
             inDir = neg(outV);
         }
         switch (sub) {
-            case OMV_FC_PID: col = enumColor((uint32_t)sp.globalFace, (uint32_t)S.faceBase.back(), false); break;
-            case OMV_FC_GID: col = enumColor((uint32_t)sp.mesh, (uint32_t)S.meshes.size(), false); break;
-            case OMV_FC_IID: col = enumColor((uint32_t)sp.inst, (uint32_t)S.instances.size(), false); break;
-            case OMV_FC_IPID: col = enumColor(fcInstBase[sp.inst] + (uint32_t)sp.face, 0u, false); break;
+            case OMV_FC_PID: col = fcPaintEnumerated<R>((uint32_t)sp.globalFace, tab, tsz, false); break;
+            case OMV_FC_GID: col = fcPaintEnumerated<R>((uint32_t)sp.mesh, tab, tsz, false); break;
+            case OMV_FC_IID: col = fcPaintEnumerated<R>((uint32_t)sp.inst, tab, tsz, false); break;
+            case OMV_FC_IPID: col = fcPaintEnumerated<R>(fcInstBase[sp.inst] + (uint32_t)sp.face, tab, tsz, false); break;
             case OMV_FC_MID:
+                /* the missing material passes id -1, painted pink (FCSubject.h:239-244) */
                 if (isMissing) col = mk(R(1000), R(0), R(1000));
-                else col = enumColor((uint32_t)mesh.matId, (uint32_t)S.materials.size(), true);
+                else col = fcPaintEnumerated<R>((uint32_t)mesh.matId, tab, tsz, true);
                 break;
             case OMV_FC_RMP: case OMV_FC_MMP: {
+                /* PBR hands omr[2] / omr[1] (PhysicallyBasedMaterial.cpp:216-219); the
+                 * OM and missing materials hand a default (zero) argument */
                 R a = R(0);
                 if (m.type == MAT_PBR) {
                     R color[4], omr[3];
                     pbrInputs(S, sp, m.pbr, color, omr);
                     a = sub == OMV_FC_RMP ? omr[2] : omr[1];
                 }
-                col = lerpCol(sub == OMV_FC_RMP ? mk(R(0), R(1), R(0)) : mk(R(0), R(0), R(1)), a);
+                col = fcPaintClamped<R>(sub, a);
                 break;
             }
-            case OMV_FC_MD: col = lerpCol(mk(R(1), R(0), R(0)), sp.density); break;
-            case OMV_FC_SG: { R g = clampv(sp.gradient, R(0), R(1)); col = mk(R(1) - g, R(1) - g, R(1) - g); break; }
-            case OMV_FC_MN:
-                if (m.type == MAT_OM) col = mk(R(0), R(1), R(0));
-                else if (m.type == MAT_PBR) col = mk(R(0), R(0), R(1));
-                else col = mk(R(1000), R(0), R(1000));
-                break;
+            case OMV_FC_MD: col = fcPaintClamped<R>(OMV_FC_MD, sp.density); break;
+            case OMV_FC_SG: col = fcPaintClamped<R>(OMV_FC_SG, sp.gradient); break;
+            case OMV_FC_MN: col = fcPaintMaterialName<R>(m.type); break;
             case OMV_FC_IN: {
                 V3<R> tfn = faceNormal(S, sp);
-                R inv = dot3(inDir, tfn) > R(0) ? R(1) : R(0);
-                col = mk(inv, R(1) - inv, R(0));
+                col = fcPaintInverted<R>(dot3(inDir, tfn) > R(0));
                 break;
             }
             default: col = mk(R(0), R(0), R(0)); break;
@@ -2679,6 +2746,26 @@ int ref_closest_hit(ref_scene* r, const float org[3], const float dir[3], float 
     if (!ok) return 0;
     *t = h.t; *u = h.u; *v = h.v; *inst = h.inst; *prim = h.prim; *tie = h.tie;
     return 1;
+}
+
+/* ---- false-colour paint rules and planted mistakes (pin tests) */
+void ref_pin_plant(int mask) { gPinPlant = mask; }
+void ref_fc_paint_clamped(int subject, float value, float out[3]) {
+    V3<float> c = fcPaintClamped<float>(subject, value);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+}
+void ref_fc_paint_enumerated(uint32_t id, const uint32_t* table, uint32_t size, int mid_style, float out[3]) {
+    V3<float> c = fcPaintEnumerated<float>(id, table, size, mid_style != 0);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+}
+/* kind: 0 OpenMaterial, 1 PBR, 2 the missing material's diffuse */
+void ref_fc_paint_material(int kind, float out[3]) {
+    V3<float> c = fcPaintMaterialName<float>(kind == 0 ? MAT_OM : kind == 1 ? MAT_PBR : MAT_DIFFUSE);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+}
+void ref_fc_paint_inverted(int flag, float out[3]) {
+    V3<float> c = fcPaintInverted<float>(flag != 0);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
 }
 
 /* ---- unit-level */

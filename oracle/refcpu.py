@@ -98,6 +98,11 @@ def lib():
         "ref_fresnel64": ([C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)], None),
         "ref_fresnel_grad": ([C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)], None),
         "ref_adjoint_plant": ([C.c_int], None),
+        "ref_pin_plant": ([C.c_int], None),
+        "ref_fc_paint_clamped": ([C.c_int, C.c_float, FP], None),
+        "ref_fc_paint_enumerated": ([C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32, C.c_int, FP], None),
+        "ref_fc_paint_material": ([C.c_int, FP], None),
+        "ref_fc_paint_inverted": ([C.c_int, FP], None),
         "ref_splat_plant": ([C.c_int], None),
         "ref_median3": ([FP, C.c_int, C.c_int], None),
         "ref_env_tables": ([vp, FP, C.POINTER(C.c_int), FP, C.POINTER(C.c_int), FP], C.c_int),
