@@ -463,8 +463,12 @@ int omv_batch_add_job(omv_batch* b, const omv_scene* s, const omv_camera* c,
 int omv_batch_job_count(const omv_batch* b, int* n);
 int omv_batch_set_tuning(omv_batch* b, const omv_batch_tuning* t);
 int omv_batch_get_tuning(const omv_batch* b, omv_batch_tuning* t);
-/* Times every (blocks_per_cu, tile_waves) candidate on the batch itself,
- * 'reps' runs each, keeps the fastest and reports it in best (may be NULL). */
+/* Times every candidate on the batch itself with plain launches (no graph
+ * capture during the sweep), 'reps' runs each, keeps the fastest and reports
+ * it in best (may be NULL): the jobs' wavefront sequences over 1, 2, 4 and 8
+ * streams (no more streams than jobs) and the persistent per-pixel kernel at
+ * each blocks_per_cu up to the occupancy limit x tile_waves 1, 2, 4.  The
+ * starting use_graph is kept: the choice is captured once, at the next run. */
 int omv_batch_autotune(omv_batch* b, int reps, omv_batch_tuning* best);
 /* Renders every job.  Synchronous. */
 int omv_batch_run(omv_batch* b);

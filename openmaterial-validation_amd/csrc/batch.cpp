@@ -166,12 +166,24 @@ int BatchRunner::prepare() {
 // whole sequence optionally captured into one graph (fork/join over streams).
 int BatchRunner::prepareWavefront(const std::vector<int>& order) {
     hipError_t e;
-    const int ns = tuning_.streams;
+    const int ns = activeStreams();
     while ((int)sres_.size() < ns) {
         sres_.emplace_back();
         StreamRes& r = sres_.back();
         if ((e = hipStreamCreate(&r.s)) != hipSuccess) return hipFail(e, "hipStreamCreate");
         if ((e = hipEventCreateWithFlags(&r.done, hipEventDisableTiming)) != hipSuccess) return hipFail(e, "hipEventCreate");
+    }
+    // Events that a capture records belong to that capture's graph: every
+    // capture gets its own fork / join events, so no event recorded into a
+    // graph that has since been destroyed is recorded or waited on again.
+    if (tuning_.use_graph) {
+        if (fork_) (void)hipEventDestroy(fork_);
+        fork_ = nullptr;
+        for (int k = 0; k < ns; ++k) {
+            if (sres_[k].done) (void)hipEventDestroy(sres_[k].done);
+            sres_[k].done = nullptr;
+            if ((e = hipEventCreateWithFlags(&sres_[k].done, hipEventDisableTiming)) != hipSuccess) return hipFail(e, "hipEventCreate");
+        }
     }
     if (!fork_ && (e = hipEventCreateWithFlags(&fork_, hipEventDisableTiming)) != hipSuccess) return hipFail(e, "hipEventCreate");
     if (!wfPerCu_[0]) wavefrontBlocksPerCU(wfPerCu_);
@@ -217,20 +229,27 @@ int BatchRunner::prepareWavefront(const std::vector<int>& order) {
     return OMV_OK;
 }
 
+// The job sequence over activeStreams() streams: fork from stream_, each
+// job's launches on its stream, join back into stream_.  A failed launch
+// stops issuing work but the side streams are still joined (so a capture
+// can end and a plain launch leaves no stream running ahead); the first
+// error is returned.
 int BatchRunner::launchWavefrontJobs() {
     hipError_t e;
-    const int ns = tuning_.streams;
+    const int ns = activeStreams();
     if ((e = launchFillU64(dCounters_.as<unsigned long long>(), 16, stream_)) != hipSuccess) return hipFail(e, "counter fill");
     if ((e = hipEventRecord(fork_, stream_)) != hipSuccess) return hipFail(e, "fork");
-    for (int k = 0; k < ns; ++k)
-        if ((e = hipStreamWaitEvent(sres_[k].s, fork_, 0)) != hipSuccess) return hipFail(e, "fork wait");
-    for (size_t i = 0; i < order_.size(); ++i) {
+    int forked = 0;
+    int rc = OMV_OK;
+    for (; forked < ns; ++forked)
+        if ((e = hipStreamWaitEvent(sres_[forked].s, fork_, 0)) != hipSuccess) { rc = hipFail(e, "fork wait"); break; }
+    for (size_t i = 0; rc == OMV_OK && i < order_.size(); ++i) {
         const Job& J = jobs_[order_[i]];
         StreamRes& r = sres_[i % ns];
         float* film = dFilm_.as<float>() + J.filmOff;
         const int spp = std::max(J.p.samples_per_pixel, 0);
         if (spp == 0) {   // PinholeCamera::stop with a zero count: rgb 0, alpha 1
-            if ((e = launchFilm(r.rad.as<float>(), 0, 0, J.npix, film, r.s)) != hipSuccess) return hipFail(e, "film kernel launch");
+            if ((e = launchFilm(r.rad.as<float>(), 0, 0, J.npix, film, r.s)) != hipSuccess) rc = hipFail(e, "film kernel launch");
             continue;
         }
         const int lvCap = lvCapOf(J.p);
@@ -244,19 +263,19 @@ int BatchRunner::launchWavefrontJobs() {
         L.blocksShade = cus_ * wfPerCu_[1];
         L.blocksGen = cus_ * wfPerCu_[3];
         const uint64_t chunkPix = std::max<uint64_t>(1, wfMaxSlots_ / (uint64_t)spp);
-        for (uint64_t pix0 = 0; pix0 < J.npix; pix0 += chunkPix) {
+        for (uint64_t pix0 = 0; rc == OMV_OK && pix0 < J.npix; pix0 += chunkPix) {
             const uint64_t np = std::min(chunkPix, J.npix - pix0);
             L.sample0 = pix0 * (uint64_t)spp;
             L.nslots = (uint32_t)(np * (uint64_t)spp);
-            if ((e = launchWavefront(L, r.s)) != hipSuccess) return hipFail(e, "wavefront launch");
-            if ((e = launchFilm(r.rad.as<float>(), spp, pix0, np, film, r.s)) != hipSuccess) return hipFail(e, "film kernel launch");
+            if ((e = launchWavefront(L, r.s)) != hipSuccess) rc = hipFail(e, "wavefront launch");
+            else if ((e = launchFilm(r.rad.as<float>(), spp, pix0, np, film, r.s)) != hipSuccess) rc = hipFail(e, "film kernel launch");
         }
     }
-    for (int k = 0; k < ns; ++k) {
-        if ((e = hipEventRecord(sres_[k].done, sres_[k].s)) != hipSuccess) return hipFail(e, "join");
-        if ((e = hipStreamWaitEvent(stream_, sres_[k].done, 0)) != hipSuccess) return hipFail(e, "join wait");
+    for (int k = 0; k < forked; ++k) {
+        if ((e = hipEventRecord(sres_[k].done, sres_[k].s)) != hipSuccess) { if (rc == OMV_OK) rc = hipFail(e, "join"); continue; }
+        if ((e = hipStreamWaitEvent(stream_, sres_[k].done, 0)) != hipSuccess && rc == OMV_OK) rc = hipFail(e, "join wait");
     }
-    return OMV_OK;
+    return rc;
 }
 
 int BatchRunner::launch() {
@@ -302,6 +321,13 @@ int BatchRunner::run() {
     return OMV_OK;
 }
 
+// Measures every candidate with plain launches (no graph capture during the
+// sweep), reps runs each, and keeps the fastest; the chosen configuration is
+// captured once, at the next run, when the starting tuning asked for a graph.
+// Candidates: the jobs' wavefront sequences over 1, 2, 4 and 8 streams (no
+// more streams than jobs), and the persistent per-pixel kernel at every
+// resident-blocks-per-CU count up to the occupancy limit x 1, 2 or 4 waves
+// of pixels per work item.
 int BatchRunner::autotune(int reps, omv_batch_tuning* best) {
     if (jobs_.empty()) { setError("batch: no jobs"); return OMV_E_STATE; }
     reps = std::max(1, reps);
@@ -313,9 +339,11 @@ int BatchRunner::autotune(int reps, omv_batch_tuning* best) {
     double bestMs = 1e300;
     const int tiles[] = {1, 2, 4};
     std::vector<omv_batch_tuning> cands;
-    for (int ns : {1, 2, 4, 8}) cands.push_back(omv_batch_tuning{0, 1, start.use_graph, ns});
+    for (int ns : {1, 2, 4, 8})
+        if (ns == 1 || ns <= (int)jobs_.size()) cands.push_back(omv_batch_tuning{0, 1, 0, ns});
     for (int bpc = 1; bpc <= occ; ++bpc)
-        for (int tw : tiles) cands.push_back(omv_batch_tuning{bpc, tw, start.use_graph, 0});
+        for (int tw : tiles) cands.push_back(omv_batch_tuning{bpc, tw, 0, 0});
+    tunedMs_.clear();
     for (const omv_batch_tuning& t : cands) {
         int rc = setTuning(t);
         if (rc) return rc;
@@ -325,8 +353,10 @@ int BatchRunner::autotune(int reps, omv_batch_tuning* best) {
             if (rc) return rc;
             m = std::min(m, stats_.render_ms);
         }
+        tunedMs_.push_back(m);
         if (m < bestMs) { bestMs = m; bestT = t; }
     }
+    bestT.use_graph = start.use_graph;
     int rc = setTuning(bestT);
     if (rc) return rc;
     if (best) *best = bestT;

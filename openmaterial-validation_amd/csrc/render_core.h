@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <deque>
 #include <new>
 #include <string>
@@ -228,6 +229,9 @@ private:
     int launch();
     int launchWavefrontJobs();
     void dropGraph();
+    // side streams in use: the tuning's count, at most one per job
+    int activeStreams() const { return std::max(1, std::min(tuning_.streams, (int)jobs_.size())); }
+    std::vector<double> tunedMs_;   // the last autotune's measured ms per candidate
     std::vector<Job> jobs_;
     bool anyNee_ = false;   // some job uses OMV_EXT_HDRI_NEE: the batch kernel with its code
     std::vector<int> order_;            // heaviest job first

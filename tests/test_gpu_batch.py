@@ -108,8 +108,10 @@ def test_batch_job_against_oracle(pkg, wl, ref):
 
 
 @pytest.mark.skip(reason="one run of this test ended in a host-side segmentation fault inside omv_batch_autotune "
-                         "(after the HDRI NEE / light-splat changes); no cause was found by reading the code, so the "
-                         "autotune sweep stays off the GPU until one is shown")
+                         "(after the HDRI NEE / light-splat changes); no cause was found by reading the code, nor by the "
+                         "CPU stand-in of the runtime under AddressSanitizer (tests/test_batch_hostsim.py).  That sweep "
+                         "captured a graph per candidate, re-recording fork/join events across destroyed graphs; it was "
+                         "replaced by the plain-launch sweep of test_batch_autotune_plain_sweep and stays off the GPU")
 def test_batch_autotune(pkg, wl):
     _, jobs, scenes = _small_batch(pkg, wl, n_jobs=6)
     plain = _plain_films(pkg, jobs, scenes)
@@ -120,3 +122,22 @@ def test_batch_autotune(pkg, wl):
     b.run()
     for j in range(len(jobs)):
         assert np.array_equal(b.film(j), plain[j])
+
+
+def test_batch_autotune_plain_sweep(pkg, wl):
+    """The autotuner times every candidate (wavefront sequences over 1, 2, 4
+    streams -- no more than the 6 jobs -- and the per-pixel kernel's
+    blocks/CU x tile sizes) with plain launches, keeps the fastest, and the
+    batch then runs the choice captured into a graph (fresh fork / join
+    events for the capture); every job's film equals the plain render's."""
+    _, jobs, scenes = _small_batch(pkg, wl, n_jobs=6)
+    plain = _plain_films(pkg, jobs, scenes)
+    b = _batch(pkg, jobs, scenes)
+    best = b.autotune(reps=1)
+    print(f"autotune choice {best}", flush=True)
+    assert best["use_graph"] == 1
+    assert best["streams"] in (1, 2, 4) or (best["streams"] == 0 and best["blocks_per_cu"] >= 1 and best["tile_waves"] in (1, 2, 4)), best
+    for rep in range(2):
+        b.run()
+        for j in range(len(jobs)):
+            assert np.array_equal(b.film(j), plain[j]), f"job {j} differs on run {rep}"
