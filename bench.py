@@ -358,100 +358,55 @@ def bench_material_fit(pkg, wl, args):
         "fit_loss": [round(losses[0], 3), round(losses[-1], 3)] if losses else None, "fit_steps": args.fit_steps,
     }
     if args.fd > 0:
-        line["fd_check"] = fd_check(pkg, wl, args.fd, dt, di, view, params, ids, T.cpu().numpy())
+        line["fd_check"] = fd_check(pkg, args.fd, di, view, params, T.cpu().numpy().astype(np.float64))
     return line
 
 
-def fd_check(pkg, wl, count, dt, di, view, params, ids, T):
-    """GPU adjoint vs central differences (h = 1e-3) of the oracle's detached
-    functional, on `count` texels drawn with seed 5002 among those the loss
-    sees, and on the OM (n, k).  Two comparisons:
-      * "float": d loss / d film from the float oracle's image, differences of
-        the float oracle's functional (the device's arithmetic), so the only
-        differences left are the rare samples whose paths diverge between
-        device and host transcendentals;
-      * "fp64": d loss / d film from the GPU image, differences of the FP64
-        oracle's functional (float and double paths also diverge where a
-        direction's rounding flips a branch)."""
-    import copy
+def fd_check(pkg, count, di, view, params, T):
+    """The GPU adjoint vs central differences (h = 1e-3) of the FP64 detached
+    functional on the device's own paths (omv_renderer_render_detached), on
+    `count` texels drawn with seed 5002 among those the loss sees and on the
+    OM (n, k); loss = sum (I - T)^2 over the detached film I."""
+    import importlib
     import torch
-    ref = ge.load_oracle()
+    gc = importlib.import_module("omv_amd.gradcheck")
     dev = torch.device("cuda")
-    s = pkg.Scene()
-    di.apply(s, pkg.pbr_desc)
+    s0, se = pkg.Scene(), pkg.Scene()
+    di.apply(s0, pkg.pbr_desc)
+    di.apply(se, pkg.pbr_desc)
     cam = pkg.Camera(view.width, view.height, view.position, view.forward, view.up, view.yfov)
     r = pkg.Renderer(pkg.default_params(**params))
-    r.set_scene(s)
-    img = r.render(cam)
-    rs = ref.Scene()
-    di.apply(rs, ref.pbr_desc)
-    p = ref.default_params(**params)
-    cd = cam.desc()
-    threads = host_cpu()["threads"]
-    img32, _ = rs.render(p, cd, threads=threads)
+    r.set_scene(s0)
+    I0 = r.render_detached(cam, se)
+    d_dev = torch.from_numpy(gc.d_loss(I0, T)).to(dev)
     tex = di.textures[0][1].shape[0]
-    mask = np.array([1, 1, 1, 0], np.float32)
+    gb = torch.zeros((tex, tex, 4), device=dev)
+    gmr = torch.zeros((tex, tex, 4), device=dev)
+    gom = torch.zeros(6, device=dev)
+    torch.cuda.synchronize()
+    r.backward(cam, d_dev.data_ptr(), textures=[(0, gb.data_ptr()), (1, gmr.data_ptr())], om_materials=[1], om_grads_ptr=gom.data_ptr())
+    torch.cuda.synchronize()
+    grads = {0: gb.cpu().numpy().astype(np.float64), 1: gmr.cpu().numpy().astype(np.float64)}
+    textures = {0: np.array(di.textures[0][1], np.float32), 1: np.array(di.textures[1][1], np.float32)}
 
-    def adjoint(base):
-        dfilm = torch.from_numpy(np.ascontiguousarray(2.0 * (base - T) * mask)).to(dev)
-        gb = torch.zeros((tex, tex, 4), device=dev)
-        gmr = torch.zeros((tex, tex, 4), device=dev)
-        gom = torch.zeros(6, device=dev)
-        r.backward(cam, dfilm.data_ptr(), textures=[(0, gb.data_ptr()), (1, gmr.data_ptr())], om_materials=[1],
-                   om_grads_ptr=gom.data_ptr())
-        return {0: gb.cpu().numpy(), 1: gmr.cpu().numpy()}, gom.cpu().numpy()
+    def near_clamp(t, y, x, c, texs):
+        if not (t == 1 and c == 1):
+            return False
+        img = texs[t][..., c]
+        h, w = img.shape
+        return float(img[np.ix_([(y + d) % h for d in (-1, 0, 1)], [(x + d) % w for d in (-1, 0, 1)])].max()) > 0.998
 
-    grads = {"float": adjoint(img32), "fp64": adjoint(img)}
-
-    def loss_of(d, fp64):
-        e = ref.Scene()
-        d.apply(e, ref.pbr_desc)
-        out = rs.render_detached(e, p, cd, fp64=fp64, threads=threads)
-        return float(((out[..., :3].astype(np.float64) - T[..., :3]) ** 2).sum())
-
-    rng = np.random.default_rng(5002)
-    g0 = grads["float"][0]
-    cands = [(t, c) for t, cs in ((0, (0, 1, 2)), (1, (1, 2))) for c in cs]
-    picks = []
-    while len(picks) < count:
-        t, c = cands[int(rng.integers(len(cands)))]
-        nz = np.argwhere(np.abs(g0[t][..., c]) > 1e-3 * np.abs(g0[t][..., c]).max())
-        y, x = nz[int(rng.integers(len(nz)))]
-        picks.append((t, int(y), int(x), c))
-
-    def mutators():
-        for t, y, x, c in picks:
-            def mut(d, dh, t=t, y=y, x=x, c=c):
-                a = d.textures[t][1]
-                a[y, x, c] = np.float32(a[y, x, c] + np.float32(dh))
-                return float(a[y, x, c])
-            yield ("tex", t, y, x, c), mut
-        for j in range(6):
-            def mut(d, dh, j=j):
-                nn, kk = np.array(d.materials[1][1], np.float32), np.array(d.materials[1][2], np.float32)
-                arr = nn if j < 3 else kk
-                arr[j % 3] = np.float32(arr[j % 3] + np.float32(dh))
-                d.materials[1] = ("om_nk", nn, kk)
-                return float(arr[j % 3])
-            yield ("om", j), mut
-
-    out = {"texels": count, "seed": 5002, "h": 1e-3}
-    for mode, fp64 in (("float", False), ("fp64", True)):
-        g, gom = grads[mode]
-        rel, om_rel = [], []
-        for key, mut in mutators():
-            vals = []
-            for sgn in (1.0, -1.0):
-                d = copy.deepcopy(di)
-                vals += [mut(d, sgn * 1e-3), loss_of(d, fp64)]
-            fd = (vals[1] - vals[3]) / (vals[0] - vals[2])
-            adj = g[key[1]][key[2], key[3], key[4]] if key[0] == "tex" else gom[key[1]]
-            (rel if key[0] == "tex" else om_rel).append(abs(adj - fd) / max(abs(fd), 1e-12))
-            print(f"# fd {mode} {key}: adjoint {adj:.6g} fd {fd:.6g}", file=sys.stderr, flush=True)
-        rel = np.array(rel)
-        out[mode] = {"median_rel": float(np.median(rel)), "p90_rel": float(np.quantile(rel, 0.9)), "max_rel": float(rel.max()),
-                     "within_1e-3": float(np.mean(rel < 1e-3)), "om_max_rel": float(max(om_rel))}
-    return out
+    picks = gc.pick_texels(grads, count, 5002, ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2)), textures, avoid=near_clamp)
+    t0 = time.perf_counter()
+    checks = gc.central_differences(r, cam, se, T, textures, picks, om=(1, di.materials[1][1], di.materials[1][2]), h=1e-3)
+    rel = gc.relative_errors(checks, grads, gom.cpu().numpy().astype(np.float64))
+    for key, adj, fd, e in rel:
+        print(f"# fd {key}: adjoint {adj:.7g} fd {fd:.7g} rel {e:.3g}", file=sys.stderr, flush=True)
+    r_tex = np.array([e[3] for e in rel if e[0][0] == "tex"])
+    r_om = np.array([e[3] for e in rel if e[0][0] == "om"])
+    return {"texels": len(picks), "seed": 5002, "h": 1e-3, "reference": "central differences of the FP64 detached functional on the device's paths",
+            "median_rel": float(np.median(r_tex)), "p90_rel": float(np.quantile(r_tex, 0.9)), "max_rel": float(r_tex.max()),
+            "within_1e-3": float(np.mean(r_tex <= 1e-3)), "om_max_rel": float(r_om.max()), "seconds": round(time.perf_counter() - t0, 2)}
 
 
 def main():

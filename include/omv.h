@@ -434,6 +434,19 @@ typedef struct omv_adjoint_targets {
  * extension return OMV_E_STATE. */
 int omv_renderer_backward(omv_renderer* r, omv_camera* c, const float* d_film, const omv_adjoint_targets* t);
 
+/* FP64 evaluation of the functional omv_renderer_backward differentiates
+ * (no reference counterpart; for checking gradients by central finite
+ * differences on the device's own paths): directions, lobes and hits are
+ * sampled from the renderer's scene exactly as omv_renderer_render samples
+ * them; every bounce weight is re-evaluated with eval's parameters (the
+ * sampled PBR lobe's w * f_eval / f_scene, an OM mirror's Fresnel term of
+ * eval's (n, k)), with material inputs, weights, throughput and radiance in
+ * double.  eval is a committed scene with the same meshes, material kinds
+ * and texture sizes (a copy with some texels or (n, k) changed).  film:
+ * W*H*4 doubles (rgb + 1), host or device pointer.  Same scope as
+ * omv_renderer_backward (OMV_E_STATE with point lights or an extension). */
+int omv_renderer_render_detached(omv_renderer* r, omv_camera* c, const omv_scene* eval, double* film);
+
 /* ------------------------------------------------------ batched jobs */
 /* Many independent validation renders (each one BDPTRenderer::render call of
  * main.cpp:489-521 with its own scene, camera and parameters) run as ONE

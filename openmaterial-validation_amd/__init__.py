@@ -289,6 +289,7 @@ def _load():
         "omv_renderer_render": [vp, vp], "omv_renderer_render_noalloc": [vp, vp, C.c_void_p, C.c_void_p, C.c_void_p], "omv_renderer_render_fc": [vp, vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)],
         "omv_renderer_get_stats": [vp, C.POINTER(RenderStats)],
         "omv_renderer_backward": [vp, vp, C.c_void_p, C.POINTER(AdjointTargets)],
+        "omv_renderer_render_detached": [vp, vp, vp, C.c_void_p],
         "omv_renderer_post": [vp, vp, C.c_float, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_uint8)],
         "omv_scene_update_texture": [vp, C.c_int, C.c_void_p],
         "omv_scene_update_material_om_nk": [vp, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)],
@@ -521,6 +522,14 @@ class Renderer:
         t.n_textures, t.texture_ids, t.texture_grads = len(textures), ids, ptrs
         t.n_om_materials, t.om_material_ids, t.om_grads = len(om_materials), oms, C.c_void_p(om_grads_ptr or None)
         self._check(self.lib.omv_renderer_backward(self.h, cam.h, C.c_void_p(d_film_ptr), C.byref(t)))
+
+    def render_detached(self, cam: Camera, eval_scene: "Scene") -> np.ndarray:
+        """FP64 film of the functional backward() differentiates: paths
+        sampled from this renderer's scene, weights with eval_scene's
+        parameters.  Returns (H, W, 4) float64."""
+        out = np.zeros(cam.width * cam.height * 4, np.float64)
+        self._check(self.lib.omv_renderer_render_detached(self.h, cam.h, eval_scene.h, C.c_void_p(out.ctypes.data)))
+        return out.reshape(cam.height, cam.width, 4)
 
     def post(self, cam: Camera, gamma=0.5, denoise=False):
         """main.cpp's post pass on the device: (gamma-corrected [+ median] float film, 8-bit image)."""

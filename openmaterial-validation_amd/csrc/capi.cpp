@@ -227,6 +227,10 @@ int omv_renderer_backward(omv_renderer* r, omv_camera* c, const float* d_film, c
     OMV_NULLCHK(r); OMV_NULLCHK(c); OMV_NULLCHK(t);
     return r->r.backward(c->c, d_film, *t);
 }
+int omv_renderer_render_detached(omv_renderer* r, omv_camera* c, const omv_scene* eval, double* film) {
+    OMV_NULLCHK(r); OMV_NULLCHK(c); OMV_NULLCHK(eval); OMV_NULLCHK(film);
+    return r->r.renderDetached(c->c, eval->s, film);
+}
 int omv_scene_env_tables(omv_scene* s, float* rp, int32_t* ra, float* cp, int32_t* ca, float* tp) {
     OMV_NULLCHK(s); OMV_NULLCHK(rp); OMV_NULLCHK(ra); OMV_NULLCHK(cp); OMV_NULLCHK(ca); OMV_NULLCHK(tp);
     return s->s.envTables(rp, ra, cp, ca, tp);

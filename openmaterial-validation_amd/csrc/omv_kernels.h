@@ -70,6 +70,11 @@ int adjointBlocksPerCU();
 hipError_t launchAdjoint(const OmvSceneView& S, const omv_camera_desc& cam, const omv_renderer_params& p, uint64_t sample0,
                          uint64_t nsamples, const float* dfilm, const int* texSlot, float* const* texGrad, const int* omSlot,
                          float* omGrad, float* lv, uint32_t* gstack, unsigned long long* counters, int blocks, hipStream_t stream);
+// FP64 detached film (adjoint.hip): directions from S0, weights with S1's
+// parameters, W*H*4 doubles; gstack holds blocks * OMV_BLOCK spill stacks.
+int detachedBlocksPerCU();
+hipError_t launchDetached(const OmvSceneView& S0, const OmvSceneView& S1, const omv_camera_desc& cam, const omv_renderer_params& p,
+                          uint64_t npix, double* film, uint32_t* gstack, unsigned long long* counters, int blocks, hipStream_t stream);
 // Film post pass (post.hip): gamma (+ 3x3 median when denoise) into outF, and
 // the 8-bit truncation of outF into outU8 when non-null; tmp holds w*h float4.
 hipError_t launchPost(const float* film, float* tmp, float* outF, uint8_t* outU8, int w, int h, float A, float gamma, bool denoise,

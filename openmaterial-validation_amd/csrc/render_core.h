@@ -160,6 +160,7 @@ public:
     int render(PinholeCamera& cam);
     int renderFc(PinholeCamera& cam, int32_t* ids, uint8_t* tie);
     int renderNoalloc(const PinholeCamera& cam, float* film, hipStream_t stream, omv_render_stats* st);
+    int renderDetached(PinholeCamera& cam, const RenderScene& eval, double* out);
     int backward(PinholeCamera& cam, const float* dFilm, const omv_adjoint_targets& t);
     int post(PinholeCamera& cam, float gamma, bool denoise, float* outF, uint8_t* outU8);
     const omv_render_stats& stats() const { return stats_; }
@@ -185,7 +186,7 @@ private:
     void* hostStage_ = nullptr;
     int cus_ = 0;
     DeviceBuffer dRad_, dLv_, dStack_, dCounters_, dFilm_, dIds_, dTie_, dArena_, dQcount_, dDfilm_, dAdjMaps_, dPost_, dSplat_;
-    DeviceBuffer dFcTable_, dFcInstBase_;
+    DeviceBuffer dFcTable_, dFcInstBase_, dDetached_;
     int fcTableSubject_ = -1;          // the table in dFcTable_ is for (subject, seed, size)
     uint32_t fcTableSeed_ = 0;
     uint64_t fcTableSize_ = 0;

@@ -210,6 +210,61 @@ int BDPTRenderer::render(PinholeCamera& cam) {
     return OMV_OK;
 }
 
+// FP64 evaluation of the functional backward() differentiates: directions
+// from this renderer's scene, weights with eval's parameters (adjoint.hip).
+int BDPTRenderer::renderDetached(PinholeCamera& cam, const RenderScene& eval, double* out) {
+    if (!scene_ || !scene_->committed()) { setError("render_detached: no committed scene"); return OMV_E_STATE; }
+    if (!eval.committed()) { setError("render_detached: evaluation scene not committed"); return OMV_E_STATE; }
+    if (!out) { setError("render_detached: null film"); return OMV_E_ARG; }
+    const OmvSceneView& S = scene_->view();
+    const OmvSceneView& E = eval.view();
+    if (S.num_lights > 0) { setError("render_detached: scenes with point lights are not differentiated"); return OMV_E_STATE; }
+    if (params_.extensions & (OMV_EXT_HDRI_NEE | OMV_EXT_OM_ROUGH)) { setError("render_detached: estimator extensions are not differentiated"); return OMV_E_STATE; }
+    // eval must differ from the scene in material parameters and texels only
+    bool same = E.num_meshes == S.num_meshes && E.num_mats == S.num_mats && E.num_texs == S.num_texs;
+    for (int i = 0; same && i < S.num_meshes; ++i) {
+        int fa = 0, fb = 0, sa = 0, sb = 0, ma = 0, mb = 0;
+        same = scene_->meshInfo(i, nullptr, &fa, &sa, &ma) == OMV_OK && eval.meshInfo(i, nullptr, &fb, &sb, &mb) == OMV_OK && fa == fb &&
+               sa == sb && ma == mb;
+    }
+    for (int i = 0; same && i < S.num_texs; ++i) {
+        int wa = 0, ha = 0, wb = 0, hb = 0;
+        same = scene_->textureSize(i, &wa, &ha) == OMV_OK && eval.textureSize(i, &wb, &hb) == OMV_OK && wa == wb && ha == hb;
+    }
+    for (int i = 0; same && i < S.num_mats; ++i) same = scene_->isOm(i) == eval.isOm(i);
+    if (!same) { setError("render_detached: the evaluation scene's meshes, materials or textures differ in structure"); return OMV_E_ARG; }
+    const omv_renderer_params& p = params_;
+    const uint64_t npix = (uint64_t)cam.desc.width * cam.desc.height;
+    std::memset(&stats_, 0, sizeof(stats_));
+    int rc = initStream();
+    if (rc) return rc;
+    const int blocks = cus_ * detachedBlocksPerCU();
+    const uint64_t lanes = (uint64_t)blocks * OMV_BLOCK;
+    hipError_t e;
+    if ((e = dStack_.reserve(lanes * (OMV_STACK_TOTAL - OMV_STACK_LDS) * sizeof(uint32_t))) != hipSuccess) return hipFail(e, "stack spill");
+    if ((e = dCounters_.reserve(8 * sizeof(unsigned long long))) != hipSuccess) return hipFail(e, "counters");
+    if ((e = dDetached_.reserve(npix * 4 * sizeof(double))) != hipSuccess) return hipFail(e, "detached film");
+    if ((e = hipMemsetAsync(dCounters_.ptr, 0, 8 * sizeof(unsigned long long), stream_)) != hipSuccess) return hipFail(e, "memset");
+    (void)hipEventRecord(ev_[0], stream_);
+    if ((e = launchDetached(S, E, cam.desc, p, npix, dDetached_.as<double>(), dStack_.as<uint32_t>(), dCounters_.as<unsigned long long>(),
+                            blocks, stream_)) != hipSuccess)
+        return hipFail(e, "detached kernel launch");
+    (void)hipEventRecord(ev_[1], stream_);
+    if ((e = hipEventSynchronize(ev_[1])) != hipSuccess) return hipFail(e, "detached kernel");
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, ev_[0], ev_[1]);
+    unsigned long long c[8] = {0};
+    if ((e = hipMemcpy(c, dCounters_.ptr, sizeof(c), hipMemcpyDeviceToHost)) != hipSuccess) return hipFail(e, "counter readback");
+    if ((e = hipMemcpy(out, dDetached_.ptr, npix * 4 * sizeof(double), hipMemcpyDefault)) != hipSuccess) return hipFail(e, "film readback");
+    stats_.samples = npix * (uint64_t)std::max(p.samples_per_pixel, 0);
+    stats_.rays = c[0]; stats_.camera_rays = c[1];
+    stats_.nodes_visited = c[4]; stats_.tris_tested = c[5]; stats_.stack_overflows = c[6];
+    stats_.render_ms = ms;
+    stats_.kernel_ms = ms;
+    if (stats_.stack_overflows) { setError("render_detached: traversal stack overflow"); return OMV_E_INTERNAL; }
+    return OMV_OK;
+}
+
 // Allocation-free render: validates on the host, then one persistent
 // per-pixel launch over the fixed scratch array (noalloc.hip).  Touches none
 // of this renderer's buffers, streams or events.
