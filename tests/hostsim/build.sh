@@ -13,5 +13,5 @@ pids=()
 for f in scene bvh renderer batch capi image_io; do g++ "${FL[@]}" -c "$C/$f.cpp" -o "$OUT/$f.o" & pids+=($!); done
 g++ "${FL[@]}" -c "$HERE/hipstub.cpp" -o "$OUT/hipstub.o" & pids+=($!)
 for p in "${pids[@]}"; do wait "$p"; done
-g++ -shared -fsanitize=address -o "$OUT/libomv_hostsim.so" "$OUT"/scene.o "$OUT"/bvh.o "$OUT"/renderer.o "$OUT"/batch.o \
-    "$OUT"/capi.o "$OUT"/image_io.o "$OUT"/hipstub.o
+g++ -shared -fsanitize=address -Wl,-z,defs -o "$OUT/libomv_hostsim.so" "$OUT"/scene.o "$OUT"/bvh.o "$OUT"/renderer.o "$OUT"/batch.o \
+    "$OUT"/capi.o "$OUT"/image_io.o "$OUT"/hipstub.o -lpthread

@@ -375,6 +375,18 @@ hipError_t launchAdjoint(const OmvSceneView&, const omv_camera_desc&, const omv_
     op(stream, "adjoint", []() {});
     return hipSuccess;
 }
+int detachedBlocksPerCU() { return 2; }
+hipError_t launchDetached(const OmvSceneView& S0, const OmvSceneView& S1, const omv_camera_desc&, const omv_renderer_params&, uint64_t npix,
+                          double* film, uint32_t* gstack, unsigned long long* counters, int blocks, hipStream_t stream) {
+    op(stream, "detached", [=]() {
+        inRange(film, npix * 32, "detached film");
+        inRange(gstack, (size_t)blocks * OMV_BLOCK * (OMV_STACK_TOTAL - OMV_STACK_LDS) * 4, "detached gstack");
+        inRange(counters, 64, "detached counters");
+        sceneCheck(S0, "detached");
+        sceneCheck(S1, "detached eval");
+    });
+    return hipSuccess;
+}
 hipError_t launchPost(const float*, float*, float*, uint8_t*, int, int, float, float, bool, hipStream_t stream) {
     op(stream, "post", []() {});
     return hipSuccess;
