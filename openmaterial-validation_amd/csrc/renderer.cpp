@@ -98,7 +98,8 @@ int BDPTRenderer::setSchedule(int s) {
 
 int BDPTRenderer::setTuning(const omv_render_tuning& t) {
     if (t.trace_keep < 0 || t.trace_keep > 64 || t.trace_blocks_per_cu < 0 || t.chunk_samples < 0 || t.traversal < 0 || t.traversal > 2 ||
-        t.sort_rays < 0 || t.sort_rays > 2 || t.chunk_streams < 0 || t.chunk_streams > 2 || t.chunk_stagger < 0) {
+        t.sort_rays < 0 || t.sort_rays > 2 || t.chunk_streams < 0 || t.chunk_streams > 2 || t.chunk_stagger < 0 || t.occlusion < 0 ||
+        t.occlusion > 1) {
         setError("set_tuning: value out of range");
         return OMV_E_RANGE;
     }
@@ -109,6 +110,7 @@ int BDPTRenderer::setTuning(const omv_render_tuning& t) {
 OmvSceneView BDPTRenderer::viewForLaunch() const {
     OmvSceneView v = scene_->view();
     if (tuning_.traversal) v.trav_wave = tuning_.traversal == 2 ? 1 : 0;
+    v.closest_only = tuning_.occlusion == 1 ? 1 : 0;
     return v;
 }
 
@@ -138,7 +140,7 @@ int BDPTRenderer::prepare(int which, uint64_t lanesWanted, int& blocks) {
     return OMV_OK;
 }
 
-int BDPTRenderer::render(PinholeCamera& cam) {
+int BDPTRenderer::render(PinholeCamera& cam, float* filmDev) {
     if (!scene_ || !scene_->committed()) { setError("render: no committed scene"); return OMV_E_STATE; }
     const omv_renderer_params& p = params_;
     if (p.samples_per_pixel < 0 || p.camera_bounces < 0 || p.light_bounces < 0) { setError("render: negative parameter"); return OMV_E_ARG; }
@@ -146,16 +148,21 @@ int BDPTRenderer::render(PinholeCamera& cam) {
     const int spp = p.samples_per_pixel;
     std::memset(&stats_, 0, sizeof(stats_));
     filmOf_ = nullptr;
-    try {
-        cam.film.resize(npix * 4);   // written from the device below
-    } catch (const std::bad_alloc&) {
-        setError("render: cannot allocate the host film");
-        return OMV_E_NOMEM;
+    if (!filmDev) {
+        try {
+            cam.film.resize(npix * 4);   // written from the device below
+        } catch (const std::bad_alloc&) {
+            setError("render: cannot allocate the host film");
+            return OMV_E_NOMEM;
+        }
     }
     int rc = initStream();
     if (rc) return rc;
     hipError_t e;
-    if ((e = dFilm_.reserve(npix * 4 * sizeof(float))) != hipSuccess) return hipFail(e, "film");
+    if (!filmDev && (e = dFilm_.reserve(npix * 4 * sizeof(float))) != hipSuccess) return hipFail(e, "film");
+    // the film the kernels write: the renderer's (read back into the camera)
+    // or the caller's device buffer (left in device memory)
+    film_ = filmDev ? filmDev : dFilm_.as<float>();
     if (!hostStage_ && (e = hipHostMalloc(&hostStage_, 8 * sizeof(unsigned long long), hipHostMallocDefault)) != hipSuccess) {
         hostStage_ = nullptr;
         return hipFail(e, "counter staging");
@@ -163,7 +170,7 @@ int BDPTRenderer::render(PinholeCamera& cam) {
     if ((e = dCounters_.reserve(8 * sizeof(unsigned long long))) != hipSuccess) return hipFail(e, "counters");
     if ((e = hipMemsetAsync(dCounters_.ptr, 0, 8 * sizeof(unsigned long long), stream_)) != hipSuccess) return hipFail(e, "memset");
     // PinholeCamera::init (PinholeCamera.cpp:65-68)
-    if ((e = hipMemsetAsync(dFilm_.ptr, 0, npix * 4 * sizeof(float), stream_)) != hipSuccess) return hipFail(e, "memset film");
+    if ((e = hipMemsetAsync(film_, 0, npix * 4 * sizeof(float), stream_)) != hipSuccess) return hipFail(e, "memset film");
     // OMV_EXT_LIGHT_SPLAT: splat sums over the whole render, added to the film at the end
     splat_ = nullptr;
     if ((p.extensions & OMV_EXT_LIGHT_SPLAT) && scene_->view().num_lights > 0 && spp > 0) {
@@ -177,19 +184,19 @@ int BDPTRenderer::render(PinholeCamera& cam) {
     if (spp > 0) {
         rc = schedule_ == OMV_SCHEDULE_WAVEFRONT ? renderWavefront(cam, kernelMs) : renderSamples(cam, kernelMs);
         if (rc) return rc;
-        if (splat_ && (e = launchSplatAdd(splat_, npix, (float)(1.0 / ((double)npix * (double)spp)), dFilm_.as<float>(), stream_)) !=
+        if (splat_ && (e = launchSplatAdd(splat_, npix, (float)(1.0 / ((double)npix * (double)spp)), film_, stream_)) !=
                           hipSuccess)
             return hipFail(e, "splat kernel launch");
     } else {
         // PinholeCamera::stop with a zero count leaves rgb = 0 and alpha = 1
         std::vector<float> z(npix * 4, 0.0f);
         for (uint64_t i = 0; i < npix; ++i) z[i * 4 + 3] = 1.0f;
-        if ((e = hipMemcpyAsync(dFilm_.ptr, z.data(), z.size() * 4, hipMemcpyHostToDevice, stream_)) != hipSuccess) return hipFail(e, "film");
+        if ((e = hipMemcpyAsync(film_, z.data(), z.size() * 4, hipMemcpyHostToDevice, stream_)) != hipSuccess) return hipFail(e, "film");
     }
     (void)hipEventRecord(ev_[1], stream_);
     // film (pinned) and counters on the same stream, one wait
     unsigned long long* hostCnt = static_cast<unsigned long long*>(hostStage_);
-    if ((e = hipMemcpyAsync(cam.film.data(), dFilm_.ptr, npix * 4 * sizeof(float), hipMemcpyDeviceToHost, stream_)) != hipSuccess)
+    if (!filmDev && (e = hipMemcpyAsync(cam.film.data(), dFilm_.ptr, npix * 4 * sizeof(float), hipMemcpyDeviceToHost, stream_)) != hipSuccess)
         return hipFail(e, "film readback");
     if ((e = hipMemcpyAsync(hostCnt, dCounters_.ptr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_)) != hipSuccess)
         return hipFail(e, "counter readback");
@@ -205,7 +212,7 @@ int BDPTRenderer::render(PinholeCamera& cam) {
     stats_.nodes_visited = c[4]; stats_.tris_tested = c[5]; stats_.stack_overflows = c[6];
     stats_.render_ms = total;
     stats_.kernel_ms = kernelMs;
-    filmOf_ = &cam;
+    filmOf_ = filmDev ? nullptr : &cam;
     if (stats_.stack_overflows) { setError("render: traversal stack overflow"); return OMV_E_INTERNAL; }
     return OMV_OK;
 }
@@ -338,7 +345,7 @@ int BDPTRenderer::renderSamples(PinholeCamera& cam, double& kernelMs) {
                             dStack_.as<uint32_t>(), dCounters_.as<unsigned long long>(), blocks, splat_, stream_)) != hipSuccess)
             return hipFail(e, "sample kernel launch");
         (void)hipEventRecord(ev_[3], stream_);
-        if ((e = launchFilm(dRad_.as<float>(), spp, pix0, np, dFilm_.as<float>(), stream_)) != hipSuccess) return hipFail(e, "film kernel launch");
+        if ((e = launchFilm(dRad_.as<float>(), spp, pix0, np, film_, stream_)) != hipSuccess) return hipFail(e, "film kernel launch");
         if ((e = hipEventSynchronize(ev_[3])) != hipSuccess) return hipFail(e, "sample kernel");
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, ev_[2], ev_[3]);
@@ -430,7 +437,7 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
         const char* what = nullptr;
         if (nstreams == 2 && k > 0 && (e = hipStreamWaitEvent(st[j], chunkEv_[1 - j], 0)) != hipSuccess) what = "hipStreamWaitEvent";
         else if ((e = launchWavefront(Lk, st[j])) != hipSuccess) what = "wavefront launch";
-        else if ((e = launchFilm(rad[j], spp, pix0, np, dFilm_.as<float>(), st[j])) != hipSuccess) what = "film kernel launch";
+        else if ((e = launchFilm(rad[j], spp, pix0, np, film_, st[j])) != hipSuccess) what = "film kernel launch";
         if (what) {
             // both streams drained before the buffers can be used again
             if (nstreams == 2) (void)hipStreamSynchronize(stream2_);

@@ -763,6 +763,7 @@ int RenderScene::commit(std::string& log) {
     v.total_faces = faceBase;
     v.fc_subject = fcSubject_;
     v.any_emissive = anyEmissive;
+    for (const OmvMaterial& m : mats_) v.any_masked |= (m.type == OMV_MAT_PBR && m.alpha_mode == OMV_ALPHA_MASK) ? 1 : 0;
     for (int a = 0; a < 3; ++a) {
         const float ext = sceneHi[a] - sceneLo[a];
         v.scene_lo[a] = std::isfinite(sceneLo[a]) ? sceneLo[a] : 0.0f;

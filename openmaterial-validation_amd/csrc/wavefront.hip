@@ -194,7 +194,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
             float tnear = a.w;
             ++nShadow;
             bool visible;
-            if (!sceneIntersect(S, xyz(a), xyz(dd), tnear, dd.w, h, st, cnt, rays)) {
+            if (!sceneIntersect(S, xyz(a), xyz(dd), tnear, dd.w, h, st, cnt, rays, connectStop(xyz(a), xyz(b)))) {
                 visible = true;
             } else {
                 f3 df = sub3(hitPosition(S, h), xyz(b));
@@ -209,13 +209,13 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
             const float4 o = q.o[e], d = q.d[e];
             float tnear = o.w;
             ++nShadow;
-            const bool hit = sceneIntersect(S, xyz(o), xyz(d), tnear, FLT_MAX, h, st, cnt, rays);
+            const bool hit = sceneIntersect(S, xyz(o), xyz(d), tnear, FLT_MAX, h, st, cnt, rays, FLT_MAX);
             vis[__float_as_uint(d.w)] = hit ? 0 : 2;
         } else if (kKind == 4) {
             const float4 a = sq.o[e], dd = sq.d[e], b = sq.b[e];
             float tnear = a.w;
             ++nShadow;
-            const bool hit = sceneIntersect(S, xyz(a), xyz(dd), tnear, dd.w, h, st, cnt, rays);
+            const bool hit = sceneIntersect(S, xyz(a), xyz(dd), tnear, dd.w, h, st, cnt, rays, FLT_MAX);
             splatAdd(splat, !hit, __float_as_uint(b.w), xyz(b));
         } else {
             const float4 o = q.o[e], d = q.d[e];
@@ -229,6 +229,65 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
         }
     }
     flushCounters(counters, rays, nCam, nLight, nShadow, cnt);
+}
+
+// Fused traversal of one camera round boundary: the camera extension rays of
+// bounce i + 1 (kind 0), then the connection rays (kind 2) and environment
+// rays (kind 3) of bounce i, one grid over the three queues back to back.
+// The three are independent given bounce i's shading, and each entry gets
+// exactly the per-ray arithmetic of its own kind, so the outputs are those
+// of three separate launches; one launch pays one latency tail instead of
+// up to three.  A null count is an empty queue.
+__global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused(
+    OmvSceneView S, WfQueue q, const uint32_t* __restrict__ qCount, WfShadowQ sq, const uint32_t* __restrict__ sqCount, WfQueue eq,
+    const uint32_t* __restrict__ eqCount, WfHits H, uint8_t* __restrict__ vis, uint8_t* __restrict__ nvis, uint32_t nslots, int lc,
+    uint32_t* __restrict__ gstack, unsigned long long* __restrict__ counters) {
+    const uint32_t gthreads = gridDim.x * blockDim.x;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st;
+    stackInit(st, gstack, gtid, gthreads);
+    Counters cnt; cnt.nodes = 0; cnt.tris = 0; cnt.overflow = 0;
+    uint32_t rays = 0, nCam = 0, nShadow = 0;
+    const uint32_t nA = qCount ? *qCount : 0u, nB = sqCount ? *sqCount : 0u, nC = eqCount ? *eqCount : 0u;
+    const uint32_t total = nA + nB + nC;
+    for (uint32_t e = gtid; e < total; e += gthreads) {
+        Hit h;
+        if (e < nA) {
+            const float4 o = q.o[e], d = q.d[e];
+            float tnear = o.w;
+            ++nCam;
+            const bool hit = sceneIntersect(S, xyz(o), xyz(d), tnear, FLT_MAX, h, st, cnt, rays);
+            float4 r; r.x = h.t; r.y = h.u; r.z = h.v; r.w = tnear;
+            int2 id; id.x = hit ? h.inst : -1; id.y = h.prim | (h.tie ? (int)0x80000000 : 0);
+            H.h[e] = r;
+            H.id[e] = id;
+        } else if (e < nA + nB) {
+            const uint32_t k = e - nA;
+            const float4 a = sq.o[k], dd = sq.d[k], b = sq.b[k];
+            float tnear = a.w;
+            ++nShadow;
+            bool visible;
+            if (!sceneIntersect(S, xyz(a), xyz(dd), tnear, dd.w, h, st, cnt, rays, connectStop(xyz(a), xyz(b)))) {
+                visible = true;
+            } else {
+                f3 df = sub3(hitPosition(S, h), xyz(b));
+                float sd = dot3(df, df);
+                float safeD = 10.0f * FLT_EPSILON * maxAbs(xyz(b));
+                visible = sd <= safeD * safeD;
+            }
+            const uint32_t id = __float_as_uint(b.w);
+            const uint32_t slot = id / (uint32_t)lc, l = id - slot * (uint32_t)lc;
+            vis[(uint64_t)l * nslots + slot] = visible ? 2 : 0;
+        } else {
+            const uint32_t k = e - nA - nB;
+            const float4 o = eq.o[k], d = eq.d[k];
+            float tnear = o.w;
+            ++nShadow;
+            const bool hit = sceneIntersect(S, xyz(o), xyz(d), tnear, FLT_MAX, h, st, cnt, rays, FLT_MAX);
+            nvis[__float_as_uint(d.w)] = hit ? 0 : 2;
+        }
+    }
+    flushCounters(counters, rays, nCam, 0u, nShadow, cnt);
 }
 
 // Persistent form of the trace kernel: each lane keeps one ray and refills
@@ -281,6 +340,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_trace_pt(OmvSceneView S, WfQ
                     itr = 10;
                     ++rays;
                     T.start(S, org, dir, tn, t1, st, cnt);
+                    T.tstop = (S.any_masked || S.closest_only) ? -1.0f : (kKind == 2 ? connectStop(org, B) : (kKind >= 3 ? FLT_MAX : -1.0f));
                     busy = true;
                 } else {
                     drained = true;
@@ -738,7 +798,9 @@ int wavefrontCounterCount(int B, int lvCap) { return 4 * lvCap + 6 * B + 16; }
 void wavefrontBlocksPerCU(int out[5]) {
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
     int t = occ(omv_wf_trace<0>, lds);
-    for (int k : {occ(omv_wf_trace<1>, lds), occ(omv_wf_trace<2>, lds), occ(omv_wf_trace<3>, lds), occ(omv_wf_trace<4>, lds)}) t = std::min(t, k);
+    for (int k : {occ(omv_wf_trace<1>, lds), occ(omv_wf_trace<2>, lds), occ(omv_wf_trace<3>, lds), occ(omv_wf_trace<4>, lds),
+                  occ(omv_wf_trace_fused, lds)})
+        t = std::min(t, k);
     int tp = occ(omv_wf_trace_pt<0>, lds);
     for (int k : {occ(omv_wf_trace_pt<1>, lds), occ(omv_wf_trace_pt<2>, lds), occ(omv_wf_trace_pt<3>, lds), occ(omv_wf_trace_pt<4>, lds)})
         tp = std::min(tp, k);
@@ -863,15 +925,26 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
         }
     }
     WfQueue in = qa, out = qb, spare = qc;   // qc is free once the light rounds are done
+    // With connection or environment rays (and the one-pass traversal), the
+    // traversal of bounce i's visibility rays is fused with that of bounce
+    // i + 1's camera rays: shade(i) -> fused trace -> connect(i) -> shade(i + 1).
+    // connect(i) still runs before shade(i + 1) (which reads the connection
+    // sum of escaping paths and overwrites the staged connections), so every
+    // sum is formed in the same order as before.
+    const bool fuse = (lights || nee) && L.traceKeep <= 0;
+    bool traced = false;   // the camera rays of round i were traced by the previous fused launch
     for (int i = 1; i <= B; ++i) {
-        mark();
-        if (L.traceKeep > 0)
-            hipLaunchKernelGGL(omv_wf_trace_pt<0>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, camCnt + i, fetchCnt + (nf++), L.traceKeep,
-                               H, W.vis, n, L.lc, L.gstack, L.counters, L.splat);
-        else
-            hipLaunchKernelGGL(omv_wf_trace<0>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, camCnt + i, H, W.vis, n, L.lc, L.gstack,
-                               L.counters, L.splat);
-        mark();
+        if (!traced) {
+            mark();
+            if (L.traceKeep > 0)
+                hipLaunchKernelGGL(omv_wf_trace_pt<0>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, camCnt + i, fetchCnt + (nf++),
+                                   L.traceKeep, H, W.vis, n, L.lc, L.gstack, L.counters, L.splat);
+            else
+                hipLaunchKernelGGL(omv_wf_trace<0>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, camCnt + i, H, W.vis, n, L.lc,
+                                   L.gstack, L.counters, L.splat);
+            mark();
+        }
+        traced = false;
         if (lights) {
             if (nee)
                 hipLaunchKernelGGL((omv_wf_shade_cam<true, true>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i, H,
@@ -879,14 +952,6 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
             else
                 hipLaunchKernelGGL((omv_wf_shade_cam<true, false>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i,
                                    H, out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad);
-            mark();
-            if (L.traceKeep > 0)
-                hipLaunchKernelGGL(omv_wf_trace_pt<2>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, shCnt + i, fetchCnt + (nf++),
-                                   L.traceKeep, H, W.vis, n, L.lc, L.gstack, L.counters, L.splat);
-            else
-                hipLaunchKernelGGL(omv_wf_trace<2>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, shCnt + i, H, W.vis, n, L.lc, L.gstack,
-                                   L.counters, L.splat);
-            mark();
         } else {
             if (nee)
                 hipLaunchKernelGGL((omv_wf_shade_cam<false, true>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i,
@@ -895,24 +960,49 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
                 hipLaunchKernelGGL((omv_wf_shade_cam<false, false>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i,
                                    H, out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad);
         }
-        if (nee) {
-            mark();
-            if (L.traceKeep > 0)
-                hipLaunchKernelGGL(omv_wf_trace_pt<3>, dim3(L.blocksTrace), blk, lds, stream, S, eq, sq, envCnt + i, fetchCnt + (nf++),
-                                   L.traceKeep, H, W.nvis, n, L.lc, L.gstack, L.counters, L.splat);
-            else
-                hipLaunchKernelGGL(omv_wf_trace<3>, dim3(L.blocksTrace), blk, lds, stream, S, eq, sq, envCnt + i, H, W.nvis, n, L.lc, L.gstack,
-                                   L.counters, L.splat);
-            mark();
-        }
-        if (lights || nee)
-            hipLaunchKernelGGL(omv_wf_connect, dim3(L.blocksShade), blk, 0, stream, W, in, camCnt + i, L.rad, lights ? 1 : 0, nee ? 1 : 0);
+        bool sorted = false;
         if (L.sortRays && i < B) {
             // reorder the next round's rays (secondary rays) into the spare queue
             hipLaunchKernelGGL(omv_wf_fill_u32, dim3(OMV_SORT_BUCKETS / 256), dim3(256), 0, stream, sortHist, OMV_SORT_BUCKETS);
             hipLaunchKernelGGL(omv_wf_sort_keys, dim3(L.blocksShade), blk, 0, stream, S, out, camCnt + i + 1, sortKeys, sortHist);
             hipLaunchKernelGGL(omv_wf_sort_scan, dim3(1), dim3(1024), 0, stream, sortHist);
             hipLaunchKernelGGL(omv_wf_sort_scatter, dim3(L.blocksShade), blk, 0, stream, out, camCnt + i + 1, sortKeys, sortHist, spare);
+            sorted = true;
+        }
+        const WfQueue next = sorted ? spare : out;
+        // the next round's hits go to H, which this round's connect does not read
+        if (fuse) {
+            mark();
+            hipLaunchKernelGGL(omv_wf_trace_fused, dim3(L.blocksTrace), blk, lds, stream, S, next, i < B ? camCnt + i + 1 : nullptr, sq,
+                               lights ? shCnt + i : nullptr, eq, nee ? envCnt + i : nullptr, H, W.vis, W.nvis, n, L.lc, L.gstack,
+                               L.counters);
+            mark();
+            traced = i < B;
+        } else {
+            if (lights) {
+                mark();
+                if (L.traceKeep > 0)
+                    hipLaunchKernelGGL(omv_wf_trace_pt<2>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, shCnt + i, fetchCnt + (nf++),
+                                       L.traceKeep, H, W.vis, n, L.lc, L.gstack, L.counters, L.splat);
+                else
+                    hipLaunchKernelGGL(omv_wf_trace<2>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, shCnt + i, H, W.vis, n, L.lc,
+                                       L.gstack, L.counters, L.splat);
+                mark();
+            }
+            if (nee) {
+                mark();
+                if (L.traceKeep > 0)
+                    hipLaunchKernelGGL(omv_wf_trace_pt<3>, dim3(L.blocksTrace), blk, lds, stream, S, eq, sq, envCnt + i, fetchCnt + (nf++),
+                                       L.traceKeep, H, W.nvis, n, L.lc, L.gstack, L.counters, L.splat);
+                else
+                    hipLaunchKernelGGL(omv_wf_trace<3>, dim3(L.blocksTrace), blk, lds, stream, S, eq, sq, envCnt + i, H, W.nvis, n, L.lc,
+                                       L.gstack, L.counters, L.splat);
+                mark();
+            }
+        }
+        if (lights || nee)
+            hipLaunchKernelGGL(omv_wf_connect, dim3(L.blocksShade), blk, 0, stream, W, in, camCnt + i, L.rad, lights ? 1 : 0, nee ? 1 : 0);
+        if (sorted) {
             const WfQueue freed = in;
             in = spare;
             spare = out;

@@ -36,10 +36,16 @@ class RenderFunction(torch.autograd.Function):
             fit.scene.update_texture(fit.mr_tex, mr_c.numpy())
         if fit.om_mat is not None:
             fit.scene.update_om(fit.om_mat, n.detach().cpu().numpy(), k.detach().cpu().numpy())
-        img = fit.renderer.render(fit.camera)
         ctx.fit = fit
         ctx.shapes = (base.shape, metal_rough.shape)
         ctx.device = base.device
+        if base.is_cuda:
+            # the film is written straight into a device tensor (no host round trip)
+            img = torch.empty((fit.camera.height, fit.camera.width, 4), dtype=torch.float32, device=base.device)
+            torch.cuda.synchronize(base.device)
+            fit.renderer.render_device(fit.camera, img.data_ptr())
+            return img
+        img = fit.renderer.render(fit.camera)
         return torch.from_numpy(np.ascontiguousarray(img)).to(base.device)
 
     @staticmethod

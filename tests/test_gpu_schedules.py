@@ -145,3 +145,62 @@ def test_wavefront_tuning_knot(pkg, wl):
         r.set_tuning(trace_keep=keep)
         imgs.append(r.render(cam))
     assert np.array_equal(imgs[0], imgs[1])
+
+
+@pytest.mark.parametrize("sched", ["wavefront", "per_sample"])
+def test_render_device_equals_render(pkg, wl, sched):
+    """omv_renderer_render_device leaves the film in a device buffer; it is
+    the film omv_renderer_render reads back, bit for bit, with both
+    schedules and with several chunks on two streams."""
+    torch = pytest.importorskip("torch")
+    desc, view = wl.mixed_test_scene(hdr=True, lights=True)
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = pkg.Camera(53, 37, view.position, view.forward, view.up, view.yfov)
+    r = pkg.Renderer(pkg.default_params(samples_per_pixel=4, camera_bounces=4, light_bounces=4, max_path_length=4, seed=9))
+    r.set_scene(s)
+    r.set_schedule(pkg.SCHEDULE_WAVEFRONT if sched == "wavefront" else pkg.SCHEDULE_PER_SAMPLE)
+    if sched == "wavefront":
+        r.set_tuning(chunk_samples=2000)   # several chunks over two streams
+    want = r.render(cam)
+    film = torch.full((cam.height, cam.width, 4), float("nan"), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    r.render_device(cam, film.data_ptr())
+    got = film.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    # the camera keeps the film of the last host render
+    assert np.array_equal(cam.film(), want)
+
+
+@pytest.mark.parametrize("case", ["lights", "nee", "splat", "instanced"])
+def test_occlusion_early_out_keeps_the_film(pkg, wl, case):
+    """Visibility rays ending at the first deciding hit (tuning occlusion 0,
+    the default) give the film of the closest-hit visibility (occlusion 1)
+    bit for bit, with both schedules, and visit fewer nodes."""
+    if case == "instanced":
+        desc, view, _ = wl.large_instanced(width=64, height=36, spp=2, depth=4, count=24, tris_per=600, hdr_w=128, hdr_h=64)
+        params = dict(samples_per_pixel=2, camera_bounces=4, light_bounces=4, max_path_length=4, seed=5)
+    else:
+        desc, view = wl.mixed_test_scene(hdr=True, lights=case != "nee", masked=False)
+        params = dict(samples_per_pixel=3, camera_bounces=4, light_bounces=4, max_path_length=4, seed=5,
+                      extensions={"lights": 0, "nee": 2, "splat": 4}[case])
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = pkg.Camera(min(view.width, 64), min(view.height, 48), view.position, view.forward, view.up, view.yfov)
+    films = {}
+    for sched in (pkg.SCHEDULE_WAVEFRONT, pkg.SCHEDULE_PER_SAMPLE):
+        for occ in (0, 1):
+            r = pkg.Renderer(pkg.default_params(**params))
+            r.set_scene(s)
+            r.set_schedule(sched)
+            r.set_tuning(occlusion=occ)
+            films[(sched, occ)] = (r.render(cam), r.stats())
+    for sched in (pkg.SCHEDULE_WAVEFRONT, pkg.SCHEDULE_PER_SAMPLE):
+        (a, sa), (b, sb) = films[(sched, 0)], films[(sched, 1)]
+        if case == "splat":   # splat sums are float atomics: order-dependent
+            np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-6)
+        else:
+            assert np.array_equal(a, b), (sched, np.argwhere(a != b)[:5].tolist())
+        for k in ("rays", "shadow_rays"):
+            assert sa[k] == sb[k], (k, sa[k], sb[k])
+        assert sa["nodes_visited"] <= sb["nodes_visited"]
