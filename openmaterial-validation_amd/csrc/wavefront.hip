@@ -124,10 +124,9 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_gen(OmvSceneView S, omv_came
         float u0 = cs.next(), u1 = cs.next();
         f3 O, D;
         cameraRay(cam, x, y, u0, u1, O, D);
-        W.thr[slot] = f4of(mk3(1.0f, 1.0f, 1.0f), 1.0f);
-        W.radc[slot] = f4of(mk3(0.0f, 0.0f, 0.0f), __int_as_float(1));
-        W.conn[slot] = f4of(mk3(0.0f, 0.0f, 0.0f), 0.0f);
-        W.csn[slot] = rngConsumed(cs);
+        // the initial camera state (throughput 1, escape weight 1, radiance 0
+        // after 1 vertex, connection sum 0, 2 draws consumed) is not stored:
+        // camera round 1 and its connect kernel start from it
         W.flags[slot] = 0u;
         if (camRays) {
             float delta = 3.0f * FLT_EPSILON * maxAbs(O);
@@ -146,16 +145,11 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_gen(OmvSceneView S, omv_came
             const int li = sampleLight(S, ls.next());
             const OmvLight Lt = S.lights[li];
             f3 dir = uniformSphere(ls);
-            const uint64_t N = n;
-            W.lv[(0 * 6 + 0) * N + slot] = Lt.pos[0];
-            W.lv[(0 * 6 + 1) * N + slot] = Lt.pos[1];
-            W.lv[(0 * 6 + 2) * N + slot] = Lt.pos[2];
-            W.lv[(0 * 6 + 3) * N + slot] = Lt.intensity[0] * P.light_scale;
-            W.lv[(0 * 6 + 4) * N + slot] = Lt.intensity[1] * P.light_scale;
-            W.lv[(0 * 6 + 5) * N + slot] = Lt.intensity[2] * P.light_scale;
+            // light vertex 0 is the sampled light itself (read from the light
+            // table where it is needed); the light throughput starts at 1
+            // in light round 1
             W.lidx[slot] = li;
             W.lplen[slot] = 1;
-            W.lthr[slot] = f4of(mk3(1.0f, 1.0f, 1.0f), 1.0f);
             W.lsn[slot] = rngConsumed(ls);
             if (W.lvCap > 1) {
                 const f3 lp = ld3(Lt.pos);
@@ -438,7 +432,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_light(OmvSceneView S, 
             const f3 pDir = xyz(dd);
             f3 outRay, brdf, erad;
             matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, false, ls);
-            float4 lt = W.lthr[slot];
+            const float4 lt = i == 1 ? f4of(mk3(1.0f, 1.0f, 1.0f), 1.0f) : W.lthr[slot];
             f3 of = xyz(lt);
             float rD = lt.w;
             float ct = fabsf(dot3(pDir, sp.n));
@@ -507,10 +501,11 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_cam(OmvSceneView S, RP
         const uint32_t slot = __float_as_uint(dd.w);
         const int2 id = H.id[e];
         const f3 pDir = xyz(dd);
-        const float4 rc = W.radc[slot];
+        const bool first = i == 1;
+        const float4 rc = first ? f4of(mk3(0.0f, 0.0f, 0.0f), __int_as_float(1)) : W.radc[slot];
         f3 radiance = xyz(rc);
         int n = __float_as_int(rc.w);
-        const float4 th = W.thr[slot];
+        const float4 th = first ? f4of(mk3(1.0f, 1.0f, 1.0f), 1.0f) : W.thr[slot];
         f3 of = xyz(th);
         bool emit = false, envRay = false;
         f3 eOrg = mk3(0.0f, 0.0f, 0.0f), eDir = eOrg;
@@ -525,7 +520,8 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_cam(OmvSceneView S, RP
                 if (kNee) bg = mk3(bg.x * th.w, bg.y * th.w, bg.z * th.w);
                 radiance = mk3(bg.x * 1.0f + radiance.x, bg.y * 1.0f + radiance.y, bg.z * 1.0f + radiance.z);
             }
-            const f3 conn = xyz(W.conn[slot]);
+            // the connection sum exists with point lights only (0 before round 1's connect)
+            const f3 conn = (kLights && !first) ? xyz(W.conn[slot]) : mk3(0.0f, 0.0f, 0.0f);
             rad[(uint64_t)slot * 3 + 0] = conn.x * 1.0f + radiance.x;
             rad[(uint64_t)slot * 3 + 1] = conn.y * 1.0f + radiance.y;
             rad[(uint64_t)slot * 3 + 2] = conn.z * 1.0f + radiance.z;
@@ -544,8 +540,9 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_cam(OmvSceneView S, RP
                 const f3 outV = neg3(pDir);
                 W.nconn[slot] = (uint32_t)max(lEnd, 0);
                 for (int l = 0; l < lEnd; ++l) {
-                    f3 lPos = mk3(W.lv[((uint64_t)l * 6 + 0) * N + slot], W.lv[((uint64_t)l * 6 + 1) * N + slot],
-                                  W.lv[((uint64_t)l * 6 + 2) * N + slot]);
+                    f3 lPos = l == 0 ? ld3(Lt.pos)
+                                     : mk3(W.lv[((uint64_t)l * 6 + 0) * N + slot], W.lv[((uint64_t)l * 6 + 1) * N + slot],
+                                           W.lv[((uint64_t)l * 6 + 2) * N + slot]);
                     f3 lRad;
                     f3 LC = normalize3(sub3(sp.pos, lPos));
                     if (l == 0) {
@@ -596,7 +593,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_cam(OmvSceneView S, RP
                 }
             }
             Rng cs;
-            rngRestore(cs, P.seed, STREAM_CAMERA, sample0 + slot, W.csn[slot]);
+            rngRestore(cs, P.seed, STREAM_CAMERA, sample0 + slot, first ? 2u : W.csn[slot]);   // 2: the camera jitter's draws
             f3 outRay, brdf, erad;
             BounceInfo bi;
             matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, true, cs, kNee ? &bi : nullptr);
@@ -627,7 +624,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_cam(OmvSceneView S, RP
             } else if (kLights || kNee) {
                 W.flags[slot] = 1u;   // written by the connect kernel of this round
             } else {
-                const f3 conn = xyz(W.conn[slot]);
+                const f3 conn = mk3(0.0f, 0.0f, 0.0f);   // no point lights: no connections
                 rad[(uint64_t)slot * 3 + 0] = conn.x * 1.0f + radiance.x;
                 rad[(uint64_t)slot * 3 + 1] = conn.y * 1.0f + radiance.y;
                 rad[(uint64_t)slot * 3 + 2] = conn.z * 1.0f + radiance.z;
@@ -655,7 +652,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_cam(OmvSceneView S, RP
 // per-sample function does), and writes the sample of paths that ended
 // this round.
 __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_connect(WfState W, WfQueue in, const uint32_t* __restrict__ inCount,
-                                                            float* __restrict__ rad, int lights, int nee) {
+                                                            float* __restrict__ rad, int lights, int nee, int round) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t total = *inCount;
@@ -664,7 +661,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_connect(WfState W, WfQueue i
         const uint32_t slot = __float_as_uint(in.d[e].w);
         const uint32_t fl = W.flags[slot];
         if (fl == 2u) continue;
-        f3 conn = xyz(W.conn[slot]);
+        f3 conn = (lights && round > 1) ? xyz(W.conn[slot]) : mk3(0.0f, 0.0f, 0.0f);
         if (lights) {
             const uint32_t nc = W.nconn[slot];
             for (uint32_t l = 0; l < nc; ++l) {
@@ -1001,7 +998,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
             }
         }
         if (lights || nee)
-            hipLaunchKernelGGL(omv_wf_connect, dim3(L.blocksShade), blk, 0, stream, W, in, camCnt + i, L.rad, lights ? 1 : 0, nee ? 1 : 0);
+            hipLaunchKernelGGL(omv_wf_connect, dim3(L.blocksShade), blk, 0, stream, W, in, camCnt + i, L.rad, lights ? 1 : 0, nee ? 1 : 0, i);
         if (sorted) {
             const WfQueue freed = in;
             in = spare;
