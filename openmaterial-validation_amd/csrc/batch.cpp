@@ -260,7 +260,8 @@ int BatchRunner::launchWavefrontJobs() {
         L.arena = r.arena.ptr; L.qcount = r.qcount.as<uint32_t>(); L.rad = r.rad.as<float>(); L.gstack = r.gstack.as<uint32_t>();
         L.counters = dCounters_.as<unsigned long long>();
         L.blocksTrace = cus_ * wfPerCu_[0];
-        L.blocksShade = cus_ * wfPerCu_[1];
+        L.blocksShade = cus_ * shadeCamBlocksPerCU(wfPerCu_, J.scene->view().num_lights > 0,
+                                                   (J.p.extensions & OMV_EXT_HDRI_NEE) && J.scene->view().env_tex_pdf);
         L.blocksGen = cus_ * wfPerCu_[3];
         const uint64_t chunkPix = std::max<uint64_t>(1, wfMaxSlots_ / (uint64_t)spp);
         for (uint64_t pix0 = 0; rc == OMV_OK && pix0 < J.npix; pix0 += chunkPix) {

@@ -63,7 +63,11 @@ struct WfLaunch {
 };
 size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat, bool sortRays);
 int wavefrontCounterCount(int B, int lvCap);
-void wavefrontBlocksPerCU(int out[5]);   // trace, shade, shade_light, gen, persistent trace
+// resident blocks per CU: [0] trace, [1] camera shading with lights + NEE,
+// [2] light shading, [3] gen, [4] persistent trace, [5] camera shading with
+// lights, [6] with NEE, [7] with neither
+void wavefrontBlocksPerCU(int out[8]);
+inline int shadeCamBlocksPerCU(const int occ[8], bool lights, bool nee) { return lights ? (nee ? occ[1] : occ[5]) : (nee ? occ[6] : occ[7]); }
 hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream);
 // Path-replay adjoint over samples [sample0, sample0 + nsamples) (adjoint.hip).
 int adjointBlocksPerCU();

@@ -388,7 +388,7 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     L.arena = dArena_.ptr; L.qcount = dQcount_.as<uint32_t>(); L.rad = dRad_.as<float>(); L.gstack = dStack_.as<uint32_t>();
     L.counters = dCounters_.as<unsigned long long>();
     L.blocksTrace = blocksTrace;
-    L.blocksShade = cus_ * (lights ? wfPerCu_[1] : wfPerCu_[1]);
+    L.blocksShade = cus_ * shadeCamBlocksPerCU(wfPerCu_, lights, nee);
     L.blocksGen = cus_ * wfPerCu_[3];
     L.traceKeep = tuning_.trace_keep;
     L.splat = splat_;

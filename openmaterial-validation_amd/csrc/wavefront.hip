@@ -29,6 +29,12 @@
 // waves per SIMD trace faster than 2, 3 or 5 (5 thrash the L2 with node
 // and triangle lines).
 #define OMV_TRACE_WAVES 4
+// Resident waves per SIMD the camera-shading kernel is compiled for (the
+// register budget it gets: 3 -> 168 VGPRs, 4 -> 128).  Measured on MI355X:
+// with point lights 4 (a few spilled registers) renders the 2M-triangle
+// instanced scene 1 % faster than 3; without them 4 spills more and gains
+// nothing on the torus knot.
+#define OMV_SHADE_WAVES(lights, nee) ((lights) && !(nee) ? 4 : 3)
 
 namespace {
 
@@ -487,7 +493,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_light(OmvSceneView S, 
 // (BDPTRenderer.cpp:266-297), the escape term (:251-258) and, with
 // OMV_EXT_HDRI_NEE, the next-event estimate staged for its visibility trace.
 template <bool kLights, bool kNee>
-__global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_cam(OmvSceneView S, RP P, uint64_t sample0, int i, WfState W, WfQueue in,
+__global__ __launch_bounds__(OMV_BLOCK, OMV_SHADE_WAVES(kLights, kNee)) void omv_wf_shade_cam(OmvSceneView S, RP P, uint64_t sample0, int i, WfState W, WfQueue in,
                                                               const uint32_t* __restrict__ inCount, WfHits H, WfQueue out,
                                                               uint32_t* __restrict__ outCount, WfShadowQ sq, uint32_t* __restrict__ sqCount,
                                                               WfQueue eq, uint32_t* __restrict__ eqCount, float* __restrict__ rad) {
@@ -792,7 +798,7 @@ size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool
 
 int wavefrontCounterCount(int B, int lvCap) { return 4 * lvCap + 6 * B + 16; }
 
-void wavefrontBlocksPerCU(int out[5]) {
+void wavefrontBlocksPerCU(int out[8]) {
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
     int t = occ(omv_wf_trace<0>, lds);
     for (int k : {occ(omv_wf_trace<1>, lds), occ(omv_wf_trace<2>, lds), occ(omv_wf_trace<3>, lds), occ(omv_wf_trace<4>, lds),
@@ -804,6 +810,9 @@ void wavefrontBlocksPerCU(int out[5]) {
     out[0] = t;
     out[4] = tp;
     out[1] = occ(omv_wf_shade_cam<true, true>, 0);
+    out[5] = occ(omv_wf_shade_cam<true, false>, 0);
+    out[6] = occ(omv_wf_shade_cam<false, true>, 0);
+    out[7] = occ(omv_wf_shade_cam<false, false>, 0);
     out[2] = occ(omv_wf_shade_light<true>, 0);
     out[3] = occ(omv_wf_gen<true>, 0);
 }

@@ -287,7 +287,7 @@ size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool
     return b;
 }
 int wavefrontCounterCount(int B, int lvCap) { return 4 * lvCap + 6 * B + 16; }
-void wavefrontBlocksPerCU(int out[5]) { out[0] = 8; out[1] = 8; out[2] = 8; out[3] = 8; out[4] = 4; }
+void wavefrontBlocksPerCU(int out[8]) { out[0] = 8; out[1] = 6; out[2] = 6; out[3] = 8; out[4] = 4; out[5] = 8; out[6] = 6; out[7] = 6; }
 int occupancyBlocksPerCU(int) { return 4; }
 int adjointBlocksPerCU() { return 4; }
 
