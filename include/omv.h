@@ -376,6 +376,15 @@ typedef struct omv_render_tuning {
                                  as sceneIntersect finds it.  Same decisions,
                                  same film; scenes with alpha-MASK materials
                                  always take the closest hit                  */
+    int xcd_bands;            /* traversal hand-out: n > 1 = the queue is cut
+                                 into chunks dealt to n bands of workgroups
+                                 (workgroup b in band b mod n, so a band lies
+                                 on one XCD when n is a multiple of the XCD
+                                 count): the rays an XCD traces at a time come
+                                 from a few stretches of the queue; 1 = plain
+                                 grid stride; 0 = default: two bands per XCD
+                                 on scenes with the wave-cooperative
+                                 traversal, else the plain stride           */
 } omv_render_tuning;
 int omv_renderer_set_tuning(omv_renderer* r, const omv_render_tuning* t);
 int omv_renderer_get_tuning(const omv_renderer* r, omv_render_tuning* t);

@@ -60,6 +60,7 @@ struct WfLaunch {
     bool sortRays;             // reorder secondary camera rays before each traversal
     hipEvent_t roundEv;        // optional: recorded once camera round roundEvAt is
     int roundEvAt;             // done (or at the end when there are fewer rounds)
+    int xcdBands;              // > 1: traversal queues dealt to this many XCDs in chunks (L2 locality); else plain stride
 };
 size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat, bool sortRays);
 int wavefrontCounterCount(int B, int lvCap);

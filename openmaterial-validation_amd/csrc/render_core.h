@@ -175,7 +175,7 @@ private:
     bool wfTimingPending_ = false;
     omv_renderer_params params_;
     int schedule_ = OMV_SCHEDULE_WAVEFRONT;
-    omv_render_tuning tuning_{0, 0, 0, 0, 0, 0, 0, 0};
+    omv_render_tuning tuning_{0, 0, 0, 0, 0, 0, 0, 0, 0};
     OmvSceneView viewForLaunch() const;   // the scene view with the tuning's traversal order
     int wfPerCu_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     RenderScene* scene_ = nullptr;
@@ -185,6 +185,7 @@ private:
     // pinned host staging for the counter read-back (hipHostMalloc)
     void* hostStage_ = nullptr;
     int cus_ = 0;
+    int xcds_ = 1;
     DeviceBuffer dRad_, dLv_, dStack_, dCounters_, dFilm_, dIds_, dTie_, dArena_, dQcount_, dDfilm_, dAdjMaps_, dPost_, dSplat_;
     DeviceBuffer dFcTable_, dFcInstBase_, dDetached_;
     int fcTableSubject_ = -1;          // the table in dFcTable_ is for (subject, seed, size)

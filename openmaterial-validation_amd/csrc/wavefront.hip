@@ -105,6 +105,38 @@ DEV void stackInit(Stack& st, uint32_t* gstack, uint32_t gtid, uint32_t gthreads
 DEV f3 xyz(const float4& v) { return mk3(v.x, v.y, v.z); }
 DEV float4 f4of(const f3& v, float w) { float4 r; r.x = v.x; r.y = v.y; r.z = v.z; r.w = w; return r; }
 
+// XCD-banded walk over a queue of `total` entries.  Workgroups are dispatched
+// to the X XCDs round-robin (workgroup b on XCD b mod X), and each XCD has an
+// L2 of its own.  The queue is cut into chunks of C entries (a power of two,
+// 128 .. 32768, about total / 16X), dealt to the XCDs round-robin, and the
+// workgroups of one XCD stride over that XCD's chunks as over one array, so
+// the rays an XCD traces at a time come from a few contiguous stretches of
+// the queue (for camera rays: of the image) and share their node and
+// triangle lines in that XCD's L2.  Every entry below `total` is visited by
+// exactly one thread: chunk g belongs to XCD g mod X as its chunk g div X,
+// and within an XCD the offsets v = local thread + k * (threads of the XCD)
+// cover 0, 1, 2, ... once each.  With X = 1, or a grid that is not a
+// multiple of X, it is the plain grid stride.  Entries must stay below 2^31.
+struct XcdWalk {
+    uint32_t x, X, logC, v, stride, total, e;
+    DEV void start(uint32_t tot, uint32_t nX) {
+        total = tot;
+        X = nX > 1u && gridDim.x % nX == 0u ? nX : 1u;
+        x = blockIdx.x % X;
+        v = (blockIdx.x / X) * blockDim.x + threadIdx.x;
+        stride = (gridDim.x / X) * blockDim.x;
+        const uint32_t want = tot / (16u * X);
+        logC = 7u;
+        while (logC < 15u && (2u << logC) <= want) ++logC;
+        e = at(v);
+    }
+    DEV uint32_t at(uint32_t vv) const {
+        return ((((vv >> logC) * X) + x) << logC) | (vv & ((1u << logC) - 1u));
+    }
+    DEV bool more() const { return e < total; }
+    DEV void next() { v += stride; e = at(v); }
+};
+
 }  // namespace
 
 // ------------------------------------------------------------------ gen
@@ -179,15 +211,16 @@ template <int kKind>
 __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSceneView S, WfQueue q, WfShadowQ sq, const uint32_t* __restrict__ count,
                                                           WfHits H, uint8_t* __restrict__ vis, uint32_t nslots, int lc,
                                                           uint32_t* __restrict__ gstack, unsigned long long* __restrict__ counters,
-                                                          float* __restrict__ splat) {
+                                                          float* __restrict__ splat, uint32_t xcds) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
     stackInit(st, gstack, gtid, gthreads);
     Counters cnt; cnt.nodes = 0; cnt.tris = 0; cnt.overflow = 0;
     uint32_t rays = 0, nCam = 0, nLight = 0, nShadow = 0;
-    const uint32_t total = *count;
-    for (uint32_t e = gtid; e < total; e += gthreads) {
+    XcdWalk w;
+    for (w.start(*count, xcds); w.more(); w.next()) {
+        const uint32_t e = w.e;
         Hit h;
         if (kKind == 2) {
             const float4 a = sq.o[e], dd = sq.d[e], b = sq.b[e];
@@ -241,7 +274,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
 __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused(
     OmvSceneView S, WfQueue q, const uint32_t* __restrict__ qCount, WfShadowQ sq, const uint32_t* __restrict__ sqCount, WfQueue eq,
     const uint32_t* __restrict__ eqCount, WfHits H, uint8_t* __restrict__ vis, uint8_t* __restrict__ nvis, uint32_t nslots, int lc,
-    uint32_t* __restrict__ gstack, unsigned long long* __restrict__ counters) {
+    uint32_t* __restrict__ gstack, unsigned long long* __restrict__ counters, uint32_t xcds) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
@@ -249,8 +282,9 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused
     Counters cnt; cnt.nodes = 0; cnt.tris = 0; cnt.overflow = 0;
     uint32_t rays = 0, nCam = 0, nShadow = 0;
     const uint32_t nA = qCount ? *qCount : 0u, nB = sqCount ? *sqCount : 0u, nC = eqCount ? *eqCount : 0u;
-    const uint32_t total = nA + nB + nC;
-    for (uint32_t e = gtid; e < total; e += gthreads) {
+    XcdWalk w;
+    for (w.start(nA + nB + nC, xcds); w.more(); w.next()) {
+        const uint32_t e = w.e;
         Hit h;
         if (e < nA) {
             const float4 o = q.o[e], d = q.d[e];
@@ -888,6 +922,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
     int nf = 0;
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
     const dim3 blk(OMV_BLOCK);
+    const uint32_t xcds = L.xcdBands > 1 ? (uint32_t)L.xcdBands : 1u;
     auto mark = [&]() {
         if (L.traceEv && L.traceEvUsed && *L.traceEvUsed < L.traceEvCap) (void)hipEventRecord(L.traceEv[(*L.traceEvUsed)++], stream);
     };
@@ -910,7 +945,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
                                    L.traceKeep, H, W.vis, n, L.lc, L.gstack, L.counters, L.splat);
             else
                 hipLaunchKernelGGL(omv_wf_trace<1>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, lightCnt + i, H, W.vis, n, L.lc, L.gstack,
-                                   L.counters, L.splat);
+                                   L.counters, L.splat, xcds);
             mark();
             if (splat) {
                 hipLaunchKernelGGL(omv_wf_shade_light<true>, dim3(L.blocksShade), blk, 0, stream, S, L.cam, P, L.sample0, i, W, in, lightCnt + i,
@@ -921,7 +956,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
                                        L.traceKeep, H, W.vis, n, L.lc, L.gstack, L.counters, L.splat);
                 else
                     hipLaunchKernelGGL(omv_wf_trace<4>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, splCnt + i, H, W.vis, n, L.lc, L.gstack,
-                                       L.counters, L.splat);
+                                       L.counters, L.splat, xcds);
                 mark();
             } else {
                 hipLaunchKernelGGL(omv_wf_shade_light<false>, dim3(L.blocksShade), blk, 0, stream, S, L.cam, P, L.sample0, i, W, in, lightCnt + i,
@@ -947,7 +982,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
                                    L.traceKeep, H, W.vis, n, L.lc, L.gstack, L.counters, L.splat);
             else
                 hipLaunchKernelGGL(omv_wf_trace<0>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, camCnt + i, H, W.vis, n, L.lc,
-                                   L.gstack, L.counters, L.splat);
+                                   L.gstack, L.counters, L.splat, xcds);
             mark();
         }
         traced = false;
@@ -981,7 +1016,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
             mark();
             hipLaunchKernelGGL(omv_wf_trace_fused, dim3(L.blocksTrace), blk, lds, stream, S, next, i < B ? camCnt + i + 1 : nullptr, sq,
                                lights ? shCnt + i : nullptr, eq, nee ? envCnt + i : nullptr, H, W.vis, W.nvis, n, L.lc, L.gstack,
-                               L.counters);
+                               L.counters, xcds);
             mark();
             traced = i < B;
         } else {
@@ -992,7 +1027,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
                                        L.traceKeep, H, W.vis, n, L.lc, L.gstack, L.counters, L.splat);
                 else
                     hipLaunchKernelGGL(omv_wf_trace<2>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, shCnt + i, H, W.vis, n, L.lc,
-                                       L.gstack, L.counters, L.splat);
+                                       L.gstack, L.counters, L.splat, xcds);
                 mark();
             }
             if (nee) {
@@ -1002,7 +1037,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
                                        L.traceKeep, H, W.nvis, n, L.lc, L.gstack, L.counters, L.splat);
                 else
                     hipLaunchKernelGGL(omv_wf_trace<3>, dim3(L.blocksTrace), blk, lds, stream, S, eq, sq, envCnt + i, H, W.nvis, n, L.lc,
-                                       L.gstack, L.counters, L.splat);
+                                       L.gstack, L.counters, L.splat, xcds);
                 mark();
             }
         }

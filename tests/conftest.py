@@ -15,6 +15,13 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def pkg():
+    # torch (device tensors in some GPU tests) initialises the HIP runtime
+    # before the render core's library does, so both share one runtime that
+    # sees the GPU, whichever test file runs first
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     return ge.load_package()
 
 

@@ -204,3 +204,30 @@ def test_occlusion_early_out_keeps_the_film(pkg, wl, case):
         for k in ("rays", "shadow_rays"):
             assert sa[k] == sb[k], (k, sa[k], sb[k])
         assert sa["nodes_visited"] <= sb["nodes_visited"]
+
+
+@pytest.mark.parametrize("size", ["small", "large"])
+def test_xcd_banded_handout_keeps_the_film(pkg, wl, size):
+    """The traversal kernels' XCD-banded hand-out of queue entries (tuning
+    xcd_bands: 0 = the device's XCD count, n = n bands, 1 = plain grid
+    stride) visits every entry exactly once: films bit for bit and ray counts
+    are those of the plain stride, for band counts that divide the grid
+    (2, 16, the default) and one that does not (3: plain stride).  The large
+    case (512 x 512 x 8 = 2.1M samples) reaches the largest chunk size."""
+    desc, view = wl.mixed_test_scene(hdr=True, lights=True, masked=False)
+    spp, w, h = (3, 96, 64) if size == "small" else (8, 512, 512)
+    params = dict(samples_per_pixel=spp, camera_bounces=4, light_bounces=4, max_path_length=4, seed=9)
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = pkg.Camera(w, h, view.position, view.forward, view.up, view.yfov)
+    out = {}
+    for bands in (1, 0, 2, 3, 16):
+        r = pkg.Renderer(pkg.default_params(**params))
+        r.set_scene(s)
+        r.set_tuning(xcd_bands=bands)
+        out[bands] = (r.render(cam), r.stats())
+    want, ws = out[1]
+    for bands, (got, gs) in out.items():
+        assert np.array_equal(got, want), (bands, np.argwhere(got != want)[:5].tolist())
+        for k in ("rays", "camera_rays", "light_rays", "shadow_rays", "samples"):
+            assert gs[k] == ws[k], (bands, k, gs[k], ws[k])
