@@ -385,6 +385,11 @@ typedef struct omv_render_tuning {
                                  grid stride; 0 = default: two bands per XCD
                                  on scenes with the wave-cooperative
                                  traversal, else the plain stride           */
+    int tail_round;           /* camera rounds from this one to the last run as
+                                 one launch, each path carried to its end by
+                                 one lane (same film): 0 = default, 1 = never
+                                 (every round as its own trace, shade and
+                                 connect launches), n >= 2 = from round n    */
 } omv_render_tuning;
 int omv_renderer_set_tuning(omv_renderer* r, const omv_render_tuning* t);
 int omv_renderer_get_tuning(const omv_renderer* r, omv_render_tuning* t);

@@ -70,7 +70,7 @@ class IorTable(C.Structure):
 class RenderTuning(C.Structure):
     _fields_ = [("trace_keep", C.c_int), ("trace_blocks_per_cu", C.c_int), ("chunk_samples", C.c_int), ("traversal", C.c_int),
                 ("sort_rays", C.c_int), ("chunk_streams", C.c_int), ("chunk_stagger", C.c_int), ("occlusion", C.c_int),
-                ("xcd_bands", C.c_int)]
+                ("xcd_bands", C.c_int), ("tail_round", C.c_int)]
 
 
 class AdjointTargets(C.Structure):
@@ -487,7 +487,7 @@ class Renderer:
 
     def set_tuning(self, **kw):
         """trace_keep, trace_blocks_per_cu, chunk_samples, traversal, sort_rays, chunk_streams, chunk_stagger,
-        occlusion, xcd_bands (the film does not depend on them)."""
+        occlusion, xcd_bands, tail_round (the film does not depend on them)."""
         t = RenderTuning()
         self._check(self.lib.omv_renderer_get_tuning(self.h, C.byref(t)))
         for k, v in kw.items():

@@ -157,6 +157,7 @@ struct Collapser {
     const std::vector<BinNode>& bin;
     int leafBase;
     std::vector<OmvNode4>& pool;
+    bool tlas;
     int maxDepth = 0;
 
     int emit(int b, int depth) {
@@ -186,17 +187,23 @@ struct Collapser {
         const int idx = (int)pool.size();
         pool.emplace_back();
         int32_t child[4];
-        uint32_t count[4];
+        uint32_t entry[4];
         for (int k = 0; k < 4; ++k) {
-            if (k >= nk) { child[k] = OMV_EMPTY_CHILD; count[k] = 0; continue; }
+            if (k >= nk) { child[k] = OMV_EMPTY_CHILD; entry[k] = OMV_ENT_NONE; continue; }
             const BinNode& c = bin[kids[k]];
-            if (c.count > 0) { child[k] = -(leafBase + c.start + 1); count[k] = (uint32_t)c.count; }
-            else { child[k] = emit(kids[k], depth + 1); count[k] = 0; }
+            if (c.count > 0) {
+                child[k] = -(leafBase + c.start + 1);
+                entry[k] = tlas ? OMV_ENT(OMV_ENT_INST, leafBase + c.start)
+                                : OMV_ENT(OMV_ENT_BLAS_LEAF, ((uint32_t)(leafBase + c.start) << 3) | ((uint32_t)(c.count - 1) & 7u));
+            } else {
+                child[k] = emit(kids[k], depth + 1);
+                entry[k] = OMV_ENT(tlas ? OMV_ENT_TLAS_NODE : OMV_ENT_BLAS_NODE, child[k]);
+            }
         }
         OmvNode4& nd = pool[idx];
         for (int k = 0; k < 4; ++k) {
             nd.child[k] = child[k];
-            nd.count[k] = count[k];
+            nd.entry[k] = entry[k];
             if (k >= nk) {
                 nd.lox[k] = nd.loy[k] = nd.loz[k] = 0.0f;
                 nd.hix[k] = nd.hiy[k] = nd.hiz[k] = 0.0f;
@@ -212,12 +219,12 @@ struct Collapser {
 
 }  // namespace
 
-Bvh4Result buildBvh4(const std::vector<BuildBox>& boxes, int leafMax, int leafBase, std::vector<OmvNode4>& pool) {
+Bvh4Result buildBvh4(const std::vector<BuildBox>& boxes, int leafMax, int leafBase, std::vector<OmvNode4>& pool, bool tlas) {
     Bvh4Result r;
     if (boxes.empty()) return r;
     Builder b(boxes, leafMax);
     b.build();
-    Collapser c{b.nodes, leafBase, pool};
+    Collapser c{b.nodes, leafBase, pool, tlas};
     r.root = c.emit(0, 1);
     r.depth = c.maxDepth;
     r.order = std::move(b.order);

@@ -22,8 +22,12 @@ struct Bvh4Result {
 
 // Builds over `boxes` (one per primitive), appending nodes to `pool`.
 // Leaves hold up to leafMax primitives; leaf slots encode
-// -(leafBase + first + 1) with count, where first indexes `order`.
-Bvh4Result buildBvh4(const std::vector<BuildBox>& boxes, int leafMax, int leafBase, std::vector<OmvNode4>& pool);
+// child = -(leafBase + first + 1), where first indexes `order`.  Entries:
+// a BLAS (tlas = false) gets OMV_ENT_BLAS_NODE / OMV_ENT_BLAS_LEAF
+// ((leafBase + first) << 3 | count - 1); a TLAS (leafMax 1) gets
+// OMV_ENT_TLAS_NODE / OMV_ENT_INST (leafBase + first, an index into `order`
+// that the caller replaces by the instance id).
+Bvh4Result buildBvh4(const std::vector<BuildBox>& boxes, int leafMax, int leafBase, std::vector<OmvNode4>& pool, bool tlas);
 
 }  // namespace omv
 #endif

@@ -239,7 +239,7 @@ __global__ void lbCollapse(const uint2* __restrict__ jobs, uint32_t njobs, const
     for (int k = 0; k < 4; ++k) {
         if (k >= nk) {
             nd.child[k] = OMV_EMPTY_CHILD;
-            nd.count[k] = 0;
+            nd.entry[k] = OMV_ENT_NONE;
             nd.lox[k] = nd.loy[k] = nd.loz[k] = nd.hix[k] = nd.hiy[k] = nd.hiz[k] = 0.0f;
             continue;
         }
@@ -250,17 +250,17 @@ __global__ void lbCollapse(const uint2* __restrict__ jobs, uint32_t njobs, const
         const int cnt = countOf(c, first, last);
         if (cnt <= leafMax) {
             nd.child[k] = -(recBase + firstOf(c, first) + 1);
-            nd.count[k] = (uint32_t)cnt;
+            nd.entry[k] = OMV_ENT(OMV_ENT_BLAS_LEAF, ((uint32_t)(recBase + firstOf(c, first)) << 3) | ((uint32_t)(cnt - 1) & 7u));
         } else {
             const uint32_t idx = atomicAdd(nodeCount, 1u);
             if (idx >= nodeCap) {   // cannot happen (one output node per binary node); keep the launch in bounds
                 atomicAdd(overflow, 1u);
                 nd.child[k] = OMV_EMPTY_CHILD;
-                nd.count[k] = 0;
+                nd.entry[k] = OMV_ENT_NONE;
                 continue;
             }
             nd.child[k] = nodeBase + (int)idx;
-            nd.count[k] = 0;
+            nd.entry[k] = OMV_ENT(OMV_ENT_BLAS_NODE, nodeBase + (int)idx);
             const uint32_t q = atomicAdd(nextCount, 1u);
             next[q] = make_uint2(c, idx);
         }

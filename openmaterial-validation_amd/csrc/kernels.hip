@@ -164,9 +164,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_fc_kernel(OmvSceneView S, omv_c
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
-    st.lds = omv_lds_stack + threadIdx.x; st.ldsStride = blockDim.x;
-    st.glb = gstack + gtid; st.glbStride = gthreads;
-    st.sp = 0;
+    st.init(gstack, gtid, gthreads);
     Counters cnt; cnt.nodes = 0; cnt.tris = 0; cnt.overflow = 0;
     uint32_t rays = 0, camRays = 0;
     const uint64_t npix = (uint64_t)cam.width * cam.height;

@@ -61,6 +61,7 @@ struct WfLaunch {
     hipEvent_t roundEv;        // optional: recorded once camera round roundEvAt is
     int roundEvAt;             // done (or at the end when there are fewer rounds)
     int xcdBands;              // > 1: traversal queues dealt to this many XCDs in chunks (L2 locality); else plain stride
+    int tailRound;             // >= 2: camera rounds tailRound .. B run as one launch (omv_wf_tail); else round by round
 };
 size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat, bool sortRays);
 int wavefrontCounterCount(int B, int lvCap);

@@ -26,10 +26,24 @@ enum { OMV_MAT_NONE = 0, OMV_MAT_OM = 1, OMV_MAT_PBR = 2, OMV_MAT_DIFFUSE = 3 };
 enum { OMV_TEX_RGBA8 = 0, OMV_TEX_RGBA32F = 1 };
 enum { OMV_BG_NONE = 0, OMV_BG_CONSTANT = 1, OMV_BG_HDR = 2 };
 
+// Traversal stack entries, 32 bits: the kind in bits 31..30 and below it a
+// BLAS or TLAS node index, an instance id, or for a BLAS leaf
+// (first triangle record << 3) | (count - 1).  OmvNode4::entry holds each
+// child's entry exactly as the traversal pushes it.
+#define OMV_ENT_SHIFT 30
+#define OMV_ENT_BLAS_NODE 0u
+#define OMV_ENT_BLAS_LEAF 1u
+#define OMV_ENT_TLAS_NODE 2u
+#define OMV_ENT_INST 3u
+#define OMV_ENT(kind, payload) (((uint32_t)(kind) << OMV_ENT_SHIFT) | (uint32_t)(payload))
+#define OMV_ENT_END 0xFFFFFFFFu    // end of an instance's BLAS (stack only)
+#define OMV_ENT_NONE 0xFFFFFFFEu   // empty child slot (never pushed)
+static_assert(OMV_BVH_LEAF_MAX >= 1 && OMV_BVH_LEAF_MAX <= 8, "leaf size must fit the entry's 3-bit count");
+
 struct alignas(64) OmvNode4 {
     float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
-    int32_t child[4];    // >= 0 internal node; < 0 leaf: -(first+1) (tri record / instance id)
-    uint32_t count[4];   // leaf size (0 for internal); empty slot: child = OMV_EMPTY_CHILD
+    int32_t child[4];    // >= 0 internal node; < 0 leaf: -(first+1) (tri record / instance id); empty: OMV_EMPTY_CHILD
+    uint32_t entry[4];   // the child's stack entry (OMV_ENT), OMV_ENT_NONE for an empty slot
 };
 static_assert(sizeof(OmvNode4) == 128, "BVH4 node must be one 128-byte line");
 

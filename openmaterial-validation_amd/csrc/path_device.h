@@ -23,6 +23,9 @@
 
 #define DEV __device__ __forceinline__
 
+// Dynamic LDS of every path kernel: the traversal stacks, [entry][lane].
+extern __shared__ uint32_t omv_lds_stack[];
+
 namespace {
 
 // ------------------------------------------------------------------ math
@@ -774,31 +777,56 @@ DEV void matGetRayAndBrdf(const OmvSceneView& S, const Ext& X, const SP& sp, con
 // ------------------------------------------------------------ traversal
 struct Counters { uint32_t nodes, tris, overflow; };
 
+// Per-lane traversal stack: entries 0 .. OMV_STACK_LDS-1 in the kernel's
+// dynamic LDS (omv_lds_stack, [entry][lane]), the rest in a global spill
+// area.  The two parts are addressed through their own address spaces, so
+// the LDS entries compile to ds_read / ds_write (LDS counter only) rather
+// than to flat accesses, which would also wait on the vector memory counter
+// behind every node and triangle fetch in flight.
+typedef __attribute__((address_space(1))) uint32_t GlobalU32;
+// Every kernel that traverses runs OMV_BLOCK-lane workgroups with
+// OMV_STACK_LDS * OMV_BLOCK words of dynamic LDS, so the LDS stride is a
+// constant (a shift, not a multiply, per push and pop).
 struct Stack {
-    uint32_t* lds;       // this lane's LDS column, stride = blockDim
-    uint32_t* glb;       // this lane's spill area, stride = spill stride
-    uint32_t ldsStride, glbStride;
+    uint32_t col;        // this lane's LDS column: omv_lds_stack[col + entry * OMV_BLOCK]
+    GlobalU32* glb;      // this lane's spill area, stride = glbStride
+    uint32_t glbStride;
     int sp;
+    // lane `tid` of `threads` (spill stacks interleaved by lane in gstack)
+    DEV void init(uint32_t* gstack, uint32_t tid, uint32_t threads) {
+        col = threadIdx.x;
+        glb = (GlobalU32*)(gstack + tid); glbStride = threads;
+        sp = 0;
+    }
     DEV void push(uint32_t e, Counters& c) {
-        if (sp < OMV_STACK_LDS) lds[sp * ldsStride] = e;
-        else if (sp < OMV_STACK_TOTAL) glb[(sp - OMV_STACK_LDS) * glbStride] = e;
+        if (sp < OMV_STACK_LDS) omv_lds_stack[col + (uint32_t)sp * OMV_BLOCK] = e;
+        else if (sp < OMV_STACK_TOTAL) glb[(uint32_t)(sp - OMV_STACK_LDS) * glbStride] = e;
         else { ++c.overflow; return; }
         ++sp;
     }
+    // push e if `valid`, without a branch on it: the slot above the top is
+    // written either way (free space) and the top moves only for a valid entry
+    DEV void pushIf(uint32_t e, bool valid, Counters& c) {
+        if (sp < OMV_STACK_LDS) omv_lds_stack[col + (uint32_t)sp * OMV_BLOCK] = e;
+        else if (sp < OMV_STACK_TOTAL) glb[(uint32_t)(sp - OMV_STACK_LDS) * glbStride] = e;
+        const bool room = sp < OMV_STACK_TOTAL;
+        c.overflow += (valid && !room) ? 1u : 0u;
+        sp += (valid && room) ? 1 : 0;
+    }
     DEV uint32_t pop() {
         --sp;
-        return sp < OMV_STACK_LDS ? lds[sp * ldsStride] : glb[(sp - OMV_STACK_LDS) * glbStride];
+        return sp < OMV_STACK_LDS ? omv_lds_stack[col + (uint32_t)sp * OMV_BLOCK] : glb[(uint32_t)(sp - OMV_STACK_LDS) * glbStride];
     }
     DEV uint32_t top() const {
         const int t = sp - 1;
-        return t < OMV_STACK_LDS ? lds[t * ldsStride] : glb[(t - OMV_STACK_LDS) * glbStride];
+        return t < OMV_STACK_LDS ? omv_lds_stack[col + (uint32_t)t * OMV_BLOCK] : glb[(uint32_t)(t - OMV_STACK_LDS) * glbStride];
     }
 };
 
 enum : uint32_t {
-    E_KIND_SHIFT = 30,
-    E_BLAS_NODE = 0u, E_BLAS_LEAF = 1u, E_TLAS_NODE = 2u, E_INST = 3u,
-    E_END = 0xFFFFFFFFu,
+    E_KIND_SHIFT = OMV_ENT_SHIFT,
+    E_BLAS_NODE = OMV_ENT_BLAS_NODE, E_BLAS_LEAF = OMV_ENT_BLAS_LEAF, E_TLAS_NODE = OMV_ENT_TLAS_NODE, E_INST = OMV_ENT_INST,
+    E_END = OMV_ENT_END, E_NONE = OMV_ENT_NONE,
 };
 
 struct Pre { int kx, ky, kz; float Sx, Sy, Sz; f3 o, d, idir; };
@@ -871,7 +899,7 @@ DEV void boxTest4(const OmvNode4& nd, const f3& o, const f3& idir, float tnear, 
         float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tnear));
         float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tfar));
         tent[c] = tmin;
-        hit[c] = nd.child[c] != OMV_EMPTY_CHILD && tmin <= tmax * kUp;
+        hit[c] = nd.entry[c] != OMV_ENT_NONE && tmin <= tmax * kUp;
     }
 }
 
@@ -883,26 +911,33 @@ DEV void loadNode(const OmvNode4* nodes, uint32_t idx, OmvNode4& nd) {
 }
 
 // push the hit children farthest first; the nearest is not pushed but
-// returned in nxt (hasNext), so the next step needs no stack round trip
-DEV void pushSorted(Stack& st, Counters& c, uint32_t e[4], float t[4], bool h[4], uint32_t& nxt, bool& hasNext) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) if (!h[i]) t[i] = FLT_MAX;
-#define OMV_CSWAP(a, b) if (t[b] > t[a]) { float tt = t[a]; t[a] = t[b]; t[b] = tt; uint32_t ee = e[a]; e[a] = e[b]; e[b] = ee; bool hh = h[a]; h[a] = h[b]; h[b] = hh; }
-    // descending by t (farthest first)
-    OMV_CSWAP(0, 1) OMV_CSWAP(2, 3) OMV_CSWAP(0, 2) OMV_CSWAP(1, 3) OMV_CSWAP(1, 2)
-#undef OMV_CSWAP
-    bool have = false;
-    uint32_t prev = 0u;
+// returned in nxt (hasNext), so the next step needs no stack round trip.
+// Branch-free: a missed child becomes (key +inf, entry E_NONE), the
+// 5-comparator network sorts descending by key with selects, so the misses
+// come first and the nearest hit (if any) last; entries are pushed unless
+// E_NONE.  Hits at equal keys may come out in either order -- the closest
+// hit does not depend on the visiting order.
+DEV void pushSorted(Stack& st, Counters& c, uint32_t e[4], float t[4], const bool h[4], uint32_t& nxt, bool& hasNext) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        if (h[i]) {
-            if (have) st.push(prev, c);
-            prev = e[i];
-            have = true;
-        }
+        t[i] = h[i] ? t[i] : INFINITY;
+        e[i] = h[i] ? e[i] : (uint32_t)E_NONE;
     }
-    nxt = prev;
-    hasNext = have;
+#define OMV_CSWAP(a, b)                                             \
+    {                                                               \
+        const bool sw = t[b] > t[a];                                \
+        const float ta = t[a], tb = t[b];                           \
+        const uint32_t ea = e[a], eb = e[b];                        \
+        t[a] = sw ? tb : ta; t[b] = sw ? ta : tb;                   \
+        e[a] = sw ? eb : ea; e[b] = sw ? ea : eb;                   \
+    }
+    OMV_CSWAP(0, 1) OMV_CSWAP(2, 3) OMV_CSWAP(0, 2) OMV_CSWAP(1, 3) OMV_CSWAP(1, 2)
+#undef OMV_CSWAP
+    st.pushIf(e[0], e[0] != (uint32_t)E_NONE, c);
+    st.pushIf(e[1], e[1] != (uint32_t)E_NONE, c);
+    st.pushIf(e[2], e[2] != (uint32_t)E_NONE, c);
+    nxt = e[3];
+    hasNext = e[3] != (uint32_t)E_NONE;
 }
 
 // Closest hit of a world-space ray over the two-level BVH, as a resumable
@@ -991,16 +1026,10 @@ struct Trav {
         OmvNode4 nd;
         loadNode(S.nodes, idx, nd);
         ++c.nodes;
-        float tent[4]; bool hit[4]; uint32_t ent[4];
+        float tent[4]; bool hit[4];
         if (tl) boxTest4(nd, wo, widir, tnear, tb, tent, hit);
         else boxTest4(nd, op.o, op.idir, tnear, tb, tent, hit);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            int32_t ch = nd.child[k];
-            if (ch >= 0) ent[k] = ((tl ? E_TLAS_NODE : E_BLAS_NODE) << E_KIND_SHIFT) | (uint32_t)ch;
-            else if (tl) ent[k] = (E_INST << E_KIND_SHIFT) | (uint32_t)(-(ch + 1));
-            else ent[k] = (E_BLAS_LEAF << E_KIND_SHIFT) | ((uint32_t)(-(ch + 1)) << 3) | ((nd.count[k] - 1u) & 7u);
-        }
+        uint32_t ent[4] = {nd.entry[0], nd.entry[1], nd.entry[2], nd.entry[3]};
         pushSorted(st, c, ent, tent, hit, nxt, hasNext);
     }
     // one entry, whatever its kind (the persistent traversal's unit of work)
@@ -1068,6 +1097,7 @@ DEV bool sceneIntersect(const OmvSceneView& S, const f3& org, const f3& dir, flo
     for (int itr = 10; itr > 0; --itr) {
         ++rays;
         if (!traceClosest(S, org, dir, tnear, tfar, h, st, c, tstop)) return false;
+        if (!S.any_masked) return true;   // no alpha-MASK material: every hit is final
         const int mesh = S.insts[h.inst].mesh_id;
         const OmvMaterial& M = S.mats[S.meshes[mesh].mat_id];
         if (M.type != OMV_MAT_PBR || M.alpha_mode != OMV_ALPHA_MASK) return true;
@@ -1383,9 +1413,6 @@ inline RP toRP(const omv_renderer_params& p) {
 }  // namespace
 
 // ================================================= per-sample path function
-// Dynamic LDS of every path kernel: the traversal stacks, [entry][lane].
-extern __shared__ uint32_t omv_lds_stack[];
-
 namespace {
 
 // Per-lane state of a path kernel: traversal stack, counters and this lane's
@@ -1401,13 +1428,107 @@ struct Lane {
 };
 
 DEV void laneInit(Lane& L, uint32_t* gstack, float* lv, uint32_t gtid, uint32_t gthreads) {
-    L.st.lds = omv_lds_stack + threadIdx.x; L.st.ldsStride = blockDim.x;
-    L.st.glb = gstack + gtid; L.st.glbStride = gthreads;
-    L.st.sp = 0;
+    L.st.init(gstack, gtid, gthreads);
     L.cnt.nodes = 0; L.cnt.tris = 0; L.cnt.overflow = 0;
     L.rays = L.camRays = L.lightRays = L.shadowRays = 0;
     L.splat = nullptr;
-    L.lv = lv + gtid; L.lvStride = gthreads;
+    L.lv = lv ? lv + gtid : nullptr; L.lvStride = gthreads;
+}
+
+// The light subpath as the camera rounds use it: the sampled light (idx < 0:
+// none), its position, intensity and range, and the vertex count.
+struct LightSub {
+    int idx, len;
+    f3 pos, intensity;
+    float range;
+};
+// Light vertices l >= 1 (position, outgoing radiance) in a lane's scratch
+struct LaneLightVerts {
+    Lane* L;
+    DEV f3 pos(int l) const { return mk3(L->lvAt(l, 0), L->lvAt(l, 1), L->lvAt(l, 2)); }
+    DEV f3 rad(int l) const { return mk3(L->lvAt(l, 3), L->lvAt(l, 4), L->lvAt(l, 5)); }
+};
+
+// Camera rounds i0 .. camera_bounces of one sample: computePath(inv = true)
+// and the connections of every camera vertex (BDPTRenderer.cpp:149-176,
+// 260-297), then the escape term (:251-258), from the state the rounds
+// before i0 left: the ray (origin already offset from its surface point,
+// direction, tnear), throughput, escape weight (OMV_EXT_HDRI_NEE), the
+// radiance and connection sums so far, the vertex count and the camera
+// stream.  Returns the sample, conn + radiance.  The per-sample function runs
+// it from round 1; the wavefront schedule hands it the paths still alive at
+// a late round with the state its kernels keep, so both give one sample bit
+// for bit.
+template <bool kLights, bool kNee, class LV>
+DEV f3 cameraRounds(const OmvSceneView& S, const RP& P, uint64_t s, int i0, f3 org, f3 pDir, float tnear, f3 of, float escW, f3 radiance,
+                    f3 conn, int n, Rng& cs, const LightSub& ls, const LV& lv, Lane& L) {
+    const int q = P.q;
+    bool exitScene = false;
+    for (int i = i0; i < P.camera_bounces + 1; ++i) {
+        ++L.camRays;
+        Hit h;
+        if (!sceneIntersect(S, org, pDir, tnear, FLT_MAX, h, L.st, L.cnt, L.rays)) { exitScene = true; break; }
+        SP sp;
+        computeSurfacePoint(S, h, sp, false);
+        modifyFrame(S, sp);
+        // connections of camera vertex j = i with light vertices l, l + j <= q
+        if (kLights && ls.idx >= 0) {
+            const int lEnd = min(ls.len, q - i + 1);
+            const f3 outV = neg3(pDir);
+            for (int l = 0; l < lEnd; ++l) {
+                f3 lPos = l == 0 ? ls.pos : lv.pos(l);
+                f3 lRad;
+                f3 LC = normalize3(sub3(sp.pos, lPos));
+                if (l == 0) {
+                    float lsc = attenuation(P, length3(sub3(sp.pos, lPos)), ls.range);
+                    float k = P.light_scale * lsc;
+                    lRad = mk3(ls.intensity.x * k, ls.intensity.y * k, ls.intensity.z * k);
+                } else {
+                    lRad = lv.rad(l);
+                }
+                float c = fabsf(dot3(LC, sp.n));
+                f3 lc = mk3(lRad.x * c * of.x, lRad.y * c * of.y, lRad.z * c * of.z);
+                if (intensity(lc) < P.cut) continue;
+                f3 brdf = matGetBrdf(S, P.x, sp, LC, outV);
+                brdf = mk3(brdf.x * c, brdf.y * c, brdf.z * c);
+                lc = mk3(brdf.x * of.x * lRad.x, brdf.y * of.y * lRad.y, brdf.z * of.z * lRad.z);
+                if (intensity(lc) < P.cut) continue;
+                if (!isConnected(S, lPos, sp.pos, L.st, L.cnt, L.rays, L.shadowRays)) continue;
+                conn = mk3(lc.x + conn.x, lc.y + conn.y, lc.z + conn.z);
+            }
+        }
+        f3 outRay, brdf, erad;
+        BounceInfo bi;
+        matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, true, cs, kNee ? &bi : nullptr);
+        float ct = fabsf(dot3(outRay, sp.n));
+        const f3 ofPrev = of;
+        of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
+        radiance = mk3(radiance.x + of.x * erad.x, radiance.y + of.y * erad.y, radiance.z + of.z * erad.z);
+        if (kNee && P.x.nee) {   // OMV_EXT_HDRI_NEE, after the vertex's emission
+            f3 c, eo, w;
+            if (neePrepare(S, P, sp, bi, ofPrev, outRay, i, s, escW, c, eo, w)) {
+                float tn = 0.0f;
+                Hit hs;
+                ++L.shadowRays;
+                if (!sceneIntersect(S, eo, w, tn, FLT_MAX, hs, L.st, L.cnt, L.rays, FLT_MAX))
+                    radiance = mk3(radiance.x + c.x, radiance.y + c.y, radiance.z + c.z);
+            }
+        }
+        ++n;
+        pDir = outRay;
+        // exact shortcuts, as in the oracle: zero throughput, or no emission past q
+        if (of.x == 0.0f && of.y == 0.0f && of.z == 0.0f) break;
+        if (!S.any_emissive && n > q) break;
+        const float delta = 3.0f * FLT_EPSILON * maxAbs(sp.pos);
+        org = mk3(sp.pos.x + delta * outRay.x, sp.pos.y + delta * outRay.y, sp.pos.z + delta * outRay.z);
+    }
+    if (exitScene && S.bg_type != OMV_BG_NONE && n <= q) {
+        f3 bg = background(S, pDir);
+        bg = mk3(bg.x * of.x, bg.y * of.y, bg.z * of.z);
+        if (kNee && P.x.nee) bg = mk3(bg.x * escW, bg.y * escW, bg.z * escW);
+        radiance = mk3(bg.x * 1.0f + radiance.x, bg.y * 1.0f + radiance.y, bg.z * 1.0f + radiance.z);
+    }
+    return mk3(conn.x * 1.0f + radiance.x, conn.y * 1.0f + radiance.y, conn.z * 1.0f + radiance.z);
 }
 
 // One sample of BDPTRenderer::computePaths (BDPTRenderer.cpp:237-301):
@@ -1484,76 +1605,13 @@ DEV f3 pathSample(const OmvSceneView& S, const omv_camera_desc& cam, const RP& P
     }
 
     // ---- camera subpath + connections
-    f3 radiance = mk3(0.0f, 0.0f, 0.0f), conn = mk3(0.0f, 0.0f, 0.0f);
-    f3 pPos = O, pDir = D, of = mk3(1.0f, 1.0f, 1.0f);
-    float tnear = 0.0f, escW = 1.0f;
-    int n = 1;
-    bool exitScene = false;
-    for (int i = 1; i < P.camera_bounces + 1; ++i) {
-        float delta = 3.0f * FLT_EPSILON * maxAbs(pPos);
-        f3 org = mk3(pPos.x + delta * pDir.x, pPos.y + delta * pDir.y, pPos.z + delta * pDir.z);
-        ++L.camRays;
-        Hit h;
-        if (!sceneIntersect(S, org, pDir, tnear, FLT_MAX, h, L.st, L.cnt, L.rays)) { exitScene = true; break; }
-        SP sp;
-        computeSurfacePoint(S, h, sp, false);
-        modifyFrame(S, sp);
-        // connections of camera vertex j = i with light vertices l, l + j <= q
-        if (kLights && lightIdx >= 0) {
-            const int lEnd = min(lpLen, q - i + 1);
-            const f3 outV = neg3(pDir);
-            for (int l = 0; l < lEnd; ++l) {
-                f3 lPos = mk3(L.lvAt(l, 0), L.lvAt(l, 1), L.lvAt(l, 2));
-                f3 lRad;
-                f3 LC = normalize3(sub3(sp.pos, lPos));
-                if (l == 0) {
-                    float lsc = attenuation(P, length3(sub3(sp.pos, lPos)), lightRange);
-                    float k = P.light_scale * lsc;
-                    lRad = mk3(lightInt.x * k, lightInt.y * k, lightInt.z * k);
-                } else {
-                    lRad = mk3(L.lvAt(l, 3), L.lvAt(l, 4), L.lvAt(l, 5));
-                }
-                float c = fabsf(dot3(LC, sp.n));
-                f3 lc = mk3(lRad.x * c * of.x, lRad.y * c * of.y, lRad.z * c * of.z);
-                if (intensity(lc) < P.cut) continue;
-                f3 brdf = matGetBrdf(S, P.x, sp, LC, outV);
-                brdf = mk3(brdf.x * c, brdf.y * c, brdf.z * c);
-                lc = mk3(brdf.x * of.x * lRad.x, brdf.y * of.y * lRad.y, brdf.z * of.z * lRad.z);
-                if (intensity(lc) < P.cut) continue;
-                if (!isConnected(S, lPos, sp.pos, L.st, L.cnt, L.rays, L.shadowRays)) continue;
-                conn = mk3(lc.x + conn.x, lc.y + conn.y, lc.z + conn.z);
-            }
-        }
-        f3 outRay, brdf, erad;
-        BounceInfo bi;
-        matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, true, cs, kNee ? &bi : nullptr);
-        float ct = fabsf(dot3(outRay, sp.n));
-        const f3 ofPrev = of;
-        of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
-        radiance = mk3(radiance.x + of.x * erad.x, radiance.y + of.y * erad.y, radiance.z + of.z * erad.z);
-        if (kNee && P.x.nee) {   // OMV_EXT_HDRI_NEE, after the vertex's emission
-            f3 c, org, w;
-            if (neePrepare(S, P, sp, bi, ofPrev, outRay, i, s, escW, c, org, w)) {
-                float tn = 0.0f;
-                Hit hs;
-                ++L.shadowRays;
-                if (!sceneIntersect(S, org, w, tn, FLT_MAX, hs, L.st, L.cnt, L.rays, FLT_MAX))
-                    radiance = mk3(radiance.x + c.x, radiance.y + c.y, radiance.z + c.z);
-            }
-        }
-        ++n;
-        pPos = sp.pos; pDir = outRay;
-        // exact shortcuts, as in the oracle: zero throughput, or no emission past q
-        if (of.x == 0.0f && of.y == 0.0f && of.z == 0.0f) break;
-        if (!S.any_emissive && n > q) break;
-    }
-    if (exitScene && S.bg_type != OMV_BG_NONE && n <= q) {
-        f3 bg = background(S, pDir);
-        bg = mk3(bg.x * of.x, bg.y * of.y, bg.z * of.z);
-        if (kNee && P.x.nee) bg = mk3(bg.x * escW, bg.y * escW, bg.z * escW);
-        radiance = mk3(bg.x * 1.0f + radiance.x, bg.y * 1.0f + radiance.y, bg.z * 1.0f + radiance.z);
-    }
-    return mk3(conn.x * 1.0f + radiance.x, conn.y * 1.0f + radiance.y, conn.z * 1.0f + radiance.z);
+    LightSub lsub;
+    lsub.idx = lightIdx; lsub.len = lpLen; lsub.pos = lightPos; lsub.intensity = lightInt; lsub.range = lightRange;
+    LaneLightVerts lvs; lvs.L = &L;
+    const float delta = 3.0f * FLT_EPSILON * maxAbs(O);
+    const f3 org = mk3(O.x + delta * D.x, O.y + delta * D.y, O.z + delta * D.z);
+    return cameraRounds<kLights, kNee>(S, P, s, 1, org, D, 0.0f, mk3(1.0f, 1.0f, 1.0f), 1.0f, mk3(0.0f, 0.0f, 0.0f),
+                                       mk3(0.0f, 0.0f, 0.0f), 1, cs, lsub, lvs, L);
 }
 
 // One pixel of the normalised film: its spp samples summed in sample order

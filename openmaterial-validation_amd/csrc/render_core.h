@@ -175,7 +175,7 @@ private:
     bool wfTimingPending_ = false;
     omv_renderer_params params_;
     int schedule_ = OMV_SCHEDULE_WAVEFRONT;
-    omv_render_tuning tuning_{0, 0, 0, 0, 0, 0, 0, 0, 0};
+    omv_render_tuning tuning_{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     OmvSceneView viewForLaunch() const;   // the scene view with the tuning's traversal order
     int wfPerCu_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     RenderScene* scene_ = nullptr;

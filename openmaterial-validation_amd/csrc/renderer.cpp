@@ -21,6 +21,9 @@ constexpr uint64_t kChunkSamples = 1ull << 24;   // per-sample radiance buffer: 
 // of one MI355X's 288 GB; chunk_samples lowers it.
 constexpr uint64_t kWfSlots = 1ull << 25;
 constexpr uint64_t kWfSlotsLights = 1ull << 25;
+// Camera round from which the wavefront hands the remaining paths to one
+// tail launch (omv_wf_tail); 0 = every round in the wavefront.
+constexpr int kTailRoundDefault = 0;
 int hipFail(hipError_t e, const char* what) {
     setError(std::string(what) + ": " + hipGetErrorString(e));
     return OMV_E_DEVICE;
@@ -99,7 +102,7 @@ int BDPTRenderer::setSchedule(int s) {
 int BDPTRenderer::setTuning(const omv_render_tuning& t) {
     if (t.trace_keep < 0 || t.trace_keep > 64 || t.trace_blocks_per_cu < 0 || t.chunk_samples < 0 || t.traversal < 0 || t.traversal > 2 ||
         t.sort_rays < 0 || t.sort_rays > 2 || t.chunk_streams < 0 || t.chunk_streams > 2 || t.chunk_stagger < 0 || t.occlusion < 0 ||
-        t.occlusion > 1 || t.xcd_bands < 0 || t.xcd_bands > 64) {
+        t.occlusion > 1 || t.xcd_bands < 0 || t.xcd_bands > 64 || t.tail_round < 0) {
         setError("set_tuning: value out of range");
         return OMV_E_RANGE;
     }
@@ -400,6 +403,7 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     // traversal 10.19 -> 9.32 ms, icosphere 256^2 x 64 1.49 -> 1.35 ms), the
     // plain stride on many-instance scenes (within 1 % either way there)
     L.xcdBands = tuning_.xcd_bands > 0 ? tuning_.xcd_bands : (S.trav_wave ? 2 * xcds_ : 1);
+    L.tailRound = tuning_.tail_round == 1 ? 0 : (tuning_.tail_round >= 2 ? tuning_.tail_round : kTailRoundDefault);
     // an event pair around every traversal launch: (2 light + 3 camera rounds) per chunk
     const uint64_t chunks = (npix + chunkPix - 1) / chunkPix;
     const size_t evNeed = (size_t)std::min<uint64_t>(1u << 14, chunks * 2 * (uint64_t)(2 * lvCap + 3 * p.camera_bounces));

@@ -571,7 +571,7 @@ int RenderScene::commit(std::string& log) {
             }
         }
         const int recBase = (int)(tris.size() / 12);
-        Bvh4Result r = buildBvh4(boxes, OMV_BVH_LEAF_MAX, recBase, nodes);
+        Bvh4Result r = buildBvh4(boxes, OMV_BVH_LEAF_MAX, recBase, nodes, false);
         d.blas_root = r.root;
         blasDepth = std::max(blasDepth, r.depth);
         for (int prim : r.order) {
@@ -673,7 +673,7 @@ int RenderScene::commit(std::string& log) {
         iboxes.push_back(b);
         iids.push_back((int)i);
     }
-    Bvh4Result tl = buildBvh4(iboxes, 1, 0, nodes);
+    Bvh4Result tl = buildBvh4(iboxes, 1, 0, nodes, true);
     float sceneLo[3] = {INFINITY, INFINITY, INFINITY}, sceneHi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (const BuildBox& b : iboxes)
         for (int a = 0; a < 3; ++a) { sceneLo[a] = std::min(sceneLo[a], b.lo[a]); sceneHi[a] = std::max(sceneHi[a], b.hi[a]); }
@@ -688,6 +688,7 @@ int RenderScene::commit(std::string& log) {
                 if (nd.child[k] >= 0) { stack.push_back(nd.child[k]); continue; }
                 const int ord = -(nd.child[k] + 1);
                 nd.child[k] = -(iids[tl.order[ord]] + 1);
+                nd.entry[k] = OMV_ENT(OMV_ENT_INST, iids[tl.order[ord]]);
             }
         }
     }
@@ -695,7 +696,10 @@ int RenderScene::commit(std::string& log) {
     if (deviceBuild) {
         for (OmvNode4& nd : nodes)
             for (int k = 0; k < 4; ++k)
-                if (nd.child[k] != OMV_EMPTY_CHILD && nd.child[k] >= 0) nd.child[k] += deviceNodes;
+                if (nd.child[k] != OMV_EMPTY_CHILD && nd.child[k] >= 0) {
+                    nd.child[k] += deviceNodes;
+                    nd.entry[k] = OMV_ENT(OMV_ENT_TLAS_NODE, nd.child[k]);
+                }
         tl.root += deviceNodes;
         hipError_t e = hipMemcpy(dNodes_.as<OmvNode4>() + deviceNodes, nodes.data(), nodes.size() * sizeof(OmvNode4), hipMemcpyHostToDevice);
         if (e != hipSuccess) return hipFail(e, "upload TLAS");

@@ -96,14 +96,23 @@ DEV uint32_t waveAppend(uint32_t* counter, bool want) {
     return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
 }
 
-DEV void stackInit(Stack& st, uint32_t* gstack, uint32_t gtid, uint32_t gthreads) {
-    st.lds = omv_lds_stack + threadIdx.x; st.ldsStride = blockDim.x;
-    st.glb = gstack + gtid; st.glbStride = gthreads;
-    st.sp = 0;
-}
+DEV void stackInit(Stack& st, uint32_t* gstack, uint32_t gtid, uint32_t gthreads) { st.init(gstack, gtid, gthreads); }
 
 DEV f3 xyz(const float4& v) { return mk3(v.x, v.y, v.z); }
 DEV float4 f4of(const f3& v, float w) { float4 r; r.x = v.x; r.y = v.y; r.z = v.z; r.w = w; return r; }
+
+// Light vertices l >= 1 of one slot in the wavefront state ([vertex][6][slot])
+struct WfLightVerts {
+    const float* lv;
+    uint64_t N;
+    uint32_t slot;
+    DEV f3 pos(int l) const {
+        return mk3(lv[((uint64_t)l * 6 + 0) * N + slot], lv[((uint64_t)l * 6 + 1) * N + slot], lv[((uint64_t)l * 6 + 2) * N + slot]);
+    }
+    DEV f3 rad(int l) const {
+        return mk3(lv[((uint64_t)l * 6 + 3) * N + slot], lv[((uint64_t)l * 6 + 4) * N + slot], lv[((uint64_t)l * 6 + 5) * N + slot]);
+    }
+};
 
 // XCD-banded walk over a queue of `total` entries.  Workgroups are dispatched
 // to the X XCDs round-robin (workgroup b on XCD b mod X), and each XCD has an
@@ -388,7 +397,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_trace_pt(OmvSceneView S, WfQ
             if (busy && !T.more(st)) {
                 // one traceClosest call finished: sceneIntersect's masked re-trace loop
                 bool done = true, hit = T.found;
-                if (hit) {
+                if (hit && S.any_masked) {
                     const int mesh = S.insts[T.h.inst].mesh_id;
                     const OmvMaterial& M = S.mats[S.meshes[mesh].mat_id];
                     if (M.type == OMV_MAT_PBR && M.alpha_mode == OMV_ALPHA_MASK &&
@@ -731,6 +740,54 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_connect(WfState W, WfQueue i
 // Paths that end when the camera loop runs out (i == B) are written by
 // shade/connect above; nothing is left over after the last round.
 
+// ----------------------------------------------------------------- tail
+// Camera rounds i0 .. B of the paths still alive after round i0 - 1, in one
+// launch: each lane takes one path of the round-i0 queue to its end with the
+// per-sample function's camera loop (cameraRounds: trace, shade, its
+// connections and next-event ray traced in place), from the state the
+// wavefront kernels keep for it (ray in the queue, throughput / escape
+// weight, radiance and vertex count, connection sum, camera stream position,
+// light subpath), and writes the path's sample.  The late rounds hold few
+// rays and cost one launch latency each for trace, shade and connect; here
+// they cost the longest remaining path once.  Sums are formed in the same
+// order, so the film is unchanged.
+template <bool kLights, bool kNee>
+__global__ __launch_bounds__(OMV_BLOCK) void omv_wf_tail(OmvSceneView S, RP P, uint64_t sample0, int i0, WfState W, WfQueue in,
+                                                         const uint32_t* __restrict__ inCount, float* __restrict__ rad,
+                                                         uint32_t* __restrict__ gstack, unsigned long long* __restrict__ counters) {
+    const uint32_t gthreads = gridDim.x * blockDim.x;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    Lane L;
+    laneInit(L, gstack, nullptr, gtid, gthreads);
+    const uint32_t total = *inCount;
+    const uint64_t N = W.n;
+    for (uint32_t e = gtid; e < total; e += gthreads) {
+        const float4 o = in.o[e], d = in.d[e];
+        const uint32_t slot = __float_as_uint(d.w);
+        const float4 rc = W.radc[slot], th = W.thr[slot];
+        LightSub ls;
+        ls.idx = -1; ls.len = 0; ls.pos = mk3(0.0f, 0.0f, 0.0f); ls.intensity = ls.pos; ls.range = FLT_MAX;
+        f3 conn = mk3(0.0f, 0.0f, 0.0f);
+        if (kLights) {
+            ls.idx = W.lidx[slot];
+            ls.len = W.lplen[slot];
+            const OmvLight Lt = S.lights[ls.idx];
+            ls.pos = ld3(Lt.pos); ls.intensity = ld3(Lt.intensity); ls.range = Lt.range;
+            conn = xyz(W.conn[slot]);
+        }
+        WfLightVerts lv;
+        lv.lv = W.lv; lv.N = N; lv.slot = slot;
+        Rng cs;
+        rngRestore(cs, P.seed, STREAM_CAMERA, sample0 + slot, W.csn[slot]);
+        const f3 r = cameraRounds<kLights, kNee>(S, P, sample0 + slot, i0, xyz(o), xyz(d), o.w, xyz(th), th.w, xyz(rc), conn,
+                                                 __float_as_int(rc.w), cs, ls, lv, L);
+        rad[(uint64_t)slot * 3 + 0] = r.x;
+        rad[(uint64_t)slot * 3 + 1] = r.y;
+        rad[(uint64_t)slot * 3 + 2] = r.z;
+    }
+    flushCounters(counters, L.rays, L.camRays, 0u, L.shadowRays, L.cnt);
+}
+
 // ----------------------------------------------------------- ray sorting
 // A round's secondary camera rays reordered by (direction octant, cell of a
 // 16^3 grid over the scene box, Morton order) with a counting sort, so that
@@ -973,8 +1030,32 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
     // sum of escaping paths and overwrites the staged connections), so every
     // sum is formed in the same order as before.
     const bool fuse = (lights || nee) && L.traceKeep <= 0;
+    // rounds i0 .. B in one launch (0: none); the round before i0 leaves
+    // its camera rays untraced for it
+    const int i0 = L.tailRound >= 2 && L.tailRound <= B ? L.tailRound : 0;
     bool traced = false;   // the camera rays of round i were traced by the previous fused launch
+    bool roundEvDone = false;
     for (int i = 1; i <= B; ++i) {
+        if (i == i0) {
+            mark();
+            if (lights) {
+                if (nee)
+                    hipLaunchKernelGGL((omv_wf_tail<true, true>), dim3(L.blocksTrace), blk, lds, stream, S, P, L.sample0, i, W, in, camCnt + i,
+                                       L.rad, L.gstack, L.counters);
+                else
+                    hipLaunchKernelGGL((omv_wf_tail<true, false>), dim3(L.blocksTrace), blk, lds, stream, S, P, L.sample0, i, W, in, camCnt + i,
+                                       L.rad, L.gstack, L.counters);
+            } else {
+                if (nee)
+                    hipLaunchKernelGGL((omv_wf_tail<false, true>), dim3(L.blocksTrace), blk, lds, stream, S, P, L.sample0, i, W, in, camCnt + i,
+                                       L.rad, L.gstack, L.counters);
+                else
+                    hipLaunchKernelGGL((omv_wf_tail<false, false>), dim3(L.blocksTrace), blk, lds, stream, S, P, L.sample0, i, W, in, camCnt + i,
+                                       L.rad, L.gstack, L.counters);
+            }
+            mark();
+            break;
+        }
         if (!traced) {
             mark();
             if (L.traceKeep > 0)
@@ -1002,7 +1083,8 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
                                    H, out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad);
         }
         bool sorted = false;
-        if (L.sortRays && i < B) {
+        const bool nextInWavefront = i < B && i + 1 != i0;   // round i + 1 is traced by this schedule's round kernels
+        if (L.sortRays && nextInWavefront) {
             // reorder the next round's rays (secondary rays) into the spare queue
             hipLaunchKernelGGL(omv_wf_fill_u32, dim3(OMV_SORT_BUCKETS / 256), dim3(256), 0, stream, sortHist, OMV_SORT_BUCKETS);
             hipLaunchKernelGGL(omv_wf_sort_keys, dim3(L.blocksShade), blk, 0, stream, S, out, camCnt + i + 1, sortKeys, sortHist);
@@ -1014,11 +1096,11 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
         // the next round's hits go to H, which this round's connect does not read
         if (fuse) {
             mark();
-            hipLaunchKernelGGL(omv_wf_trace_fused, dim3(L.blocksTrace), blk, lds, stream, S, next, i < B ? camCnt + i + 1 : nullptr, sq,
-                               lights ? shCnt + i : nullptr, eq, nee ? envCnt + i : nullptr, H, W.vis, W.nvis, n, L.lc, L.gstack,
+            hipLaunchKernelGGL(omv_wf_trace_fused, dim3(L.blocksTrace), blk, lds, stream, S, next, nextInWavefront ? camCnt + i + 1 : nullptr,
+                               sq, lights ? shCnt + i : nullptr, eq, nee ? envCnt + i : nullptr, H, W.vis, W.nvis, n, L.lc, L.gstack,
                                L.counters, xcds);
             mark();
-            traced = i < B;
+            traced = nextInWavefront;
         } else {
             if (lights) {
                 mark();
@@ -1051,9 +1133,10 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
         } else {
             std::swap(in, out);
         }
-        if (L.roundEv && i == std::min(L.roundEvAt, B)) (void)hipEventRecord(L.roundEv, stream);
+        if (L.roundEv && i == std::min(L.roundEvAt, B)) { (void)hipEventRecord(L.roundEv, stream); roundEvDone = true; }
     }
-    if (L.roundEv && B < 1) (void)hipEventRecord(L.roundEv, stream);
+    // fewer rounds, or the rounds from roundEvAt on ran in the tail launch
+    if (L.roundEv && !roundEvDone) (void)hipEventRecord(L.roundEv, stream);
     return hipGetLastError();
 }
 

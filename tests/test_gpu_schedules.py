@@ -231,3 +231,41 @@ def test_xcd_banded_handout_keeps_the_film(pkg, wl, size):
         assert np.array_equal(got, want), (bands, np.argwhere(got != want)[:5].tolist())
         for k in ("rays", "camera_rays", "light_rays", "shadow_rays", "samples"):
             assert gs[k] == ws[k], (bands, k, gs[k], ws[k])
+
+
+@pytest.mark.parametrize("case", ["lights", "nee", "hdri", "masked", "splat", "instanced", "chunks"])
+def test_tail_launch_keeps_the_film(pkg, wl, case):
+    """The late camera rounds run as one launch (tuning tail_round = n: each
+    path alive at round n carried to its end by one lane) give the film of
+    the round-by-round wavefront (tail_round = 1) bit for bit, with the same
+    ray counts, from every starting round; also with several chunks on two
+    streams (the chunk hand-over event is recorded in the tail case too)."""
+    depth = 5
+    params = dict(samples_per_pixel=3, camera_bounces=depth, light_bounces=depth, max_path_length=depth, seed=11)
+    if case == "instanced":
+        desc, view, _ = wl.large_instanced(width=64, height=36, spp=2, depth=depth, count=24, tris_per=600, hdr_w=128, hdr_h=64)
+    elif case == "hdri":
+        desc, view = wl.mixed_test_scene(hdr=True, lights=False, masked=False)
+    else:
+        desc, view = wl.mixed_test_scene(hdr=True, lights=case != "nee", masked=case == "masked")
+        params["extensions"] = {"nee": 2, "splat": 4}.get(case, 0)
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = pkg.Camera(min(view.width, 64), min(view.height, 48), view.position, view.forward, view.up, view.yfov)
+    out = {}
+    for tail in (1, 2, 3, depth):
+        r = pkg.Renderer(pkg.default_params(**params))
+        r.set_scene(s)
+        kw = dict(tail_round=tail)
+        if case == "chunks":
+            kw.update(chunk_samples=cam.width * 7 * params["samples_per_pixel"])
+        r.set_tuning(**kw)
+        out[tail] = (r.render(cam), r.stats())
+    want, ws = out[1]
+    for tail, (got, gs) in out.items():
+        if case == "splat":   # splat sums are float atomics: order-dependent
+            np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-6)
+        else:
+            assert np.array_equal(got, want), (tail, np.argwhere(got != want)[:5].tolist())
+        for k in ("rays", "camera_rays", "light_rays", "shadow_rays", "samples"):
+            assert gs[k] == ws[k], (tail, k, gs[k], ws[k])
