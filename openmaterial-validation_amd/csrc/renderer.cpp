@@ -22,8 +22,12 @@ constexpr uint64_t kChunkSamples = 1ull << 24;   // per-sample radiance buffer: 
 constexpr uint64_t kWfSlots = 1ull << 25;
 constexpr uint64_t kWfSlotsLights = 1ull << 25;
 // Camera round from which the wavefront hands the remaining paths to one
-// tail launch (omv_wf_tail); 0 = every round in the wavefront.
-constexpr int kTailRoundDefault = 0;
+// tail launch (omv_wf_tail).  Measured on MI355X (kernel ms, round by round
+// -> tail from round 3 / 4 / 5 / 6): 2M-triangle instanced scene 1080p x 16
+// 17.97 -> 21.22 / 17.99 / 17.64 / 17.81; torus knot 512^2 x 64 11.43 ->
+// 11.55 / 10.73 / 10.82 / 10.99.  From round 5 both gain; earlier the tail
+// launch holds too many paths, each traced one lane at a time.
+constexpr int kTailRoundDefault = 5;
 int hipFail(hipError_t e, const char* what) {
     setError(std::string(what) + ": " + hipGetErrorString(e));
     return OMV_E_DEVICE;
