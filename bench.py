@@ -544,7 +544,8 @@ def main():
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "achieved_model": "cold-cache algorithmic bytes: rays x B_ray, B_ray = 32 + 16 + 64 x nodes + 48 x tris "
                                       "of the oracle's canonical binary SAH BVH over the CPU sample's rays",
-                    "kernel": "omv_wf_trace (all traversal launches of one render)" if t_ms > 0 else "omv_bdpt_kernel",
+                    "kernel": "traversal launches of one render: omv_wf_trace<k>, omv_wf_trace_fused, and the tail launch "
+                              "omv_wf_tail (which also shades the late rounds)" if t_ms > 0 else "omv_bdpt_kernel",
                     "b_ray": round(b_ray, 1), "rays_per_render": int(rays_per_launch), "kernel_ms": round(dom_ms, 3),
                     "render_kernels_ms": round(k_ms, 3)}
         # the same model over the device's own BVH4 (128-byte nodes, 48-byte
