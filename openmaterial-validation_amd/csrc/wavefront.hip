@@ -101,6 +101,42 @@ DEV void stackInit(Stack& st, uint32_t* gstack, uint32_t gtid, uint32_t gthreads
 DEV f3 xyz(const float4& v) { return mk3(v.x, v.y, v.z); }
 DEV float4 f4of(const f3& v, float w) { float4 r; r.x = v.x; r.y = v.y; r.z = v.z; r.w = w; return r; }
 
+// Round-1 rays of sample s: the camera ray (PinholeCamera::getRay, 2 draws
+// of the camera stream) offset from the camera position -- computed where it
+// is used (the round-1 camera traversal, shading and connect kernels)
+// instead of stored by the generation kernel -- and the light choice and
+// emitted ray (tryComputeLightPath, BDPTRenderer.cpp:194-206) with the
+// light-stream draws they consumed.
+DEV void genCameraRay(const omv_camera_desc& cam, uint32_t seed, int spp, uint64_t s, f3& org, f3& dir) {
+    const uint64_t pix = s / (uint64_t)spp;
+    const int x = (int)(pix % (uint64_t)cam.width), y = (int)(pix / (uint64_t)cam.width);
+    Rng cs; cs.init(seed, STREAM_CAMERA, s);
+    const float u0 = cs.next(), u1 = cs.next();
+    f3 O;
+    cameraRay(cam, x, y, u0, u1, O, dir);
+    const float delta = 3.0f * FLT_EPSILON * maxAbs(O);
+    org = mk3(O.x + delta * dir.x, O.y + delta * dir.y, O.z + delta * dir.z);
+}
+DEV int genLightRay(const OmvSceneView& S, uint32_t seed, uint64_t s, f3& org, f3& dir, uint32_t& consumed) {
+    Rng ls; ls.init(seed, STREAM_LIGHT, s);
+    const int li = sampleLight(S, ls.next());
+    dir = uniformSphere(ls);
+    consumed = rngConsumed(ls);
+    const f3 lp = ld3(S.lights[li].pos);
+    const float delta = 3.0f * FLT_EPSILON * maxAbs(lp);
+    org = mk3(lp.x + delta * dir.x, lp.y + delta * dir.y, lp.z + delta * dir.z);
+    return li;
+}
+// Where the round-1 camera kernels find their rays: the camera queue, or
+// (gen) computed from the entry, which is the slot
+struct WfGen {
+    int gen;                 // 1: round-1 rays are computed, entry e is slot e
+    omv_camera_desc cam;
+    uint32_t seed;
+    int spp;
+    uint64_t sample0;
+};
+
 // Light vertices l >= 1 of one slot in the wavefront state ([vertex][6][slot])
 struct WfLightVerts {
     const float* lv;
@@ -151,10 +187,13 @@ struct XcdWalk {
 // ------------------------------------------------------------------ gen
 // Camera ray of every slot (PinholeCamera::getRay, 2 draws) and, with
 // lights, the light-subpath start (tryComputeLightPath, BDPTRenderer.cpp:194-206).
+// With writeQueues = 0 the round-1 camera rays are not stored: the round-1
+// camera kernels compute them (WfGen).  Light rays are always stored: their
+// direction comes from a rejection loop, cheaper to keep than to redo.
 template <bool kLights>
 __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_gen(OmvSceneView S, omv_camera_desc cam, RP P, int spp, uint64_t sample0,
                                                         WfState W, WfQueue camQ, WfQueue lightQ, uint32_t* __restrict__ camCount,
-                                                        uint32_t* __restrict__ lightCount, float* __restrict__ rad) {
+                                                        uint32_t* __restrict__ lightCount, float* __restrict__ rad, int writeQueues) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = W.n;
@@ -165,21 +204,17 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_gen(OmvSceneView S, omv_came
     }
     for (uint32_t slot = gtid; slot < n; slot += gthreads) {
         const uint64_t s = sample0 + slot;
-        const uint64_t pix = s / (uint64_t)spp;
-        const int x = (int)(pix % (uint64_t)cam.width), y = (int)(pix / (uint64_t)cam.width);
-        Rng cs; cs.init(P.seed, STREAM_CAMERA, s);
-        float u0 = cs.next(), u1 = cs.next();
-        f3 O, D;
-        cameraRay(cam, x, y, u0, u1, O, D);
         // the initial camera state (throughput 1, escape weight 1, radiance 0
         // after 1 vertex, connection sum 0, 2 draws consumed) is not stored:
         // camera round 1 and its connect kernel start from it
         W.flags[slot] = 0u;
         if (camRays) {
-            float delta = 3.0f * FLT_EPSILON * maxAbs(O);
-            f3 org = mk3(O.x + delta * D.x, O.y + delta * D.y, O.z + delta * D.z);
-            camQ.o[slot] = f4of(org, 0.0f);
-            camQ.d[slot] = f4of(D, __uint_as_float(slot));
+            if (writeQueues) {
+                f3 org, D;
+                genCameraRay(cam, P.seed, spp, s, org, D);
+                camQ.o[slot] = f4of(org, 0.0f);
+                camQ.d[slot] = f4of(D, __uint_as_float(slot));
+            }
         } else {
             // no camera vertex: conn * 1 + radiance with both zero
             rad[(uint64_t)slot * 3 + 0] = 0.0f;
@@ -188,20 +223,16 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_gen(OmvSceneView S, omv_came
             W.flags[slot] = 2u;
         }
         if (kLights) {
-            Rng ls; ls.init(P.seed, STREAM_LIGHT, s);
-            const int li = sampleLight(S, ls.next());
-            const OmvLight Lt = S.lights[li];
-            f3 dir = uniformSphere(ls);
             // light vertex 0 is the sampled light itself (read from the light
             // table where it is needed); the light throughput starts at 1
             // in light round 1
+            f3 org, dir;
+            uint32_t consumed;
+            const int li = genLightRay(S, P.seed, s, org, dir, consumed);
             W.lidx[slot] = li;
             W.lplen[slot] = 1;
-            W.lsn[slot] = rngConsumed(ls);
+            W.lsn[slot] = consumed;
             if (W.lvCap > 1) {
-                const f3 lp = ld3(Lt.pos);
-                float delta = 3.0f * FLT_EPSILON * maxAbs(lp);
-                f3 org = mk3(lp.x + delta * dir.x, lp.y + delta * dir.y, lp.z + delta * dir.z);
                 lightQ.o[slot] = f4of(org, 0.0f);
                 lightQ.d[slot] = f4of(dir, __uint_as_float(slot));
             }
@@ -220,7 +251,7 @@ template <int kKind>
 __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSceneView S, WfQueue q, WfShadowQ sq, const uint32_t* __restrict__ count,
                                                           WfHits H, uint8_t* __restrict__ vis, uint32_t nslots, int lc,
                                                           uint32_t* __restrict__ gstack, unsigned long long* __restrict__ counters,
-                                                          float* __restrict__ splat, uint32_t xcds) {
+                                                          float* __restrict__ splat, uint32_t xcds, WfGen G) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
@@ -260,10 +291,16 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
             const bool hit = sceneIntersect(S, xyz(a), xyz(dd), tnear, dd.w, h, st, cnt, rays, FLT_MAX);
             splatAdd(splat, !hit, __float_as_uint(b.w), xyz(b));
         } else {
-            const float4 o = q.o[e], d = q.d[e];
-            float tnear = o.w;
+            f3 org, dir;
+            float tnear = 0.0f;
+            if (kKind == 0 && G.gen) {   // round 1: the camera ray of slot e
+                genCameraRay(G.cam, G.seed, G.spp, G.sample0 + e, org, dir);
+            } else {
+                const float4 o = q.o[e], d = q.d[e];
+                org = xyz(o); dir = xyz(d); tnear = o.w;
+            }
             if (kKind == 0) ++nCam; else ++nLight;
-            const bool hit = sceneIntersect(S, xyz(o), xyz(d), tnear, FLT_MAX, h, st, cnt, rays);
+            const bool hit = sceneIntersect(S, org, dir, tnear, FLT_MAX, h, st, cnt, rays);
             float4 r; r.x = h.t; r.y = h.u; r.z = h.v; r.w = tnear;
             int2 id; id.x = hit ? h.inst : -1; id.y = h.prim | (h.tie ? (int)0x80000000 : 0);
             H.h[e] = r;
@@ -333,118 +370,148 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused
     flushCounters(counters, rays, nCam, 0u, nShadow, cnt);
 }
 
-// Persistent form of the trace kernel: each lane keeps one ray and refills
-// from the queue (one atomic per wave) as soon as its ray is finished, once
-// fewer than `keep` lanes of the wave are still busy, so lanes do not idle
-// behind the longest traversal of their wave.  Same per-ray arithmetic
-// (Trav + the masked re-trace loop of sceneIntersect) and the same outputs.
+// Arguments of the refilling traversal: the queues of one launch (kKind
+// 0..4 as omv_wf_trace: q or sq with its count; kKind 5: the fused launch's
+// camera, connection and environment queues back to back).
+struct WfTraceArgs {
+    WfQueue q; const uint32_t* qCount;
+    WfShadowQ sq; const uint32_t* sqCount;
+    WfQueue eq; const uint32_t* eqCount;
+    WfHits H;
+    uint8_t* vis; uint8_t* nvis;
+    uint32_t nslots; int lc;
+    uint32_t* gstack; unsigned long long* counters; float* splat;
+    uint32_t xcds;
+    int keep;
+};
+
+// Refilling form of the traversal (tuning trace_keep > 0): each lane holds
+// one ray and steps its traversal entry by entry; once fewer than `keep`
+// lanes of the wave are busy, every idle lane takes the next ray of its own
+// list (the XCD-banded walk over the launch's entries, the same hand-out as
+// omv_wf_trace: no atomics, each entry once), so lanes do not idle behind
+// the longest traversal of their wave.  Same per-ray arithmetic (Trav + the
+// masked re-trace loop of sceneIntersect, the occlusion early-out) and the
+// same outputs as omv_wf_trace / omv_wf_trace_fused.
 template <int kKind>
-__global__ __launch_bounds__(OMV_BLOCK) void omv_wf_trace_pt(OmvSceneView S, WfQueue q, WfShadowQ sq, const uint32_t* __restrict__ count,
-                                                             uint32_t* __restrict__ fetch, int keep, WfHits H, uint8_t* __restrict__ vis,
-                                                             uint32_t nslots, int lc, uint32_t* __restrict__ gstack,
-                                                             unsigned long long* __restrict__ counters, float* __restrict__ splat) {
+__global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_pt(OmvSceneView S, WfTraceArgs A) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
     Stack st;
-    stackInit(st, gstack, gtid, gthreads);
+    stackInit(st, A.gstack, gtid, gthreads);
     Counters cnt; cnt.nodes = 0; cnt.tris = 0; cnt.overflow = 0;
     uint32_t rays = 0, nCam = 0, nLight = 0, nShadow = 0;
-    const uint32_t total = *count;
+    // entries [0, nA) camera / light rays of q, [nA, nA + nB) rays of sq,
+    // [nA + nB, total) environment rays of eq (by kKind)
+    uint32_t nA = 0, nB = 0, nC = 0;
+    if (kKind == 5) {
+        nA = A.qCount ? *A.qCount : 0u; nB = A.sqCount ? *A.sqCount : 0u; nC = A.eqCount ? *A.eqCount : 0u;
+    } else if (kKind == 2 || kKind == 4) {
+        nB = *A.sqCount;
+    } else if (kKind == 3) {
+        nC = *A.eqCount;
+    } else {
+        nA = *A.qCount;
+    }
+    XcdWalk w;
+    w.start(nA + nB + nC, A.xcds);
     Trav T;
-    bool busy = false, drained = false;
-    uint32_t e = 0;
-    int itr = 0;
+    bool busy = false;
+    int kind = 0, itr = 0;   // kind of the held ray: 0 q (camera / light), 2 connection / pinhole, 3 environment
+    uint32_t k = 0;          // its index in its own queue
     f3 org = mk3(0.0f, 0.0f, 0.0f), dir = org, B = org;
     float t1 = 0.0f;
     for (;;) {
-        // refill idle lanes
-        const bool need = !busy && !drained;
-        const uint64_t m = __ballot(need);
-        if (m) {
-            const int leader = __ffsll((unsigned long long)m) - 1;
-            uint32_t base = 0u;
-            if (lane == leader) base = atomicAdd(fetch, (uint32_t)__popcll(m));
-            base = __shfl(base, leader, 64);
-            if (need) {
-                const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                if (k < total) {
-                    e = k;
-                    float tn;
-                    if (kKind == 2 || kKind == 4) {
-                        const float4 a = sq.o[e], dd = sq.d[e], b = sq.b[e];
-                        org = xyz(a); dir = xyz(dd); B = xyz(b); tn = a.w; t1 = dd.w;
-                        ++nShadow;
-                    } else {
-                        const float4 o = q.o[e], d = q.d[e];
-                        org = xyz(o); dir = xyz(d); tn = o.w; t1 = FLT_MAX;
-                        if (kKind == 0) ++nCam; else if (kKind == 1) ++nLight; else ++nShadow;
-                    }
-                    itr = 10;
-                    ++rays;
-                    T.start(S, org, dir, tn, t1, st, cnt);
-                    T.tstop = (S.any_masked || S.closest_only) ? -1.0f : (kKind == 2 ? connectStop(org, B) : (kKind >= 3 ? FLT_MAX : -1.0f));
-                    busy = true;
-                } else {
-                    drained = true;
-                }
+        // idle lanes take the next entry of their list
+        if (!busy && w.more()) {
+            const uint32_t e = w.e;
+            w.next();
+            float tn, ts;
+            if (e < nA) {
+                k = e;
+                kind = 0;
+                const float4 o = A.q.o[k], d = A.q.d[k];
+                org = xyz(o); dir = xyz(d); tn = o.w; t1 = FLT_MAX; ts = -1.0f;
+                if (kKind == 1) ++nLight; else ++nCam;
+            } else if (e < nA + nB) {
+                k = e - nA;
+                kind = 2;
+                const float4 a = A.sq.o[k], dd = A.sq.d[k], b = A.sq.b[k];
+                org = xyz(a); dir = xyz(dd); B = xyz(b); tn = a.w; t1 = dd.w;
+                ts = kKind == 4 ? FLT_MAX : connectStop(org, B);
+                ++nShadow;
+            } else {
+                k = e - nA - nB;
+                kind = 3;
+                const float4 o = A.eq.o[k], d = A.eq.d[k];
+                org = xyz(o); dir = xyz(d); tn = o.w; t1 = FLT_MAX; ts = FLT_MAX;
+                ++nShadow;
             }
+            itr = 10;
+            ++rays;
+            // a zero or NaN direction leaves nothing to traverse: its
+            // result (no hit) is written at the next step
+            T.start(S, org, dir, tn, t1, st, cnt);
+            T.tstop = (S.any_masked || S.closest_only) ? -1.0f : ts;
+            busy = true;
         }
         if (__ballot(busy) == 0ull) break;
-        // traverse until too few lanes of the wave are busy
+        // step until fewer than keep lanes of the wave are busy
         for (;;) {
-            if (busy && T.more(st)) T.step(S, st, cnt);
-            if (busy && !T.more(st)) {
-                // one traceClosest call finished: sceneIntersect's masked re-trace loop
-                bool done = true, hit = T.found;
-                if (hit && S.any_masked) {
-                    const int mesh = S.insts[T.h.inst].mesh_id;
-                    const OmvMaterial& M = S.mats[S.meshes[mesh].mat_id];
-                    if (M.type == OMV_MAT_PBR && M.alpha_mode == OMV_ALPHA_MASK &&
-                        isMaskedHit(S, mesh, T.h.prim, mk3(1.0f - T.h.u - T.h.v, T.h.u, T.h.v))) {
-                        f3 o2 = mk3(org.x + T.h.t * dir.x, org.y + T.h.t * dir.y, org.z + T.h.t * dir.z);
-                        float v = maxAbs(o2);
-                        v = v > T.h.t ? v : T.h.t;
-                        float delta = 3.0f * FLT_EPSILON * v;
-                        const float tn = T.h.t + delta;
-                        hit = false;
-                        T.tnear = tn;
-                        if (!(t1 <= tn) && --itr > 0) {
-                            ++rays;
-                            T.start(S, org, dir, tn, t1, st, cnt);
-                            done = false;
+            if (busy) {
+                if (T.more(st)) T.step(S, st, cnt);
+                if (!T.more(st)) {
+                    // one traceClosest call finished: sceneIntersect's masked re-trace loop
+                    bool done = true, hit = T.found;
+                    if (hit && S.any_masked) {
+                        const int mesh = S.insts[T.h.inst].mesh_id;
+                        const OmvMaterial& M = S.mats[S.meshes[mesh].mat_id];
+                        if (M.type == OMV_MAT_PBR && M.alpha_mode == OMV_ALPHA_MASK &&
+                            isMaskedHit(S, mesh, T.h.prim, mk3(1.0f - T.h.u - T.h.v, T.h.u, T.h.v))) {
+                            f3 o2 = mk3(org.x + T.h.t * dir.x, org.y + T.h.t * dir.y, org.z + T.h.t * dir.z);
+                            float v = maxAbs(o2);
+                            v = v > T.h.t ? v : T.h.t;
+                            float delta = 3.0f * FLT_EPSILON * v;
+                            const float tn = T.h.t + delta;
+                            hit = false;
+                            T.tnear = tn;
+                            if (!(t1 <= tn) && --itr > 0) {
+                                ++rays;
+                                T.start(S, org, dir, tn, t1, st, cnt);
+                                done = false;
+                            }
                         }
                     }
-                }
-                if (done) {
-                    if (kKind == 2) {
-                        bool visible = true;
-                        if (hit) {
-                            f3 df = sub3(hitPosition(S, T.h), B);
-                            float sd = dot3(df, df);
-                            float safeD = 10.0f * FLT_EPSILON * maxAbs(B);
-                            visible = sd <= safeD * safeD;
+                    if (done) {
+                        if (kind == 2 && kKind != 4) {
+                            bool visible = true;
+                            if (hit) {
+                                f3 df = sub3(hitPosition(S, T.h), B);
+                                float sd = dot3(df, df);
+                                float safeD = 10.0f * FLT_EPSILON * maxAbs(B);
+                                visible = sd <= safeD * safeD;
+                            }
+                            const uint32_t id = __float_as_uint(A.sq.b[k].w);
+                            const uint32_t slot = id / (uint32_t)A.lc, l = id - slot * (uint32_t)A.lc;
+                            A.vis[(uint64_t)l * A.nslots + slot] = visible ? 2 : 0;
+                        } else if (kind == 2) {
+                            splatAdd(A.splat, !hit, __float_as_uint(A.sq.b[k].w), B);
+                        } else if (kind == 3) {
+                            A.nvis[__float_as_uint(A.eq.d[k].w)] = hit ? 0 : 2;
+                        } else {
+                            float4 r; r.x = T.h.t; r.y = T.h.u; r.z = T.h.v; r.w = T.tnear;
+                            int2 id; id.x = hit ? T.h.inst : -1; id.y = T.h.prim | (T.h.tie ? (int)0x80000000 : 0);
+                            A.H.h[k] = r;
+                            A.H.id[k] = id;
                         }
-                        const uint32_t id = __float_as_uint(sq.b[e].w);
-                        const uint32_t slot = id / (uint32_t)lc, l = id - slot * (uint32_t)lc;
-                        vis[(uint64_t)l * nslots + slot] = visible ? 2 : 0;
-                    } else if (kKind == 3) {
-                        vis[__float_as_uint(q.d[e].w)] = hit ? 0 : 2;
-                    } else if (kKind == 4) {
-                        splatAdd(splat, !hit, __float_as_uint(sq.b[e].w), B);
-                    } else {
-                        float4 r; r.x = T.h.t; r.y = T.h.u; r.z = T.h.v; r.w = T.tnear;
-                        int2 id; id.x = hit ? T.h.inst : -1; id.y = T.h.prim | (T.h.tie ? (int)0x80000000 : 0);
-                        H.h[e] = r;
-                        H.id[e] = id;
+                        busy = false;
                     }
-                    busy = false;
                 }
             }
-            if (__popcll(__ballot(busy)) < keep) break;
+            if (__popcll(__ballot(busy)) < A.keep) break;
         }
     }
-    flushCounters(counters, rays, nCam, nLight, nShadow, cnt);
+    flushCounters(A.counters, rays, nCam, nLight, nShadow, cnt);
 }
 
 // ---------------------------------------------------------- shade light
@@ -462,6 +529,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_light(OmvSceneView S, 
     for (uint32_t e = gtid; e < total; e += gthreads) {
         const float4 dd = in.d[e];
         const uint32_t slot = __float_as_uint(dd.w);
+        const f3 pDir = xyz(dd);
         const int2 id = H.id[e];
         bool emit = false, spl = false;
         f3 nPos = mk3(0.0f, 0.0f, 0.0f), nDir = nPos, sPos = nPos, sDir = nPos, ctb = nPos;
@@ -478,7 +546,6 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_light(OmvSceneView S, 
             const OmvLight Lt = S.lights[W.lidx[slot]];
             const f3 lightPos = ld3(Lt.pos);
             const f3 lRad = mk3(Lt.intensity[0] * P.light_scale, Lt.intensity[1] * P.light_scale, Lt.intensity[2] * P.light_scale);
-            const f3 pDir = xyz(dd);
             f3 outRay, brdf, erad;
             matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, false, ls);
             const float4 lt = i == 1 ? f4of(mk3(1.0f, 1.0f, 1.0f), 1.0f) : W.lthr[slot];
@@ -539,17 +606,26 @@ template <bool kLights, bool kNee>
 __global__ __launch_bounds__(OMV_BLOCK, OMV_SHADE_WAVES(kLights, kNee)) void omv_wf_shade_cam(OmvSceneView S, RP P, uint64_t sample0, int i, WfState W, WfQueue in,
                                                               const uint32_t* __restrict__ inCount, WfHits H, WfQueue out,
                                                               uint32_t* __restrict__ outCount, WfShadowQ sq, uint32_t* __restrict__ sqCount,
-                                                              WfQueue eq, uint32_t* __restrict__ eqCount, float* __restrict__ rad) {
+                                                              WfQueue eq, uint32_t* __restrict__ eqCount, float* __restrict__ rad, WfGen G) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t total = *inCount;
     const uint64_t N = W.n;
     const int q = P.q;
+    const bool gen = G.gen && i == 1;
     for (uint32_t e = gtid; e < total; e += gthreads) {
-        const float4 dd = in.d[e];
-        const uint32_t slot = __float_as_uint(dd.w);
+        uint32_t slot;
+        f3 pDir;
+        if (gen) {   // round 1: the camera ray of slot e
+            f3 org;
+            slot = e;
+            genCameraRay(G.cam, G.seed, G.spp, G.sample0 + e, org, pDir);
+        } else {
+            const float4 dd = in.d[e];
+            slot = __float_as_uint(dd.w);
+            pDir = xyz(dd);
+        }
         const int2 id = H.id[e];
-        const f3 pDir = xyz(dd);
         const bool first = i == 1;
         const float4 rc = first ? f4of(mk3(0.0f, 0.0f, 0.0f), __int_as_float(1)) : W.radc[slot];
         f3 radiance = xyz(rc);
@@ -701,13 +777,14 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_SHADE_WAVES(kLights, kNee)) void omv
 // per-sample function does), and writes the sample of paths that ended
 // this round.
 __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_connect(WfState W, WfQueue in, const uint32_t* __restrict__ inCount,
-                                                            float* __restrict__ rad, int lights, int nee, int round) {
+                                                            float* __restrict__ rad, int lights, int nee, int round, int gen) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t total = *inCount;
     const uint64_t N = W.n;
+    const bool slotIsEntry = gen && round == 1;   // round-1 rays computed from the slot: entry e is slot e
     for (uint32_t e = gtid; e < total; e += gthreads) {
-        const uint32_t slot = __float_as_uint(in.d[e].w);
+        const uint32_t slot = slotIsEntry ? e : __float_as_uint(in.d[e].w);
         const uint32_t fl = W.flags[slot];
         if (fl == 2u) continue;
         f3 conn = (lights && round > 1) ? xyz(W.conn[slot]) : mk3(0.0f, 0.0f, 0.0f);
@@ -887,7 +964,8 @@ size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool
     return b;
 }
 
-int wavefrontCounterCount(int B, int lvCap) { return 4 * lvCap + 6 * B + 16; }
+// lightCnt, camCnt, shCnt, envCnt, splCnt (see launchWavefront)
+int wavefrontCounterCount(int B, int lvCap) { return 2 * (lvCap + 2) + 3 * (B + 2); }
 
 void wavefrontBlocksPerCU(int out[8]) {
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
@@ -896,7 +974,8 @@ void wavefrontBlocksPerCU(int out[8]) {
                   occ(omv_wf_trace_fused, lds)})
         t = std::min(t, k);
     int tp = occ(omv_wf_trace_pt<0>, lds);
-    for (int k : {occ(omv_wf_trace_pt<1>, lds), occ(omv_wf_trace_pt<2>, lds), occ(omv_wf_trace_pt<3>, lds), occ(omv_wf_trace_pt<4>, lds)})
+    for (int k : {occ(omv_wf_trace_pt<1>, lds), occ(omv_wf_trace_pt<2>, lds), occ(omv_wf_trace_pt<3>, lds), occ(omv_wf_trace_pt<4>, lds),
+                  occ(omv_wf_trace_pt<5>, lds)})
         tp = std::min(tp, k);
     out[0] = t;
     out[4] = tp;
@@ -975,22 +1054,37 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
     uint32_t* shCnt = camCnt + (B + 2);
     uint32_t* envCnt = shCnt + (B + 2);
     uint32_t* splCnt = envCnt + (B + 2);
-    uint32_t* fetchCnt = splCnt + (L.lvCap + 2);     // one per persistent traversal launch
-    int nf = 0;
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
     const dim3 blk(OMV_BLOCK);
     const uint32_t xcds = L.xcdBands > 1 ? (uint32_t)L.xcdBands : 1u;
+    // refilling traversal (trace_keep > 0): one argument block per launch
+    auto ptArgs = [&](const WfQueue& q, const uint32_t* qc, const uint32_t* sqc, const WfQueue& e, const uint32_t* ec) {
+        WfTraceArgs a;
+        a.q = q; a.qCount = qc; a.sq = sq; a.sqCount = sqc; a.eq = e; a.eqCount = ec; a.H = H;
+        a.vis = W.vis; a.nvis = W.nvis; a.nslots = n; a.lc = L.lc;
+        a.gstack = L.gstack; a.counters = L.counters; a.splat = L.splat; a.xcds = xcds; a.keep = L.traceKeep;
+        return a;
+    };
     auto mark = [&]() {
         if (L.traceEv && L.traceEvUsed && *L.traceEvUsed < L.traceEvCap) (void)hipEventRecord(L.traceEv[(*L.traceEvUsed)++], stream);
     };
     hipLaunchKernelGGL(omv_wf_fill_u32, dim3((ncnt + 63) / 64), dim3(64), 0, stream, L.qcount, ncnt);
-    // camera rays of round 1 go to qa, light rays of round 1 to qb
+    // Round-1 camera rays: computed by the round-1 camera kernels from the
+    // slot (Gen1), or with the refilling traversal stored by gen in qa;
+    // round-1 light rays are stored in qb
+    WfGen Gen1;
+    std::memset(&Gen1, 0, sizeof(Gen1));
+    Gen1.gen = L.traceKeep > 0 ? 0 : 1;
+    Gen1.cam = L.cam; Gen1.seed = P.seed; Gen1.spp = L.p.samples_per_pixel; Gen1.sample0 = L.sample0;
+    WfGen Gen0 = Gen1;
+    Gen0.gen = 0;
+    const int writeQueues = Gen1.gen ? 0 : 1;
     if (lights)
         hipLaunchKernelGGL(omv_wf_gen<true>, dim3(L.blocksGen), blk, 0, stream, S, L.cam, P, L.p.samples_per_pixel, L.sample0, W, qa, qb,
-                           camCnt + 1, lightCnt + 1, L.rad);
+                           camCnt + 1, lightCnt + 1, L.rad, writeQueues);
     else
         hipLaunchKernelGGL(omv_wf_gen<false>, dim3(L.blocksGen), blk, 0, stream, S, L.cam, P, L.p.samples_per_pixel, L.sample0, W, qa, qb,
-                           camCnt + 1, lightCnt + 1, L.rad);
+                           camCnt + 1, lightCnt + 1, L.rad, writeQueues);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (lights) {
@@ -998,22 +1092,22 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
         for (int i = 1; i < L.lvCap; ++i) {
             mark();
             if (L.traceKeep > 0)
-                hipLaunchKernelGGL(omv_wf_trace_pt<1>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, lightCnt + i, fetchCnt + (nf++),
-                                   L.traceKeep, H, W.vis, n, L.lc, L.gstack, L.counters, L.splat);
+                hipLaunchKernelGGL(omv_wf_trace_pt<1>, dim3(L.blocksTrace), blk, lds, stream, S,
+                                   ptArgs(in, lightCnt + i, nullptr, eq, nullptr));
             else
                 hipLaunchKernelGGL(omv_wf_trace<1>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, lightCnt + i, H, W.vis, n, L.lc, L.gstack,
-                                   L.counters, L.splat, xcds);
+                                   L.counters, L.splat, xcds, Gen0);
             mark();
             if (splat) {
                 hipLaunchKernelGGL(omv_wf_shade_light<true>, dim3(L.blocksShade), blk, 0, stream, S, L.cam, P, L.sample0, i, W, in, lightCnt + i,
                                    H, out, lightCnt + i + 1, sq, splCnt + i);
                 mark();
                 if (L.traceKeep > 0)
-                    hipLaunchKernelGGL(omv_wf_trace_pt<4>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, splCnt + i, fetchCnt + (nf++),
-                                       L.traceKeep, H, W.vis, n, L.lc, L.gstack, L.counters, L.splat);
+                    hipLaunchKernelGGL(omv_wf_trace_pt<4>, dim3(L.blocksTrace), blk, lds, stream, S,
+                                       ptArgs(in, nullptr, splCnt + i, eq, nullptr));
                 else
                     hipLaunchKernelGGL(omv_wf_trace<4>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, splCnt + i, H, W.vis, n, L.lc, L.gstack,
-                                       L.counters, L.splat, xcds);
+                                       L.counters, L.splat, xcds, Gen0);
                 mark();
             } else {
                 hipLaunchKernelGGL(omv_wf_shade_light<false>, dim3(L.blocksShade), blk, 0, stream, S, L.cam, P, L.sample0, i, W, in, lightCnt + i,
@@ -1029,7 +1123,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
     // connect(i) still runs before shade(i + 1) (which reads the connection
     // sum of escaping paths and overwrites the staged connections), so every
     // sum is formed in the same order as before.
-    const bool fuse = (lights || nee) && L.traceKeep <= 0;
+    const bool fuse = lights || nee;
     // rounds i0 .. B in one launch (0: none); the round before i0 leaves
     // its camera rays untraced for it
     const int i0 = L.tailRound >= 2 && L.tailRound <= B ? L.tailRound : 0;
@@ -1059,28 +1153,28 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
         if (!traced) {
             mark();
             if (L.traceKeep > 0)
-                hipLaunchKernelGGL(omv_wf_trace_pt<0>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, camCnt + i, fetchCnt + (nf++),
-                                   L.traceKeep, H, W.vis, n, L.lc, L.gstack, L.counters, L.splat);
+                hipLaunchKernelGGL(omv_wf_trace_pt<0>, dim3(L.blocksTrace), blk, lds, stream, S,
+                                   ptArgs(in, camCnt + i, nullptr, eq, nullptr));
             else
                 hipLaunchKernelGGL(omv_wf_trace<0>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, camCnt + i, H, W.vis, n, L.lc,
-                                   L.gstack, L.counters, L.splat, xcds);
+                                   L.gstack, L.counters, L.splat, xcds, i == 1 ? Gen1 : Gen0);
             mark();
         }
         traced = false;
         if (lights) {
             if (nee)
                 hipLaunchKernelGGL((omv_wf_shade_cam<true, true>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i, H,
-                                   out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad);
+                                   out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad, Gen1);
             else
                 hipLaunchKernelGGL((omv_wf_shade_cam<true, false>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i,
-                                   H, out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad);
+                                   H, out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad, Gen1);
         } else {
             if (nee)
                 hipLaunchKernelGGL((omv_wf_shade_cam<false, true>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i,
-                                   H, out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad);
+                                   H, out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad, Gen1);
             else
                 hipLaunchKernelGGL((omv_wf_shade_cam<false, false>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i,
-                                   H, out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad);
+                                   H, out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad, Gen1);
         }
         bool sorted = false;
         const bool nextInWavefront = i < B && i + 1 != i0;   // round i + 1 is traced by this schedule's round kernels
@@ -1096,35 +1190,41 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
         // the next round's hits go to H, which this round's connect does not read
         if (fuse) {
             mark();
-            hipLaunchKernelGGL(omv_wf_trace_fused, dim3(L.blocksTrace), blk, lds, stream, S, next, nextInWavefront ? camCnt + i + 1 : nullptr,
-                               sq, lights ? shCnt + i : nullptr, eq, nee ? envCnt + i : nullptr, H, W.vis, W.nvis, n, L.lc, L.gstack,
-                               L.counters, xcds);
+            if (L.traceKeep > 0)
+                hipLaunchKernelGGL(omv_wf_trace_pt<5>, dim3(L.blocksTrace), blk, lds, stream, S,
+                                   ptArgs(next, nextInWavefront ? camCnt + i + 1 : nullptr, lights ? shCnt + i : nullptr, eq,
+                                          nee ? envCnt + i : nullptr));
+            else
+                hipLaunchKernelGGL(omv_wf_trace_fused, dim3(L.blocksTrace), blk, lds, stream, S, next, nextInWavefront ? camCnt + i + 1 : nullptr,
+                                   sq, lights ? shCnt + i : nullptr, eq, nee ? envCnt + i : nullptr, H, W.vis, W.nvis, n, L.lc, L.gstack,
+                                   L.counters, xcds);
             mark();
             traced = nextInWavefront;
         } else {
             if (lights) {
                 mark();
                 if (L.traceKeep > 0)
-                    hipLaunchKernelGGL(omv_wf_trace_pt<2>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, shCnt + i, fetchCnt + (nf++),
-                                       L.traceKeep, H, W.vis, n, L.lc, L.gstack, L.counters, L.splat);
+                    hipLaunchKernelGGL(omv_wf_trace_pt<2>, dim3(L.blocksTrace), blk, lds, stream, S,
+                                       ptArgs(in, nullptr, shCnt + i, eq, nullptr));
                 else
                     hipLaunchKernelGGL(omv_wf_trace<2>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, shCnt + i, H, W.vis, n, L.lc,
-                                       L.gstack, L.counters, L.splat, xcds);
+                                       L.gstack, L.counters, L.splat, xcds, Gen0);
                 mark();
             }
             if (nee) {
                 mark();
                 if (L.traceKeep > 0)
-                    hipLaunchKernelGGL(omv_wf_trace_pt<3>, dim3(L.blocksTrace), blk, lds, stream, S, eq, sq, envCnt + i, fetchCnt + (nf++),
-                                       L.traceKeep, H, W.nvis, n, L.lc, L.gstack, L.counters, L.splat);
+                    hipLaunchKernelGGL(omv_wf_trace_pt<3>, dim3(L.blocksTrace), blk, lds, stream, S,
+                                       ptArgs(in, nullptr, nullptr, eq, envCnt + i));
                 else
                     hipLaunchKernelGGL(omv_wf_trace<3>, dim3(L.blocksTrace), blk, lds, stream, S, eq, sq, envCnt + i, H, W.nvis, n, L.lc,
-                                       L.gstack, L.counters, L.splat, xcds);
+                                       L.gstack, L.counters, L.splat, xcds, Gen0);
                 mark();
             }
         }
         if (lights || nee)
-            hipLaunchKernelGGL(omv_wf_connect, dim3(L.blocksShade), blk, 0, stream, W, in, camCnt + i, L.rad, lights ? 1 : 0, nee ? 1 : 0, i);
+            hipLaunchKernelGGL(omv_wf_connect, dim3(L.blocksShade), blk, 0, stream, W, in, camCnt + i, L.rad, lights ? 1 : 0, nee ? 1 : 0, i,
+                               Gen1.gen);
         if (sorted) {
             const WfQueue freed = in;
             in = spare;

@@ -286,7 +286,7 @@ size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool
     if (sortRays) b += upb(N * 4) + upb(4096 * 4);
     return b;
 }
-int wavefrontCounterCount(int B, int lvCap) { return 4 * lvCap + 6 * B + 16; }
+int wavefrontCounterCount(int B, int lvCap) { return 2 * (lvCap + 2) + 3 * (B + 2); }
 void wavefrontBlocksPerCU(int out[8]) { out[0] = 8; out[1] = 6; out[2] = 6; out[3] = 8; out[4] = 4; out[5] = 8; out[6] = 6; out[7] = 6; }
 int occupancyBlocksPerCU(int) { return 4; }
 int adjointBlocksPerCU() { return 4; }

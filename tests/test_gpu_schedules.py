@@ -269,3 +269,35 @@ def test_tail_launch_keeps_the_film(pkg, wl, case):
             assert np.array_equal(got, want), (tail, np.argwhere(got != want)[:5].tolist())
         for k in ("rays", "camera_rays", "light_rays", "shadow_rays", "samples"):
             assert gs[k] == ws[k], (tail, k, gs[k], ws[k])
+
+
+@pytest.mark.parametrize("case", ["nee", "splat", "instanced"])
+def test_refilling_traversal_keeps_the_film(pkg, wl, case):
+    """The refilling traversal (trace_keep > 0: lanes take the next ray of
+    their own list once few lanes of the wave are busy), also in the fused
+    launch with environment and connection rays, gives the plain
+    traversal's film and ray counts."""
+    if case == "instanced":
+        desc, view, _ = wl.large_instanced(width=64, height=36, spp=2, depth=6, count=24, tris_per=600, hdr_w=128, hdr_h=64)
+        params = dict(samples_per_pixel=2, camera_bounces=6, light_bounces=6, max_path_length=6, seed=13)
+    else:
+        desc, view = wl.mixed_test_scene(hdr=True, lights=True, masked=True)
+        params = dict(samples_per_pixel=3, camera_bounces=5, light_bounces=5, max_path_length=5, seed=13,
+                      extensions={"nee": 2, "splat": 4}[case])
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = pkg.Camera(min(view.width, 64), min(view.height, 48), view.position, view.forward, view.up, view.yfov)
+    out = {}
+    for keep in (0, 16, 48, 64):
+        r = pkg.Renderer(pkg.default_params(**params))
+        r.set_scene(s)
+        r.set_tuning(trace_keep=keep)
+        out[keep] = (r.render(cam), r.stats())
+    want, ws = out[0]
+    for keep, (got, gs) in out.items():
+        if case == "splat":   # splat sums are float atomics: order-dependent
+            np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-6)
+        else:
+            assert np.array_equal(got, want), (keep, np.argwhere(got != want)[:5].tolist())
+        for k in ("rays", "camera_rays", "light_rays", "shadow_rays", "samples"):
+            assert gs[k] == ws[k], (keep, k, gs[k], ws[k])
