@@ -46,7 +46,7 @@ struct WfLaunch {
     uint64_t sample0;          // first sample of the chunk
     uint32_t nslots;           // samples in the chunk
     int lvCap, lc;             // light vertices kept, connections per camera vertex
-    void* arena;               // wavefrontArenaBytes(nslots, lvCap, lc, lights, nee, splat, sortRays)
+    void* arena;               // wavefrontArenaBytes(nslots, lvCap, lc, lights, nee, splat, sortRays, blocksShade)
     uint32_t* qcount;          // wavefrontCounterCount(B, lvCap) counters
     float* rad;                // 3 floats per sample of the chunk
     uint32_t* gstack;          // spill stacks of blocksTrace * OMV_BLOCK lanes
@@ -63,7 +63,12 @@ struct WfLaunch {
     int xcdBands;              // > 1: traversal queues dealt to this many XCDs in chunks (L2 locality); else plain stride
     int tailRound;             // >= 2: camera rounds tailRound .. B run as one launch (omv_wf_tail); else round by round
 };
-size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat, bool sortRays);
+// Ray queues are appended to through OMV_QSEG counters each (one per group of
+// producing workgroups); wavefrontSegCap is the entries of one group's region
+// for a producing launch of gridShade workgroups (blocksShade).
+#define OMV_QSEG 64
+uint32_t wavefrontSegCap(uint32_t nslots, int gridShade);
+size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat, bool sortRays, int gridShade);
 int wavefrontCounterCount(int B, int lvCap);
 // resident blocks per CU: [0] trace, [1] camera shading with lights + NEE,
 // [2] light shading, [3] gen, [4] persistent trace, [5] camera shading with

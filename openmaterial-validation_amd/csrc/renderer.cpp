@@ -386,8 +386,9 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     if ((e = dStack_.reserve((uint64_t)blocksTrace * OMV_BLOCK * (OMV_STACK_TOTAL - OMV_STACK_LDS) * sizeof(uint32_t))) != hipSuccess)
         return hipFail(e, "stack spill");
     const bool nee = (p.extensions & OMV_EXT_HDRI_NEE) && S.env_tex_pdf;
-    if ((e = dArena_.reserve(wavefrontArenaBytes((uint32_t)slots, lvCap, lc, lights, nee, splat_ != nullptr, tuning_.sort_rays == 2))) !=
-        hipSuccess)
+    const int blocksShade = cus_ * shadeCamBlocksPerCU(wfPerCu_, lights, nee);
+    if ((e = dArena_.reserve(wavefrontArenaBytes((uint32_t)slots, lvCap, lc, lights, nee, splat_ != nullptr, tuning_.sort_rays == 2,
+                                                 blocksShade))) != hipSuccess)
         return hipFail(e, "wavefront state");
     if ((e = dQcount_.reserve(wavefrontCounterCount(p.camera_bounces, lvCap) * sizeof(uint32_t))) != hipSuccess) return hipFail(e, "queue counters");
     if ((e = dRad_.reserve(slots * 3 * sizeof(float))) != hipSuccess) return hipFail(e, "sample buffer");
@@ -397,7 +398,7 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     L.arena = dArena_.ptr; L.qcount = dQcount_.as<uint32_t>(); L.rad = dRad_.as<float>(); L.gstack = dStack_.as<uint32_t>();
     L.counters = dCounters_.as<unsigned long long>();
     L.blocksTrace = blocksTrace;
-    L.blocksShade = cus_ * shadeCamBlocksPerCU(wfPerCu_, lights, nee);
+    L.blocksShade = blocksShade;
     L.blocksGen = cus_ * wfPerCu_[3];
     L.traceKeep = tuning_.trace_keep;
     L.splat = splat_;

@@ -98,6 +98,52 @@ DEV uint32_t waveAppend(uint32_t* counter, bool want) {
 
 DEV void stackInit(Stack& st, uint32_t* gstack, uint32_t gtid, uint32_t gthreads) { st.init(gstack, gtid, gthreads); }
 
+// A queue's entries in K regions of cap entries: region r holds the physical
+// entries [r*cap, r*cap + cnt[r]).  A producer appends to the region of its
+// workgroup (blockIdx mod K), so the append atomics of a launch are spread
+// over K counters instead of serialising on one (a wave-aggregated atomic on
+// one address costs ~11 ns of that address's time on MI355X; over 64
+// addresses ~0.35 ns).  Consumers walk the logical index 0 .. sum(cnt) - 1
+// region by region.  The host sizes cap so that no region can overflow (a
+// region is fed by ceil(grid/K) workgroups, each handling at most
+// OMV_BLOCK * ceil(N / threads) input entries, with at most `per` appends
+// each); a full region still drops the append and counts it in `lost`
+// rather than write out of bounds.  K = 1 is a plain compact queue.
+struct WfSeg {
+    uint32_t* cnt;    // K counters (null: an empty queue)
+    uint32_t K, cap;
+};
+inline WfSeg segOf(uint32_t* cnt, uint32_t K, uint32_t cap) { WfSeg g; g.cnt = cnt; g.K = K; g.cap = cap; return g; }
+DEV uint32_t segTotal(const WfSeg& g) {
+    uint32_t t = 0;
+    if (g.cnt)
+        for (uint32_t r = 0; r < g.K; ++r) t += min(g.cnt[r], g.cap);
+    return t;
+}
+// logical -> physical index for a non-decreasing sequence of logical indices
+struct SegCursor {
+    const uint32_t* cnt;
+    uint32_t K, cap, r, lo, hi;
+    DEV void init(const WfSeg& g) {
+        cnt = g.cnt; K = g.K; cap = g.cap; r = 0; lo = 0;
+        hi = g.cnt ? min(g.cnt[0], cap) : 0u;
+    }
+    DEV uint32_t phys(uint32_t L) {
+        while (L >= hi && r + 1 < K) { ++r; lo = hi; hi += min(cnt[r], cap); }
+        return r * cap + (L - lo);
+    }
+};
+// Physical slot of the calling lane's entry, or OMV_SEG_NONE when it does not
+// append (or its region is full); all active lanes of the wave make the call.
+#define OMV_SEG_NONE 0xFFFFFFFFu
+DEV uint32_t segAppend(const WfSeg& g, bool want, unsigned long long* lost) {
+    const uint32_t r = blockIdx.x % g.K;
+    const uint32_t k = waveAppend(g.cnt + r, want);
+    if (!want) return OMV_SEG_NONE;
+    if (k >= g.cap) { atomicAdd(lost, 1ull); return OMV_SEG_NONE; }
+    return r * g.cap + k;
+}
+
 DEV f3 xyz(const float4& v) { return mk3(v.x, v.y, v.z); }
 DEV float4 f4of(const f3& v, float w) { float4 r; r.x = v.x; r.y = v.y; r.z = v.z; r.w = w; return r; }
 
@@ -248,7 +294,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_gen(OmvSceneView S, omv_came
 // 4: pinhole rays of OMV_EXT_LIGHT_SPLAT (visible iff nothing is hit; the
 //    visible contributions are added to the splat film).
 template <int kKind>
-__global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSceneView S, WfQueue q, WfShadowQ sq, const uint32_t* __restrict__ count,
+__global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSceneView S, WfQueue q, WfShadowQ sq, WfSeg count,
                                                           WfHits H, uint8_t* __restrict__ vis, uint32_t nslots, int lc,
                                                           uint32_t* __restrict__ gstack, unsigned long long* __restrict__ counters,
                                                           float* __restrict__ splat, uint32_t xcds, WfGen G) {
@@ -259,11 +305,14 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
     Counters cnt; cnt.nodes = 0; cnt.tris = 0; cnt.overflow = 0;
     uint32_t rays = 0, nCam = 0, nLight = 0, nShadow = 0;
     XcdWalk w;
-    for (w.start(*count, xcds); w.more(); w.next()) {
-        const uint32_t e = w.e;
+    SegCursor cur;
+    cur.init(count);
+    for (w.start(segTotal(count), xcds); w.more(); w.next()) {
+        const uint32_t e = w.e;                                       // logical entry (hit records)
+        const uint32_t p = kKind == 0 && G.gen ? e : cur.phys(e);   // its queue slot
         Hit h;
         if (kKind == 2) {
-            const float4 a = sq.o[e], dd = sq.d[e], b = sq.b[e];
+            const float4 a = sq.o[p], dd = sq.d[p], b = sq.b[p];
             float tnear = a.w;
             ++nShadow;
             bool visible;
@@ -279,13 +328,13 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
             const uint32_t slot = id / (uint32_t)lc, l = id - slot * (uint32_t)lc;
             vis[(uint64_t)l * nslots + slot] = visible ? 2 : 0;
         } else if (kKind == 3) {
-            const float4 o = q.o[e], d = q.d[e];
+            const float4 o = q.o[p], d = q.d[p];
             float tnear = o.w;
             ++nShadow;
             const bool hit = sceneIntersect(S, xyz(o), xyz(d), tnear, FLT_MAX, h, st, cnt, rays, FLT_MAX);
             vis[__float_as_uint(d.w)] = hit ? 0 : 2;
         } else if (kKind == 4) {
-            const float4 a = sq.o[e], dd = sq.d[e], b = sq.b[e];
+            const float4 a = sq.o[p], dd = sq.d[p], b = sq.b[p];
             float tnear = a.w;
             ++nShadow;
             const bool hit = sceneIntersect(S, xyz(a), xyz(dd), tnear, dd.w, h, st, cnt, rays, FLT_MAX);
@@ -296,7 +345,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
             if (kKind == 0 && G.gen) {   // round 1: the camera ray of slot e
                 genCameraRay(G.cam, G.seed, G.spp, G.sample0 + e, org, dir);
             } else {
-                const float4 o = q.o[e], d = q.d[e];
+                const float4 o = q.o[p], d = q.d[p];
                 org = xyz(o); dir = xyz(d); tnear = o.w;
             }
             if (kKind == 0) ++nCam; else ++nLight;
@@ -318,22 +367,25 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
 // of three separate launches; one launch pays one latency tail instead of
 // up to three.  A null count is an empty queue.
 __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused(
-    OmvSceneView S, WfQueue q, const uint32_t* __restrict__ qCount, WfShadowQ sq, const uint32_t* __restrict__ sqCount, WfQueue eq,
-    const uint32_t* __restrict__ eqCount, WfHits H, uint8_t* __restrict__ vis, uint8_t* __restrict__ nvis, uint32_t nslots, int lc,
-    uint32_t* __restrict__ gstack, unsigned long long* __restrict__ counters, uint32_t xcds) {
+    OmvSceneView S, WfQueue q, WfSeg qCount, WfShadowQ sq, WfSeg sqCount, WfQueue eq, WfSeg eqCount, WfHits H, uint8_t* __restrict__ vis,
+    uint8_t* __restrict__ nvis, uint32_t nslots, int lc, uint32_t* __restrict__ gstack, unsigned long long* __restrict__ counters,
+    uint32_t xcds) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
     stackInit(st, gstack, gtid, gthreads);
     Counters cnt; cnt.nodes = 0; cnt.tris = 0; cnt.overflow = 0;
     uint32_t rays = 0, nCam = 0, nShadow = 0;
-    const uint32_t nA = qCount ? *qCount : 0u, nB = sqCount ? *sqCount : 0u, nC = eqCount ? *eqCount : 0u;
+    const uint32_t nA = segTotal(qCount), nB = segTotal(sqCount), nC = segTotal(eqCount);
+    SegCursor cA, cB, cC;
+    cA.init(qCount); cB.init(sqCount); cC.init(eqCount);
     XcdWalk w;
     for (w.start(nA + nB + nC, xcds); w.more(); w.next()) {
         const uint32_t e = w.e;
         Hit h;
         if (e < nA) {
-            const float4 o = q.o[e], d = q.d[e];
+            const uint32_t p = cA.phys(e);
+            const float4 o = q.o[p], d = q.d[p];
             float tnear = o.w;
             ++nCam;
             const bool hit = sceneIntersect(S, xyz(o), xyz(d), tnear, FLT_MAX, h, st, cnt, rays);
@@ -342,7 +394,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused
             H.h[e] = r;
             H.id[e] = id;
         } else if (e < nA + nB) {
-            const uint32_t k = e - nA;
+            const uint32_t k = cB.phys(e - nA);
             const float4 a = sq.o[k], dd = sq.d[k], b = sq.b[k];
             float tnear = a.w;
             ++nShadow;
@@ -359,7 +411,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused
             const uint32_t slot = id / (uint32_t)lc, l = id - slot * (uint32_t)lc;
             vis[(uint64_t)l * nslots + slot] = visible ? 2 : 0;
         } else {
-            const uint32_t k = e - nA - nB;
+            const uint32_t k = cC.phys(e - nA - nB);
             const float4 o = eq.o[k], d = eq.d[k];
             float tnear = o.w;
             ++nShadow;
@@ -374,9 +426,9 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused
 // 0..4 as omv_wf_trace: q or sq with its count; kKind 5: the fused launch's
 // camera, connection and environment queues back to back).
 struct WfTraceArgs {
-    WfQueue q; const uint32_t* qCount;
-    WfShadowQ sq; const uint32_t* sqCount;
-    WfQueue eq; const uint32_t* eqCount;
+    WfQueue q; WfSeg qCount;
+    WfShadowQ sq; WfSeg sqCount;
+    WfQueue eq; WfSeg eqCount;
     WfHits H;
     uint8_t* vis; uint8_t* nvis;
     uint32_t nslots; int lc;
@@ -403,16 +455,9 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_pt(Om
     uint32_t rays = 0, nCam = 0, nLight = 0, nShadow = 0;
     // entries [0, nA) camera / light rays of q, [nA, nA + nB) rays of sq,
     // [nA + nB, total) environment rays of eq (by kKind)
-    uint32_t nA = 0, nB = 0, nC = 0;
-    if (kKind == 5) {
-        nA = A.qCount ? *A.qCount : 0u; nB = A.sqCount ? *A.sqCount : 0u; nC = A.eqCount ? *A.eqCount : 0u;
-    } else if (kKind == 2 || kKind == 4) {
-        nB = *A.sqCount;
-    } else if (kKind == 3) {
-        nC = *A.eqCount;
-    } else {
-        nA = *A.qCount;
-    }
+    const uint32_t nA = segTotal(A.qCount), nB = segTotal(A.sqCount), nC = segTotal(A.eqCount);
+    SegCursor cA, cB, cC;
+    cA.init(A.qCount); cB.init(A.sqCount); cC.init(A.eqCount);
     XcdWalk w;
     w.start(nA + nB + nC, A.xcds);
     Trav T;
@@ -428,20 +473,21 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_pt(Om
             w.next();
             float tn, ts;
             if (e < nA) {
-                k = e;
+                k = e;                       // logical: the hit record's index
                 kind = 0;
-                const float4 o = A.q.o[k], d = A.q.d[k];
+                const uint32_t p = cA.phys(e);
+                const float4 o = A.q.o[p], d = A.q.d[p];
                 org = xyz(o); dir = xyz(d); tn = o.w; t1 = FLT_MAX; ts = -1.0f;
                 if (kKind == 1) ++nLight; else ++nCam;
             } else if (e < nA + nB) {
-                k = e - nA;
+                k = cB.phys(e - nA);         // physical: re-read at the end
                 kind = 2;
                 const float4 a = A.sq.o[k], dd = A.sq.d[k], b = A.sq.b[k];
                 org = xyz(a); dir = xyz(dd); B = xyz(b); tn = a.w; t1 = dd.w;
                 ts = kKind == 4 ? FLT_MAX : connectStop(org, B);
                 ++nShadow;
             } else {
-                k = e - nA - nB;
+                k = cC.phys(e - nA - nB);
                 kind = 3;
                 const float4 o = A.eq.o[k], d = A.eq.d[k];
                 org = xyz(o); dir = xyz(d); tn = o.w; t1 = FLT_MAX; ts = FLT_MAX;
@@ -520,14 +566,16 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_pt(Om
 // the visibility trace.
 template <bool kSplat>
 __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_light(OmvSceneView S, omv_camera_desc cam, RP P, uint64_t sample0, int i, WfState W,
-                                                                WfQueue in, const uint32_t* __restrict__ inCount, WfHits H, WfQueue out,
-                                                                uint32_t* __restrict__ outCount, WfShadowQ sq, uint32_t* __restrict__ sqCount) {
+                                                                WfQueue in, WfSeg inCount, WfHits H, WfQueue out, WfSeg outCount, WfShadowQ sq,
+                                                                WfSeg sqCount, unsigned long long* __restrict__ lost) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t total = *inCount;
+    const uint32_t total = segTotal(inCount);
     const uint64_t N = W.n;
+    SegCursor cur;
+    cur.init(inCount);
     for (uint32_t e = gtid; e < total; e += gthreads) {
-        const float4 dd = in.d[e];
+        const float4 dd = in.d[cur.phys(e)];
         const uint32_t slot = __float_as_uint(dd.w);
         const f3 pDir = xyz(dd);
         const int2 id = H.id[e];
@@ -581,14 +629,14 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_light(OmvSceneView S, 
                 }
             }
         }
-        const uint32_t k = waveAppend(outCount, emit);
-        if (emit) {
+        const uint32_t k = segAppend(outCount, emit, lost);
+        if (k != OMV_SEG_NONE) {
             out.o[k] = f4of(nPos, nTnear);
             out.d[k] = f4of(nDir, __uint_as_float(slot));
         }
         if (kSplat) {
-            const uint32_t ks = waveAppend(sqCount, spl);
-            if (spl) {
+            const uint32_t ks = segAppend(sqCount, spl, lost);
+            if (ks != OMV_SEG_NONE) {
                 sq.o[ks] = f4of(sPos, sTn);
                 sq.d[ks] = f4of(sDir, sD);
                 sq.b[ks] = f4of(ctb, __uint_as_float(pix));
@@ -604,15 +652,17 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_light(OmvSceneView S, 
 // OMV_EXT_HDRI_NEE, the next-event estimate staged for its visibility trace.
 template <bool kLights, bool kNee>
 __global__ __launch_bounds__(OMV_BLOCK, OMV_SHADE_WAVES(kLights, kNee)) void omv_wf_shade_cam(OmvSceneView S, RP P, uint64_t sample0, int i, WfState W, WfQueue in,
-                                                              const uint32_t* __restrict__ inCount, WfHits H, WfQueue out,
-                                                              uint32_t* __restrict__ outCount, WfShadowQ sq, uint32_t* __restrict__ sqCount,
-                                                              WfQueue eq, uint32_t* __restrict__ eqCount, float* __restrict__ rad, WfGen G) {
+                                                              WfSeg inCount, WfHits H, WfQueue out, WfSeg outCount, WfShadowQ sq,
+                                                              WfSeg sqCount, WfQueue eq, WfSeg eqCount, float* __restrict__ rad, WfGen G,
+                                                              unsigned long long* __restrict__ lost) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t total = *inCount;
+    const uint32_t total = segTotal(inCount);
     const uint64_t N = W.n;
     const int q = P.q;
     const bool gen = G.gen && i == 1;
+    SegCursor cur;
+    cur.init(inCount);
     for (uint32_t e = gtid; e < total; e += gthreads) {
         uint32_t slot;
         f3 pDir;
@@ -621,7 +671,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_SHADE_WAVES(kLights, kNee)) void omv
             slot = e;
             genCameraRay(G.cam, G.seed, G.spp, G.sample0 + e, org, pDir);
         } else {
-            const float4 dd = in.d[e];
+            const float4 dd = in.d[cur.phys(e)];
             slot = __float_as_uint(dd.w);
             pDir = xyz(dd);
         }
@@ -708,9 +758,10 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_SHADE_WAVES(kLights, kNee)) void omv
                             W.cc[((uint64_t)l * 3 + 2) * N + slot] = lc.z;
                         }
                     }
+                    const uint32_t k = segAppend(sqCount, ray, lost);
+                    if (ray && k == OMV_SEG_NONE) v = 0;   // a full region (cannot happen with the host's sizing)
                     W.vis[(uint64_t)l * N + slot] = v;
-                    const uint32_t k = waveAppend(sqCount, ray);
-                    if (ray) {
+                    if (k != OMV_SEG_NONE) {
                         sq.o[k] = f4of(A, tnear);
                         sq.d[k] = f4of(dir, tfar);
                         sq.b[k] = f4of(B, __uint_as_float(slot * (uint32_t)W.lc + (uint32_t)l));
@@ -756,14 +807,15 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_SHADE_WAVES(kLights, kNee)) void omv
                 W.flags[slot] = 2u;
             }
         }
-        const uint32_t k = waveAppend(outCount, emit);
-        if (emit) {
+        const uint32_t k = segAppend(outCount, emit, lost);
+        if (k != OMV_SEG_NONE) {
             out.o[k] = f4of(nPos, nTnear);
             out.d[k] = f4of(nDir, __uint_as_float(slot));
         }
         if (kNee) {
-            const uint32_t ke = waveAppend(eqCount, envRay);
-            if (envRay) {
+            const uint32_t ke = segAppend(eqCount, envRay, lost);
+            if (envRay && ke == OMV_SEG_NONE) W.nvis[slot] = 0;
+            if (ke != OMV_SEG_NONE) {
                 eq.o[ke] = f4of(eOrg, 0.0f);
                 eq.d[ke] = f4of(eDir, __uint_as_float(slot));
             }
@@ -776,15 +828,17 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_SHADE_WAVES(kLights, kNee)) void omv
 // visible next-event estimate (after the vertex's emission, as the
 // per-sample function does), and writes the sample of paths that ended
 // this round.
-__global__ __launch_bounds__(OMV_BLOCK) void omv_wf_connect(WfState W, WfQueue in, const uint32_t* __restrict__ inCount,
-                                                            float* __restrict__ rad, int lights, int nee, int round, int gen) {
+__global__ __launch_bounds__(OMV_BLOCK) void omv_wf_connect(WfState W, WfQueue in, WfSeg inCount, float* __restrict__ rad, int lights,
+                                                            int nee, int round, int gen) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t total = *inCount;
+    const uint32_t total = segTotal(inCount);
     const uint64_t N = W.n;
     const bool slotIsEntry = gen && round == 1;   // round-1 rays computed from the slot: entry e is slot e
+    SegCursor cur;
+    cur.init(inCount);
     for (uint32_t e = gtid; e < total; e += gthreads) {
-        const uint32_t slot = slotIsEntry ? e : __float_as_uint(in.d[e].w);
+        const uint32_t slot = slotIsEntry ? e : __float_as_uint(in.d[cur.phys(e)].w);
         const uint32_t fl = W.flags[slot];
         if (fl == 2u) continue;
         f3 conn = (lights && round > 1) ? xyz(W.conn[slot]) : mk3(0.0f, 0.0f, 0.0f);
@@ -830,16 +884,19 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_connect(WfState W, WfQueue i
 // order, so the film is unchanged.
 template <bool kLights, bool kNee>
 __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_tail(OmvSceneView S, RP P, uint64_t sample0, int i0, WfState W, WfQueue in,
-                                                         const uint32_t* __restrict__ inCount, float* __restrict__ rad,
-                                                         uint32_t* __restrict__ gstack, unsigned long long* __restrict__ counters) {
+                                                         WfSeg inCount, float* __restrict__ rad, uint32_t* __restrict__ gstack,
+                                                         unsigned long long* __restrict__ counters) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Lane L;
     laneInit(L, gstack, nullptr, gtid, gthreads);
-    const uint32_t total = *inCount;
+    const uint32_t total = segTotal(inCount);
     const uint64_t N = W.n;
+    SegCursor cur;
+    cur.init(inCount);
     for (uint32_t e = gtid; e < total; e += gthreads) {
-        const float4 o = in.o[e], d = in.d[e];
+        const uint32_t p = cur.phys(e);
+        const float4 o = in.o[p], d = in.d[p];
         const uint32_t slot = __float_as_uint(d.w);
         const float4 rc = W.radc[slot], th = W.thr[slot];
         LightSub ls;
@@ -950,22 +1007,34 @@ constexpr size_t kAlign = 256;
 size_t up(size_t b) { return (b + kAlign - 1) & ~(kAlign - 1); }
 }  // namespace
 
-size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat, bool sortRays) {
+// Entries per append region of a queue fed by a launch of gridShade
+// workgroups over at most nslots input entries with one append each: the
+// workgroups of one region (ceil(grid / K)) handle at most OMV_BLOCK *
+// ceil(nslots / threads) entries each.  A queue holds OMV_QSEG regions
+// (>= nslots entries in all).
+uint32_t wavefrontSegCap(uint32_t nslots, int gridShade) {
+    const uint64_t g = (uint64_t)std::max(gridShade, 1), threads = g * OMV_BLOCK;
+    const uint64_t perBlock = (uint64_t)OMV_BLOCK * ((nslots + threads - 1) / threads);
+    return (uint32_t)(((g + OMV_QSEG - 1) / OMV_QSEG) * perBlock);
+}
+
+size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat, bool sortRays, int gridShade) {
     const size_t N = nslots;
-    size_t b = 6 * up(N * 16)                        // three ray queues
+    const size_t Q = (size_t)OMV_QSEG * wavefrontSegCap(nslots, gridShade);   // queue entries
+    size_t b = 6 * up(Q * 16)                        // three ray queues
                + up(N * 16) + up(N * 8)              // hits
                + 3 * up(N * 16) + 3 * up(N * 4);     // thr, radc, conn, csn, flags, nconn
     if (lights)
         b += up(N * 16) + 3 * up(N * 4) + up(N * (size_t)lvCap * 24) + up(N * (size_t)lc * 12) + up(N * (size_t)lc) +
-             3 * up(N * (size_t)lc * 16);
-    if (nee) b += up(N * 16) + up(N) + 2 * up(N * 16);   // neeC, nvis, environment-ray queue
+             3 * up(Q * (size_t)lc * 16);
+    if (nee) b += up(N * 16) + up(N) + 2 * up(Q * 16);   // neeC, nvis, environment-ray queue
     if (lights && splat) b += up(N * 16);                 // lbeta
     if (sortRays) b += up(N * 4) + up(OMV_SORT_BUCKETS * 4);   // keys, bucket counts / offsets
     return b;
 }
 
-// lightCnt, camCnt, shCnt, envCnt, splCnt (see launchWavefront)
-int wavefrontCounterCount(int B, int lvCap) { return 2 * (lvCap + 2) + 3 * (B + 2); }
+// lightCnt, camCnt, shCnt, envCnt, splCnt, OMV_QSEG counters each (see launchWavefront)
+int wavefrontCounterCount(int B, int lvCap) { return OMV_QSEG * (2 * (lvCap + 2) + 3 * (B + 2)); }
 
 void wavefrontBlocksPerCU(int out[8]) {
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
@@ -998,9 +1067,12 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
     const bool splat = P.x.splat;
     const uint32_t n = L.nslots;
     const size_t N = n;
+    // queues: OMV_QSEG append regions of segCap entries
+    const uint32_t segCap = wavefrontSegCap(n, L.blocksShade);
+    const size_t Q = (size_t)OMV_QSEG * segCap;
     uint8_t* cur = static_cast<uint8_t*>(L.arena);
     auto take = [&](size_t bytes) { uint8_t* p = cur; cur += up(bytes); return p; };
-    auto queue = [&]() { WfQueue q; q.o = reinterpret_cast<float4*>(take(N * 16)); q.d = reinterpret_cast<float4*>(take(N * 16)); return q; };
+    auto queue = [&]() { WfQueue q; q.o = reinterpret_cast<float4*>(take(Q * 16)); q.d = reinterpret_cast<float4*>(take(Q * 16)); return q; };
     WfQueue qa = queue(), qb = queue(), qc = queue();
     WfHits H;
     H.h = reinterpret_cast<float4*>(take(N * 16));
@@ -1024,9 +1096,9 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
         W.lv = reinterpret_cast<float*>(take(N * (size_t)L.lvCap * 24));
         W.cc = reinterpret_cast<float*>(take(N * (size_t)L.lc * 12));
         W.vis = reinterpret_cast<uint8_t*>(take(N * (size_t)L.lc));
-        sq.o = reinterpret_cast<float4*>(take(N * (size_t)L.lc * 16));
-        sq.d = reinterpret_cast<float4*>(take(N * (size_t)L.lc * 16));
-        sq.b = reinterpret_cast<float4*>(take(N * (size_t)L.lc * 16));
+        sq.o = reinterpret_cast<float4*>(take(Q * (size_t)L.lc * 16));
+        sq.d = reinterpret_cast<float4*>(take(Q * (size_t)L.lc * 16));
+        sq.b = reinterpret_cast<float4*>(take(Q * (size_t)L.lc * 16));
     }
     if (splat) W.lbeta = reinterpret_cast<float4*>(take(N * 16));
     uint32_t* sortKeys = nullptr;
@@ -1049,16 +1121,29 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
     // splCnt[i] (the connection-ray queue is free during the light rounds)
     const int B = L.p.camera_bounces;
     const int ncnt = wavefrontCounterCount(B, L.lvCap);
-    uint32_t* lightCnt = L.qcount;
-    uint32_t* camCnt = lightCnt + (L.lvCap + 2);
-    uint32_t* shCnt = camCnt + (B + 2);
-    uint32_t* envCnt = shCnt + (B + 2);
-    uint32_t* splCnt = envCnt + (B + 2);
+    const uint32_t K = OMV_QSEG;
+    uint32_t* lightBase = L.qcount;
+    uint32_t* camBase = lightBase + K * (L.lvCap + 2);
+    uint32_t* shBase = camBase + K * (B + 2);
+    uint32_t* envBase = shBase + K * (B + 2);
+    uint32_t* splBase = envBase + K * (B + 2);
+    // Segmented counters of round r's queues.  The round-1 queues (written by
+    // gen at the slot's index) and camera queues that are reordered by the
+    // ray sort are compact (one region).
+    const uint32_t camK = L.sortRays ? 1u : K, camCap = L.sortRays ? (uint32_t)Q : segCap;
+    auto lightCnt = [&](int r) { return r == 1 ? segOf(lightBase + K, 1u, (uint32_t)Q) : segOf(lightBase + K * r, K, segCap); };
+    auto camCnt = [&](int r) { return r == 1 ? segOf(camBase + K, 1u, (uint32_t)Q) : segOf(camBase + K * r, camK, camCap); };
+    auto shCnt = [&](int r) { return segOf(shBase + K * r, K, segCap * (uint32_t)L.lc); };
+    auto envCnt = [&](int r) { return segOf(envBase + K * r, K, segCap); };
+    auto splCnt = [&](int r) { return segOf(splBase + K * r, K, segCap); };
+    WfSeg none;
+    std::memset(&none, 0, sizeof(none));
+    unsigned long long* lost = L.counters + 6;   // the stats' overflow counter
     const size_t lds = (size_t)OMV_STACK_LDS * OMV_BLOCK * sizeof(uint32_t);
     const dim3 blk(OMV_BLOCK);
     const uint32_t xcds = L.xcdBands > 1 ? (uint32_t)L.xcdBands : 1u;
     // refilling traversal (trace_keep > 0): one argument block per launch
-    auto ptArgs = [&](const WfQueue& q, const uint32_t* qc, const uint32_t* sqc, const WfQueue& e, const uint32_t* ec) {
+    auto ptArgs = [&](const WfQueue& q, const WfSeg& qc, const WfSeg& sqc, const WfQueue& e, const WfSeg& ec) {
         WfTraceArgs a;
         a.q = q; a.qCount = qc; a.sq = sq; a.sqCount = sqc; a.eq = e; a.eqCount = ec; a.H = H;
         a.vis = W.vis; a.nvis = W.nvis; a.nslots = n; a.lc = L.lc;
@@ -1081,10 +1166,10 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
     const int writeQueues = Gen1.gen ? 0 : 1;
     if (lights)
         hipLaunchKernelGGL(omv_wf_gen<true>, dim3(L.blocksGen), blk, 0, stream, S, L.cam, P, L.p.samples_per_pixel, L.sample0, W, qa, qb,
-                           camCnt + 1, lightCnt + 1, L.rad, writeQueues);
+                           camCnt(1).cnt, lightCnt(1).cnt, L.rad, writeQueues);
     else
         hipLaunchKernelGGL(omv_wf_gen<false>, dim3(L.blocksGen), blk, 0, stream, S, L.cam, P, L.p.samples_per_pixel, L.sample0, W, qa, qb,
-                           camCnt + 1, lightCnt + 1, L.rad, writeQueues);
+                           camCnt(1).cnt, lightCnt(1).cnt, L.rad, writeQueues);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (lights) {
@@ -1093,25 +1178,25 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
             mark();
             if (L.traceKeep > 0)
                 hipLaunchKernelGGL(omv_wf_trace_pt<1>, dim3(L.blocksTrace), blk, lds, stream, S,
-                                   ptArgs(in, lightCnt + i, nullptr, eq, nullptr));
+                                   ptArgs(in, lightCnt(i), none, eq, none));
             else
-                hipLaunchKernelGGL(omv_wf_trace<1>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, lightCnt + i, H, W.vis, n, L.lc, L.gstack,
+                hipLaunchKernelGGL(omv_wf_trace<1>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, lightCnt(i), H, W.vis, n, L.lc, L.gstack,
                                    L.counters, L.splat, xcds, Gen0);
             mark();
             if (splat) {
-                hipLaunchKernelGGL(omv_wf_shade_light<true>, dim3(L.blocksShade), blk, 0, stream, S, L.cam, P, L.sample0, i, W, in, lightCnt + i,
-                                   H, out, lightCnt + i + 1, sq, splCnt + i);
+                hipLaunchKernelGGL(omv_wf_shade_light<true>, dim3(L.blocksShade), blk, 0, stream, S, L.cam, P, L.sample0, i, W, in, lightCnt(i),
+                                   H, out, lightCnt(i + 1), sq, splCnt(i), lost);
                 mark();
                 if (L.traceKeep > 0)
                     hipLaunchKernelGGL(omv_wf_trace_pt<4>, dim3(L.blocksTrace), blk, lds, stream, S,
-                                       ptArgs(in, nullptr, splCnt + i, eq, nullptr));
+                                       ptArgs(in, none, splCnt(i), eq, none));
                 else
-                    hipLaunchKernelGGL(omv_wf_trace<4>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, splCnt + i, H, W.vis, n, L.lc, L.gstack,
+                    hipLaunchKernelGGL(omv_wf_trace<4>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, splCnt(i), H, W.vis, n, L.lc, L.gstack,
                                        L.counters, L.splat, xcds, Gen0);
                 mark();
             } else {
-                hipLaunchKernelGGL(omv_wf_shade_light<false>, dim3(L.blocksShade), blk, 0, stream, S, L.cam, P, L.sample0, i, W, in, lightCnt + i,
-                                   H, out, lightCnt + i + 1, sq, splCnt + i);
+                hipLaunchKernelGGL(omv_wf_shade_light<false>, dim3(L.blocksShade), blk, 0, stream, S, L.cam, P, L.sample0, i, W, in, lightCnt(i),
+                                   H, out, lightCnt(i + 1), sq, splCnt(i), lost);
             }
             std::swap(in, out);
         }
@@ -1134,17 +1219,17 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
             mark();
             if (lights) {
                 if (nee)
-                    hipLaunchKernelGGL((omv_wf_tail<true, true>), dim3(L.blocksTrace), blk, lds, stream, S, P, L.sample0, i, W, in, camCnt + i,
+                    hipLaunchKernelGGL((omv_wf_tail<true, true>), dim3(L.blocksTrace), blk, lds, stream, S, P, L.sample0, i, W, in, camCnt(i),
                                        L.rad, L.gstack, L.counters);
                 else
-                    hipLaunchKernelGGL((omv_wf_tail<true, false>), dim3(L.blocksTrace), blk, lds, stream, S, P, L.sample0, i, W, in, camCnt + i,
+                    hipLaunchKernelGGL((omv_wf_tail<true, false>), dim3(L.blocksTrace), blk, lds, stream, S, P, L.sample0, i, W, in, camCnt(i),
                                        L.rad, L.gstack, L.counters);
             } else {
                 if (nee)
-                    hipLaunchKernelGGL((omv_wf_tail<false, true>), dim3(L.blocksTrace), blk, lds, stream, S, P, L.sample0, i, W, in, camCnt + i,
+                    hipLaunchKernelGGL((omv_wf_tail<false, true>), dim3(L.blocksTrace), blk, lds, stream, S, P, L.sample0, i, W, in, camCnt(i),
                                        L.rad, L.gstack, L.counters);
                 else
-                    hipLaunchKernelGGL((omv_wf_tail<false, false>), dim3(L.blocksTrace), blk, lds, stream, S, P, L.sample0, i, W, in, camCnt + i,
+                    hipLaunchKernelGGL((omv_wf_tail<false, false>), dim3(L.blocksTrace), blk, lds, stream, S, P, L.sample0, i, W, in, camCnt(i),
                                        L.rad, L.gstack, L.counters);
             }
             mark();
@@ -1154,36 +1239,36 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
             mark();
             if (L.traceKeep > 0)
                 hipLaunchKernelGGL(omv_wf_trace_pt<0>, dim3(L.blocksTrace), blk, lds, stream, S,
-                                   ptArgs(in, camCnt + i, nullptr, eq, nullptr));
+                                   ptArgs(in, camCnt(i), none, eq, none));
             else
-                hipLaunchKernelGGL(omv_wf_trace<0>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, camCnt + i, H, W.vis, n, L.lc,
+                hipLaunchKernelGGL(omv_wf_trace<0>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, camCnt(i), H, W.vis, n, L.lc,
                                    L.gstack, L.counters, L.splat, xcds, i == 1 ? Gen1 : Gen0);
             mark();
         }
         traced = false;
         if (lights) {
             if (nee)
-                hipLaunchKernelGGL((omv_wf_shade_cam<true, true>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i, H,
-                                   out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad, Gen1);
+                hipLaunchKernelGGL((omv_wf_shade_cam<true, true>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt(i), H,
+                                   out, camCnt(i + 1), sq, shCnt(i), eq, envCnt(i), L.rad, Gen1, lost);
             else
-                hipLaunchKernelGGL((omv_wf_shade_cam<true, false>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i,
-                                   H, out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad, Gen1);
+                hipLaunchKernelGGL((omv_wf_shade_cam<true, false>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt(i),
+                                   H, out, camCnt(i + 1), sq, shCnt(i), eq, envCnt(i), L.rad, Gen1, lost);
         } else {
             if (nee)
-                hipLaunchKernelGGL((omv_wf_shade_cam<false, true>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i,
-                                   H, out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad, Gen1);
+                hipLaunchKernelGGL((omv_wf_shade_cam<false, true>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt(i),
+                                   H, out, camCnt(i + 1), sq, shCnt(i), eq, envCnt(i), L.rad, Gen1, lost);
             else
-                hipLaunchKernelGGL((omv_wf_shade_cam<false, false>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt + i,
-                                   H, out, camCnt + i + 1, sq, shCnt + i, eq, envCnt + i, L.rad, Gen1);
+                hipLaunchKernelGGL((omv_wf_shade_cam<false, false>), dim3(L.blocksShade), blk, 0, stream, S, P, L.sample0, i, W, in, camCnt(i),
+                                   H, out, camCnt(i + 1), sq, shCnt(i), eq, envCnt(i), L.rad, Gen1, lost);
         }
         bool sorted = false;
         const bool nextInWavefront = i < B && i + 1 != i0;   // round i + 1 is traced by this schedule's round kernels
         if (L.sortRays && nextInWavefront) {
             // reorder the next round's rays (secondary rays) into the spare queue
             hipLaunchKernelGGL(omv_wf_fill_u32, dim3(OMV_SORT_BUCKETS / 256), dim3(256), 0, stream, sortHist, OMV_SORT_BUCKETS);
-            hipLaunchKernelGGL(omv_wf_sort_keys, dim3(L.blocksShade), blk, 0, stream, S, out, camCnt + i + 1, sortKeys, sortHist);
+            hipLaunchKernelGGL(omv_wf_sort_keys, dim3(L.blocksShade), blk, 0, stream, S, out, camCnt(i + 1).cnt, sortKeys, sortHist);
             hipLaunchKernelGGL(omv_wf_sort_scan, dim3(1), dim3(1024), 0, stream, sortHist);
-            hipLaunchKernelGGL(omv_wf_sort_scatter, dim3(L.blocksShade), blk, 0, stream, out, camCnt + i + 1, sortKeys, sortHist, spare);
+            hipLaunchKernelGGL(omv_wf_sort_scatter, dim3(L.blocksShade), blk, 0, stream, out, camCnt(i + 1).cnt, sortKeys, sortHist, spare);
             sorted = true;
         }
         const WfQueue next = sorted ? spare : out;
@@ -1192,11 +1277,11 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
             mark();
             if (L.traceKeep > 0)
                 hipLaunchKernelGGL(omv_wf_trace_pt<5>, dim3(L.blocksTrace), blk, lds, stream, S,
-                                   ptArgs(next, nextInWavefront ? camCnt + i + 1 : nullptr, lights ? shCnt + i : nullptr, eq,
-                                          nee ? envCnt + i : nullptr));
+                                   ptArgs(next, nextInWavefront ? camCnt(i + 1) : none, lights ? shCnt(i) : none, eq,
+                                          nee ? envCnt(i) : none));
             else
-                hipLaunchKernelGGL(omv_wf_trace_fused, dim3(L.blocksTrace), blk, lds, stream, S, next, nextInWavefront ? camCnt + i + 1 : nullptr,
-                                   sq, lights ? shCnt + i : nullptr, eq, nee ? envCnt + i : nullptr, H, W.vis, W.nvis, n, L.lc, L.gstack,
+                hipLaunchKernelGGL(omv_wf_trace_fused, dim3(L.blocksTrace), blk, lds, stream, S, next, nextInWavefront ? camCnt(i + 1) : none,
+                                   sq, lights ? shCnt(i) : none, eq, nee ? envCnt(i) : none, H, W.vis, W.nvis, n, L.lc, L.gstack,
                                    L.counters, xcds);
             mark();
             traced = nextInWavefront;
@@ -1205,9 +1290,9 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
                 mark();
                 if (L.traceKeep > 0)
                     hipLaunchKernelGGL(omv_wf_trace_pt<2>, dim3(L.blocksTrace), blk, lds, stream, S,
-                                       ptArgs(in, nullptr, shCnt + i, eq, nullptr));
+                                       ptArgs(in, none, shCnt(i), eq, none));
                 else
-                    hipLaunchKernelGGL(omv_wf_trace<2>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, shCnt + i, H, W.vis, n, L.lc,
+                    hipLaunchKernelGGL(omv_wf_trace<2>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, shCnt(i), H, W.vis, n, L.lc,
                                        L.gstack, L.counters, L.splat, xcds, Gen0);
                 mark();
             }
@@ -1215,15 +1300,15 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
                 mark();
                 if (L.traceKeep > 0)
                     hipLaunchKernelGGL(omv_wf_trace_pt<3>, dim3(L.blocksTrace), blk, lds, stream, S,
-                                       ptArgs(in, nullptr, nullptr, eq, envCnt + i));
+                                       ptArgs(in, none, none, eq, envCnt(i)));
                 else
-                    hipLaunchKernelGGL(omv_wf_trace<3>, dim3(L.blocksTrace), blk, lds, stream, S, eq, sq, envCnt + i, H, W.nvis, n, L.lc,
+                    hipLaunchKernelGGL(omv_wf_trace<3>, dim3(L.blocksTrace), blk, lds, stream, S, eq, sq, envCnt(i), H, W.nvis, n, L.lc,
                                        L.gstack, L.counters, L.splat, xcds, Gen0);
                 mark();
             }
         }
         if (lights || nee)
-            hipLaunchKernelGGL(omv_wf_connect, dim3(L.blocksShade), blk, 0, stream, W, in, camCnt + i, L.rad, lights ? 1 : 0, nee ? 1 : 0, i,
+            hipLaunchKernelGGL(omv_wf_connect, dim3(L.blocksShade), blk, 0, stream, W, in, camCnt(i), L.rad, lights ? 1 : 0, nee ? 1 : 0, i,
                                Gen1.gen);
         if (sorted) {
             const WfQueue freed = in;
