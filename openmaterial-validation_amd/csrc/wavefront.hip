@@ -98,6 +98,13 @@ DEV uint32_t waveAppend(uint32_t* counter, bool want) {
 
 DEV void stackInit(Stack& st, uint32_t* gstack, uint32_t gtid, uint32_t gthreads) { st.init(gstack, gtid, gthreads); }
 
+// The traversal launches add their ray and visit counts into one of
+// OMV_STAT_SLOTS slots (128 bytes apart) by workgroup, for the same reason
+// the queues have several append counters; omv_wf_stats_sum folds the slots
+// into the renderer's counters once per chunk.
+#define OMV_STAT_SLOTS 64
+DEV unsigned long long* statSlot(unsigned long long* base) { return base + (blockIdx.x % OMV_STAT_SLOTS) * 16u; }
+
 // A queue's entries in K regions of cap entries: region r holds the physical
 // entries [r*cap, r*cap + cnt[r]).  A producer appends to the region of its
 // workgroup (blockIdx mod K), so the append atomics of a launch are spread
@@ -356,7 +363,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
             H.id[e] = id;
         }
     }
-    flushCounters(counters, rays, nCam, nLight, nShadow, cnt);
+    flushCounters(statSlot(counters), rays, nCam, nLight, nShadow, cnt);
 }
 
 // Fused traversal of one camera round boundary: the camera extension rays of
@@ -419,7 +426,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused
             nvis[__float_as_uint(d.w)] = hit ? 0 : 2;
         }
     }
-    flushCounters(counters, rays, nCam, 0u, nShadow, cnt);
+    flushCounters(statSlot(counters), rays, nCam, 0u, nShadow, cnt);
 }
 
 // Arguments of the refilling traversal: the queues of one launch (kKind
@@ -557,7 +564,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_pt(Om
             if (__popcll(__ballot(busy)) < A.keep) break;
         }
     }
-    flushCounters(A.counters, rays, nCam, nLight, nShadow, cnt);
+    flushCounters(statSlot(A.counters), rays, nCam, nLight, nShadow, cnt);
 }
 
 // ---------------------------------------------------------- shade light
@@ -919,7 +926,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_tail(OmvSceneView S, RP P, u
         rad[(uint64_t)slot * 3 + 1] = r.y;
         rad[(uint64_t)slot * 3 + 2] = r.z;
     }
-    flushCounters(counters, L.rays, L.camRays, 0u, L.shadowRays, L.cnt);
+    flushCounters(statSlot(counters), L.rays, L.camRays, 0u, L.shadowRays, L.cnt);
 }
 
 // ----------------------------------------------------------- ray sorting
@@ -993,6 +1000,16 @@ __global__ void omv_wf_fill_u32(uint32_t* p, int n) {
     if (i < n) p[i] = 0u;
 }
 
+// the OMV_STAT_SLOTS stat slots of a chunk added into the renderer's counters
+__global__ void omv_wf_stats_sum(const unsigned long long* __restrict__ slots, unsigned long long* __restrict__ out) {
+    const int t = threadIdx.x;
+    if (t < 7) {
+        unsigned long long v = 0;
+        for (int k = 0; k < OMV_STAT_SLOTS; ++k) v += slots[k * 16 + t];
+        if (v) atomicAdd(out + t, v);
+    }
+}
+
 // ------------------------------------------------------------ launchers
 namespace omv {
 
@@ -1030,6 +1047,7 @@ size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool
     if (nee) b += up(N * 16) + up(N) + 2 * up(Q * 16);   // neeC, nvis, environment-ray queue
     if (lights && splat) b += up(N * 16);                 // lbeta
     if (sortRays) b += up(N * 4) + up(OMV_SORT_BUCKETS * 4);   // keys, bucket counts / offsets
+    b += up(OMV_STAT_SLOTS * 16 * 8);                     // traversal stat slots
     return b;
 }
 
@@ -1103,6 +1121,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
     if (splat) W.lbeta = reinterpret_cast<float4*>(take(N * 16));
     uint32_t* sortKeys = nullptr;
     uint32_t* sortHist = nullptr;
+    unsigned long long* stats = reinterpret_cast<unsigned long long*>(take(OMV_STAT_SLOTS * 16 * 8));
     if (L.sortRays) {
         sortKeys = reinterpret_cast<uint32_t*>(take(N * 4));
         sortHist = reinterpret_cast<uint32_t*>(take(OMV_SORT_BUCKETS * 4));
@@ -1147,13 +1166,15 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
         WfTraceArgs a;
         a.q = q; a.qCount = qc; a.sq = sq; a.sqCount = sqc; a.eq = e; a.eqCount = ec; a.H = H;
         a.vis = W.vis; a.nvis = W.nvis; a.nslots = n; a.lc = L.lc;
-        a.gstack = L.gstack; a.counters = L.counters; a.splat = L.splat; a.xcds = xcds; a.keep = L.traceKeep;
+        a.gstack = L.gstack; a.counters = stats; a.splat = L.splat; a.xcds = xcds; a.keep = L.traceKeep;
         return a;
     };
     auto mark = [&]() {
         if (L.traceEv && L.traceEvUsed && *L.traceEvUsed < L.traceEvCap) (void)hipEventRecord(L.traceEv[(*L.traceEvUsed)++], stream);
     };
     hipLaunchKernelGGL(omv_wf_fill_u32, dim3((ncnt + 63) / 64), dim3(64), 0, stream, L.qcount, ncnt);
+    hipLaunchKernelGGL(omv_wf_fill_u32, dim3(OMV_STAT_SLOTS * 32 / 64), dim3(64), 0, stream, reinterpret_cast<uint32_t*>(stats),
+                       OMV_STAT_SLOTS * 32);
     // Round-1 camera rays: computed by the round-1 camera kernels from the
     // slot (Gen1), or with the refilling traversal stored by gen in qa;
     // round-1 light rays are stored in qb
@@ -1181,7 +1202,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
                                    ptArgs(in, lightCnt(i), none, eq, none));
             else
                 hipLaunchKernelGGL(omv_wf_trace<1>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, lightCnt(i), H, W.vis, n, L.lc, L.gstack,
-                                   L.counters, L.splat, xcds, Gen0);
+                                   stats, L.splat, xcds, Gen0);
             mark();
             if (splat) {
                 hipLaunchKernelGGL(omv_wf_shade_light<true>, dim3(L.blocksShade), blk, 0, stream, S, L.cam, P, L.sample0, i, W, in, lightCnt(i),
@@ -1192,7 +1213,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
                                        ptArgs(in, none, splCnt(i), eq, none));
                 else
                     hipLaunchKernelGGL(omv_wf_trace<4>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, splCnt(i), H, W.vis, n, L.lc, L.gstack,
-                                       L.counters, L.splat, xcds, Gen0);
+                                       stats, L.splat, xcds, Gen0);
                 mark();
             } else {
                 hipLaunchKernelGGL(omv_wf_shade_light<false>, dim3(L.blocksShade), blk, 0, stream, S, L.cam, P, L.sample0, i, W, in, lightCnt(i),
@@ -1220,17 +1241,17 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
             if (lights) {
                 if (nee)
                     hipLaunchKernelGGL((omv_wf_tail<true, true>), dim3(L.blocksTrace), blk, lds, stream, S, P, L.sample0, i, W, in, camCnt(i),
-                                       L.rad, L.gstack, L.counters);
+                                       L.rad, L.gstack, stats);
                 else
                     hipLaunchKernelGGL((omv_wf_tail<true, false>), dim3(L.blocksTrace), blk, lds, stream, S, P, L.sample0, i, W, in, camCnt(i),
-                                       L.rad, L.gstack, L.counters);
+                                       L.rad, L.gstack, stats);
             } else {
                 if (nee)
                     hipLaunchKernelGGL((omv_wf_tail<false, true>), dim3(L.blocksTrace), blk, lds, stream, S, P, L.sample0, i, W, in, camCnt(i),
-                                       L.rad, L.gstack, L.counters);
+                                       L.rad, L.gstack, stats);
                 else
                     hipLaunchKernelGGL((omv_wf_tail<false, false>), dim3(L.blocksTrace), blk, lds, stream, S, P, L.sample0, i, W, in, camCnt(i),
-                                       L.rad, L.gstack, L.counters);
+                                       L.rad, L.gstack, stats);
             }
             mark();
             break;
@@ -1242,7 +1263,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
                                    ptArgs(in, camCnt(i), none, eq, none));
             else
                 hipLaunchKernelGGL(omv_wf_trace<0>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, camCnt(i), H, W.vis, n, L.lc,
-                                   L.gstack, L.counters, L.splat, xcds, i == 1 ? Gen1 : Gen0);
+                                   L.gstack, stats, L.splat, xcds, i == 1 ? Gen1 : Gen0);
             mark();
         }
         traced = false;
@@ -1282,7 +1303,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
             else
                 hipLaunchKernelGGL(omv_wf_trace_fused, dim3(L.blocksTrace), blk, lds, stream, S, next, nextInWavefront ? camCnt(i + 1) : none,
                                    sq, lights ? shCnt(i) : none, eq, nee ? envCnt(i) : none, H, W.vis, W.nvis, n, L.lc, L.gstack,
-                                   L.counters, xcds);
+                                   stats, xcds);
             mark();
             traced = nextInWavefront;
         } else {
@@ -1293,7 +1314,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
                                        ptArgs(in, none, shCnt(i), eq, none));
                 else
                     hipLaunchKernelGGL(omv_wf_trace<2>, dim3(L.blocksTrace), blk, lds, stream, S, in, sq, shCnt(i), H, W.vis, n, L.lc,
-                                       L.gstack, L.counters, L.splat, xcds, Gen0);
+                                       L.gstack, stats, L.splat, xcds, Gen0);
                 mark();
             }
             if (nee) {
@@ -1303,7 +1324,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
                                        ptArgs(in, none, none, eq, envCnt(i)));
                 else
                     hipLaunchKernelGGL(omv_wf_trace<3>, dim3(L.blocksTrace), blk, lds, stream, S, eq, sq, envCnt(i), H, W.nvis, n, L.lc,
-                                       L.gstack, L.counters, L.splat, xcds, Gen0);
+                                       L.gstack, stats, L.splat, xcds, Gen0);
                 mark();
             }
         }
@@ -1322,6 +1343,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
     }
     // fewer rounds, or the rounds from roundEvAt on ran in the tail launch
     if (L.roundEv && !roundEvDone) (void)hipEventRecord(L.roundEv, stream);
+    hipLaunchKernelGGL(omv_wf_stats_sum, dim3(1), dim3(64), 0, stream, stats, L.counters);
     return hipGetLastError();
 }
 

@@ -290,6 +290,7 @@ size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool
     if (nee) b += upb(N * 16) + upb(N) + 2 * upb(Q * 16);
     if (lights && splat) b += upb(N * 16);
     if (sortRays) b += upb(N * 4) + upb(4096 * 4);
+    b += upb(64 * 16 * 8);   // traversal stat slots
     return b;
 }
 int wavefrontCounterCount(int B, int lvCap) { return OMV_QSEG * (2 * (lvCap + 2) + 3 * (B + 2)); }
