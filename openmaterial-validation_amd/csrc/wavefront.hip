@@ -121,25 +121,37 @@ struct WfSeg {
     uint32_t K, cap;
 };
 inline WfSeg segOf(uint32_t* cnt, uint32_t K, uint32_t cap) { WfSeg g; g.cnt = cnt; g.K = K; g.cap = cap; return g; }
-DEV uint32_t segTotal(const WfSeg& g) {
-    uint32_t t = 0;
-    if (g.cnt)
-        for (uint32_t r = 0; r < g.K; ++r) t += min(g.cnt[r], g.cap);
-    return t;
-}
-// logical -> physical index for a non-decreasing sequence of logical indices
-struct SegCursor {
-    const uint32_t* cnt;
-    uint32_t K, cap, r, lo, hi;
-    DEV void init(const WfSeg& g) {
-        cnt = g.cnt; K = g.K; cap = g.cap; r = 0; lo = 0;
-        hi = g.cnt ? min(g.cnt[0], cap) : 0u;
-    }
-    DEV uint32_t phys(uint32_t L) {
-        while (L >= hi && r + 1 < K) { ++r; lo = hi; hi += min(cnt[r], cap); }
-        return r * cap + (L - lo);
+// Region prefix of a segmented queue, per workgroup in LDS: pre[r] = the
+// entries in the regions before r.  The first wave builds it (lane r loads
+// region r's count, then a wave scan), so the logical -> physical map is a
+// 6-step search in LDS rather than a dependent walk over the counters.
+struct SegTable {
+    uint32_t pre[OMV_QSEG + 1];
+    uint32_t K, cap;
+    DEV uint32_t total() const { return pre[K]; }
+    DEV uint32_t phys(uint32_t L) const {
+        uint32_t r = 0;
+#pragma unroll
+        for (uint32_t st = OMV_QSEG / 2; st > 0; st >>= 1)
+            if (r + st < K && pre[r + st] <= L) r += st;
+        return r * cap + (L - pre[r]);
     }
 };
+static_assert(OMV_QSEG <= 64, "one wave builds a queue's region prefix");
+// All threads of the workgroup call it: it synchronises the workgroup.
+DEV void segLoad(SegTable& t, const WfSeg& g) {
+    if (threadIdx.x < 64) {
+        const uint32_t lane = threadIdx.x;
+        uint32_t v = (g.cnt && lane < g.K) ? min(g.cnt[lane], g.cap) : 0u;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(v, d, 64);
+            if ((int)lane >= d) v += y;
+        }
+        t.pre[lane + 1] = v;
+        if (lane == 0) { t.pre[0] = 0u; t.K = g.cnt ? g.K : 1u; t.cap = g.cap; }
+    }
+    __syncthreads();
+}
 // Physical slot of the calling lane's entry, or OMV_SEG_NONE when it does not
 // append (or its region is full); all active lanes of the wave make the call.
 #define OMV_SEG_NONE 0xFFFFFFFFu
@@ -312,9 +324,9 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
     Counters cnt; cnt.nodes = 0; cnt.tris = 0; cnt.overflow = 0;
     uint32_t rays = 0, nCam = 0, nLight = 0, nShadow = 0;
     XcdWalk w;
-    SegCursor cur;
-    cur.init(count);
-    for (w.start(segTotal(count), xcds); w.more(); w.next()) {
+    __shared__ SegTable cur;
+    segLoad(cur, count);
+    for (w.start(cur.total(), xcds); w.more(); w.next()) {
         const uint32_t e = w.e;                                       // logical entry (hit records)
         const uint32_t p = kKind == 0 && G.gen ? e : cur.phys(e);   // its queue slot
         Hit h;
@@ -383,9 +395,9 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused
     stackInit(st, gstack, gtid, gthreads);
     Counters cnt; cnt.nodes = 0; cnt.tris = 0; cnt.overflow = 0;
     uint32_t rays = 0, nCam = 0, nShadow = 0;
-    const uint32_t nA = segTotal(qCount), nB = segTotal(sqCount), nC = segTotal(eqCount);
-    SegCursor cA, cB, cC;
-    cA.init(qCount); cB.init(sqCount); cC.init(eqCount);
+    __shared__ SegTable cA, cB, cC;
+    segLoad(cA, qCount); segLoad(cB, sqCount); segLoad(cC, eqCount);
+    const uint32_t nA = cA.total(), nB = cB.total(), nC = cC.total();
     XcdWalk w;
     for (w.start(nA + nB + nC, xcds); w.more(); w.next()) {
         const uint32_t e = w.e;
@@ -462,9 +474,9 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_pt(Om
     uint32_t rays = 0, nCam = 0, nLight = 0, nShadow = 0;
     // entries [0, nA) camera / light rays of q, [nA, nA + nB) rays of sq,
     // [nA + nB, total) environment rays of eq (by kKind)
-    const uint32_t nA = segTotal(A.qCount), nB = segTotal(A.sqCount), nC = segTotal(A.eqCount);
-    SegCursor cA, cB, cC;
-    cA.init(A.qCount); cB.init(A.sqCount); cC.init(A.eqCount);
+    __shared__ SegTable cA, cB, cC;
+    segLoad(cA, A.qCount); segLoad(cB, A.sqCount); segLoad(cC, A.eqCount);
+    const uint32_t nA = cA.total(), nB = cB.total(), nC = cC.total();
     XcdWalk w;
     w.start(nA + nB + nC, A.xcds);
     Trav T;
@@ -577,10 +589,10 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_light(OmvSceneView S, 
                                                                 WfSeg sqCount, unsigned long long* __restrict__ lost) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t total = segTotal(inCount);
+    __shared__ SegTable cur;
+    segLoad(cur, inCount);
+    const uint32_t total = cur.total();
     const uint64_t N = W.n;
-    SegCursor cur;
-    cur.init(inCount);
     for (uint32_t e = gtid; e < total; e += gthreads) {
         const float4 dd = in.d[cur.phys(e)];
         const uint32_t slot = __float_as_uint(dd.w);
@@ -664,12 +676,12 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_SHADE_WAVES(kLights, kNee)) void omv
                                                               unsigned long long* __restrict__ lost) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t total = segTotal(inCount);
+    __shared__ SegTable cur;
+    segLoad(cur, inCount);
+    const uint32_t total = cur.total();
     const uint64_t N = W.n;
     const int q = P.q;
     const bool gen = G.gen && i == 1;
-    SegCursor cur;
-    cur.init(inCount);
     for (uint32_t e = gtid; e < total; e += gthreads) {
         uint32_t slot;
         f3 pDir;
@@ -839,11 +851,11 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_connect(WfState W, WfQueue i
                                                             int nee, int round, int gen) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t total = segTotal(inCount);
+    __shared__ SegTable cur;
+    segLoad(cur, inCount);
+    const uint32_t total = cur.total();
     const uint64_t N = W.n;
     const bool slotIsEntry = gen && round == 1;   // round-1 rays computed from the slot: entry e is slot e
-    SegCursor cur;
-    cur.init(inCount);
     for (uint32_t e = gtid; e < total; e += gthreads) {
         const uint32_t slot = slotIsEntry ? e : __float_as_uint(in.d[cur.phys(e)].w);
         const uint32_t fl = W.flags[slot];
@@ -897,10 +909,10 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_tail(OmvSceneView S, RP P, u
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Lane L;
     laneInit(L, gstack, nullptr, gtid, gthreads);
-    const uint32_t total = segTotal(inCount);
+    __shared__ SegTable cur;
+    segLoad(cur, inCount);
+    const uint32_t total = cur.total();
     const uint64_t N = W.n;
-    SegCursor cur;
-    cur.init(inCount);
     for (uint32_t e = gtid; e < total; e += gthreads) {
         const uint32_t p = cur.phys(e);
         const float4 o = in.o[p], d = in.d[p];
