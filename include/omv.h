@@ -350,10 +350,10 @@ typedef struct omv_render_tuning {
                                  (0 = occupancy limit)                          */
     int chunk_samples;        /* samples per wavefront chunk, rounded down to
                                  whole pixels (0 = default: 2^25)               */
-    int traversal;            /* 0 = by scene (wave-cooperative up to 4
-                                 instances), 1 = entry by entry, 2 = wave-
-                                 cooperative: node entries until every lane
-                                 of the wave holds a leaf, then the leaves    */
+    int traversal;            /* 0 = default (entry by entry), 1 = entry by
+                                 entry, 2 = wave-cooperative: node entries
+                                 until every lane of the wave holds a leaf,
+                                 then the leaves                              */
     int sort_rays;            /* secondary camera rays reordered by
                                  (direction octant, origin cell of a 16^3
                                  grid over the scene) before each traversal:
@@ -383,8 +383,8 @@ typedef struct omv_render_tuning {
                                  count): the rays an XCD traces at a time come
                                  from a few stretches of the queue; 1 = plain
                                  grid stride; 0 = default: two bands per XCD
-                                 on scenes with the wave-cooperative
-                                 traversal, else the plain stride           */
+                                 on scenes of at most 4 instances, else the
+                                 plain stride                               */
     int tail_round;           /* camera rounds from this one to the last run as
                                  one launch, each path carried to its end by
                                  one lane (same film): 0 = default, 1 = never

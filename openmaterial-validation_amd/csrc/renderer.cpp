@@ -403,11 +403,11 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     L.traceKeep = tuning_.trace_keep;
     L.splat = splat_;
     L.sortRays = tuning_.sort_rays == 2;
-    // default hand-out (measured on MI355X): two bands per XCD on scenes with
-    // the wave-cooperative traversal (single-BLAS scenes: torus knot 512^2 x 64
-    // traversal 10.19 -> 9.32 ms, icosphere 256^2 x 64 1.49 -> 1.35 ms), the
-    // plain stride on many-instance scenes (within 1 % either way there)
-    L.xcdBands = tuning_.xcd_bands > 0 ? tuning_.xcd_bands : (S.trav_wave ? 2 * xcds_ : 1);
+    // default hand-out (measured on MI355X, entry-by-entry traversal): two
+    // bands per XCD on scenes of at most 4 instances (torus knot 512^2 x 64
+    // kernels 9.50 -> 8.95 ms, icosphere 256^2 x 64 1.64 -> 1.53 ms), the plain
+    // stride on many-instance scenes (instanced 1080p 15.85 vs 16.03 ms)
+    L.xcdBands = tuning_.xcd_bands > 0 ? tuning_.xcd_bands : (S.num_insts <= 4 ? 2 * xcds_ : 1);
     L.tailRound = tuning_.tail_round == 1 ? 0 : (tuning_.tail_round >= 2 ? tuning_.tail_round : kTailRoundDefault);
     // an event pair around every traversal launch: (2 light + 3 camera rounds) per chunk
     const uint64_t chunks = (npix + chunkPix - 1) / chunkPix;

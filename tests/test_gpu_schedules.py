@@ -29,7 +29,7 @@ def _both(pkg, desc, view, **params):
     return out
 
 
-def _same(out):
+def _same(out, visit_tol=0.1):
     (a, sa), (b, sb) = out
     assert np.all(np.isfinite(a))
     assert np.array_equal(a, b), f"films differ at {np.argwhere(a != b)[:5].tolist()}"
@@ -38,7 +38,7 @@ def _same(out):
     # node / triangle counts depend on which rays share a wave (the traversal
     # postpones leaves wave-cooperatively), the closest hits do not
     for k in ("nodes_visited", "tris_tested"):
-        assert abs(sa[k] - sb[k]) <= 0.1 * sb[k], (k, sa[k], sb[k])
+        assert abs(sa[k] - sb[k]) <= visit_tol * sb[k], (k, sa[k], sb[k])
     assert sa["stack_overflows"] == 0
 
 
