@@ -350,10 +350,10 @@ typedef struct omv_render_tuning {
                                  (0 = occupancy limit)                          */
     int chunk_samples;        /* samples per wavefront chunk, rounded down to
                                  whole pixels (0 = default: 2^25)               */
-    int traversal;            /* 0 = default (entry by entry), 1 = entry by
-                                 entry, 2 = wave-cooperative: node entries
-                                 until every lane of the wave holds a leaf,
-                                 then the leaves                              */
+    int traversal;            /* 0 = default, 1 = entry by entry: the same
+                                 order (the wave-cooperative order, 2, was
+                                 removed: slower on every scene measured
+                                 once node visits were branch-free)         */
     int sort_rays;            /* secondary camera rays reordered by
                                  (direction octant, origin cell of a 16^3
                                  grid over the scene) before each traversal:
@@ -435,7 +435,7 @@ int omv_encode_image(const char* ext, const float* rgba, int width, int height, 
  * refused (OMV_E_STATE). */
 #define OMV_NOALLOC_SCRATCH_BYTES      (64u << 20)
 #define OMV_NOALLOC_MIN_BLOCKS         256   /* blocks of 128 lanes the scratch must hold */
-#define OMV_NOALLOC_MAX_LIGHT_VERTICES 69
+#define OMV_NOALLOC_MAX_LIGHT_VERTICES 68
 int omv_renderer_render_noalloc(omv_renderer* r, const omv_camera* c, float* film_dev, void* stream, omv_render_stats* stats);
 
 /* ---------------------------------------------------------- adjoint */

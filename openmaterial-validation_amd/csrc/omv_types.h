@@ -18,7 +18,7 @@
 
 #define OMV_MAX_UV_CHANNELS 8
 #define OMV_BVH_LEAF_MAX 2   // triangles per BVH4 leaf: 2 traced fastest of 1-4 (100k- and 2M-triangle scenes)
-#define OMV_STACK_LDS 32      // per-lane traversal stack entries kept in LDS
+#define OMV_STACK_LDS 28      // per-lane traversal stack entries kept in LDS
 #define OMV_STACK_TOTAL 128   // LDS + global overflow; the builder proves the bound
 #define OMV_EMPTY_CHILD ((int32_t)0x80000000)
 
@@ -118,8 +118,6 @@ struct OmvSceneView {
     int32_t any_masked;   // some PBR material uses alpha MASK (re-traced cut-outs): no occlusion early-out
     int32_t closest_only; // 1: visibility rays take the closest hit too (the occlusion early-out off)
     float   scene_lo[3], scene_inv[3];   // world box of the instances, 1 / its extent (0 if flat)
-    int32_t trav_wave;    // traversal order: 1 wave-cooperative (leaves postponed until every
-                          // lane of the wave holds one), 0 entry by entry; the hits do not depend on it
     int32_t bg_type, bg_w, bg_h;
     float   bg_const[3];
     float   bg_rot[9];

@@ -980,11 +980,6 @@ struct Trav {
         return st.pop();
     }
     DEV static bool isLeaf(uint32_t e) { return e != E_END && (e >> E_KIND_SHIFT) == E_BLAS_LEAF; }
-    // kind of the next entry without consuming it (more() must hold)
-    DEV bool nextIsBlasNode(const Stack& st) const {
-        const uint32_t e = hasNext ? nxt : st.top();
-        return e != E_END && (e >> E_KIND_SHIFT) == E_BLAS_NODE;
-    }
     // triangles of a BLAS leaf entry (object ray of the current instance)
     DEV void leaf(const OmvSceneView& S, uint32_t e, Stack& st, Counters& c) {
         const uint32_t idx = e & 0x3FFFFFFFu;
@@ -1038,30 +1033,6 @@ struct Trav {
         if (isLeaf(e)) leaf(S, e, st, c);
         else visit(S, e, st, c);
     }
-    // The whole traversal, wave-cooperative (Aila & Laine 2009, speculative
-    // while-while): node entries are taken until every active lane of the
-    // wave holds a leaf or has nothing left -- a lane that already holds one
-    // goes on speculatively through BLAS nodes of the same instance -- and
-    // then the lanes test their leaves together, so node and triangle work do
-    // not alternate under divergence.  Postponing a leaf only loosens culling
-    // for a while; the closest hit does not depend on the order.
-    DEV void run(const OmvSceneView& S, Stack& st, Counters& c) {
-        bool hasPend = false;
-        uint32_t pend = 0u;
-        while (more(st) || hasPend) {
-            for (;;) {
-                bool go = !hasPend && more(st);
-                if (!go && hasPend && more(st)) go = nextIsBlasNode(st);
-                if (go) {
-                    const uint32_t e = take(st);
-                    if (isLeaf(e)) { pend = e; hasPend = true; }
-                    else visit(S, e, st, c);
-                }
-                if (__ballot(!hasPend && more(st)) == 0ull) break;
-            }
-            if (hasPend) { leaf(S, pend, st, c); hasPend = false; }
-        }
-    }
 };
 
 DEV bool traceClosest(const OmvSceneView& S, const f3& o, const f3& d, float tnear, float tfar, Hit& h, Stack& st, Counters& c,
@@ -1070,8 +1041,7 @@ DEV bool traceClosest(const OmvSceneView& S, const f3& o, const f3& d, float tne
     const bool ok = T.start(S, o, d, tnear, tfar, st, c);
     T.tstop = tstop;
     if (ok) {
-        if (S.trav_wave) T.run(S, st, c);
-        else while (T.more(st)) T.step(S, st, c);
+        while (T.more(st)) T.step(S, st, c);
     }
     h = T.h;
     return ok && T.found;

@@ -104,7 +104,7 @@ int BDPTRenderer::setSchedule(int s) {
 }
 
 int BDPTRenderer::setTuning(const omv_render_tuning& t) {
-    if (t.trace_keep < 0 || t.trace_keep > 64 || t.trace_blocks_per_cu < 0 || t.chunk_samples < 0 || t.traversal < 0 || t.traversal > 2 ||
+    if (t.trace_keep < 0 || t.trace_keep > 64 || t.trace_blocks_per_cu < 0 || t.chunk_samples < 0 || t.traversal < 0 || t.traversal > 1 ||
         t.sort_rays < 0 || t.sort_rays > 2 || t.chunk_streams < 0 || t.chunk_streams > 2 || t.chunk_stagger < 0 || t.occlusion < 0 ||
         t.occlusion > 1 || t.xcd_bands < 0 || t.xcd_bands > 64 || t.tail_round < 0) {
         setError("set_tuning: value out of range");
@@ -116,7 +116,6 @@ int BDPTRenderer::setTuning(const omv_render_tuning& t) {
 
 OmvSceneView BDPTRenderer::viewForLaunch() const {
     OmvSceneView v = scene_->view();
-    if (tuning_.traversal) v.trav_wave = tuning_.traversal == 2 ? 1 : 0;
     v.closest_only = tuning_.occlusion == 1 ? 1 : 0;
     return v;
 }

@@ -773,9 +773,6 @@ int RenderScene::commit(std::string& log) {
         v.scene_lo[a] = std::isfinite(sceneLo[a]) ? sceneLo[a] : 0.0f;
         v.scene_inv[a] = (std::isfinite(ext) && ext > 0.0f) ? 1.0f / ext : 0.0f;
     }
-    // entry-by-entry traversal by default: with branch-free node visits it is
-    // faster than the wave-cooperative order on every scene measured
-    v.trav_wave = 0;
     v.bg_type = bgType_; v.bg_w = bgW_; v.bg_h = bgH_;
     for (int c = 0; c < 3; ++c) v.bg_const[c] = bgConst_[c];
     std::memcpy(v.bg_rot, bgRot_, sizeof(bgRot_));

@@ -24,11 +24,13 @@
 
 #include "path_device.h"
 
-// Traversal kernels are held to 4 waves per SIMD (<= 128 VGPRs, no spill):
-// measured on the headline and the 2M-triangle instanced scene, 4 resident
-// waves per SIMD trace faster than 2, 3 or 5 (5 thrash the L2 with node
-// and triangle lines).
-#define OMV_TRACE_WAVES 4
+// Traversal kernels are compiled for 5 waves per SIMD (<= 96 VGPRs; a few
+// bytes of spill in the camera and fused kernels) with a 28-entry LDS stack,
+// so 10 workgroups fit a CU's 160 KB of LDS.  Measured on MI355X against 4
+// waves (<= 128 VGPRs, 32-entry LDS stack): 2M-triangle instanced scene
+// 15.85 -> 15.23 ms, torus knot 8.93 -> 8.53 ms per render; 3 waves
+// (6 workgroups per CU) was 16 % slower than 4.
+#define OMV_TRACE_WAVES 5
 // Resident waves per SIMD the camera-shading kernel is compiled for (the
 // register budget it gets: 3 -> 168 VGPRs, 4 -> 128).  Measured on MI355X:
 // with point lights 4 (a few spilled registers) renders the 2M-triangle
@@ -465,7 +467,7 @@ struct WfTraceArgs {
 // masked re-trace loop of sceneIntersect, the occlusion early-out) and the
 // same outputs as omv_wf_trace / omv_wf_trace_fused.
 template <int kKind>
-__global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_pt(OmvSceneView S, WfTraceArgs A) {
+__global__ __launch_bounds__(OMV_BLOCK, 4) void omv_wf_trace_pt(OmvSceneView S, WfTraceArgs A) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;

@@ -35,8 +35,8 @@ def _same(out, visit_tol=0.1):
     assert np.array_equal(a, b), f"films differ at {np.argwhere(a != b)[:5].tolist()}"
     for k in ("samples", "rays", "camera_rays", "light_rays", "shadow_rays"):
         assert sa[k] == sb[k], (k, sa[k], sb[k])
-    # node / triangle counts depend on which rays share a wave (the traversal
-    # postpones leaves wave-cooperatively), the closest hits do not
+    # node / triangle counts: both schedules traverse each ray entry by entry
+    # with the same early-outs; checked loosely, the hits and rays exactly
     for k in ("nodes_visited", "tris_tested"):
         assert abs(sa[k] - sb[k]) <= visit_tol * sb[k], (k, sa[k], sb[k])
     assert sa["stack_overflows"] == 0
@@ -107,7 +107,7 @@ def test_wavefront_multi_chunk(pkg, wl, lights, streams):
 
 @pytest.mark.parametrize("tuning", [dict(trace_keep=1), dict(trace_keep=32), dict(trace_keep=64),
                                     dict(trace_keep=48, trace_blocks_per_cu=1, chunk_samples=1000),
-                                    dict(traversal=1), dict(traversal=2), dict(traversal=2, trace_blocks_per_cu=3),
+                                    dict(traversal=1), dict(traversal=1, trace_blocks_per_cu=3),
                                     dict(sort_rays=2), dict(sort_rays=2, chunk_samples=1000, traversal=1),
                                     dict(chunk_streams=2, chunk_samples=1000), dict(chunk_streams=2, chunk_samples=777, chunk_stagger=1, trace_keep=32),
                                     dict(chunk_streams=2, chunk_samples=1000, chunk_stagger=9, sort_rays=2)])

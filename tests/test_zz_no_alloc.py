@@ -91,7 +91,7 @@ def test_noalloc_refusals(pkg, wl):
     r.set_scene(scene)
     with pytest.raises(RuntimeError, match="LIGHT_SPLAT"):
         r.render_noalloc(cam, film.data_ptr())
-    lim = 69   # OMV_NOALLOC_MAX_LIGHT_VERTICES
+    lim = 68   # OMV_NOALLOC_MAX_LIGHT_VERTICES
     r2 = pkg.Renderer(pkg.default_params(**dict(params, light_bounces=lim, max_path_length=lim + 1, camera_bounces=1)))
     r2.set_scene(scene)
     with pytest.raises(RuntimeError, match="more vertices than the fixed scratch"):
