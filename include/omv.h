@@ -350,10 +350,12 @@ typedef struct omv_render_tuning {
                                  (0 = occupancy limit)                          */
     int chunk_samples;        /* samples per wavefront chunk, rounded down to
                                  whole pixels (0 = default: 2^25)               */
-    int traversal;            /* 0 = default, 1 = entry by entry: the same
-                                 order (the wave-cooperative order, 2, was
-                                 removed: slower on every scene measured
-                                 once node visits were branch-free)         */
+    int traversal;            /* 0 = default; 1 = every ray entry by entry,
+                                 one lane each; 2 = the first bounce's camera
+                                 rays (64 consecutive samples per wave) as
+                                 wave packets: one stack of (entry, lane mask)
+                                 per wave, node and triangle records read
+                                 once per wave.  Same hits and film      */
     int sort_rays;            /* secondary camera rays reordered by
                                  (direction octant, origin cell of a 16^3
                                  grid over the scene) before each traversal:

@@ -62,6 +62,7 @@ struct WfLaunch {
     int roundEvAt;             // done (or at the end when there are fewer rounds)
     int xcdBands;              // > 1: traversal queues dealt to this many XCDs in chunks (L2 locality); else plain stride
     int tailRound;             // >= 2: camera rounds tailRound .. B run as one launch (omv_wf_tail); else round by round
+    int blocksPacket;          // > 0: round-1 camera rays traced as wave packets (omv_wf_trace_cam1) on this many workgroups
 };
 // Ray queues are appended to through OMV_QSEG counters each (one per group of
 // producing workgroups); wavefrontSegCap is the entries of one group's region
@@ -72,9 +73,9 @@ size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool
 int wavefrontCounterCount(int B, int lvCap);
 // resident blocks per CU: [0] trace, [1] camera shading with lights + NEE,
 // [2] light shading, [3] gen, [4] persistent trace, [5] camera shading with
-// lights, [6] with NEE, [7] with neither
-void wavefrontBlocksPerCU(int out[8]);
-inline int shadeCamBlocksPerCU(const int occ[8], bool lights, bool nee) { return lights ? (nee ? occ[1] : occ[5]) : (nee ? occ[6] : occ[7]); }
+// lights, [6] with NEE, [7] with neither, [8] round-1 packet trace
+void wavefrontBlocksPerCU(int out[9]);
+inline int shadeCamBlocksPerCU(const int occ[9], bool lights, bool nee) { return lights ? (nee ? occ[1] : occ[5]) : (nee ? occ[6] : occ[7]); }
 hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream);
 // Path-replay adjoint over samples [sample0, sample0 + nsamples) (adjoint.hip).
 int adjointBlocksPerCU();

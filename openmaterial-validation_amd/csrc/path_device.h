@@ -1060,6 +1060,24 @@ struct RP {   // renderer parameters used per sample
 // is returned (instead of the closest), which decides "hit before tstop"
 // exactly as the closest hit would.  Scenes with alpha-MASK materials keep the
 // closest hit (a cut-out hit is re-traced past, up to 10 times).
+// One step of sceneIntersect's mask loop after a trace that found h: 0 = the
+// hit is final, 1 = an alpha-MASK cut-out (tnear moved past it, tfar back to
+// t1: trace again), 2 = a cut-out with nothing left beyond it (no hit).
+DEV int maskStep(const OmvSceneView& S, const f3& org, const f3& dir, const Hit& h, float& tnear, float& tfar, float t1) {
+    if (!S.any_masked) return 0;   // no alpha-MASK material: every hit is final
+    const int mesh = S.insts[h.inst].mesh_id;
+    const OmvMaterial& M = S.mats[S.meshes[mesh].mat_id];
+    if (M.type != OMV_MAT_PBR || M.alpha_mode != OMV_ALPHA_MASK) return 0;
+    if (!isMaskedHit(S, mesh, h.prim, mk3(1.0f - h.u - h.v, h.u, h.v))) return 0;
+    f3 o2 = mk3(org.x + h.t * dir.x, org.y + h.t * dir.y, org.z + h.t * dir.z);
+    float v = maxAbs(o2);
+    v = v > h.t ? v : h.t;
+    float delta = 3.0f * FLT_EPSILON * v;
+    tnear = h.t + delta;
+    tfar = t1;
+    return tfar <= tnear ? 2 : 1;
+}
+
 DEV bool sceneIntersect(const OmvSceneView& S, const f3& org, const f3& dir, float& tnear, float tfar, Hit& h, Stack& st, Counters& c, uint32_t& rays,
                         float tstop = -1.0f) {
     const float t1 = tfar;
@@ -1067,18 +1085,9 @@ DEV bool sceneIntersect(const OmvSceneView& S, const f3& org, const f3& dir, flo
     for (int itr = 10; itr > 0; --itr) {
         ++rays;
         if (!traceClosest(S, org, dir, tnear, tfar, h, st, c, tstop)) return false;
-        if (!S.any_masked) return true;   // no alpha-MASK material: every hit is final
-        const int mesh = S.insts[h.inst].mesh_id;
-        const OmvMaterial& M = S.mats[S.meshes[mesh].mat_id];
-        if (M.type != OMV_MAT_PBR || M.alpha_mode != OMV_ALPHA_MASK) return true;
-        if (!isMaskedHit(S, mesh, h.prim, mk3(1.0f - h.u - h.v, h.u, h.v))) return true;
-        f3 o2 = mk3(org.x + h.t * dir.x, org.y + h.t * dir.y, org.z + h.t * dir.z);
-        float v = maxAbs(o2);
-        v = v > h.t ? v : h.t;
-        float delta = 3.0f * FLT_EPSILON * v;
-        tnear = h.t + delta;
-        tfar = t1;
-        if (tfar <= tnear) return false;
+        const int m = maskStep(S, org, dir, h, tnear, tfar, t1);
+        if (m == 0) return true;
+        if (m == 2) return false;
     }
     return false;
 }

@@ -177,7 +177,7 @@ private:
     int schedule_ = OMV_SCHEDULE_WAVEFRONT;
     omv_render_tuning tuning_{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     OmvSceneView viewForLaunch() const;   // the scene view with the tuning's traversal order
-    int wfPerCu_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int wfPerCu_[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     RenderScene* scene_ = nullptr;
     omv_render_stats stats_{};
     hipStream_t stream_ = nullptr;
@@ -246,7 +246,7 @@ private:
     };
     std::deque<StreamRes> sres_;
     hipEvent_t fork_ = nullptr;
-    int wfPerCu_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int wfPerCu_[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     uint64_t wfMaxSlots_ = 0;
     bool dirty_ = true;
     int blocks_ = 0, cus_ = 0;

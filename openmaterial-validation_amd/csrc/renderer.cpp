@@ -28,6 +28,15 @@ constexpr uint64_t kWfSlotsLights = 1ull << 25;
 // 11.55 / 10.73 / 10.82 / 10.99.  From round 5 both gain; earlier the tail
 // launch holds too many paths, each traced one lane at a time.
 constexpr int kTailRoundDefault = 5;
+// Round-1 camera rays are traced as wave packets (omv_wf_trace_cam1) by
+// default from this many samples per pixel on (a wave's 64 consecutive
+// samples then lie in one or two pixels).  Measured on MI355X (kernel ms per render, per-ray -> packets):
+// 64 samples per pixel: torus knot 512^2 8.55 -> 8.38, instanced scene
+// 960x540 15.06 -> 14.89; 16: torus knot 1024^2 8.26 -> 8.18, instanced
+// 1920x1080 15.06 -> 15.13; 4: instanced 1920x1080 5.78 -> 6.28 (the wider
+// a wave's footprint, the more nodes its packet visits for the lanes that
+// do not need them).
+constexpr int kPacketMinSpp = 64;
 int hipFail(hipError_t e, const char* what) {
     setError(std::string(what) + ": " + hipGetErrorString(e));
     return OMV_E_DEVICE;
@@ -104,7 +113,7 @@ int BDPTRenderer::setSchedule(int s) {
 }
 
 int BDPTRenderer::setTuning(const omv_render_tuning& t) {
-    if (t.trace_keep < 0 || t.trace_keep > 64 || t.trace_blocks_per_cu < 0 || t.chunk_samples < 0 || t.traversal < 0 || t.traversal > 1 ||
+    if (t.trace_keep < 0 || t.trace_keep > 64 || t.trace_blocks_per_cu < 0 || t.chunk_samples < 0 || t.traversal < 0 || t.traversal > 2 ||
         t.sort_rays < 0 || t.sort_rays > 2 || t.chunk_streams < 0 || t.chunk_streams > 2 || t.chunk_stagger < 0 || t.occlusion < 0 ||
         t.occlusion > 1 || t.xcd_bands < 0 || t.xcd_bands > 64 || t.tail_round < 0) {
         setError("set_tuning: value out of range");
@@ -408,6 +417,8 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     // stride on many-instance scenes (instanced 1080p 15.85 vs 16.03 ms)
     L.xcdBands = tuning_.xcd_bands > 0 ? tuning_.xcd_bands : (S.num_insts <= 4 ? 2 * xcds_ : 1);
     L.tailRound = tuning_.tail_round == 1 ? 0 : (tuning_.tail_round >= 2 ? tuning_.tail_round : kTailRoundDefault);
+    const bool packets = tuning_.traversal == 2 || (tuning_.traversal == 0 && spp >= kPacketMinSpp);
+    L.blocksPacket = packets ? cus_ * wfPerCu_[8] : 0;
     // an event pair around every traversal launch: (2 light + 3 camera rounds) per chunk
     const uint64_t chunks = (npix + chunkPix - 1) / chunkPix;
     const size_t evNeed = (size_t)std::min<uint64_t>(1u << 14, chunks * 2 * (uint64_t)(2 * lvCap + 3 * p.camera_bounces));

@@ -294,7 +294,7 @@ size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool
     return b;
 }
 int wavefrontCounterCount(int B, int lvCap) { return OMV_QSEG * (2 * (lvCap + 2) + 3 * (B + 2)); }
-void wavefrontBlocksPerCU(int out[8]) { out[0] = 8; out[1] = 6; out[2] = 6; out[3] = 8; out[4] = 4; out[5] = 8; out[6] = 6; out[7] = 6; }
+void wavefrontBlocksPerCU(int out[9]) { out[0] = 8; out[1] = 6; out[2] = 6; out[3] = 8; out[4] = 4; out[5] = 8; out[6] = 6; out[7] = 6; out[8] = 8; }
 int occupancyBlocksPerCU(int) { return 4; }
 int adjointBlocksPerCU() { return 4; }
 

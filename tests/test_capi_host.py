@@ -188,7 +188,7 @@ def test_tuning_round_trip_and_ranges(pkg):
     got = pkg.RenderTuning()
     assert L.omv_renderer_get_tuning(r, C.byref(got)) == 0
     assert (got.chunk_streams, got.chunk_stagger, got.chunk_samples, got.trace_keep) == (2, 3, 1000, 32)
-    for field, bad in (("chunk_streams", 3), ("chunk_stagger", -1), ("traversal", 2), ("sort_rays", 3), ("trace_keep", 65)):
+    for field, bad in (("chunk_streams", 3), ("chunk_stagger", -1), ("traversal", 3), ("sort_rays", 3), ("trace_keep", 65)):
         b = pkg.RenderTuning()
         assert L.omv_renderer_get_tuning(r, C.byref(b)) == 0
         setattr(b, field, bad)

@@ -108,6 +108,7 @@ def test_wavefront_multi_chunk(pkg, wl, lights, streams):
 @pytest.mark.parametrize("tuning", [dict(trace_keep=1), dict(trace_keep=32), dict(trace_keep=64),
                                     dict(trace_keep=48, trace_blocks_per_cu=1, chunk_samples=1000),
                                     dict(traversal=1), dict(traversal=1, trace_blocks_per_cu=3),
+                                    dict(traversal=2), dict(traversal=2, tail_round=2, chunk_samples=999),
                                     dict(sort_rays=2), dict(sort_rays=2, chunk_samples=1000, traversal=1),
                                     dict(chunk_streams=2, chunk_samples=1000), dict(chunk_streams=2, chunk_samples=777, chunk_stagger=1, trace_keep=32),
                                     dict(chunk_streams=2, chunk_samples=1000, chunk_stagger=9, sort_rays=2)])
@@ -301,3 +302,37 @@ def test_refilling_traversal_keeps_the_film(pkg, wl, case):
             assert np.array_equal(got, want), (keep, np.argwhere(got != want)[:5].tolist())
         for k in ("rays", "camera_rays", "light_rays", "shadow_rays", "samples"):
             assert gs[k] == ws[k], (keep, k, gs[k], ws[k])
+
+
+@pytest.mark.parametrize("case", ["masked", "lights", "nee", "instanced", "knot", "chunks"])
+def test_packet_traversal_keeps_the_film(pkg, wl, case):
+    """Round-1 camera rays traced as wave packets (tuning traversal = 2: one
+    wave-uniform stack of (entry, lane mask), records read once per wave)
+    give the per-ray traversal's film and ray counts bit for bit: with the
+    alpha-mask re-trace loop, instancing, a single-mesh scene, NEE, and
+    chunks whose sample counts leave the last wave part-filled."""
+    depth = 4
+    params = dict(samples_per_pixel=3, camera_bounces=depth, light_bounces=depth, max_path_length=depth, seed=17)
+    if case == "instanced":
+        desc, view, _ = wl.large_instanced(width=64, height=36, spp=2, depth=depth, count=24, tris_per=600, hdr_w=128, hdr_h=64)
+    elif case == "knot":
+        desc, view, params = wl.hdri_torus_knot(width=40, height=33, spp=16, tex=128, hdr_w=256, hdr_h=128, segments=200, tube=16)
+    else:
+        desc, view = wl.mixed_test_scene(hdr=True, lights=case != "nee", masked=case in ("masked", "chunks"))
+        params["extensions"] = 2 if case == "nee" else 0
+    s = pkg.Scene()
+    desc.apply(s, pkg.pbr_desc)
+    cam = pkg.Camera(min(view.width, 61), min(view.height, 47), view.position, view.forward, view.up, view.yfov)
+    out = {}
+    for trav in (1, 2):
+        r = pkg.Renderer(pkg.default_params(**params))
+        r.set_scene(s)
+        kw = dict(traversal=trav)
+        if case == "chunks":
+            kw.update(chunk_samples=cam.width * 5 * params["samples_per_pixel"] + 7)
+        r.set_tuning(**kw)
+        out[trav] = (r.render(cam), r.stats())
+    # a lane visits the leaves in the packet's order (its leader lane's
+    # nearest first), not its own, so its closest t culls a little less:
+    # node and triangle counts are checked to 25 %
+    _same([out[2], out[1]], visit_tol=0.25)
