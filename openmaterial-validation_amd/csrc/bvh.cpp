@@ -204,9 +204,9 @@ struct Collapser {
         for (int k = 0; k < 4; ++k) {
             nd.child[k] = child[k];
             nd.entry[k] = entry[k];
-            if (k >= nk) {
-                nd.lox[k] = nd.loy[k] = nd.loz[k] = 0.0f;
-                nd.hix[k] = nd.hiy[k] = nd.hiz[k] = 0.0f;
+            if (k >= nk) {   // empty slot: a box at +inf on every plane, which no ray hits
+                nd.lox[k] = nd.loy[k] = nd.loz[k] = INFINITY;
+                nd.hix[k] = nd.hiy[k] = nd.hiz[k] = INFINITY;
                 continue;
             }
             const BinNode& c = bin[kids[k]];

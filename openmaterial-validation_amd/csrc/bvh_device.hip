@@ -240,7 +240,7 @@ __global__ void lbCollapse(const uint2* __restrict__ jobs, uint32_t njobs, const
         if (k >= nk) {
             nd.child[k] = OMV_EMPTY_CHILD;
             nd.entry[k] = OMV_ENT_NONE;
-            nd.lox[k] = nd.loy[k] = nd.loz[k] = nd.hix[k] = nd.hiy[k] = nd.hiz[k] = 0.0f;
+            nd.lox[k] = nd.loy[k] = nd.loz[k] = nd.hix[k] = nd.hiy[k] = nd.hiz[k] = INFINITY;   // no ray hits it
             continue;
         }
         const uint32_t c = kids[k];

@@ -21,6 +21,7 @@
 #define OMV_STACK_LDS 28      // per-lane traversal stack entries kept in LDS
 #define OMV_STACK_TOTAL 128   // LDS + global overflow; the builder proves the bound
 #define OMV_EMPTY_CHILD ((int32_t)0x80000000)
+#define OMV_MAX_NODES ((size_t)1 << 25)   // node byte offsets (node * 128) stay below 2^32
 
 enum { OMV_MAT_NONE = 0, OMV_MAT_OM = 1, OMV_MAT_PBR = 2, OMV_MAT_DIFFUSE = 3 };
 enum { OMV_TEX_RGBA8 = 0, OMV_TEX_RGBA32F = 1 };
@@ -41,7 +42,7 @@ enum { OMV_BG_NONE = 0, OMV_BG_CONSTANT = 1, OMV_BG_HDR = 2 };
 static_assert(OMV_BVH_LEAF_MAX >= 1 && OMV_BVH_LEAF_MAX <= 8, "leaf size must fit the entry's 3-bit count");
 
 struct alignas(64) OmvNode4 {
-    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];   // child boxes; an empty slot's planes are all +inf
     int32_t child[4];    // >= 0 internal node; < 0 leaf: -(first+1) (tri record / instance id); empty: OMV_EMPTY_CHILD
     uint32_t entry[4];   // the child's stack entry (OMV_ENT), OMV_ENT_NONE for an empty slot
 };

@@ -888,7 +888,10 @@ DEV void offer(Hit& h, bool& found, float t, float u, float v, int inst, int pri
     }
 }
 
-// 4-wide slab test, conservative (Ize 2013: tmax scaled by 1 + 2*gamma(3))
+// 4-wide slab test, conservative (Ize 2013: tmax scaled by 1 + 2*gamma(3)).
+// An empty slot (planes at +inf, entry E_NONE) tests as a miss or, when
+// tfar * kUp overflows to +inf, as a hit at +inf; either way its E_NONE
+// entry is never pushed (pushSorted), so no entry check is needed here.
 DEV void boxTest4(const OmvNode4& nd, const f3& o, const f3& idir, float tnear, float tfar, float tent[4], bool hit[4]) {
     const float kUp = 1.0000004f;
 #pragma unroll
@@ -899,15 +902,49 @@ DEV void boxTest4(const OmvNode4& nd, const f3& o, const f3& idir, float tnear, 
         float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tnear));
         float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tfar));
         tent[c] = tmin;
-        hit[c] = nd.entry[c] != OMV_ENT_NONE && tmin <= tmax * kUp;
+        hit[c] = tmin <= tmax * kUp;
     }
 }
 
-DEV void loadNode(const OmvNode4* nodes, uint32_t idx, OmvNode4& nd) {
-    const float4* src = reinterpret_cast<const float4*>(nodes + idx);
-    float4* dst = reinterpret_cast<float4*>(&nd);
+// Byte offsets of a ray's near plane rows within a node (OmvNode4's float4
+// rows lox hix loy hiy loz hiz child entry), packed x | y << 8 | z << 16:
+// the near x plane of every child is hix for a ray going -x (sign bit of
+// 1/d.x set, -0 included) and lox otherwise; the far row is the other one
+// of the pair (offset ^ 16).
+DEV uint32_t nearRows(const f3& idir) {
+    const uint32_t sx = __float_as_uint(idir.x) >> 31, sy = __float_as_uint(idir.y) >> 31, sz = __float_as_uint(idir.z) >> 31;
+    return (sx << 4) | ((32u | (sy << 4)) << 8) | ((64u | (sz << 4)) << 16);
+}
+DEV float4 ldRow(const char* base, uint32_t off) { return *reinterpret_cast<const float4*>(base + off); }
+// boxTest4 of node idx with the planes fetched by the ray's direction signs:
+// the near distance on an axis is (near - o) * idir and the far one
+// (far - o) * idir, the same values as the min / max of the two (the
+// product is monotonic in the plane), so no per-axis min / max is needed.
+// Where the min / max form would drop a NaN distance (o on a plane with a
+// zero direction component) the max / min below drop it too, so no box
+// that form accepts is rejected.  Also returns the children's entries.
+// Node byte offsets must fit in 32 bits (OMV_MAX_NODES).
+DEV void boxTest4Rows(const OmvNode4* nodes, uint32_t idx, uint32_t rows, const f3& o, const f3& idir, float tnear, float tfar, float tent[4],
+                      bool hit[4], uint32_t ent[4]) {
+    const char* base = reinterpret_cast<const char*>(nodes);
+    const uint32_t nb = idx << 7;
+    const uint32_t ax = nb | (rows & 0xFFu), ay = nb | ((rows >> 8) & 0xFFu), az = nb | (rows >> 16);
+    const float4 nx = ldRow(base, ax), fx = ldRow(base, ax ^ 16u);
+    const float4 ny = ldRow(base, ay), fy = ldRow(base, ay ^ 16u);
+    const float4 nz = ldRow(base, az), fz = ldRow(base, az ^ 16u);
+    const uint4 en = *reinterpret_cast<const uint4*>(base + (nb | 112u));
+    const float kUp = 1.0000004f;
+    const float NX[4] = {nx.x, nx.y, nx.z, nx.w}, FX[4] = {fx.x, fx.y, fx.z, fx.w};
+    const float NY[4] = {ny.x, ny.y, ny.z, ny.w}, FY[4] = {fy.x, fy.y, fy.z, fy.w};
+    const float NZ[4] = {nz.x, nz.y, nz.z, nz.w}, FZ[4] = {fz.x, fz.y, fz.z, fz.w};
 #pragma unroll
-    for (int i = 0; i < 8; ++i) dst[i] = src[i];
+    for (int c = 0; c < 4; ++c) {
+        const float tmin = fmaxf(fmaxf((NX[c] - o.x) * idir.x, (NY[c] - o.y) * idir.y), fmaxf((NZ[c] - o.z) * idir.z, tnear));
+        const float tmax = fminf(fminf((FX[c] - o.x) * idir.x, (FY[c] - o.y) * idir.y), fminf((FZ[c] - o.z) * idir.z, tfar));
+        tent[c] = tmin;
+        hit[c] = tmin <= tmax * kUp;
+    }
+    ent[0] = en.x; ent[1] = en.y; ent[2] = en.z; ent[3] = en.w;
 }
 
 // push the hit children farthest first; the nearest is not pushed but
@@ -951,6 +988,7 @@ DEV void pushSorted(Stack& st, Counters& c, uint32_t e[4], float t[4], const boo
 struct Trav {
     f3 wo, wd, widir;   // world ray (TLAS boxes, instance transforms)
     Pre op;             // object ray of the current instance (BLAS boxes, triangles)
+    uint32_t wrows, orows;   // nearRows of the world and object rays
     Hit h;
     float tnear, tfar;
     float tstop;     // occlusion early-out: the traversal ends at the first hit with t <= tstop (< 0: closest hit only)
@@ -970,6 +1008,8 @@ struct Trav {
         if (!(ax > 0.0f || ay > 0.0f || az > 0.0f)) return false;   // zero / NaN direction: no hit (as rayPrep)
         wo = o; wd = d;
         widir = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+        wrows = nearRows(widir);
+        orows = wrows;
         curInst = -1;
         nxt = (E_TLAS_NODE << E_KIND_SHIFT) | (uint32_t)S.tlas_root;
         hasNext = true;
@@ -1010,6 +1050,7 @@ struct Trav {
                         I[8] * o.x + I[9] * o.y + I[10] * o.z + I[11]);
             f3 od = mk3(I[0] * d.x + I[1] * d.y + I[2] * d.z, I[4] * d.x + I[5] * d.y + I[6] * d.z, I[8] * d.x + I[9] * d.y + I[10] * d.z);
             if (!rayPrep(oo, od, op)) return;
+            orows = nearRows(op.idir);
             curInst = inst;
             st.push(E_END, c);
             nxt = (E_BLAS_NODE << E_KIND_SHIFT) | (uint32_t)S.meshes[in.mesh_id].blas_root;
@@ -1018,13 +1059,9 @@ struct Trav {
         }
         // internal node (TLAS uses the world ray, BLAS the object ray)
         const bool tl = kind == E_TLAS_NODE;
-        OmvNode4 nd;
-        loadNode(S.nodes, idx, nd);
         ++c.nodes;
-        float tent[4]; bool hit[4];
-        if (tl) boxTest4(nd, wo, widir, tnear, tb, tent, hit);
-        else boxTest4(nd, op.o, op.idir, tnear, tb, tent, hit);
-        uint32_t ent[4] = {nd.entry[0], nd.entry[1], nd.entry[2], nd.entry[3]};
+        float tent[4]; bool hit[4]; uint32_t ent[4];
+        boxTest4Rows(S.nodes, idx, tl ? wrows : orows, tl ? wo : op.o, tl ? widir : op.idir, tnear, tb, tent, hit, ent);
         pushSorted(st, c, ent, tent, hit, nxt, hasNext);
     }
     // one entry, whatever its kind (the persistent traversal's unit of work)

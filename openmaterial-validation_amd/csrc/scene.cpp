@@ -710,6 +710,12 @@ int RenderScene::commit(std::string& log) {
         setError("commit: BVH too deep for the traversal stack (" + std::to_string(stackBound_) + " entries)");
         return OMV_E_SCENE;
     }
+    // the traversal addresses a node's plane rows by 32-bit byte offsets (node * 128 + row)
+    const size_t totalNodes = (deviceBuild ? (size_t)deviceNodes : 0) + nodes.size();
+    if (totalNodes > OMV_MAX_NODES) {
+        setError("commit: BVH of " + std::to_string(totalNodes) + " nodes exceeds the " + std::to_string(OMV_MAX_NODES) + "-node limit");
+        return OMV_E_SCENE;
+    }
     // light CDF (RenderScene.cpp:213-220, DistanceRandom::calculate)
     std::vector<float> cdf;
     float acc = 0.0f;

@@ -546,7 +546,7 @@ DEV void packetTrace(const OmvSceneView& S, PacketRay& R, uint64_t M0, uint32_t*
         if (act) ++c.nodes;
         uint64_t m[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) m[k] = __ballot(act && hit[k]);
+        for (int k = 0; k < 4; ++k) m[k] = nd.entry[k] != E_NONE ? __ballot(act && hit[k]) : 0ull;   // empty slots: never pushed
         const uint64_t any = m[0] | m[1] | m[2] | m[3];
         if (any == 0ull) continue;
         // order by the first hitting lane's entry distances; a child that
