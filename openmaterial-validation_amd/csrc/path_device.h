@@ -850,15 +850,35 @@ DEV bool rayPrep(const f3& o, const f3& d, Pre& p) {
     return true;
 }
 
+// Component selection for the ray's axis permutation by bit masks: k in
+// {0, 1, 2} gives m1 = all ones iff k == 1 and m2 = all ones iff k == 2,
+// and pick() takes y over x by m1, then z by m2 (two v_bfi_b32).  The masks
+// pass through an empty asm so the optimiser does not turn the selects back
+// into a compare chain (lowered as branches) or a dynamic vector index
+// (lowered through scratch memory).
+struct PermMask { uint32_t m1, m2; };
+DEV PermMask permMask(int k) {
+    PermMask r;
+    r.m1 = 0u - (uint32_t)(k & 1);
+    r.m2 = 0u - (uint32_t)((k >> 1) & 1);
+    asm("" : "+v"(r.m1), "+v"(r.m2));
+    return r;
+}
+DEV float pick(const f3& v, const PermMask& m) {
+    const uint32_t a = (__float_as_uint(v.x) & ~m.m1) | (__float_as_uint(v.y) & m.m1);
+    return __uint_as_float((a & ~m.m2) | (__float_as_uint(v.z) & m.m2));
+}
+
 // Woop, Benthin & Wald 2013; t in (tnear, tfar]
 DEV bool triIntersect(const Pre& p, const float4 r0, const float4 r1, const float4 r2, float tnear, float tfar, float& t, float& u, float& v) {
     f3 A = mk3(r0.x - p.o.x, r0.y - p.o.y, r0.z - p.o.z);
     f3 B = mk3(r1.x - p.o.x, r1.y - p.o.y, r1.z - p.o.z);
     f3 C = mk3(r2.x - p.o.x, r2.y - p.o.y, r2.z - p.o.z);
-    const float Akz = comp(A, p.kz), Bkz = comp(B, p.kz), Ckz = comp(C, p.kz);
-    float Ax = comp(A, p.kx) - p.Sx * Akz, Ay = comp(A, p.ky) - p.Sy * Akz;
-    float Bx = comp(B, p.kx) - p.Sx * Bkz, By = comp(B, p.ky) - p.Sy * Bkz;
-    float Cx = comp(C, p.kx) - p.Sx * Ckz, Cy = comp(C, p.ky) - p.Sy * Ckz;
+    const PermMask mx = permMask(p.kx), my = permMask(p.ky), mz = permMask(p.kz);
+    const float Akz = pick(A, mz), Bkz = pick(B, mz), Ckz = pick(C, mz);
+    float Ax = pick(A, mx) - p.Sx * Akz, Ay = pick(A, my) - p.Sy * Akz;
+    float Bx = pick(B, mx) - p.Sx * Bkz, By = pick(B, my) - p.Sy * Bkz;
+    float Cx = pick(C, mx) - p.Sx * Ckz, Cy = pick(C, my) - p.Sy * Ckz;
     float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
     if (U == 0.0f || V == 0.0f || W == 0.0f) {
         double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
