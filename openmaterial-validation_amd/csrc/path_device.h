@@ -113,9 +113,13 @@ struct Rng {
         b0 = x0; b1 = x1; b2 = x2; b3 = x3;
         ++c0; idx = 0;
     }
+    // The block is kept rotated so that b0 is always the next draw (idx
+    // counts the draws taken from it): selecting b[idx] by a per-lane idx
+    // would compile to a branchy switch.
     DEV float next() {
         if (idx == 4) refill();
-        uint32_t v = idx == 0 ? b0 : (idx == 1 ? b1 : (idx == 2 ? b2 : b3));
+        const uint32_t v = b0;
+        b0 = b1; b1 = b2; b2 = b3;
         ++idx;
         return (float)(v >> 8) * (1.0f / 16777216.0f);
     }

@@ -82,7 +82,10 @@ DEV void rngRestore(Rng& r, uint32_t seed, uint32_t stream, uint64_t sample, uin
     const uint32_t block = (n - 1u) >> 2;
     r.c0 = block;
     r.refill();
-    r.idx = (int)(n - 4u * block);
+    // drop the draws already taken from this block (b0 is the next draw)
+    const int k = (int)(n - 4u * block);
+    for (int j = 0; j < k; ++j) { r.b0 = r.b1; r.b1 = r.b2; r.b2 = r.b3; }
+    r.idx = k;
 }
 
 // Slot of the calling lane in a queue: one atomic per wave.  Lanes that do
