@@ -817,8 +817,24 @@ struct Stack {
         c.overflow += (valid && !room) ? 1u : 0u;
         sp += (valid && room) ? 1 : 0;
     }
+    // pushIf of three entries; when no active lane of the wave could reach
+    // the spill area (a wave-uniform test) the entries go to LDS without the
+    // per-entry LDS / spill / overflow branches
+    DEV void push3If(const uint32_t e[3], const bool v[3], Counters& c) {
+        if (__ballot(sp > OMV_STACK_LDS - 3) == 0ull) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                omv_lds_stack[col + (uint32_t)sp * OMV_BLOCK] = e[k];
+                sp += v[k] ? 1 : 0;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) pushIf(e[k], v[k], c);
+        }
+    }
     DEV uint32_t pop() {
         --sp;
+        if (__ballot(sp >= OMV_STACK_LDS) == 0ull) return omv_lds_stack[col + (uint32_t)sp * OMV_BLOCK];   // wave-uniform: LDS only
         return sp < OMV_STACK_LDS ? omv_lds_stack[col + (uint32_t)sp * OMV_BLOCK] : glb[(uint32_t)(sp - OMV_STACK_LDS) * glbStride];
     }
     DEV uint32_t top() const {
@@ -832,27 +848,6 @@ enum : uint32_t {
     E_BLAS_NODE = OMV_ENT_BLAS_NODE, E_BLAS_LEAF = OMV_ENT_BLAS_LEAF, E_TLAS_NODE = OMV_ENT_TLAS_NODE, E_INST = OMV_ENT_INST,
     E_END = OMV_ENT_END, E_NONE = OMV_ENT_NONE,
 };
-
-struct Pre { int kx, ky, kz; float Sx, Sy, Sz; f3 o, d, idir; };
-
-DEV bool rayPrep(const f3& o, const f3& d, Pre& p) {
-    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
-    if (!(ax > 0.0f || ay > 0.0f || az > 0.0f)) return false;   // zero / NaN direction: no hit
-    int kz = 0; float m = ax;
-    if (ay > m) { kz = 1; m = ay; }
-    if (az > m) { kz = 2; }
-    int kx = kz + 1; if (kx == 3) kx = 0;
-    int ky = kx + 1; if (ky == 3) ky = 0;
-    float dkz = comp(d, kz);
-    if (dkz < 0.0f) { int t = kx; kx = ky; ky = t; }
-    p.kx = kx; p.ky = ky; p.kz = kz;
-    p.Sx = comp(d, kx) / dkz;
-    p.Sy = comp(d, ky) / dkz;
-    p.Sz = 1.0f / dkz;
-    p.o = o; p.d = d;
-    p.idir = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    return true;
-}
 
 // Component selection for the ray's axis permutation by bit masks: k in
 // {0, 1, 2} gives m1 = all ones iff k == 1 and m2 = all ones iff k == 2,
@@ -873,6 +868,27 @@ DEV float pick(const f3& v, const PermMask& m) {
     return __uint_as_float((a & ~m.m2) | (__float_as_uint(v.z) & m.m2));
 }
 
+struct Pre { int kx, ky, kz; float Sx, Sy, Sz; f3 o, d, idir; };
+
+DEV bool rayPrep(const f3& o, const f3& d, Pre& p) {
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    if (!(ax > 0.0f || ay > 0.0f || az > 0.0f)) return false;   // zero / NaN direction: no hit
+    int kz = 0; float m = ax;
+    if (ay > m) { kz = 1; m = ay; }
+    if (az > m) { kz = 2; }
+    int kx = kz + 1; if (kx == 3) kx = 0;
+    int ky = kx + 1; if (ky == 3) ky = 0;
+    float dkz = pick(d, permMask(kz));
+    if (dkz < 0.0f) { int t = kx; kx = ky; ky = t; }
+    p.kx = kx; p.ky = ky; p.kz = kz;
+    p.Sx = pick(d, permMask(kx)) / dkz;
+    p.Sy = pick(d, permMask(ky)) / dkz;
+    p.Sz = 1.0f / dkz;
+    p.o = o; p.d = d;
+    p.idir = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    return true;
+}
+
 // Woop, Benthin & Wald 2013; t in (tnear, tfar]
 DEV bool triIntersect(const Pre& p, const float4 r0, const float4 r1, const float4 r2, float tnear, float tfar, float& t, float& u, float& v) {
     f3 A = mk3(r0.x - p.o.x, r0.y - p.o.y, r0.z - p.o.z);
@@ -884,7 +900,10 @@ DEV bool triIntersect(const Pre& p, const float4 r0, const float4 r1, const floa
     float Bx = pick(B, mx) - p.Sx * Bkz, By = pick(B, my) - p.Sy * Bkz;
     float Cx = pick(C, mx) - p.Sx * Ckz, Cy = pick(C, my) - p.Sy * Ckz;
     float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
-    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    const bool zero = U == 0.0f || V == 0.0f || W == 0.0f;
+    // the FP64 recomputation is rare: a wave-uniform branch around it keeps
+    // the compiler from computing it for every triangle and selecting
+    if (__ballot(zero) != 0ull && zero) {
         double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
         U = (float)(CxBy - CyBx);
         double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
@@ -994,9 +1013,8 @@ DEV void pushSorted(Stack& st, Counters& c, uint32_t e[4], float t[4], const boo
     }
     OMV_CSWAP(0, 1) OMV_CSWAP(2, 3) OMV_CSWAP(0, 2) OMV_CSWAP(1, 3) OMV_CSWAP(1, 2)
 #undef OMV_CSWAP
-    st.pushIf(e[0], e[0] != (uint32_t)E_NONE, c);
-    st.pushIf(e[1], e[1] != (uint32_t)E_NONE, c);
-    st.pushIf(e[2], e[2] != (uint32_t)E_NONE, c);
+    const bool v[3] = {e[0] != (uint32_t)E_NONE, e[1] != (uint32_t)E_NONE, e[2] != (uint32_t)E_NONE};
+    st.push3If(e, v, c);
     nxt = e[3];
     hasNext = e[3] != (uint32_t)E_NONE;
 }
