@@ -105,10 +105,11 @@ struct Rng {
 #pragma unroll
         for (int r = 0; r < 10; ++r) {
             if (r) { ka += 0x9E3779B9u; kb += 0xBB67AE85u; }
-            uint32_t lo0 = 0xD2511F53u * x0, hi0 = __umulhi(0xD2511F53u, x0);
-            uint32_t lo1 = 0xCD9E8D57u * x2, hi1 = __umulhi(0xCD9E8D57u, x2);
-            uint32_t n0 = hi1 ^ x1 ^ ka, n2 = hi0 ^ x3 ^ kb;
-            x0 = n0; x1 = lo1; x2 = n2; x3 = lo0;
+            // one 32x32 -> 64 product each (v_mad_u64_u32) instead of a
+            // separate low and high multiply
+            const uint64_t p0 = (uint64_t)0xD2511F53u * x0, p1 = (uint64_t)0xCD9E8D57u * x2;
+            uint32_t n0 = (uint32_t)(p1 >> 32) ^ x1 ^ ka, n2 = (uint32_t)(p0 >> 32) ^ x3 ^ kb;
+            x0 = n0; x1 = (uint32_t)p1; x2 = n2; x3 = (uint32_t)p0;
         }
         b0 = x0; b1 = x1; b2 = x2; b3 = x3;
         ++c0; idx = 0;

@@ -178,8 +178,14 @@ DEV float4 f4of(const f3& v, float w) { float4 r; r.x = v.x; r.y = v.y; r.z = v.
 // emitted ray (tryComputeLightPath, BDPTRenderer.cpp:194-206) with the
 // light-stream draws they consumed.
 DEV void genCameraRay(const omv_camera_desc& cam, uint32_t seed, int spp, uint64_t s, f3& org, f3& dir) {
-    const uint64_t pix = s / (uint64_t)spp;
-    const int x = (int)(pix % (uint64_t)cam.width), y = (int)(pix / (uint64_t)cam.width);
+    int x, y;
+    if (s < (1ull << 32)) {   // 32-bit division (the 64-bit one expands to a long sequence)
+        const uint32_t pix = (uint32_t)s / (uint32_t)spp;
+        x = (int)(pix % (uint32_t)cam.width); y = (int)(pix / (uint32_t)cam.width);
+    } else {
+        const uint64_t pix = s / (uint64_t)spp;
+        x = (int)(pix % (uint64_t)cam.width); y = (int)(pix / (uint64_t)cam.width);
+    }
     Rng cs; cs.init(seed, STREAM_CAMERA, s);
     const float u0 = cs.next(), u1 = cs.next();
     f3 O;
