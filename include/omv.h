@@ -350,7 +350,7 @@ typedef struct omv_render_tuning {
                                  (0 = occupancy limit)                          */
     int chunk_samples;        /* samples per wavefront chunk, rounded down to
                                  whole pixels (0 = default: 2^25)               */
-    int traversal;            /* 0 = default; 1 = every ray entry by entry,
+    int traversal;            /* 0 = default = 1: every ray entry by entry,
                                  one lane each; 2 = the first bounce's camera
                                  rays (64 consecutive samples per wave) as
                                  wave packets: one stack of (entry, lane mask)

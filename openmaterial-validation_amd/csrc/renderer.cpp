@@ -28,15 +28,14 @@ constexpr uint64_t kWfSlotsLights = 1ull << 25;
 // 11.55 / 10.73 / 10.82 / 10.99.  From round 5 both gain; earlier the tail
 // launch holds too many paths, each traced one lane at a time.
 constexpr int kTailRoundDefault = 5;
-// Round-1 camera rays are traced as wave packets (omv_wf_trace_cam1) by
-// default from this many samples per pixel on (a wave's 64 consecutive
-// samples then lie in one or two pixels).  Measured on MI355X (kernel ms per render, per-ray -> packets):
-// 64 samples per pixel: torus knot 512^2 8.55 -> 8.38, instanced scene
-// 960x540 15.06 -> 14.89; 16: torus knot 1024^2 8.26 -> 8.18, instanced
-// 1920x1080 15.06 -> 15.13; 4: instanced 1920x1080 5.78 -> 6.28 (the wider
-// a wave's footprint, the more nodes its packet visits for the lanes that
-// do not need them).
-constexpr int kPacketMinSpp = 64;
+// Round-1 camera rays are traced as wave packets (omv_wf_trace_cam1) only on
+// request (tuning traversal = 2).  Measured on MI355X (kernel ms per render,
+// per-ray -> packets), first with the branchy triangle setup: at 64 samples
+// per pixel torus knot 512^2 8.55 -> 8.38, instanced 960x540 15.06 ->
+// 14.89; at 16: instanced 1080p 15.06 -> 15.13.  Once the per-ray traversal
+// got the branch-free triangle setup and stack fast paths: torus knot 512^2
+// x 64 6.36 -> 6.41, 1024^2 x 16 6.23 -> 6.26, instanced 1080p x 16 12.48
+// -> 12.61, x 4 4.74 -> 5.12.
 int hipFail(hipError_t e, const char* what) {
     setError(std::string(what) + ": " + hipGetErrorString(e));
     return OMV_E_DEVICE;
@@ -417,7 +416,7 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     // stride on many-instance scenes (instanced 1080p 15.85 vs 16.03 ms)
     L.xcdBands = tuning_.xcd_bands > 0 ? tuning_.xcd_bands : (S.num_insts <= 4 ? 2 * xcds_ : 1);
     L.tailRound = tuning_.tail_round == 1 ? 0 : (tuning_.tail_round >= 2 ? tuning_.tail_round : kTailRoundDefault);
-    const bool packets = tuning_.traversal == 2 || (tuning_.traversal == 0 && spp >= kPacketMinSpp);
+    const bool packets = tuning_.traversal == 2;
     L.blocksPacket = packets ? cus_ * wfPerCu_[8] : 0;
     // an event pair around every traversal launch: (2 light + 3 camera rounds) per chunk
     const uint64_t chunks = (npix + chunkPix - 1) / chunkPix;
