@@ -584,9 +584,25 @@ DEV void pbrInputs(const OmvSceneView& S, const SP& sp, const OmvMaterial& M, fl
     if (pbrTex(S, sp, M, OMV_PBR_METALLIC_ROUGHNESS, t)) { omr[0] *= t.x; omr[1] *= t.z; omr[2] *= t.y; }
     if (M.occ_separate && pbrTex(S, sp, M, OMV_PBR_OCCLUSION, t)) omr[0] = t.x;
 }
-DEV void pbrSetup(const OmvSceneView& S, const SP& sp, const OmvMaterial& M, Bsdf& b, float omr[3]) {
+// PBR texture inputs at a surface point (pbrInputs), read once per vertex
+// where several BRDF evaluations there need them (the connections to the
+// light vertices, then the bounce): the same texel reads, done once
+struct MatIn { float col[4], omr[3]; };
+DEV void matInputs(const OmvSceneView& S, const SP& sp, MatIn& mi) {
+    const OmvMaterial& M = S.mats[S.meshes[sp.mesh].mat_id];
+    if (M.type == OMV_MAT_PBR) pbrInputs(S, sp, M, mi.col, mi.omr);
+}
+// mi: the inputs from matInputs at this point, or null to read them here
+DEV void pbrSetup(const OmvSceneView& S, const SP& sp, const OmvMaterial& M, Bsdf& b, float omr[3], const MatIn* mi = nullptr) {
     float col[4];
-    pbrInputs(S, sp, M, col, omr);
+    if (mi) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) col[k] = mi->col[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) omr[k] = mi->omr[k];
+    } else {
+        pbrInputs(S, sp, M, col, omr);
+    }
     b.c0 = col[0]; b.c1 = col[1]; b.c2 = col[2]; b.c3 = col[3];
     if (M.alpha_mode != OMV_ALPHA_BLEND) b.c3 = 1.0f;
     b.transmissive = M.double_sided || b.c3 < 1.0f;
@@ -718,7 +734,7 @@ DEV f3 omRoughSample(const OmvMaterial& M, float alpha, const f3& I, const SP& s
 }
 
 // RenderMaterial::getBrdf of the three materials for a connection
-DEV f3 matGetBrdf(const OmvSceneView& S, const Ext& X, const SP& sp, const f3& inV, const f3& outV) {
+DEV f3 matGetBrdf(const OmvSceneView& S, const Ext& X, const SP& sp, const f3& inV, const f3& outV, const MatIn* mi = nullptr) {
     const OmvMaterial& M = S.mats[S.meshes[sp.mesh].mat_id];
     if (M.type == OMV_MAT_OM) {   // OpenMaterial::getBrdf (OpenMaterial.cpp:95-163)
         if (!M.has_base) return mk3(0.0f, 0.0f, 0.0f);
@@ -731,7 +747,7 @@ DEV f3 matGetBrdf(const OmvSceneView& S, const Ext& X, const SP& sp, const f3& i
         Bsdf b; bsdfInit(b);
         float omr[3];
         b.I = inV; b.O = outV; b.N = sp.n;
-        pbrSetup(S, sp, M, b, omr);
+        pbrSetup(S, sp, M, b, omr, mi);
         if (dot3(inV, sp.n) > 0.0f && b.transmissive) b.N = neg3(b.N);
         pbrFinish(M, b, omr);
         return evaluateDirect(b);
@@ -750,7 +766,7 @@ struct BounceInfo {
     float occ, roughRaw, fCos;
 };
 DEV void matGetRayAndBrdf(const OmvSceneView& S, const Ext& X, const SP& sp, const f3& inV, f3& outV, f3& brdf, f3& erad, bool inv,
-                          Rng& r, BounceInfo* info = nullptr) {
+                          Rng& r, BounceInfo* info = nullptr, const MatIn* mi = nullptr) {
     const OmvMaterial& M = S.mats[S.meshes[sp.mesh].mat_id];
     erad = mk3(0.0f, 0.0f, 0.0f);
     if (info) info->kind = 0;
@@ -765,7 +781,7 @@ DEV void matGetRayAndBrdf(const OmvSceneView& S, const Ext& X, const SP& sp, con
         Bsdf b; bsdfInit(b);
         float omr[3];
         b.I = inV; b.N = sp.n; b.T = sp.t;
-        pbrSetup(S, sp, M, b, omr);
+        pbrSetup(S, sp, M, b, omr, mi);
         pbrFinish(M, b, omr);
         b.eta = 1.0f / (M.double_sided ? 1.0f : M.ior);
         brdf = sampleIndirect(b, r);
@@ -1530,6 +1546,8 @@ DEV f3 cameraRounds(const OmvSceneView& S, const RP& P, uint64_t s, int i0, f3 o
         SP sp;
         computeSurfacePoint(S, h, sp, false);
         modifyFrame(S, sp);
+        MatIn mi;   // PBR texture inputs, read once for the connections and the bounce (kLights)
+        if (kLights) matInputs(S, sp, mi);
         // connections of camera vertex j = i with light vertices l, l + j <= q
         if (kLights && ls.idx >= 0) {
             const int lEnd = min(ls.len, q - i + 1);
@@ -1548,7 +1566,7 @@ DEV f3 cameraRounds(const OmvSceneView& S, const RP& P, uint64_t s, int i0, f3 o
                 float c = fabsf(dot3(LC, sp.n));
                 f3 lc = mk3(lRad.x * c * of.x, lRad.y * c * of.y, lRad.z * c * of.z);
                 if (intensity(lc) < P.cut) continue;
-                f3 brdf = matGetBrdf(S, P.x, sp, LC, outV);
+                f3 brdf = matGetBrdf(S, P.x, sp, LC, outV, &mi);
                 brdf = mk3(brdf.x * c, brdf.y * c, brdf.z * c);
                 lc = mk3(brdf.x * of.x * lRad.x, brdf.y * of.y * lRad.y, brdf.z * of.z * lRad.z);
                 if (intensity(lc) < P.cut) continue;
@@ -1558,7 +1576,7 @@ DEV f3 cameraRounds(const OmvSceneView& S, const RP& P, uint64_t s, int i0, f3 o
         }
         f3 outRay, brdf, erad;
         BounceInfo bi;
-        matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, true, cs, kNee ? &bi : nullptr);
+        matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, true, cs, kNee ? &bi : nullptr, kLights ? &mi : nullptr);
         float ct = fabsf(dot3(outRay, sp.n));
         const f3 ofPrev = of;
         of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);

@@ -945,6 +945,8 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_SHADE_WAVES(kLights, kNee)) void omv
             SP sp;
             computeSurfacePoint(S, h, sp, false);
             modifyFrame(S, sp);
+            MatIn mi;   // PBR texture inputs, read once for the connections and the bounce
+            if (kLights) matInputs(S, sp, mi);
             if (kLights) {
                 const int li = W.lidx[slot];
                 const OmvLight Lt = S.lights[li];
@@ -973,7 +975,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_SHADE_WAVES(kLights, kNee)) void omv
                     f3 A = lPos, B = sp.pos, dir = mk3(0.0f, 0.0f, 0.0f);
                     float tnear = 0.0f, tfar = 0.0f;
                     if (!(intensity(lc) < P.cut)) {
-                        f3 brdf = matGetBrdf(S, P.x, sp, LC, outV);
+                        f3 brdf = matGetBrdf(S, P.x, sp, LC, outV, &mi);
                         brdf = mk3(brdf.x * c, brdf.y * c, brdf.z * c);
                         lc = mk3(brdf.x * of.x * lRad.x, brdf.y * of.y * lRad.y, brdf.z * of.z * lRad.z);
                         if (!(intensity(lc) < P.cut)) {
@@ -1010,7 +1012,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_SHADE_WAVES(kLights, kNee)) void omv
             rngRestore(cs, P.seed, STREAM_CAMERA, sample0 + slot, first ? 2u : W.csn[slot]);   // 2: the camera jitter's draws
             f3 outRay, brdf, erad;
             BounceInfo bi;
-            matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, true, cs, kNee ? &bi : nullptr);
+            matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, true, cs, kNee ? &bi : nullptr, kLights ? &mi : nullptr);
             float ct = fabsf(dot3(outRay, sp.n));
             const f3 ofPrev = of;
             of = mk3(of.x * brdf.x * ct, of.y * brdf.y * ct, of.z * brdf.z * ct);
