@@ -412,26 +412,38 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused
     XcdWalk w;
     for (w.start(nA + nB + nC, xcds); w.more(); w.next()) {
         const uint32_t e = w.e;
-        Hit h;
-        if (e < nA) {
+        // the entry's ray: one traversal call site for the three kinds (the
+        // arguments each kind's own launch would pass), then its epilogue
+        const int kind = e < nA ? 0 : (e < nA + nB ? 2 : 3);
+        float4 o, d, b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float tfar = FLT_MAX, tstop = -1.0f;
+        if (kind == 0) {
             const uint32_t p = cA.phys(e);
-            const float4 o = q.o[p], d = q.d[p];
-            float tnear = o.w;
+            o = q.o[p]; d = q.d[p];
             ++nCam;
-            const bool hit = sceneIntersect(S, xyz(o), xyz(d), tnear, FLT_MAX, h, st, cnt, rays);
+        } else if (kind == 2) {
+            const uint32_t k = cB.phys(e - nA);
+            o = sq.o[k]; d = sq.d[k]; b = sq.b[k];
+            tfar = d.w;
+            tstop = connectStop(xyz(o), xyz(b));
+            ++nShadow;
+        } else {
+            const uint32_t k = cC.phys(e - nA - nB);
+            o = eq.o[k]; d = eq.d[k];
+            tstop = FLT_MAX;
+            ++nShadow;
+        }
+        Hit h;
+        float tnear = o.w;
+        const bool hit = sceneIntersect(S, xyz(o), xyz(d), tnear, tfar, h, st, cnt, rays, tstop);
+        if (kind == 0) {
             float4 r; r.x = h.t; r.y = h.u; r.z = h.v; r.w = tnear;
             int2 id; id.x = hit ? h.inst : -1; id.y = h.prim | (h.tie ? (int)0x80000000 : 0);
             H.h[e] = r;
             H.id[e] = id;
-        } else if (e < nA + nB) {
-            const uint32_t k = cB.phys(e - nA);
-            const float4 a = sq.o[k], dd = sq.d[k], b = sq.b[k];
-            float tnear = a.w;
-            ++nShadow;
-            bool visible;
-            if (!sceneIntersect(S, xyz(a), xyz(dd), tnear, dd.w, h, st, cnt, rays, connectStop(xyz(a), xyz(b)))) {
-                visible = true;
-            } else {
+        } else if (kind == 2) {
+            bool visible = true;
+            if (hit) {
                 f3 df = sub3(hitPosition(S, h), xyz(b));
                 float sd = dot3(df, df);
                 float safeD = 10.0f * FLT_EPSILON * maxAbs(xyz(b));
@@ -441,11 +453,6 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused
             const uint32_t slot = id / (uint32_t)lc, l = id - slot * (uint32_t)lc;
             vis[(uint64_t)l * nslots + slot] = visible ? 2 : 0;
         } else {
-            const uint32_t k = cC.phys(e - nA - nB);
-            const float4 o = eq.o[k], d = eq.d[k];
-            float tnear = o.w;
-            ++nShadow;
-            const bool hit = sceneIntersect(S, xyz(o), xyz(d), tnear, FLT_MAX, h, st, cnt, rays, FLT_MAX);
             nvis[__float_as_uint(d.w)] = hit ? 0 : 2;
         }
     }
