@@ -553,6 +553,10 @@ def main():
         b_dev = B_RAY_IN + B_RAY_OUT + 128.0 * nodes / max(rays, 1) + B_TRI * tris / max(rays, 1)
         roofline["b_ray_device_bvh4"] = round(b_dev, 1)
         roofline["achieved_device_bvh4"] = round(rays_per_launch * b_dev / (dom_ms * 1e-3) / 1e9, 1)
+        if achieved > HBM_PEAK_GBS:
+            roofline["frac_note"] = ("the cache-cold model's bytes exceed what HBM could deliver in this time: repeated node and "
+                                     "triangle fetches are served by L2 and the Infinity Cache, so frac > 1 measures that reuse; "
+                                     "measured_hbm_frac is the kernel's share of HBM bandwidth")
         if traffic:
             # what the memory system actually moved (rocprofv3 counters): the
             # traversal's working set sits in L2 and the Infinity Cache, so this
