@@ -65,7 +65,8 @@ struct OmvInstance {
     float inv[12];   // world -> object, the double-precision inverse rounded to float
     float nrm[9];    // normal matrix: inverse of the transposed 3x3 (RenderMesh.h:36-43)
     int32_t mesh_id; // -1 = invalid instance
-    int32_t pad[2];
+    int32_t blas_root;   // the mesh's BLAS root node (a copy of OmvMesh::blas_root: one dependent load less per instance entry)
+    int32_t pad;
 };
 
 struct OmvMaterial {

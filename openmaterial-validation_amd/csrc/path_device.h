@@ -1112,7 +1112,7 @@ struct Trav {
             orows = nearRows(op.idir);
             curInst = inst;
             st.push(E_END, c);
-            nxt = (E_BLAS_NODE << E_KIND_SHIFT) | (uint32_t)S.meshes[in.mesh_id].blas_root;
+            nxt = (E_BLAS_NODE << E_KIND_SHIFT) | (uint32_t)in.blas_root;
             hasNext = true;
             return;
         }

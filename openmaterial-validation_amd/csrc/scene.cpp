@@ -647,7 +647,9 @@ int RenderScene::commit(std::string& log) {
         std::memcpy(d.inv, in.inv, sizeof(d.inv));
         std::memcpy(d.nrm, in.nrm, sizeof(d.nrm));
         d.mesh_id = in.meshId;
+        d.blas_root = -1;
         if (in.meshId < 0) continue;
+        d.blas_root = dmeshes[in.meshId].blas_root;
         const OmvNode4& root = deviceBuild ? blasRoots[in.meshId] : nodes[dmeshes[in.meshId].blas_root];
         float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (int k = 0; k < 4; ++k) {

@@ -543,7 +543,7 @@ DEV void packetTrace(const OmvSceneView& S, PacketRay& R, uint64_t M0, uint32_t*
             const uint64_t M2 = __ballot(ok);
             if (M2 == 0ull) continue;
             curInst = inst;
-            e = (E_BLAS_NODE << E_KIND_SHIFT) | wuni((uint32_t)S.meshes[in.mesh_id].blas_root);
+            e = (E_BLAS_NODE << E_KIND_SHIFT) | wuni((uint32_t)in.blas_root);
             M = M2;
             has = true;
             continue;
