@@ -273,7 +273,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_adjoint_kernel(OmvSceneView S, 
             BounceInfo bi;
             matGetRayAndBrdf(S, P.x, sp, pDir, outRay, brdf, erad, true, cs, &bi);
             const float wc[3] = {A[0] * rem[0], A[1] * rem[1], A[2] * rem[2]};
-            const OmvMaterial& M = S.mats[S.meshes[sp.mesh].mat_id];
+            const OmvMaterial& M = S.mats[sp.mat];
             if (bi.kind == 1 && bi.b.lobe != 0) {
                 const LobeGeo g = lobeGeo(bi.b);
                 const float col[3] = {bi.b.c0, bi.b.c1, bi.b.c2};
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_adjoint_kernel(OmvSceneView S, 
                 if (bi.roughRaw > 0.00001f && bi.roughRaw < 0.99999f)
                     scatterTex(S, T, M, sp, OMV_PBR_METALLIC_ROUGHNESS, 1, gAlpha * 2.0f * bi.b.rough * M.rough);
             } else if (bi.kind == 2) {
-                const int slot = T.omSlot[S.meshes[sp.mesh].mat_id];
+                const int slot = T.omSlot[sp.mat];
                 if (slot >= 0) {
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {

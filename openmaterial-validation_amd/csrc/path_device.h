@@ -466,6 +466,7 @@ DEV f4 texLookup(const OmvSceneView& S, int id, float u, float v) {
 // -------------------------------------------------------- surface point
 struct SP {
     int inst, mesh, face;
+    int mat;              // the mesh's material slot (read with the mesh record once per hit)
     f3 pos, n, t, b, bary;
     float tw;
     bool texNorm;
@@ -479,7 +480,7 @@ struct Hit { float t, u, v; int inst, prim; bool tie; };
 DEV void computeSurfacePoint(const OmvSceneView& S, const Hit& h, SP& sp, bool fcDensity) {
     const OmvInstance& in = S.insts[h.inst];
     const OmvMesh m = S.meshes[in.mesh_id];
-    sp.inst = h.inst; sp.mesh = in.mesh_id; sp.face = h.prim;
+    sp.inst = h.inst; sp.mesh = in.mesh_id; sp.face = h.prim; sp.mat = m.mat_id;
     sp.bary = mk3(1.0f - h.u - h.v, h.u, h.v);
     sp.texNorm = false;
     const uint32_t* f = S.faces + m.face_offset + 3u * (uint32_t)h.prim;
@@ -589,7 +590,7 @@ DEV void pbrInputs(const OmvSceneView& S, const SP& sp, const OmvMaterial& M, fl
 // light vertices, then the bounce): the same texel reads, done once
 struct MatIn { float col[4], omr[3]; };
 DEV void matInputs(const OmvSceneView& S, const SP& sp, MatIn& mi) {
-    const OmvMaterial& M = S.mats[S.meshes[sp.mesh].mat_id];
+    const OmvMaterial& M = S.mats[sp.mat];
     if (M.type == OMV_MAT_PBR) pbrInputs(S, sp, M, mi.col, mi.omr);
 }
 // mi: the inputs from matInputs at this point, or null to read them here
@@ -618,7 +619,7 @@ DEV void pbrFinish(const OmvMaterial& M, Bsdf& b, const float omr[3]) {
 // PhysicallyBasedMaterial::modifyFrame (:147-167); the other materials
 // apply the identity texture normal (OpenMaterial.cpp:66-70, DiffuseColorMaterial.cpp:35-39)
 DEV void modifyFrame(const OmvSceneView& S, SP& sp) {
-    const OmvMaterial& M = S.mats[S.meshes[sp.mesh].mat_id];
+    const OmvMaterial& M = S.mats[sp.mat];
     f3 tn = mk3(0.0f, 0.0f, 1.0f);
     if (M.type == OMV_MAT_PBR) {
         f4 t;
@@ -735,7 +736,7 @@ DEV f3 omRoughSample(const OmvMaterial& M, float alpha, const f3& I, const SP& s
 
 // RenderMaterial::getBrdf of the three materials for a connection
 DEV f3 matGetBrdf(const OmvSceneView& S, const Ext& X, const SP& sp, const f3& inV, const f3& outV, const MatIn* mi = nullptr) {
-    const OmvMaterial& M = S.mats[S.meshes[sp.mesh].mat_id];
+    const OmvMaterial& M = S.mats[sp.mat];
     if (M.type == OMV_MAT_OM) {   // OpenMaterial::getBrdf (OpenMaterial.cpp:95-163)
         if (!M.has_base) return mk3(0.0f, 0.0f, 0.0f);
         if (X.omRough) return omRoughEval(M, X.omAlpha, inV, outV, sp.n);
@@ -767,7 +768,7 @@ struct BounceInfo {
 };
 DEV void matGetRayAndBrdf(const OmvSceneView& S, const Ext& X, const SP& sp, const f3& inV, f3& outV, f3& brdf, f3& erad, bool inv,
                           Rng& r, BounceInfo* info = nullptr, const MatIn* mi = nullptr) {
-    const OmvMaterial& M = S.mats[S.meshes[sp.mesh].mat_id];
+    const OmvMaterial& M = S.mats[sp.mat];
     erad = mk3(0.0f, 0.0f, 0.0f);
     if (info) info->kind = 0;
     if (M.type == OMV_MAT_OM) {
