@@ -51,7 +51,7 @@ struct WfShadowQ {     // connection visibility rays (isConnected, BDPTRenderer.
     float4* b;         // B xyz (camera vertex), as_float(slot * lc + l)
 };
 struct WfHits {
-    float4* h;         // t, u, v, tnear after the masked re-trace loop
+    float4* h;         // t, u, v, tnear after the masked re-trace loop (written for hits only)
     int2* id;          // instance (-1: miss), primitive | tie << 31
 };
 struct WfState {
@@ -382,7 +382,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
             const bool hit = sceneIntersect(S, org, dir, tnear, FLT_MAX, h, st, cnt, rays);
             float4 r; r.x = h.t; r.y = h.u; r.z = h.v; r.w = tnear;
             int2 id; id.x = hit ? h.inst : -1; id.y = h.prim | (h.tie ? (int)0x80000000 : 0);
-            H.h[e] = r;
+            if (hit) H.h[e] = r;   // read by the shading kernels for hits only
             H.id[e] = id;
         }
     }
@@ -439,7 +439,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused
         if (kind == 0) {
             float4 r; r.x = h.t; r.y = h.u; r.z = h.v; r.w = tnear;
             int2 id; id.x = hit ? h.inst : -1; id.y = h.prim | (h.tie ? (int)0x80000000 : 0);
-            H.h[e] = r;
+            if (hit) H.h[e] = r;   // read by the shading kernels for hits only
             H.id[e] = id;
         } else if (kind == 2) {
             bool visible = true;
@@ -660,7 +660,7 @@ __global__ __launch_bounds__(OMV_BLOCK, 6) void omv_wf_trace_cam1(OmvSceneView S
         if (has) {
             float4 r; r.x = R.h.t; r.y = R.h.u; r.z = R.h.v; r.w = R.tnear;
             int2 id; id.x = hit ? R.h.inst : -1; id.y = R.h.prim | (R.h.tie ? (int)0x80000000 : 0);
-            H.h[e] = r;
+            if (hit) H.h[e] = r;   // read by the shading kernels for hits only
             H.id[e] = id;
         }
     }
@@ -792,7 +792,7 @@ __global__ __launch_bounds__(OMV_BLOCK, 4) void omv_wf_trace_pt(OmvSceneView S, 
                         } else {
                             float4 r; r.x = T.h.t; r.y = T.h.u; r.z = T.h.v; r.w = T.tnear;
                             int2 id; id.x = hit ? T.h.inst : -1; id.y = T.h.prim | (T.h.tie ? (int)0x80000000 : 0);
-                            A.H.h[k] = r;
+                            if (hit) A.H.h[k] = r;   // read by the shading kernels for hits only
                             A.H.id[k] = id;
                         }
                         busy = false;
