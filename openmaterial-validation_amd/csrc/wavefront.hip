@@ -820,15 +820,15 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_shade_light(OmvSceneView S, 
     const uint32_t total = cur.total();
     const uint64_t N = W.n;
     for (uint32_t e = gtid; e < total; e += gthreads) {
-        const float4 dd = in.d[cur.phys(e)];
-        const uint32_t slot = __float_as_uint(dd.w);
-        const f3 pDir = xyz(dd);
         const int2 id = H.id[e];
         bool emit = false, spl = false;
         f3 nPos = mk3(0.0f, 0.0f, 0.0f), nDir = nPos, sPos = nPos, sDir = nPos, ctb = nPos;
         float nTnear = 0.0f, sTn = 0.0f, sD = 0.0f;
-        uint32_t pix = 0;
-        if (id.x >= 0) {
+        uint32_t pix = 0, slot = 0;
+        if (id.x >= 0) {   // a missed light ray ends its subpath: nothing to read or write
+            const float4 dd = in.d[cur.phys(e)];
+            slot = __float_as_uint(dd.w);
+            const f3 pDir = xyz(dd);
             const float4 hr = H.h[e];
             Hit h; h.t = hr.x; h.u = hr.y; h.v = hr.z; h.inst = id.x; h.prim = id.y & 0x7FFFFFFF; h.tie = id.y < 0;
             SP sp;
