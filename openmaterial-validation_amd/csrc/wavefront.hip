@@ -168,6 +168,32 @@ DEV uint32_t segAppend(const WfSeg& g, bool want, unsigned long long* lost) {
     return r * g.cap + k;
 }
 
+// segAppend with the wave's entries ordered by key (key < nkeys <= 8): one
+// atomic per wave as before; a lane's slot in the wave's chunk is the count
+// of appending lanes with a smaller key, then of lanes before it with the
+// same key.  Connection rays use the target light as the key, so the rays
+// a traversal wave takes from one chunk head to one light together.
+DEV uint32_t segAppendKeyed(const WfSeg& g, bool want, uint32_t key, uint32_t nkeys, unsigned long long* lost) {
+    const uint64_t m = __ballot(want);
+    if (m == 0ull) return OMV_SEG_NONE;
+    const int lane = threadIdx.x & 63;
+    const uint64_t before = (1ull << lane) - 1ull;
+    uint32_t rank = 0;
+    for (uint32_t k = 0; k < nkeys; ++k) {
+        const uint64_t mk = __ballot(want && key == k);
+        rank += k < key ? (uint32_t)__popcll(mk) : (k == key ? (uint32_t)__popcll(mk & before) : 0u);
+    }
+    const uint32_t r = blockIdx.x % g.K;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0u;
+    if (lane == leader) base = atomicAdd(g.cnt + r, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (!want) return OMV_SEG_NONE;
+    const uint32_t k = base + rank;
+    if (k >= g.cap) { atomicAdd(lost, 1ull); return OMV_SEG_NONE; }
+    return r * g.cap + k;
+}
+
 DEV f3 xyz(const float4& v) { return mk3(v.x, v.y, v.z); }
 DEV float4 f4of(const f3& v, float w) { float4 r; r.x = v.x; r.y = v.y; r.z = v.z; r.w = w; return r; }
 
@@ -1005,7 +1031,11 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_SHADE_WAVES(kLights, kNee)) void omv
                             W.cc[((uint64_t)l * 3 + 2) * N + slot] = lc.z;
                         }
                     }
-                    const uint32_t k = segAppend(sqCount, ray, lost);
+                    // connections to the light itself grouped by light (key = light index),
+                    // to light subpath vertices after them (key = lights)
+                    const uint32_t nk = S.num_lights < 8 ? (uint32_t)S.num_lights + 1u : 1u;
+                    const uint32_t key = nk > 1u ? (l == 0 ? (uint32_t)li : (uint32_t)S.num_lights) : 0u;
+                    const uint32_t k = segAppendKeyed(sqCount, ray, key, nk, lost);
                     if (ray && k == OMV_SEG_NONE) v = 0;   // a full region (cannot happen with the host's sizing)
                     W.vis[(uint64_t)l * N + slot] = v;
                     if (k != OMV_SEG_NONE) {
