@@ -32,7 +32,6 @@ namespace {
 struct f3 { float x, y, z; };
 struct f4 { float x, y, z, w; };
 DEV f3 mk3(float a, float b, float c) { f3 r; r.x = a; r.y = b; r.z = c; return r; }
-DEV float comp(const f3& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
 // UtilityFunctions.h:128-247, same operation order
 DEV float dot3(const f3& a, const f3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 DEV f3 cross3(const f3& a, const f3& b) { return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
@@ -854,10 +853,6 @@ struct Stack {
         --sp;
         if (__ballot(sp >= OMV_STACK_LDS) == 0ull) return omv_lds_stack[col + (uint32_t)sp * OMV_BLOCK];   // wave-uniform: LDS only
         return sp < OMV_STACK_LDS ? omv_lds_stack[col + (uint32_t)sp * OMV_BLOCK] : glb[(uint32_t)(sp - OMV_STACK_LDS) * glbStride];
-    }
-    DEV uint32_t top() const {
-        const int t = sp - 1;
-        return t < OMV_STACK_LDS ? omv_lds_stack[col + (uint32_t)t * OMV_BLOCK] : glb[(uint32_t)(t - OMV_STACK_LDS) * glbStride];
     }
 };
 
