@@ -290,20 +290,26 @@ struct XcdWalk {
 // Camera ray of every slot (PinholeCamera::getRay, 2 draws) and, with
 // lights, the light-subpath start (tryComputeLightPath, BDPTRenderer.cpp:194-206).
 // With writeQueues = 0 the round-1 camera rays are not stored: the round-1
-// camera kernels compute them (WfGen).  Light rays are always stored: their
-// direction comes from a rejection loop, cheaper to keep than to redo.
+// camera kernels compute them (WfGen).  Light rays are stored (their
+// direction comes from a rejection loop, cheaper to keep than to redo) by
+// appending to the segmented round-1 light queue (lightSeg; the launch uses
+// the shading grid, for which the regions are sized) -- except a ray that
+// misses all four child boxes of the TLAS root: the traversal's first step
+// is that same test (boxTest4Rows, tnear 0, tfar FLT_MAX), after which it
+// ends without a hit, so such a ray is not traced.  Its trace is still
+// counted (one ray, one light ray, the root's node visit), and light round
+// 1 leaves its subpath at the light vertex, as for a traced miss.
 template <bool kLights>
 __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_gen(OmvSceneView S, omv_camera_desc cam, RP P, int spp, uint64_t sample0,
                                                         WfState W, WfQueue camQ, WfQueue lightQ, uint32_t* __restrict__ camCount,
-                                                        uint32_t* __restrict__ lightCount, float* __restrict__ rad, int writeQueues) {
+                                                        WfSeg lightSeg, float* __restrict__ rad, int writeQueues,
+                                                        unsigned long long* __restrict__ counters, unsigned long long* __restrict__ lost) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = W.n;
     const bool camRays = P.camera_bounces >= 1;
-    if (gtid == 0) {
-        *camCount = camRays ? n : 0u;
-        if (kLights) *lightCount = W.lvCap > 1 ? n : 0u;
-    }
+    if (gtid == 0) *camCount = camRays ? n : 0u;
+    uint32_t culled = 0;
     for (uint32_t slot = gtid; slot < n; slot += gthreads) {
         const uint64_t s = sample0 + slot;
         // the initial camera state (throughput 1, escape weight 1, radiance 0
@@ -335,10 +341,27 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_gen(OmvSceneView S, omv_came
             W.lplen[slot] = 1;
             W.lsn[slot] = consumed;
             if (W.lvCap > 1) {
-                lightQ.o[slot] = f4of(org, 0.0f);
-                lightQ.d[slot] = f4of(dir, __uint_as_float(slot));
+                bool trace = true;
+                if (S.tlas_root >= 0) {
+                    const f3 idir = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+                    float tent[4]; bool hit[4]; uint32_t ent[4];
+                    boxTest4Rows(S.nodes, (uint32_t)S.tlas_root, nearRows(idir), org, idir, 0.0f, FLT_MAX, tent, hit, ent);
+                    trace = false;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) trace = trace || (hit[k] && ent[k] != (uint32_t)E_NONE);
+                }
+                const uint32_t k = segAppend(lightSeg, trace, lost);
+                if (k != OMV_SEG_NONE) {
+                    lightQ.o[k] = f4of(org, 0.0f);
+                    lightQ.d[k] = f4of(dir, __uint_as_float(slot));
+                }
+                culled += trace ? 0u : 1u;
             }
         }
+    }
+    if (kLights) {
+        Counters c; c.nodes = culled; c.tris = 0; c.overflow = 0;
+        flushCounters(statSlot(counters), culled, 0u, culled, 0u, c);
     }
 }
 
@@ -1421,7 +1444,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
     // gen at the slot's index) and camera queues that are reordered by the
     // ray sort are compact (one region).
     const uint32_t camK = L.sortRays ? 1u : K, camCap = L.sortRays ? (uint32_t)Q : segCap;
-    auto lightCnt = [&](int r) { return r == 1 ? segOf(lightBase + K, 1u, (uint32_t)Q) : segOf(lightBase + K * r, K, segCap); };
+    auto lightCnt = [&](int r) { return segOf(lightBase + K * r, K, segCap); };
     auto camCnt = [&](int r) { return r == 1 ? segOf(camBase + K, 1u, (uint32_t)Q) : segOf(camBase + K * r, camK, camCap); };
     auto shCnt = [&](int r) { return segOf(shBase + K * r, K, segCap * (uint32_t)L.lc); };
     auto envCnt = [&](int r) { return segOf(envBase + K * r, K, segCap); };
@@ -1456,12 +1479,14 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
     WfGen Gen0 = Gen1;
     Gen0.gen = 0;
     const int writeQueues = Gen1.gen ? 0 : 1;
+    // with lights, gen appends the round-1 light rays: it runs on the shading
+    // grid, for which the queue regions are sized (wavefrontSegCap)
     if (lights)
-        hipLaunchKernelGGL(omv_wf_gen<true>, dim3(L.blocksGen), blk, 0, stream, S, L.cam, P, L.p.samples_per_pixel, L.sample0, W, qa, qb,
-                           camCnt(1).cnt, lightCnt(1).cnt, L.rad, writeQueues);
+        hipLaunchKernelGGL(omv_wf_gen<true>, dim3(L.blocksShade), blk, 0, stream, S, L.cam, P, L.p.samples_per_pixel, L.sample0, W, qa, qb,
+                           camCnt(1).cnt, lightCnt(1), L.rad, writeQueues, stats, lost);
     else
         hipLaunchKernelGGL(omv_wf_gen<false>, dim3(L.blocksGen), blk, 0, stream, S, L.cam, P, L.p.samples_per_pixel, L.sample0, W, qa, qb,
-                           camCnt(1).cnt, lightCnt(1).cnt, L.rad, writeQueues);
+                           camCnt(1).cnt, lightCnt(1), L.rad, writeQueues, stats, lost);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (lights) {
