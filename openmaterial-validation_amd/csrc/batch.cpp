@@ -203,7 +203,7 @@ int BatchRunner::prepareWavefront(const std::vector<int>& order) {
         if (slots * (uint64_t)lc >= (1ull << 32)) { setError("batch: too many samples per pixel for one chunk"); return OMV_E_RANGE; }
         const bool nee = (J.p.extensions & OMV_EXT_HDRI_NEE) && J.scene->view().env_tex_pdf;
         arena = std::max(arena, wavefrontArenaBytes((uint32_t)slots, lvCap, lc, lights, nee, false, false,
-                                                    cus_ * shadeCamBlocksPerCU(wfPerCu_, lights, nee)));
+                                                    cus_ * shadeCamBlocksPerCU(wfPerCu_, lights, nee), cus_ * wfPerCu_[3]));
         rad = std::max(rad, (size_t)slots * 3 * sizeof(float));
         ncnt = std::max(ncnt, wavefrontCounterCount(J.p.camera_bounces, lvCap));
     }

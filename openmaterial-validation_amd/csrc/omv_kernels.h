@@ -46,7 +46,7 @@ struct WfLaunch {
     uint64_t sample0;          // first sample of the chunk
     uint32_t nslots;           // samples in the chunk
     int lvCap, lc;             // light vertices kept, connections per camera vertex
-    void* arena;               // wavefrontArenaBytes(nslots, lvCap, lc, lights, nee, splat, sortRays, blocksShade)
+    void* arena;               // wavefrontArenaBytes(nslots, lvCap, lc, lights, nee, splat, sortRays, blocksShade, blocksGen)
     uint32_t* qcount;          // wavefrontCounterCount(B, lvCap) counters
     float* rad;                // 3 floats per sample of the chunk
     uint32_t* gstack;          // spill stacks of blocksTrace * OMV_BLOCK lanes
@@ -68,8 +68,8 @@ struct WfLaunch {
 // producing workgroups); wavefrontSegCap is the entries of one group's region
 // for a producing launch of gridShade workgroups (blocksShade).
 #define OMV_QSEG 64
-uint32_t wavefrontSegCap(uint32_t nslots, int gridShade);
-size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat, bool sortRays, int gridShade);
+uint32_t wavefrontSegCap(uint32_t nslots, int gridShade, int gridGen);
+size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat, bool sortRays, int gridShade, int gridGen);
 int wavefrontCounterCount(int B, int lvCap);
 // resident blocks per CU: [0] trace, [1] camera shading with lights + NEE,
 // [2] light shading, [3] gen, [4] persistent trace, [5] camera shading with

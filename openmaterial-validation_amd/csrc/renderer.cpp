@@ -395,7 +395,7 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     const bool nee = (p.extensions & OMV_EXT_HDRI_NEE) && S.env_tex_pdf;
     const int blocksShade = cus_ * shadeCamBlocksPerCU(wfPerCu_, lights, nee);
     if ((e = dArena_.reserve(wavefrontArenaBytes((uint32_t)slots, lvCap, lc, lights, nee, splat_ != nullptr, tuning_.sort_rays == 2,
-                                                 blocksShade))) != hipSuccess)
+                                                 blocksShade, cus_ * wfPerCu_[3]))) != hipSuccess)
         return hipFail(e, "wavefront state");
     if ((e = dQcount_.reserve(wavefrontCounterCount(p.camera_bounces, lvCap) * sizeof(uint32_t))) != hipSuccess) return hipFail(e, "queue counters");
     if ((e = dRad_.reserve(slots * 3 * sizeof(float))) != hipSuccess) return hipFail(e, "sample buffer");

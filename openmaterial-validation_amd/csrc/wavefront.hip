@@ -292,8 +292,8 @@ struct XcdWalk {
 // With writeQueues = 0 the round-1 camera rays are not stored: the round-1
 // camera kernels compute them (WfGen).  Light rays are stored (their
 // direction comes from a rejection loop, cheaper to keep than to redo) by
-// appending to the segmented round-1 light queue (lightSeg; the launch uses
-// the shading grid, for which the regions are sized) -- except a ray that
+// appending to the segmented round-1 light queue (lightSeg, whose regions
+// are sized for this launch's grid) -- except a ray that
 // misses all four child boxes of the TLAS root: the traversal's first step
 // is that same test (boxTest4Rows, tnear 0, tfar FLT_MAX), after which it
 // ends without a hit, so such a ray is not traced.  Its trace is still
@@ -1317,20 +1317,25 @@ constexpr size_t kAlign = 256;
 size_t up(size_t b) { return (b + kAlign - 1) & ~(kAlign - 1); }
 }  // namespace
 
-// Entries per append region of a queue fed by a launch of gridShade
+// Entries per append region of a queue fed by a launch of `grid`
 // workgroups over at most nslots input entries with one append each: the
 // workgroups of one region (ceil(grid / K)) handle at most OMV_BLOCK *
-// ceil(nslots / threads) entries each.  A queue holds OMV_QSEG regions
-// (>= nslots entries in all).
-uint32_t wavefrontSegCap(uint32_t nslots, int gridShade) {
-    const uint64_t g = (uint64_t)std::max(gridShade, 1), threads = g * OMV_BLOCK;
+// ceil(nslots / threads) entries each.  The producers are the shading
+// kernels (gridShade) and, for the round-1 light queue, generation
+// (gridGen); regions are sized for the larger of the two.  A queue holds
+// OMV_QSEG regions (>= nslots entries in all).
+static uint32_t segCapFor(uint32_t nslots, int grid) {
+    const uint64_t g = (uint64_t)std::max(grid, 1), threads = g * OMV_BLOCK;
     const uint64_t perBlock = (uint64_t)OMV_BLOCK * ((nslots + threads - 1) / threads);
     return (uint32_t)(((g + OMV_QSEG - 1) / OMV_QSEG) * perBlock);
 }
+uint32_t wavefrontSegCap(uint32_t nslots, int gridShade, int gridGen) {
+    return std::max(segCapFor(nslots, gridShade), segCapFor(nslots, gridGen));
+}
 
-size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat, bool sortRays, int gridShade) {
+size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat, bool sortRays, int gridShade, int gridGen) {
     const size_t N = nslots;
-    const size_t Q = (size_t)OMV_QSEG * wavefrontSegCap(nslots, gridShade);   // queue entries
+    const size_t Q = (size_t)OMV_QSEG * wavefrontSegCap(nslots, gridShade, gridGen);   // queue entries
     size_t b = 6 * up(Q * 16)                        // three ray queues
                + up(N * 16) + up(N * 8)              // hits
                + 3 * up(N * 16) + 3 * up(N * 4);     // thr, radc, conn, csn, flags, nconn
@@ -1380,7 +1385,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
     const uint32_t n = L.nslots;
     const size_t N = n;
     // queues: OMV_QSEG append regions of segCap entries
-    const uint32_t segCap = wavefrontSegCap(n, L.blocksShade);
+    const uint32_t segCap = wavefrontSegCap(n, L.blocksShade, L.blocksGen);
     const size_t Q = (size_t)OMV_QSEG * segCap;
     uint8_t* cur = static_cast<uint8_t*>(L.arena);
     auto take = [&](size_t bytes) { uint8_t* p = cur; cur += up(bytes); return p; };
@@ -1479,10 +1484,10 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
     WfGen Gen0 = Gen1;
     Gen0.gen = 0;
     const int writeQueues = Gen1.gen ? 0 : 1;
-    // with lights, gen appends the round-1 light rays: it runs on the shading
-    // grid, for which the queue regions are sized (wavefrontSegCap)
+    // with lights, gen appends the round-1 light rays (the queue regions are
+    // sized for its grid too: wavefrontSegCap)
     if (lights)
-        hipLaunchKernelGGL(omv_wf_gen<true>, dim3(L.blocksShade), blk, 0, stream, S, L.cam, P, L.p.samples_per_pixel, L.sample0, W, qa, qb,
+        hipLaunchKernelGGL(omv_wf_gen<true>, dim3(L.blocksGen), blk, 0, stream, S, L.cam, P, L.p.samples_per_pixel, L.sample0, W, qa, qb,
                            camCnt(1).cnt, lightCnt(1), L.rad, writeQueues, stats, lost);
     else
         hipLaunchKernelGGL(omv_wf_gen<false>, dim3(L.blocksGen), blk, 0, stream, S, L.cam, P, L.p.samples_per_pixel, L.sample0, W, qa, qb,

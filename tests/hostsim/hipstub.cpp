@@ -275,14 +275,18 @@ namespace omv {
 
 static size_t upb(size_t b) { return (b + 255) & ~(size_t)255; }
 
-uint32_t wavefrontSegCap(uint32_t nslots, int gridShade) {
-    const uint64_t g = (uint64_t)(gridShade > 1 ? gridShade : 1), threads = g * OMV_BLOCK;
+static uint32_t segCapFor(uint32_t nslots, int grid) {
+    const uint64_t g = (uint64_t)(grid > 1 ? grid : 1), threads = g * OMV_BLOCK;
     const uint64_t perBlock = (uint64_t)OMV_BLOCK * ((nslots + threads - 1) / threads);
     return (uint32_t)(((g + OMV_QSEG - 1) / OMV_QSEG) * perBlock);
 }
-size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat, bool sortRays, int gridShade) {
+uint32_t wavefrontSegCap(uint32_t nslots, int gridShade, int gridGen) {
+    const uint32_t a = segCapFor(nslots, gridShade), b = segCapFor(nslots, gridGen);
+    return a > b ? a : b;
+}
+size_t wavefrontArenaBytes(uint32_t nslots, int lvCap, int lc, bool lights, bool nee, bool splat, bool sortRays, int gridShade, int gridGen) {
     const size_t N = nslots;
-    const size_t Q = (size_t)OMV_QSEG * wavefrontSegCap(nslots, gridShade);
+    const size_t Q = (size_t)OMV_QSEG * wavefrontSegCap(nslots, gridShade, gridGen);
     size_t b = 6 * upb(Q * 16) + upb(N * 16) + upb(N * 8) + 3 * upb(N * 16) + 3 * upb(N * 4);
     if (lights)
         b += upb(N * 16) + 3 * upb(N * 4) + upb(N * (size_t)lvCap * 24) + upb(N * (size_t)lc * 12) + upb(N * (size_t)lc) +
@@ -314,7 +318,7 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
         const bool lights = c.S.num_lights > 0;
         const bool nee = (c.p.extensions & OMV_EXT_HDRI_NEE) && c.S.env_tex_pdf;
         const bool splat = lights && c.splat && (c.p.extensions & OMV_EXT_LIGHT_SPLAT);
-        inRange(c.arena, wavefrontArenaBytes(c.nslots, c.lvCap, c.lc, lights, nee, splat, c.sortRays, c.blocksShade), "wavefront arena");
+        inRange(c.arena, wavefrontArenaBytes(c.nslots, c.lvCap, c.lc, lights, nee, splat, c.sortRays, c.blocksShade, c.blocksGen), "wavefront arena");
         inRange(c.qcount, wavefrontCounterCount(c.p.camera_bounces, c.lvCap) * 4, "wavefront qcount");
         inRange(c.rad, (size_t)c.nslots * 12, "wavefront rad");
         inRange(c.gstack, (size_t)c.blocksTrace * OMV_BLOCK * (OMV_STACK_TOTAL - OMV_STACK_LDS) * 4, "wavefront gstack");
