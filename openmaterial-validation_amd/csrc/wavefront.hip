@@ -33,10 +33,10 @@
 #define OMV_TRACE_WAVES 5
 // Resident waves per SIMD the camera-shading kernel is compiled for (the
 // register budget it gets: 3 -> 168 VGPRs, 4 -> 128).  Measured on MI355X:
-// with point lights 4 (a few spilled registers) renders the 2M-triangle
-// instanced scene 1 % faster than 3; without them 4 spills more and gains
-// nothing on the torus knot.
-#define OMV_SHADE_WAVES(lights, nee) ((lights) && !(nee) ? 4 : 3)
+// with point lights 4 (24 bytes of spill) renders the 2M-triangle instanced
+// scene 0.2 ms faster than 3; without lights or NEE 4 (68 bytes of spill)
+// takes the torus knot's shading from 1.95 to 1.89 ms per render.
+#define OMV_SHADE_WAVES(lights, nee) (!(nee) ? 4 : 3)
 
 namespace {
 
