@@ -219,15 +219,31 @@ DEV void genCameraRay(const omv_camera_desc& cam, uint32_t seed, int spp, uint64
     const float delta = 3.0f * FLT_EPSILON * maxAbs(O);
     org = mk3(O.x + delta * dir.x, O.y + delta * dir.y, O.z + delta * dir.z);
 }
-DEV int genLightRay(const OmvSceneView& S, uint32_t seed, uint64_t s, f3& org, f3& dir, uint32_t& consumed) {
-    Rng ls; ls.init(seed, STREAM_LIGHT, s);
-    const int li = sampleLight(S, ls.next());
+DEV void genLightRay(Rng& ls, const f3& lp, f3& org, f3& dir) {   // after the light choice's draw
     dir = uniformSphere(ls);
-    consumed = rngConsumed(ls);
-    const f3 lp = ld3(S.lights[li].pos);
     const float delta = 3.0f * FLT_EPSILON * maxAbs(lp);
     org = mk3(lp.x + delta * dir.x, lp.y + delta * dir.y, lp.z + delta * dir.z);
-    return li;
+}
+// True when every light ray from lp misses the children of the TLAS root,
+// whatever direction it draws: each non-empty child lies below lp on some
+// axis by 2e-30 or more.  The direction is in the closed positive octant
+// (uniformSphere), so the origin lp + delta * dir is >= lp on every axis,
+// and on that axis the far-plane distance (hi - o) * idir of the root-box
+// test is at most about -2e-30 times idir (idir of about 1 or more: a
+// negative normal float, or -inf for a zero component); its tmax is then
+// below its tmin >= 0 and the child is missed.  The root-box test culls
+// such a ray for every direction, so the direction need not be drawn.
+DEV bool lightBelowRoot(const OmvSceneView& S, const f3& lp) {
+    const char* base = reinterpret_cast<const char*>(S.nodes) + ((size_t)(uint32_t)S.tlas_root << 7);
+    const float4 hx = ldRow(base, 16u), hy = ldRow(base, 48u), hz = ldRow(base, 80u);   // the +x, +y, +z far planes
+    const uint4 en = *reinterpret_cast<const uint4*>(base + 112u);
+    const float HX[4] = {hx.x, hx.y, hx.z, hx.w}, HY[4] = {hy.x, hy.y, hy.z, hy.w}, HZ[4] = {hz.x, hz.y, hz.z, hz.w};
+    const uint32_t E[4] = {en.x, en.y, en.z, en.w};
+    bool below = true;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        below = below && (E[k] == (uint32_t)E_NONE || lp.x - HX[k] >= 2e-30f || lp.y - HY[k] >= 2e-30f || lp.z - HZ[k] >= 2e-30f);
+    return below;
 }
 // Where the round-1 camera kernels find their rays: the camera queue, or
 // (gen) computed from the entry, which is the slot
@@ -298,7 +314,9 @@ struct XcdWalk {
 // is that same test (boxTest4Rows, tnear 0, tfar FLT_MAX), after which it
 // ends without a hit, so such a ray is not traced.  Its trace is still
 // counted (one ray, one light ray, the root's node visit), and light round
-// 1 leaves its subpath at the light vertex, as for a traced miss.
+// 1 leaves its subpath at the light vertex, as for a traced miss.  A light
+// that lies above every root child on some axis has all its rays culled
+// (lightBelowRoot); their directions are not drawn.
 template <bool kLights>
 __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_gen(OmvSceneView S, omv_camera_desc cam, RP P, int spp, uint64_t sample0,
                                                         WfState W, WfQueue camQ, WfQueue lightQ, uint32_t* __restrict__ camCount,
@@ -334,21 +352,27 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_gen(OmvSceneView S, omv_came
             // light vertex 0 is the sampled light itself (read from the light
             // table where it is needed); the light throughput starts at 1
             // in light round 1
-            f3 org, dir;
-            uint32_t consumed;
-            const int li = genLightRay(S, P.seed, s, org, dir, consumed);
+            // W.lsn (the light-stream draws consumed) is read only where a
+            // traced light ray hits, so a culled ray need not store it
+            Rng ls; ls.init(P.seed, STREAM_LIGHT, s);
+            const int li = sampleLight(S, ls.next());
             W.lidx[slot] = li;
             W.lplen[slot] = 1;
-            W.lsn[slot] = consumed;
             if (W.lvCap > 1) {
-                bool trace = true;
-                if (S.tlas_root >= 0) {
-                    const f3 idir = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
-                    float tent[4]; bool hit[4]; uint32_t ent[4];
-                    boxTest4Rows(S.nodes, (uint32_t)S.tlas_root, nearRows(idir), org, idir, 0.0f, FLT_MAX, tent, hit, ent);
-                    trace = false;
+                const f3 lp = ld3(S.lights[li].pos);
+                bool trace = !(S.tlas_root >= 0 && lightBelowRoot(S, lp));
+                f3 org = lp, dir = mk3(0.0f, 0.0f, 1.0f);
+                if (trace) {
+                    genLightRay(ls, lp, org, dir);
+                    W.lsn[slot] = rngConsumed(ls);
+                    if (S.tlas_root >= 0) {
+                        const f3 idir = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+                        float tent[4]; bool hit[4]; uint32_t ent[4];
+                        boxTest4Rows(S.nodes, (uint32_t)S.tlas_root, nearRows(idir), org, idir, 0.0f, FLT_MAX, tent, hit, ent);
+                        trace = false;
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) trace = trace || (hit[k] && ent[k] != (uint32_t)E_NONE);
+                        for (int k = 0; k < 4; ++k) trace = trace || (hit[k] && ent[k] != (uint32_t)E_NONE);
+                    }
                 }
                 const uint32_t k = segAppend(lightSeg, trace, lost);
                 if (k != OMV_SEG_NONE) {

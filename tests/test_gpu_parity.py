@@ -154,6 +154,25 @@ def test_bdpt_radiance_mixed(pkg, wl, ref, hdr, lights, f32):
     assert abs(st["light_rays"] - rst["light_rays"]) <= 0.002 * rst["light_rays"] + 4
 
 
+def test_bdpt_radiance_lights_above_and_inside(pkg, wl, ref):
+    """One light above the whole scene (its light rays all leave it and are
+    culled before their directions are drawn) and one inside it (its light
+    subpaths bounce): radiance against the oracle.  The mesh with the
+    missing material is given a material here: under point lights it makes
+    fireflies (5 pixels holding 88 % of the oracle image's sum) whose
+    values follow last-ulp differences of their paths, which the image-mean
+    bar cannot take."""
+    desc, view = wl.mixed_test_scene(hdr=False, lights=False, masked=True, missing=False)
+    desc.lights.append(((3.0, 4.0, 3.5), (40.0, 40.0, 40.0), -1.0))
+    desc.lights.append(((-1.2, -0.6, -1.0), (30.0, 20.0, 10.0), -1.0))
+    img, rimg, st, rst = _render_pair(pkg, ref, desc, view, samples_per_pixel=8, camera_bounces=4, light_bounces=4,
+                                      max_path_length=5, seed=17)
+    _radiance_close(img, rimg)
+    assert st["stack_overflows"] == 0
+    assert abs(st["light_rays"] - rst["light_rays"]) <= 0.002 * rst["light_rays"] + 4
+    assert rst["light_rays"] > view.width * view.height * 8
+
+
 def test_bdpt_radiance_torus_knot_small(pkg, wl, ref):
     desc, view, params = wl.hdri_torus_knot(width=64, height=64, spp=4, depth=5, tex=128, hdr_w=256, hdr_h=128, segments=200, tube=20)
     params["samples_per_pixel"] = 4

@@ -66,6 +66,27 @@ def test_schedules_agree_mixed(pkg, wl, name):
     _same(_both(pkg, desc, view, **params))
 
 
+@pytest.mark.parametrize("place", ["above", "below", "one_axis", "both"])
+def test_light_placement_keeps_the_film(pkg, wl, place):
+    """Light rays draw directions in the positive octant only, so a light
+    above every child of the TLAS root on some axis has all its rays culled
+    before their directions are drawn (the wavefront schedule), while the
+    per-sample schedule traces every one.  Lights above the scene, below it
+    (rays traced, light subpaths grow), above it on x only, and one of each:
+    films and ray counts must agree bit for bit."""
+    lights = {"above": [((3.0, 4.0, 3.5), (40.0, 40.0, 40.0), -1.0)],
+              "below": [((-1.2, -0.6, -1.0), (30.0, 20.0, 10.0), -1.0)],
+              "one_axis": [((6.0, -0.5, -0.5), (30.0, 30.0, 30.0), 20.0)],
+              "both": [((3.0, 4.0, 3.5), (40.0, 40.0, 40.0), -1.0), ((-1.2, -0.6, -1.0), (30.0, 20.0, 10.0), -1.0)]}[place]
+    desc, view = wl.mixed_test_scene(hdr=True, lights=False, masked=True)
+    desc.lights.extend(lights)
+    view.width, view.height = 64, 48
+    out = _both(pkg, desc, view, samples_per_pixel=3, camera_bounces=4, light_bounces=4, max_path_length=6, seed=13)
+    _same(out)
+    if place in ("below", "both"):
+        assert out[0][1]["light_rays"] > view.width * view.height * 3   # light subpaths past their first ray
+
+
 def test_schedules_agree_torus_knot(pkg, wl):
     desc, view, params = wl.hdri_torus_knot(width=96, height=96, spp=4, tex=128, hdr_w=256, hdr_h=128, segments=200, tube=16)
     _same(_both(pkg, desc, view, **params))
