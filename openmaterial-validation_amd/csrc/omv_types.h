@@ -119,6 +119,8 @@ struct OmvSceneView {
     int32_t any_emissive;
     int32_t any_masked;   // some PBR material uses alpha MASK (re-traced cut-outs): no occlusion early-out
     int32_t closest_only; // 1: visibility rays take the closest hit too (the occlusion early-out off)
+    int32_t lights_leave; // 1: every light lies above each child box of the TLAS root on some axis, so every
+                          // light ray (positive-octant directions) misses the scene and no light round has rays
     float   scene_lo[3], scene_inv[3];   // world box of the instances, 1 / its extent (0 if flat)
     int32_t bg_type, bg_w, bg_h;
     float   bg_const[3];

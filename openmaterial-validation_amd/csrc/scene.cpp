@@ -694,6 +694,21 @@ int RenderScene::commit(std::string& log) {
             }
         }
     }
+    // Light rays take directions in the closed positive octant only
+    // (uniformSphere), so a light that lies above every child box of the TLAS
+    // root on some axis -- by 2e-30 or more, the generation kernel's own test
+    // (lightBelowRoot) -- emits rays that all miss the scene.  When every
+    // light does, no light round has rays and the renderer skips them.
+    int lightsLeave = 0;
+    if (tl.root >= 0 && (size_t)tl.root < nodes.size() && !lights_.empty()) {
+        const OmvNode4& r = nodes[tl.root];
+        lightsLeave = 1;
+        for (const OmvLight& L : lights_)
+            for (int k = 0; k < 4; ++k)
+                if (r.entry[k] != OMV_ENT_NONE &&
+                    !(L.pos[0] - r.hix[k] >= 2e-30f || L.pos[1] - r.hiy[k] >= 2e-30f || L.pos[2] - r.hiz[k] >= 2e-30f))
+                    lightsLeave = 0;
+    }
     // device build: the TLAS (built above from index 0) goes after the BLAS nodes
     if (deviceBuild) {
         for (OmvNode4& nd : nodes)
@@ -772,6 +787,7 @@ int RenderScene::commit(std::string& log) {
     v.num_meshes = (int)meshes_.size(); v.num_insts = (int)insts_.size(); v.num_mats = (int)mats_.size();
     v.num_texs = (int)texs_.size(); v.num_lights = (int)lights_.size();
     v.tlas_root = tl.root;
+    v.lights_leave = lightsLeave;
     v.total_faces = faceBase;
     v.fc_subject = fcSubject_;
     v.any_emissive = anyEmissive;

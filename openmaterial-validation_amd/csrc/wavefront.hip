@@ -360,7 +360,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_gen(OmvSceneView S, omv_came
             W.lplen[slot] = 1;
             if (W.lvCap > 1) {
                 const f3 lp = ld3(S.lights[li].pos);
-                bool trace = !(S.tlas_root >= 0 && lightBelowRoot(S, lp));
+                bool trace = !(S.tlas_root >= 0 && (S.lights_leave || lightBelowRoot(S, lp)));
                 f3 org = lp, dir = mk3(0.0f, 0.0f, 1.0f);
                 if (trace) {
                     genLightRay(ls, lp, org, dir);
@@ -1518,7 +1518,9 @@ hipError_t launchWavefront(const WfLaunch& L, hipStream_t stream) {
                            camCnt(1).cnt, lightCnt(1), L.rad, writeQueues, stats, lost);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (lights) {
+    // with S.lights_leave generation queues no light ray, and the light
+    // rounds would only find empty queues
+    if (lights && !S.lights_leave) {
         WfQueue in = qb, out = qc;
         for (int i = 1; i < L.lvCap; ++i) {
             mark();
