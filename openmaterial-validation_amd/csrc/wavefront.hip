@@ -357,7 +357,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_gen(OmvSceneView S, omv_came
             Rng ls; ls.init(P.seed, STREAM_LIGHT, s);
             const int li = sampleLight(S, ls.next());
             W.lidx[slot] = li;
-            W.lplen[slot] = 1;
+            if (!S.lights_leave) W.lplen[slot] = 1;   // with lights_leave every subpath keeps 1 vertex: not stored
             if (W.lvCap > 1) {
                 const f3 lp = ld3(S.lights[li].pos);
                 bool trace = !(S.tlas_root >= 0 && (S.lights_leave || lightBelowRoot(S, lp)));
@@ -1006,7 +1006,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_SHADE_WAVES(kLights, kNee)) void omv
         float nTnear = 0.0f;
         if (id.x < 0) {
             // escape (BDPTRenderer.cpp:251-258); no connections this round
-            if (kLights) W.nconn[slot] = 0u;
+            // (the path is written here, so the connect kernel reads nothing else of it)
             if (S.bg_type != OMV_BG_NONE && n <= q) {
                 f3 bg = background(S, pDir);
                 bg = mk3(bg.x * of.x, bg.y * of.y, bg.z * of.z);
@@ -1031,7 +1031,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_SHADE_WAVES(kLights, kNee)) void omv
                 const int li = W.lidx[slot];
                 const OmvLight Lt = S.lights[li];
                 const f3 lightInt = ld3(Lt.intensity);
-                const int lEnd = min(W.lplen[slot], q - i + 1);
+                const int lEnd = min(S.lights_leave ? 1 : W.lplen[slot], q - i + 1);
                 const f3 outV = neg3(pDir);
                 W.nconn[slot] = (uint32_t)max(lEnd, 0);
                 for (int l = 0; l < lEnd; ++l) {
@@ -1176,8 +1176,12 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_connect(WfState W, WfQueue i
             }
             W.conn[slot] = f4of(conn, 0.0f);
         }
+        // the radiance so far is needed by a visible next-event estimate and
+        // by a path that ends this round
+        const bool neeVis = nee && W.nvis[slot] == 2;
+        if (!neeVis && fl != 1u) continue;
         float4 rc = W.radc[slot];
-        if (nee && W.nvis[slot] == 2) {
+        if (neeVis) {
             const f3 c = xyz(W.neeC[slot]);
             rc.x = rc.x + c.x; rc.y = rc.y + c.y; rc.z = rc.z + c.z;
             W.radc[slot] = rc;
@@ -1228,7 +1232,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_tail(OmvSceneView S, RP P, u
         f3 conn = mk3(0.0f, 0.0f, 0.0f);
         if (kLights) {
             ls.idx = W.lidx[slot];
-            ls.len = W.lplen[slot];
+            ls.len = S.lights_leave ? 1 : W.lplen[slot];
             const OmvLight Lt = S.lights[ls.idx];
             ls.pos = ld3(Lt.pos); ls.intensity = ld3(Lt.intensity); ls.range = Lt.range;
             conn = xyz(W.conn[slot]);
