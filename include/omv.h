@@ -391,7 +391,9 @@ typedef struct omv_render_tuning {
                                  one launch, each path carried to its end by
                                  one lane (same film): 0 = default, 1 = never
                                  (every round as its own trace, shade and
-                                 connect launches), n >= 2 = from round n    */
+                                 connect launches), n >= 2 = from round n;
+                                 the default is 5 with point lights or NEE,
+                                 else 4                                      */
 } omv_render_tuning;
 int omv_renderer_set_tuning(omv_renderer* r, const omv_render_tuning* t);
 int omv_renderer_get_tuning(const omv_renderer* r, omv_render_tuning* t);

@@ -26,8 +26,14 @@ constexpr uint64_t kWfSlotsLights = 1ull << 25;
 // -> tail from round 3 / 4 / 5 / 6): 2M-triangle instanced scene 1080p x 16
 // 17.97 -> 21.22 / 17.99 / 17.64 / 17.81; torus knot 512^2 x 64 11.43 ->
 // 11.55 / 10.73 / 10.82 / 10.99.  From round 5 both gain; earlier the tail
-// launch holds too many paths, each traced one lane at a time.
-constexpr int kTailRoundDefault = 5;
+// launch holds too many paths, each traced one lane at a time.  Re-measured
+// once the traversal and shading had become about 40 % faster (round by round
+// -> tail from round 4 / 5 / 6 / 7): instanced (point lights: a connection
+// trace per bounce) 11.13 -> 11.31 / 10.93 / 11.01 / 10.97; torus knot (no
+// visibility rays) 6.68 -> 6.25 / 6.31 / 6.48 / 6.56.  So from round 5 with
+// point lights or NEE, from round 4 without.
+constexpr int kTailRoundVisibility = 5;
+constexpr int kTailRoundPlain = 4;
 // Round-1 camera rays are traced as wave packets (omv_wf_trace_cam1) only on
 // request (tuning traversal = 2).  Measured on MI355X (kernel ms per render,
 // per-ray -> packets), first with the branchy triangle setup: at 64 samples
@@ -415,7 +421,8 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     // kernels 9.50 -> 8.95 ms, icosphere 256^2 x 64 1.64 -> 1.53 ms), the plain
     // stride on many-instance scenes (instanced 1080p 15.85 vs 16.03 ms)
     L.xcdBands = tuning_.xcd_bands > 0 ? tuning_.xcd_bands : (S.num_insts <= 4 ? 2 * xcds_ : 1);
-    L.tailRound = tuning_.tail_round == 1 ? 0 : (tuning_.tail_round >= 2 ? tuning_.tail_round : kTailRoundDefault);
+    L.tailRound = tuning_.tail_round == 1 ? 0
+                 : (tuning_.tail_round >= 2 ? tuning_.tail_round : (lights || nee ? kTailRoundVisibility : kTailRoundPlain));
     const bool packets = tuning_.traversal == 2;
     L.blocksPacket = packets ? cus_ * wfPerCu_[8] : 0;
     // an event pair around every traversal launch: (2 light + 3 camera rounds) per chunk
