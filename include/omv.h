@@ -384,7 +384,7 @@ typedef struct omv_render_tuning {
                                  on one XCD when n is a multiple of the XCD
                                  count): the rays an XCD traces at a time come
                                  from a few stretches of the queue; 1 = plain
-                                 grid stride; 0 = default: two bands per XCD
+                                 grid stride; 0 = default: four bands per XCD
                                  on scenes of at most 4 instances, else the
                                  plain stride                               */
     int tail_round;           /* camera rounds from this one to the last run as

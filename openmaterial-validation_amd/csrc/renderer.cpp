@@ -416,11 +416,15 @@ int BDPTRenderer::renderWavefront(PinholeCamera& cam, double& kernelMs) {
     L.traceKeep = tuning_.trace_keep;
     L.splat = splat_;
     L.sortRays = tuning_.sort_rays == 2;
-    // default hand-out (measured on MI355X, entry-by-entry traversal): two
-    // bands per XCD on scenes of at most 4 instances (torus knot 512^2 x 64
-    // kernels 9.50 -> 8.95 ms, icosphere 256^2 x 64 1.64 -> 1.53 ms), the plain
-    // stride on many-instance scenes (instanced 1080p 15.85 vs 16.03 ms)
-    L.xcdBands = tuning_.xcd_bands > 0 ? tuning_.xcd_bands : (S.num_insts <= 4 ? 2 * xcds_ : 1);
+    // default hand-out (measured on MI355X, entry-by-entry traversal): bands
+    // of the queue dealt to the XCDs on scenes of at most 4 instances -- two
+    // per XCD when introduced (torus knot 512^2 x 64 kernels 9.50 -> 8.95 ms,
+    // icosphere 256^2 x 64 1.64 -> 1.53 ms), four per XCD once the kernels had
+    // become faster (2 / 4 / 8 per XCD: torus knot 6.27 / 6.16 / 6.23 ms,
+    // icosphere 0.565 / 0.550 / 0.558 ms) -- and the plain stride on
+    // many-instance scenes (instanced 1080p 15.85 vs 16.03 ms; later 10.92 vs
+    // 11.00 with one band per XCD)
+    L.xcdBands = tuning_.xcd_bands > 0 ? tuning_.xcd_bands : (S.num_insts <= 4 ? 4 * xcds_ : 1);
     L.tailRound = tuning_.tail_round == 1 ? 0
                  : (tuning_.tail_round >= 2 ? tuning_.tail_round : (lights || nee ? kTailRoundVisibility : kTailRoundPlain));
     const bool packets = tuning_.traversal == 2;

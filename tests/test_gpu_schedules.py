@@ -231,10 +231,12 @@ def test_occlusion_early_out_keeps_the_film(pkg, wl, case):
 @pytest.mark.parametrize("size", ["small", "large"])
 def test_xcd_banded_handout_keeps_the_film(pkg, wl, size):
     """The traversal kernels' XCD-banded hand-out of queue entries (tuning
-    xcd_bands: 0 = the device's XCD count, n = n bands, 1 = plain grid
-    stride) visits every entry exactly once: films bit for bit and ray counts
-    are those of the plain stride, for band counts that divide the grid
-    (2, 16, the default) and one that does not (3: plain stride).  The large
+    xcd_bands: 0 = default, four bands per XCD on scenes of at most 4
+    instances, n = n bands, 1 = plain grid stride) visits every entry exactly
+    once: films bit for bit and ray counts are those of the plain stride, for
+    band counts that divide the grid (2, 16, 32 -- the default count on a
+    few-instance scene -- and the default) and one that does not (3: plain
+    stride).  The large
     case (512 x 512 x 8 = 2.1M samples) reaches the largest chunk size."""
     desc, view = wl.mixed_test_scene(hdr=True, lights=True, masked=False)
     spp, w, h = (3, 96, 64) if size == "small" else (8, 512, 512)
@@ -243,7 +245,7 @@ def test_xcd_banded_handout_keeps_the_film(pkg, wl, size):
     desc.apply(s, pkg.pbr_desc)
     cam = pkg.Camera(w, h, view.position, view.forward, view.up, view.yfov)
     out = {}
-    for bands in (1, 0, 2, 3, 16):
+    for bands in (1, 0, 2, 3, 16, 32):
         r = pkg.Renderer(pkg.default_params(**params))
         r.set_scene(s)
         r.set_tuning(xcd_bands=bands)
