@@ -488,7 +488,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused
         // the entry's ray: one traversal call site for the three kinds (the
         // arguments each kind's own launch would pass), then its epilogue
         const int kind = e < nA ? 0 : (e < nA + nB ? 2 : 3);
-        float4 o, d, b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float4 o, d;
         float tfar = FLT_MAX, tstop = -1.0f;
         if (kind == 0) {
             const uint32_t p = cA.phys(e);
@@ -496,9 +496,9 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused
             ++nCam;
         } else if (kind == 2) {
             const uint32_t k = cB.phys(e - nA);
-            o = sq.o[k]; d = sq.d[k]; b = sq.b[k];
+            o = sq.o[k]; d = sq.d[k];
             tfar = d.w;
-            tstop = connectStop(xyz(o), xyz(b));
+            tstop = connectStop(xyz(o), xyz(sq.b[k]));   // the target is read again after the traversal
             ++nShadow;
         } else {
             const uint32_t k = cC.phys(e - nA - nB);
@@ -515,6 +515,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused
             if (hit) H.h[e] = r;   // read by the shading kernels for hits only
             H.id[e] = id;
         } else if (kind == 2) {
+            const float4 b = sq.b[cB.phys(e - nA)];   // not held in registers across the traversal
             bool visible = true;
             if (hit) {
                 f3 df = sub3(hitPosition(S, h), xyz(b));
@@ -526,7 +527,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused
             const uint32_t slot = id / (uint32_t)lc, l = id - slot * (uint32_t)lc;
             vis[(uint64_t)l * nslots + slot] = visible ? 2 : 0;
         } else {
-            nvis[__float_as_uint(d.w)] = hit ? 0 : 2;
+            nvis[__float_as_uint(eq.d[cC.phys(e - nA - nB)].w)] = hit ? 0 : 2;   // the slot, read again
         }
     }
     flushCounters(statSlot(counters), rays, nCam, 0u, nShadow, cnt);
