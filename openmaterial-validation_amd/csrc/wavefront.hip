@@ -406,7 +406,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
     Stack st;
     stackInit(st, gstack, gtid, gthreads);
     Counters cnt; cnt.nodes = 0; cnt.tris = 0; cnt.overflow = 0;
-    uint32_t rays = 0, nCam = 0, nLight = 0, nShadow = 0;
+    uint32_t rays = 0;
     XcdWalk w;
     __shared__ SegTable cur;
     segLoad(cur, count);
@@ -417,7 +417,6 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
         if (kKind == 2) {
             const float4 a = sq.o[p], dd = sq.d[p], b = sq.b[p];
             float tnear = a.w;
-            ++nShadow;
             bool visible;
             if (!sceneIntersect(S, xyz(a), xyz(dd), tnear, dd.w, h, st, cnt, rays, connectStop(xyz(a), xyz(b)))) {
                 visible = true;
@@ -433,13 +432,11 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
         } else if (kKind == 3) {
             const float4 o = q.o[p], d = q.d[p];
             float tnear = o.w;
-            ++nShadow;
             const bool hit = sceneIntersect(S, xyz(o), xyz(d), tnear, FLT_MAX, h, st, cnt, rays, FLT_MAX);
             vis[__float_as_uint(d.w)] = hit ? 0 : 2;
         } else if (kKind == 4) {
             const float4 a = sq.o[p], dd = sq.d[p], b = sq.b[p];
             float tnear = a.w;
-            ++nShadow;
             const bool hit = sceneIntersect(S, xyz(a), xyz(dd), tnear, dd.w, h, st, cnt, rays, FLT_MAX);
             splatAdd(splat, !hit, __float_as_uint(b.w), xyz(b));
         } else {
@@ -451,7 +448,6 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
                 const float4 o = q.o[p], d = q.d[p];
                 org = xyz(o); dir = xyz(d); tnear = o.w;
             }
-            if (kKind == 0) ++nCam; else ++nLight;
             const bool hit = sceneIntersect(S, org, dir, tnear, FLT_MAX, h, st, cnt, rays);
             float4 r; r.x = h.t; r.y = h.u; r.z = h.v; r.w = tnear;
             int2 id; id.x = hit ? h.inst : -1; id.y = h.prim | (h.tie ? (int)0x80000000 : 0);
@@ -459,7 +455,10 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace(OmvSc
             H.id[e] = id;
         }
     }
-    flushCounters(statSlot(counters), rays, nCam, nLight, nShadow, cnt);
+    // every entry below the total is traced once, so the ray kinds are
+    // counted once per launch rather than per lane
+    const uint32_t all = blockIdx.x == 0 && threadIdx.x == 0 ? cur.total() : 0u;
+    flushCounters(statSlot(counters), rays, kKind == 0 ? all : 0u, kKind == 1 ? all : 0u, kKind >= 2 ? all : 0u, cnt);
 }
 
 // Fused traversal of one camera round boundary: the camera extension rays of
@@ -478,7 +477,7 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused
     Stack st;
     stackInit(st, gstack, gtid, gthreads);
     Counters cnt; cnt.nodes = 0; cnt.tris = 0; cnt.overflow = 0;
-    uint32_t rays = 0, nCam = 0, nShadow = 0;
+    uint32_t rays = 0;
     __shared__ SegTable cA, cB, cC;
     segLoad(cA, qCount); segLoad(cB, sqCount); segLoad(cC, eqCount);
     const uint32_t nA = cA.total(), nB = cB.total(), nC = cC.total();
@@ -493,18 +492,15 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused
         if (kind == 0) {
             const uint32_t p = cA.phys(e);
             o = q.o[p]; d = q.d[p];
-            ++nCam;
         } else if (kind == 2) {
             const uint32_t k = cB.phys(e - nA);
             o = sq.o[k]; d = sq.d[k];
             tfar = d.w;
             tstop = connectStop(xyz(o), xyz(sq.b[k]));   // the target is read again after the traversal
-            ++nShadow;
         } else {
             const uint32_t k = cC.phys(e - nA - nB);
             o = eq.o[k]; d = eq.d[k];
             tstop = FLT_MAX;
-            ++nShadow;
         }
         Hit h;
         float tnear = o.w;
@@ -530,7 +526,9 @@ __global__ __launch_bounds__(OMV_BLOCK, OMV_TRACE_WAVES) void omv_wf_trace_fused
             nvis[__float_as_uint(eq.d[cC.phys(e - nA - nB)].w)] = hit ? 0 : 2;   // the slot, read again
         }
     }
-    flushCounters(statSlot(counters), rays, nCam, 0u, nShadow, cnt);
+    // every entry is traced once: the kinds are counted once per launch
+    const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+    flushCounters(statSlot(counters), rays, first ? nA : 0u, 0u, first ? nB + nC : 0u, cnt);
 }
 
 // ------------------------------------------------ packet traversal (round 1)
