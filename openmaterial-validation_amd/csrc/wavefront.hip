@@ -37,6 +37,14 @@
 // scene 0.2 ms faster than 3; without lights or NEE 4 (68 bytes of spill)
 // takes the torus knot's shading from 1.95 to 1.89 ms per render.
 #define OMV_SHADE_WAVES(lights, nee) (!(nee) ? 4 : 3)
+// Resident waves per SIMD the tail launch is compiled for (1: no bound, the
+// register count the compiler picks).  Measured on MI355X: the light-free
+// tail at 4 (128 VGPRs, 152 bytes of spill) instead of 3 (167 VGPRs) takes
+// the torus knot 512^2 x 64 from 6.14 to 6.00 ms per render -- the tail
+// holds few paths per wave and gains from more waves in flight; with point
+// lights 3 waves (168 VGPRs) instead of 2 (221) is 0.03 ms slower on the
+// instanced scene.
+#define OMV_TAIL_WAVES(lights, nee) ((lights) || (nee) ? 1 : 4)
 
 namespace {
 
@@ -1210,7 +1218,7 @@ __global__ __launch_bounds__(OMV_BLOCK) void omv_wf_connect(WfState W, WfQueue i
 // they cost the longest remaining path once.  Sums are formed in the same
 // order, so the film is unchanged.
 template <bool kLights, bool kNee>
-__global__ __launch_bounds__(OMV_BLOCK) void omv_wf_tail(OmvSceneView S, RP P, uint64_t sample0, int i0, WfState W, WfQueue in,
+__global__ __launch_bounds__(OMV_BLOCK, OMV_TAIL_WAVES(kLights, kNee)) void omv_wf_tail(OmvSceneView S, RP P, uint64_t sample0, int i0, WfState W, WfQueue in,
                                                          WfSeg inCount, float* __restrict__ rad, uint32_t* __restrict__ gstack,
                                                          unsigned long long* __restrict__ counters) {
     const uint32_t gthreads = gridDim.x * blockDim.x;
