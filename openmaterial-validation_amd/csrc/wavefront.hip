@@ -43,7 +43,8 @@
 // the torus knot 512^2 x 64 from 6.14 to 6.00 ms per render -- the tail
 // holds few paths per wave and gains from more waves in flight; with point
 // lights 3 waves (168 VGPRs) instead of 2 (221) is 0.03 ms slower on the
-// instanced scene.
+// instanced scene, and with NEE alone 3 instead of 2 (196 VGPRs) is equal on
+// the torus knot (9.50 ms).
 #define OMV_TAIL_WAVES(lights, nee) ((lights) || (nee) ? 1 : 4)
 
 namespace {
